@@ -1,0 +1,180 @@
+/*
+ * pyto_hip.h -- C ABI of libpytohip.so, the MI355X (gfx950) implementation of
+ * PyTO's voxel-hex structural FEA path: the SIMP-scaled 8-node-hex operator,
+ * the Jacobi-preconditioned deflated PCG, the cone filter and the element
+ * kernels of the compliance and p-norm stress sensitivities.
+ *
+ * The reference (UW-ERSL/PyTO) is pure Python with no FFI layer; each entry
+ * point below names the reference function or expression it replaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative PYTO_E* code; the
+ *     message of the last error on the calling thread is pyto_last_error().
+ *     No C++ exception crosses this boundary.
+ *   - All array arguments are caller-owned DEVICE pointers (e.g. torch tensor
+ *     data_ptr()), except where a name ends in _host.  Vectors over DOFs use
+ *     the reference's AoS order: dof = 3*node + component, node index
+ *     ix + (nx+1)*iy + (nx+1)*(ny+1)*iz (src/hex_mesher.py:104-109,1085-1093).
+ *     Element arrays use element index ix + nx*iy + nx*ny*iz
+ *     (src/hex_mesher.py:117-124).
+ *   - `stream` is a hipStream_t passed as void* (0 = null stream).  Calls are
+ *     asynchronous except the drivers that return scalars to the host.
+ *   - A handle owns only its constants and scratch workspace, which
+ *     pyto_destroy frees.  One thread at a time may use a handle.
+ */
+#ifndef PYTO_HIP_H
+#define PYTO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYTO_OK 0
+#define PYTO_EINVAL -1     /* bad argument (shape, range, null pointer) */
+#define PYTO_EHIP -2       /* HIP runtime error */
+#define PYTO_ENOMEM -3     /* device allocation failed */
+#define PYTO_ESTATE -4     /* call out of order (e.g. solve before density) */
+#define PYTO_ENOCONV -5    /* solver stopped at maxIters (result still written) */
+
+typedef struct pyto_handle pyto_handle;
+
+/* Solver statistics written by the solve drivers. */
+typedef struct pyto_solve_stats {
+  int iterations;        /* inner PCG iterations (sum over refinement steps) */
+  int outer_iterations;  /* refinement steps (1 for the plain FP64 solve) */
+  int converged;         /* 1 if the stopping rule was met */
+  int matvecs64;         /* FP64 operator applications */
+  int matvecs32;         /* FP32 operator applications */
+  double rel;            /* final sqrt(r^T M r / r0^T M r0) */
+  double rel_true;       /* ||f - K u|| / ||f|| recomputed in FP64 at exit */
+  double setup_ms;       /* density set-up time inside the call, if any */
+} pyto_solve_stats;
+
+const char* pyto_last_error(void);
+int pyto_version(void);
+
+/* Element stiffness KE (24x24, row-major FP64) of one brick element.
+ * Replaces hex8_stiffness_matrix_structural, src/hex_element_stiffness.py:231-286. */
+int pyto_ke_hex8(double E, double nu, double hx, double hy, double hz, double* ke_host);
+
+/* Create the grid operator on the current device.
+ * nelx/nely/nelz, element size and material replace HexMesher.grid_mesh +
+ * HexStructuralFEA.__init__ (src/hex_mesher.py:72-169,
+ * src/hex_structural_fea.py:21-52).  fixed_mask: uint8 per node, bit c set if
+ * DOF 3*node+c is fixed (BC.fixed_dofs, src/bound_cond.py:8-32); only zero
+ * Dirichlet values are supported.  penal/eps_void parametrise the SIMP law
+ * E_e = E0*(eps + x^penal (1-eps)) (src/topopt_material_model.py:88). */
+int pyto_create(pyto_handle** out, int nelx, int nely, int nelz,
+                double hx, double hy, double hz, double E0, double nu,
+                double penal, double eps_void, const uint8_t* fixed_mask,
+                void* stream);
+int pyto_destroy(pyto_handle* h);
+int pyto_info(pyto_handle* h, int64_t* nnodes, int64_t* nelems, int64_t* ndof,
+              int64_t* workspace_bytes);
+
+/* Densities x (FP64, per element) -> E_e (FP64 and FP32 copies) and the
+ * Jacobi inverse diagonal; if deflation groups are set, also assembles the
+ * coarse Galerkin matrix sym(W^T K W) + 1e-10 I and factorises it (FP64).
+ * Replaces the per-solve part of HexStructuralFEA.solve
+ * (src/hex_structural_fea.py:87-102), _jacobi_preconditioner
+ * (src/linear_solvers.py:71-88) and the set-up of deflatedPCG_CPU
+ * (src/deflation.py:514-521). */
+int pyto_set_density(pyto_handle* h, const double* x, void* stream);
+
+/* Deflation groups: node_group (int32 per node, 0..ngroups-1) and group
+ * centres (FP64, ngroups x 3).  W is implicit (rigid-body modes about the
+ * centres, column map u:{0,4,5} v:{1,3,5} w:{2,3,4}); replaces
+ * DeflationSolver.create_deflation_matrix (src/deflation.py:386-458).
+ * node_group and centers are device pointers; the handle keeps copies. */
+int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
+                    const double* centers, void* stream);
+
+/* y = K_ff u (fixed DOFs of u ignored, of y zeroed): the SIMP-scaled
+ * matrix-free product replacing K@p with the assembled CSR K
+ * (src/deflation.py:540,558,573; src/hex_structural_fea.py:101). */
+int pyto_matvec_f64(pyto_handle* h, const double* u, double* y, void* stream);
+int pyto_matvec_f32(pyto_handle* h, const float* u, float* y, void* stream);
+
+/* diag(K_ff) (fixed DOFs 0); replaces A.diagonal() in
+ * _jacobi_preconditioner (src/linear_solvers.py:84). */
+int pyto_diagonal_f64(pyto_handle* h, double* d, void* stream);
+
+/* Coarse operators (need groups + density).  restrict: t = W^T v (6G);
+ * prolong: v = W mu (fixed DOFs 0); coarse_solve: mu = E^{-1} t with the
+ * FP64 factor; replaces WT@v, W@mu, spsolve(WKW, .) (src/deflation.py:533-551). */
+int pyto_restrict_f64(pyto_handle* h, const double* v, double* t, void* stream);
+int pyto_prolong_f64(pyto_handle* h, const double* mu, double* v, void* stream);
+int pyto_coarse_solve_f64(pyto_handle* h, const double* t, double* mu, void* stream);
+/* Diagonal shift of the coarse matrix: E = sym(W^T K W) + (abs + rel*|E_ii|) on
+ * the diagonal.  Default abs = 1e-10, rel = 0: the reference's
+ * `WKW += eye * 1e-10` (src/deflation.py:521).  That absolute shift makes the
+ * deflation inexact (the iteration stalls below rtol ~1e-9, and earlier when
+ * void elements make coarse entries small); abs = rel = 0 gives the exact
+ * Galerkin matrix.  Takes effect at the next pyto_set_density. */
+int pyto_set_coarse_shift(pyto_handle* h, double abs_shift, double rel_shift);
+/* Copy the coarse Galerkin matrix (6G x 6G FP64, device) for inspection. */
+int pyto_coarse_matrix_f64(pyto_handle* h, double* E, void* stream);
+
+/* Plain FP64 deflated PCG: the reference algorithm of deflatedPCG_CPU
+ * (src/deflation.py:504-584) with the stopping rule sqrt(rz/rz0) <= rtol and
+ * maxIters; f and u are full-length DOF vectors (u[fixed] = 0), i.e.
+ * linear_solvers.solve(..., Solvers.DPCG, ...) (src/linear_solvers.py:177-202).
+ * Synchronises `stream`; returns PYTO_ENOCONV (u still written) at maxIters. */
+int pyto_solve_dpcg_f64(pyto_handle* h, const double* f, double* u, double rtol,
+                        int max_iters, pyto_solve_stats* stats, void* stream);
+
+/* FP32 deflated PCG inside FP64 iterative refinement: the outer residual
+ * f - K u is recomputed in FP64, inner products accumulate in FP64, the coarse
+ * factor is FP64; stops when the FP64 sqrt(r^T M r / r0^T M r0) <= rtol
+ * (the reference's rule on the true residual).  No reference counterpart:
+ * this is the precision form of the same solve. */
+int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rtol,
+                          double inner_rtol, int max_iters, int max_outer,
+                          pyto_solve_stats* stats, void* stream);
+
+/* Compliance: ce_e = u_e^T KE u_e, C = sum E(x_e) ce_e (FP64 reduction into
+ * *C, a device scalar), grad_e = -E'(x_e) ce_e; replaces
+ * compute_compliance_and_gradient (src/topopt_obj_cons_sensitivities.py:148-179).
+ * ce and grad may be null. */
+int pyto_compliance(pyto_handle* h, const double* u, const double* x, double* ce,
+                    double* grad, double* C, void* stream);
+
+/* Bilinear element term be_e = a_e^T KE b_e (FP64); with a = adjoint and
+ * b = u it is the T2 factor of src/topopt_obj_cons_sensitivities.py:279-282. */
+int pyto_element_bilinear(pyto_handle* h, const double* a, const double* b,
+                          double* be, void* stream);
+
+/* Centre-point engineering strains and stresses (N_el x 6 each, order
+ * xx,yy,zz,xy,xz,yz; stresses relaxed by x^q), von Mises (N_el), element
+ * strain energy 0.5 eps.sigma (N_el) and sum vm^p (device scalar); replaces
+ * HexStructuralFEA.postprocess (src/hex_structural_fea.py:152-232).
+ * strain, stress and energy may be null. */
+int pyto_stress(pyto_handle* h, const double* u, const double* x, double q,
+                double pexp, double* strain, double* stress, double* vm,
+                double* energy, double* sum_vm_p, void* stream);
+
+/* p-norm adjoint right-hand side g (N_dof) and direct term T1 (N_el), given
+ * vm and sum_vm_p from pyto_stress; the intended math of
+ * src/topopt_obj_cons_sensitivities.py:229-266 (g_e as a flat 24-vector). */
+int pyto_pnorm_adjoint_rhs(pyto_handle* h, const double* u, const double* x,
+                           const double* vm, const double* sum_vm_p, double q,
+                           double pexp, double* g, double* T1, void* stream);
+
+/* Cone filter of radius rel_radius*hx (src/topopt_filters.py:8-39) as a fixed
+ * stencil.  mode 0: out = H v; 1: out = (H v)/Hs (density filter,
+ * src/topopt_mma.py:255); 2: out = (H (x v))/Hs/x with x = aux (sensitivity
+ * filter, src/topopt_ocm.py:239); 3: out = H (v/Hs) (density-filter chain
+ * rule, src/topopt_mma.py:342-345); 4: out = Hs (v ignored). */
+int pyto_filter_apply(pyto_handle* h, double rel_radius, int mode, const double* v,
+                      const double* aux, double* out, void* stream);
+/* Stencil of the filter (host arrays): *count offsets (dx,dy,dz) and weights;
+ * capacity is the size of the output arrays. */
+int pyto_filter_stencil(pyto_handle* h, double rel_radius, int capacity, int* count,
+                        int32_t* offsets_host, double* weights_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYTO_HIP_H */

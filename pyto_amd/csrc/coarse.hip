@@ -1,0 +1,702 @@
+// Deflation coarse space on the GPU: implicit rigid-body W, restriction and
+// prolongation, the Galerkin matrix sym(W^T K W) + 1e-10 I assembled from the
+// elements that are not rigid under W, and its FP64 Cholesky factor and
+// inverse (blocked, 64x64 tiles).
+//
+// W (src/deflation.py:386-458): node n of group g contributes rows 3n+c and
+// columns 6g..6g+5 with the rigid-body modes about the group centre, column
+// map u:{0,4,5} v:{1,3,5} w:{2,3,4}.  Rows of fixed DOFs are dropped
+// (W = dsolver.W[free_dofs], src/linear_solvers.py:184).
+//
+// "Special" elements: an element whose 8 nodes are in one group and carry no
+// fixed DOF sees W as a rigid motion, so KE * (W restricted to it) = 0.  Only
+// the other elements contribute to W^T K v and to W^T K W; they are listed
+// once per set of groups, sorted by that set, and processed one wave per
+// chunk of up to 64 elements with a wave reduction before the FP64 atomics.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <vector>
+
+#include "element.hpp"
+
+namespace pyto {
+
+// rigid-mode weights of node (ix,iy,iz) relative to centre c, per component:
+// returns the 6 column coefficients for component comp.
+__device__ __forceinline__ void w_row(int comp, double rx, double ry, double rz, double w[6]) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) w[j] = 0.0;
+  if (comp == 0) { w[0] = 1.0; w[4] = rz; w[5] = -ry; }
+  else if (comp == 1) { w[1] = 1.0; w[3] = -rz; w[5] = rx; }
+  else { w[2] = 1.0; w[3] = ry; w[4] = -rx; }
+}
+
+// ---------------------------------------------------------------------------
+// t = W^T v over all nodes (fixed DOFs excluded).  One thread per node, wave
+// segmented reduction by group, one atomic per (wave, group, column).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict_full(const T* __restrict__ v,
+                                                       const int32_t* __restrict__ grp,
+                                                       const double* __restrict__ centers,
+                                                       const uint8_t* __restrict__ mask,
+                                                       GridDims g, double* __restrict__ t) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double c[6] = {0, 0, 0, 0, 0, 0};
+  int gid = -1;
+  if (n < g.nnodes) {
+    int ix, iy, iz;
+    node_coords(n, g, ix, iy, iz);
+    gid = grp[n];
+    const double rx = ix * g.hx - centers[3 * gid];
+    const double ry = iy * g.hy - centers[3 * gid + 1];
+    const double rz = iz * g.hz - centers[3 * gid + 2];
+    const uint8_t m = mask[n];
+    const double vx = (m & 1) ? 0.0 : (double)v[3 * n];
+    const double vy = (m & 2) ? 0.0 : (double)v[3 * n + 1];
+    const double vz = (m & 4) ? 0.0 : (double)v[3 * n + 2];
+    c[0] = vx; c[1] = vy; c[2] = vz;
+    c[3] = -rz * vy + ry * vz;
+    c[4] = rz * vx - rx * vz;
+    c[5] = -ry * vx + rx * vy;
+  }
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(gid >= 0);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lg = __shfl(gid, leader, 64);
+    const uint64_t members = __ballot(gid == lg);
+    const bool mine = gid == lg;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double s = wave_reduce_sum(mine ? c[j] : 0.0);
+      if (lane == leader) atomicAdd(&t[6 * lg + j], s);
+    }
+    pending &= ~members;
+  }
+}
+
+// out = W mu (fixed DOFs 0), or p = z + beta p - W mu (UPDATE).
+template <typename T, bool UPDATE>
+__global__ __launch_bounds__(256) void k_prolong(T* __restrict__ out, const T* __restrict__ z,
+                                                 const double* __restrict__ mu,
+                                                 const int32_t* __restrict__ grp,
+                                                 const double* __restrict__ centers,
+                                                 const uint8_t* __restrict__ mask, GridDims g,
+                                                 const double* __restrict__ beta_ptr,
+                                                 bool first, const int* __restrict__ done) {
+  if (done && *done) return;
+  const double beta = (UPDATE && !first) ? *beta_ptr : 0.0;
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < g.nnodes;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    int ix, iy, iz;
+    node_coords(n, g, ix, iy, iz);
+    double wm[3] = {0, 0, 0};
+    if (grp) {
+      const int gid = grp[n];
+      const double rx = ix * g.hx - centers[3 * gid];
+      const double ry = iy * g.hy - centers[3 * gid + 1];
+      const double rz = iz * g.hz - centers[3 * gid + 2];
+      const double* m6 = mu + 6 * gid;
+      wm[0] = m6[0] + rz * m6[4] - ry * m6[5];
+      wm[1] = m6[1] - rz * m6[3] + rx * m6[5];
+      wm[2] = m6[2] + ry * m6[3] - rx * m6[4];
+    }
+    const uint8_t m = mask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T val;
+      if (UPDATE) {
+        const double pold = first ? 0.0 : (double)out[3 * n + c];
+        val = (T)((double)z[3 * n + c] + beta * pold - wm[c]);
+      } else {
+        val = (T)wm[c];
+      }
+      out[3 * n + c] = ((m >> c) & 1) ? T(0) : val;
+    }
+  }
+}
+
+// Gather of an element's 24 nodal values (fixed DOFs zeroed), node ids and
+// the node coordinates relative to their group centre.
+struct ElemNodes {
+  int64_t node[8];
+  int gid[8];
+  uint8_t m[8];
+  double r[8][3];
+};
+
+__device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
+                                           const int32_t* __restrict__ grp,
+                                           const double* __restrict__ centers,
+                                           const uint8_t* __restrict__ mask, ElemNodes& en) {
+  int ix, iy, iz;
+  elem_coords(e, g, ix, iy, iz);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int a = corner_a(j), b = corner_b(j), c = corner_c(j);
+    const int64_t n = (ix + a) + (int64_t)g.sx * ((iy + b) + (int64_t)g.sy * (iz + c));
+    en.node[j] = n;
+    const int gid = grp[n];
+    en.gid[j] = gid;
+    en.m[j] = mask[n];
+    en.r[j][0] = (ix + a) * g.hx - centers[3 * gid];
+    en.r[j][1] = (iy + b) * g.hy - centers[3 * gid + 1];
+    en.r[j][2] = (iz + c) * g.hz - centers[3 * gid + 2];
+  }
+}
+
+// t += W^T K v over the special elements (one wave per chunk).
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict_special(
+    const T* __restrict__ v, const int32_t* __restrict__ special,
+    const SpecialChunk* __restrict__ chunks, int nchunks, const T* __restrict__ Ee,
+    const int32_t* __restrict__ grp, const double* __restrict__ centers,
+    const uint8_t* __restrict__ mask, GridDims g, ModalConsts<T> mc, double* __restrict__ t,
+    const int* __restrict__ done) {
+  if (done && *done) return;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= nchunks) return;
+  const SpecialChunk ch = chunks[wave];
+  const bool active = lane < ch.count;
+  double acc[8][6];
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[s][j] = 0.0;
+  if (active) {
+    const int64_t e = special[ch.start + lane];
+    ElemNodes en;
+    elem_nodes(e, g, grp, centers, mask, en);
+    T u[8][3], f[8][3];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) u[j][c] = ((en.m[j] >> c) & 1) ? T(0) : v[3 * en.node[j] + c];
+    element_apply<T>(u, Ee[e], mc, f);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const double fx = (en.m[j] & 1) ? 0.0 : (double)f[j][0];
+      const double fy = (en.m[j] & 2) ? 0.0 : (double)f[j][1];
+      const double fz = (en.m[j] & 4) ? 0.0 : (double)f[j][2];
+      const double rx = en.r[j][0], ry = en.r[j][1], rz = en.r[j][2];
+      double w6[6];
+      w6[0] = fx; w6[1] = fy; w6[2] = fz;
+      w6[3] = -rz * fy + ry * fz;
+      w6[4] = rz * fx - rx * fz;
+      w6[5] = -ry * fx + rx * fy;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        if (s < ch.ng && en.gid[j] == ch.groups[s]) {
+#pragma unroll
+          for (int jj = 0; jj < 6; ++jj) acc[s][jj] += w6[jj];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    if (s < ch.ng) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const double tot = wave_reduce_sum(acc[s][j]);
+        if (lane == 0) atomicAdd(&t[6 * ch.groups[s] + j], tot);
+      }
+    }
+  }
+}
+
+// E += W_b^T K W_a over the special elements; E is nc_pad x nc_pad row-major.
+__global__ __launch_bounds__(256) void k_galerkin(
+    const int32_t* __restrict__ special, const SpecialChunk* __restrict__ chunks, int nchunks,
+    const double* __restrict__ Ee, const int32_t* __restrict__ grp,
+    const double* __restrict__ centers, const uint8_t* __restrict__ mask, GridDims g,
+    ModalConsts<double> mc, double* __restrict__ E, int ld) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= nchunks) return;
+  const SpecialChunk ch = chunks[wave];
+  const bool active = lane < ch.count;
+  ElemNodes en;
+  double Ev = 0.0;
+  if (active) {
+    const int64_t e = special[ch.start + lane];
+    elem_nodes(e, g, grp, centers, mask, en);
+    Ev = Ee[e];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      en.node[j] = 0; en.gid[j] = -1; en.m[j] = 7;
+      en.r[j][0] = en.r[j][1] = en.r[j][2] = 0.0;
+    }
+  }
+  for (int sa = 0; sa < ch.ng; ++sa) {
+    const int ga = ch.groups[sa];
+    for (int ja = 0; ja < 6; ++ja) {
+      double u[8][3], f[8][3];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        double w[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          w_row(c, en.r[j][0], en.r[j][1], en.r[j][2], w);
+          u[j][c] = (en.gid[j] == ga && !((en.m[j] >> c) & 1)) ? w[ja] : 0.0;
+        }
+      }
+      element_apply<double>(u, Ev, mc, f);
+      for (int sb = 0; sb < ch.ng; ++sb) {
+        const int gb = ch.groups[sb];
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          if (en.gid[j] != gb) continue;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            if ((en.m[j] >> c) & 1) continue;
+            double w[6];
+            w_row(c, en.r[j][0], en.r[j][1], en.r[j][2], w);
+#pragma unroll
+            for (int jb = 0; jb < 6; ++jb) acc[jb] += w[jb] * f[j][c];
+          }
+        }
+#pragma unroll
+        for (int jb = 0; jb < 6; ++jb) {
+          const double tot = wave_reduce_sum(acc[jb]);
+          if (lane == 0) atomicAdd(&E[(int64_t)(6 * gb + jb) * ld + (6 * ga + ja)], tot);
+        }
+      }
+    }
+  }
+}
+
+// E <- (E + E^T)/2 + (sa + sr |E_ii|) I on the nc x nc block, identity on the
+// padding.  A diagonal that is exactly zero (a group whose nodes are all
+// fixed, so W has no rows for it) becomes 1: that unknown decouples.
+__global__ void k_symmetrize(double* __restrict__ E, int nc, int ld, double sa, double sr) {
+  const int i = blockIdx.y;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j <= i && j < ld; j += gridDim.x * blockDim.x) {
+    if (i < nc && j < nc) {
+      double v = 0.5 * (E[(int64_t)i * ld + j] + E[(int64_t)j * ld + i]);
+      if (i == j) v = (v == 0.0) ? 1.0 : v + sa + sr * fabs(v);
+      E[(int64_t)i * ld + j] = v;
+      E[(int64_t)j * ld + i] = v;
+    } else {
+      const double v = (i == j) ? 1.0 : 0.0;
+      E[(int64_t)i * ld + j] = v;
+      E[(int64_t)j * ld + i] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense FP64 blocks (n multiple of 64, row-major, leading dimension ld).
+// ---------------------------------------------------------------------------
+constexpr int NB = 64;
+
+// In-place Cholesky of the 64x64 diagonal block at (k0,k0), lower; also writes
+// its triangular inverse to Dinv (64x64).  flag[0] set on a non-positive pivot.
+__global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, int ld, int k0,
+                                                    double* __restrict__ Dinv,
+                                                    int* __restrict__ flag) {
+  __shared__ double L[NB][NB + 1];
+  __shared__ double X[NB][NB + 1];
+  for (int e = threadIdx.x; e < NB * NB; e += 256) {
+    const int i = e / NB, j = e % NB;
+    L[i][j] = (j <= i) ? A[(int64_t)(k0 + i) * ld + (k0 + j)] : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < NB; ++j) {
+    if (threadIdx.x == 0) {
+      double d = L[j][j];
+      if (!(d > 0.0)) {
+        flag[0] = 1;
+        d = 1e-300;
+      }
+      L[j][j] = sqrt(d);
+    }
+    __syncthreads();
+    const double ljj = L[j][j];
+    for (int i = j + 1 + threadIdx.x; i < NB; i += 256) L[i][j] /= ljj;
+    __syncthreads();
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+      const int i = e / NB, l = e % NB;
+      if (i > j && l > j && l <= i) L[i][l] -= L[i][j] * L[l][j];
+    }
+    __syncthreads();
+  }
+  // triangular inverse X = L^{-1}: column c solved by thread c (forward sub)
+  if (threadIdx.x < NB) {
+    const int c = threadIdx.x;
+    for (int i = 0; i < NB; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int l = c; l < i; ++l) s -= L[i][l] * X[l][c];
+      X[i][c] = (i < c) ? 0.0 : s / L[i][i];
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NB * NB; e += 256) {
+    const int i = e / NB, j = e % NB;
+    if (j <= i) A[(int64_t)(k0 + i) * ld + (k0 + j)] = L[i][j];
+    Dinv[(int64_t)i * NB + j] = X[i][j];
+  }
+}
+
+// C (MxN) = alpha * op(A) op(B) + beta * C, all dims multiples of 64.
+// op(A)[i][k] = TA ? A[k*lda+i] : A[i*lda+k]; likewise B.  LOWER: only tiles
+// with tile_row >= tile_col.
+template <bool TA, bool TB, bool LOWER>
+__global__ __launch_bounds__(256) void k_gemm64(int M, int N, int K, double alpha,
+                                                const double* __restrict__ A, int lda,
+                                                const double* __restrict__ B, int ldb,
+                                                double beta, double* __restrict__ C, int ldc) {
+  const int tr = blockIdx.y, tc = blockIdx.x;
+  if (LOWER && tr < tc) return;
+  __shared__ double As[16][NB + 1];
+  __shared__ double Bs[16][NB + 1];
+  const int ti = threadIdx.x / 16, tj = threadIdx.x % 16;
+  double acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int e = threadIdx.x; e < 16 * NB; e += 256) {
+      const int kk = e / NB, ii = e % NB;
+      const int gi = tr * NB + ii, gk = k0 + kk;
+      As[kk][ii] = TA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
+      const int gj = tc * NB + ii;
+      Bs[kk][ii] = TB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[r] = As[kk][ti + 16 * r];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = Bs[kk][tj + 16 * q];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[r][q] += a[r] * b[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int gi = tr * NB + ti + 16 * r, gj = tc * NB + tj + 16 * q;
+      double* cp = C + (int64_t)gi * ldc + gj;
+      *cp = alpha * acc[r][q] + (beta == 0.0 ? 0.0 : beta * *cp);
+    }
+}
+
+// Y block row i: Y_i = Dinv_i * R where R holds I_i - L_{i,<i} Y_{<i}.
+__global__ void k_set_identity_rows(double* __restrict__ R, int ld, int row0, int ncols) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < NB * ncols; e += gridDim.x * blockDim.x) {
+    const int i = e / ncols, j = e % ncols;
+    R[(int64_t)i * ld + j] = (row0 + i == j) ? 1.0 : 0.0;
+  }
+}
+
+// mu = Einv t (one wave per row), optional: out2 = t - E mu residual form.
+__global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ A, int ld, int n,
+                                              const double* __restrict__ x,
+                                              const double* __restrict__ b, double sign,
+                                              double* __restrict__ y,
+                                              const int* __restrict__ done) {
+  if (done && *done) return;
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.0;
+  const double* a = A + (int64_t)row * ld;
+  for (int j = lane; j < n; j += 64) s += a[j] * x[j];
+  s = wave_reduce_sum(s);
+  if (lane == 0) y[row] = (b ? b[row] : 0.0) + sign * s;
+}
+
+__global__ void k_axpy_small(double* __restrict__ y, const double* __restrict__ x, int n,
+                             const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
+static int round_up64(int n) { return (n + 63) / 64 * 64; }
+
+// mu = E^{-1} t with one step of refinement on the coarse residual:
+// mu0 = Einv t; r = t - E mu0; mu = mu0 + Einv r.
+int coarse_apply(Handle* h, const double* t, double* mu, const int* done, hipStream_t s) {
+  const int n = h->nc, ld = round_up64(h->nc);
+  const int blocks = (n * 64 + 255) / 256;
+  hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Einv, ld, n, t,
+                     (const double*)nullptr, 1.0, mu, done);
+  hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Ec, ld, n, mu, t, -1.0, h->cres,
+                     done);
+  hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Einv, ld, n, h->cres,
+                     (const double*)nullptr, 1.0, h->cwork + (int64_t)ld * ld, done);
+  hipLaunchKernelGGL(k_axpy_small, dim3((n + 255) / 256), dim3(256), 0, s, mu,
+                     h->cwork + (int64_t)ld * ld, n, done);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+template <typename T>
+int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
+  PYTO_CHECK_HIP(hipMemsetAsync(t, 0, sizeof(double) * h->nc, s));
+  const int blocks = (int)((h->g.nnodes + 255) / 256);
+  hipLaunchKernelGGL(k_restrict_full<T>, dim3(blocks), dim3(256), 0, s, v, h->node_group,
+                     h->centers, h->fixmask, h->g, t);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+template <typename T>
+int launch_restrict_special(Handle* h, const T* v, const T* Ee, const ModalConsts<T>& mc,
+                            double* t, const int* done, hipStream_t s) {
+  if (h->nchunks == 0) return PYTO_OK;
+  const int blocks = (h->nchunks * 64 + 255) / 256;
+  hipLaunchKernelGGL(k_restrict_special<T>, dim3(blocks), dim3(256), 0, s, v, h->special,
+                     h->chunks, h->nchunks, Ee, h->node_group, h->centers, h->fixmask, h->g,
+                     mc, t, done);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+template <typename T, bool UPDATE>
+int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
+                   bool first, const int* done, hipStream_t s) {
+  const int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 8192);
+  hipLaunchKernelGGL((k_prolong<T, UPDATE>), dim3(blocks), dim3(256), 0, s, out, z, mu,
+                     h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
+                     h->fixmask, h->g, beta, first, done);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+template int launch_restrict_full<double>(Handle*, const double*, double*, hipStream_t);
+template int launch_restrict_full<float>(Handle*, const float*, double*, hipStream_t);
+template int launch_restrict_special<double>(Handle*, const double*, const double*,
+                                             const ModalConsts<double>&, double*, const int*,
+                                             hipStream_t);
+template int launch_restrict_special<float>(Handle*, const float*, const float*,
+                                            const ModalConsts<float>&, double*, const int*,
+                                            hipStream_t);
+template int launch_prolong<double, false>(Handle*, double*, const double*, const double*,
+                                           const double*, bool, const int*, hipStream_t);
+template int launch_prolong<double, true>(Handle*, double*, const double*, const double*,
+                                          const double*, bool, const int*, hipStream_t);
+template int launch_prolong<float, false>(Handle*, float*, const float*, const double*,
+                                          const double*, bool, const int*, hipStream_t);
+template int launch_prolong<float, true>(Handle*, float*, const float*, const double*,
+                                         const double*, bool, const int*, hipStream_t);
+
+// Cholesky factor of Ec into cwork (lower), then Einv = L^{-T} L^{-1}.
+static int factorize_coarse(Handle* h, hipStream_t s) {
+  const int ld = round_up64(h->nc);
+  const int nb = ld / NB;
+  double* L = h->cwork;                       // ld x ld factor
+  double* Y = h->Einv;                        // L^{-1} then overwritten
+  double* Dinv = h->cwork + 2 * (int64_t)ld * ld;   // nb blocks of 64x64
+  double* Tmp = h->cwork + (int64_t)ld * ld;        // ld x ld scratch (first ld*64 used)
+  PYTO_CHECK_HIP(hipMemcpyAsync(L, h->Ec, sizeof(double) * ld * ld, hipMemcpyDeviceToDevice, s));
+  for (int kb = 0; kb < nb; ++kb) {
+    const int k0 = kb * NB;
+    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, s, L, ld, k0,
+                       Dinv + (int64_t)kb * NB * NB, h->flags + F_COARSE_PIVOT);
+    const int m = ld - k0 - NB;
+    if (m > 0) {
+      double* P = L + (int64_t)(k0 + NB) * ld + k0;
+      // P <- P * Dinv^T (in place is safe: each 64-row tile reads only itself, K = 64)
+      hipLaunchKernelGGL((k_gemm64<false, true, false>), dim3(1, m / NB), dim3(256), 0, s, m,
+                         NB, NB, 1.0, P, ld, Dinv + (int64_t)kb * NB * NB, NB, 0.0, P, ld);
+      double* Tr = L + (int64_t)(k0 + NB) * ld + (k0 + NB);
+      hipLaunchKernelGGL((k_gemm64<false, true, true>), dim3(m / NB, m / NB), dim3(256), 0, s,
+                         m, m, NB, -1.0, P, ld, P, ld, 1.0, Tr, ld);
+    }
+  }
+  // Y = L^{-1}, block row by block row: Y_i = Dinv_i (I_i - L_{i,<i} Y_{<i})
+  for (int ib = 0; ib < nb; ++ib) {
+    const int r0 = ib * NB;
+    const int ncols = r0 + NB;
+    hipLaunchKernelGGL(k_set_identity_rows, dim3(64), dim3(256), 0, s, Tmp, ld, r0, ncols);
+    if (ib > 0)
+      hipLaunchKernelGGL((k_gemm64<false, false, false>), dim3(ncols / NB, 1), dim3(256), 0, s,
+                         NB, ncols, r0, -1.0, L + (int64_t)r0 * ld, ld, Y, ld, 1.0, Tmp, ld);
+    hipLaunchKernelGGL((k_gemm64<false, false, false>), dim3(ncols / NB, 1), dim3(256), 0, s,
+                       NB, ncols, NB, 1.0, Dinv + (int64_t)ib * NB * NB, NB, Tmp, ld, 0.0,
+                       Y + (int64_t)r0 * ld, ld);
+    // zero the strictly upper part of this block row (columns >= ncols)
+    if (ncols < ld)
+      hipLaunchKernelGGL(k_set_identity_rows, dim3(64), dim3(256), 0, s,
+                         Y + (int64_t)r0 * ld + ncols, ld, -1000000, ld - ncols);
+  }
+  // Einv = Y^T Y  (into Tmp region, then copy)
+  double* Out = h->cwork + (int64_t)ld * ld;
+  hipLaunchKernelGGL((k_gemm64<true, false, false>), dim3(nb, nb), dim3(256), 0, s, ld, ld, ld,
+                     1.0, Y, ld, Y, ld, 0.0, Out, ld);
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->Einv, Out, sizeof(double) * ld * ld, hipMemcpyDeviceToDevice, s));
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+int coarse_setup(Handle* h, hipStream_t s) {
+  const int ld = round_up64(h->nc);
+  PYTO_CHECK_HIP(hipMemsetAsync(h->Ec, 0, sizeof(double) * ld * ld, s));
+  if (h->nchunks > 0) {
+    const int blocks = (h->nchunks * 64 + 255) / 256;
+    hipLaunchKernelGGL(k_galerkin, dim3(blocks), dim3(256), 0, s, h->special, h->chunks,
+                       h->nchunks, h->Ee64, h->node_group, h->centers, h->fixmask, h->g,
+                       h->mc64, h->Ec, ld);
+    PYTO_CHECK_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_symmetrize, dim3((ld + 255) / 256, ld), dim3(256), 0, s, h->Ec, h->nc, ld,
+                     h->shift_abs, h->shift_rel);
+  PYTO_CHECK_HIP(hipGetLastError());
+  PYTO_CHECK_HIP(hipMemsetAsync(h->flags + F_COARSE_PIVOT, 0, sizeof(int), s));
+  PYTO_TRY(factorize_coarse(h, s));
+  h->coarse_ready = true;
+  return PYTO_OK;
+}
+
+}  // namespace pyto
+
+using namespace pyto;
+
+extern "C" {
+
+int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
+                    const double* centers, void* stream) {
+  PYTO_REQUIRE(h, PYTO_EINVAL, "null handle");
+  PYTO_REQUIRE(ngroups >= 0, PYTO_EINVAL, "ngroups must be >= 0");
+  PYTO_REQUIRE(6 * (int64_t)ngroups <= 12288, PYTO_EINVAL,
+               "coarse space of %d groups (%d unknowns) exceeds the dense GPU limit of 2048 "
+               "groups; use fewer, larger deflation groups", ngroups, 6 * ngroups);
+  hipStream_t s = as_stream(stream);
+  // release a previous coarse space
+  void* olds[] = {h->node_group, h->centers, h->special, h->chunks, h->Ec, h->Einv, h->cwork,
+                  h->ct, h->cmu, h->cres};
+  for (void* p : olds)
+    if (p) hipFree(p);
+  h->node_group = nullptr; h->centers = nullptr; h->special = nullptr; h->chunks = nullptr;
+  h->Ec = h->Einv = h->cwork = h->ct = h->cmu = h->cres = nullptr;
+  h->ngroups = 0; h->nc = 0; h->nspecial = 0; h->nchunks = 0; h->coarse_ready = false;
+  if (ngroups == 0) return PYTO_OK;
+  PYTO_REQUIRE(node_group && centers, PYTO_EINVAL, "null group arrays");
+  const int64_t nn = h->g.nnodes;
+  std::vector<int32_t> grp(nn);
+  std::vector<uint8_t> mask(nn);
+  PYTO_CHECK_HIP(hipMemcpyAsync(grp.data(), node_group, nn * 4, hipMemcpyDeviceToHost, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(mask.data(), h->fixmask, nn, hipMemcpyDeviceToHost, s));
+  PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  for (int64_t n = 0; n < nn; ++n)
+    PYTO_REQUIRE(grp[n] >= 0 && grp[n] < ngroups, PYTO_EINVAL,
+                 "node_group[%lld] = %d out of range [0,%d)", (long long)n, grp[n], ngroups);
+  // special elements and their group sets
+  const GridDims& g = h->g;
+  std::vector<std::pair<std::array<int, 8>, int32_t>> list;
+  for (int iz = 0; iz < g.nz; ++iz)
+    for (int iy = 0; iy < g.ny; ++iy)
+      for (int ix = 0; ix < g.nx; ++ix) {
+        std::array<int, 8> gs;
+        bool fixed = false;
+        for (int j = 0; j < 8; ++j) {
+          const int64_t n = (ix + corner_a(j)) + (int64_t)g.sx * ((iy + corner_b(j)) +
+                                                                  (int64_t)g.sy * (iz + corner_c(j)));
+          gs[j] = grp[n];
+          fixed |= mask[n] != 0;
+        }
+        std::sort(gs.begin(), gs.end());
+        const int nu = (int)(std::unique(gs.begin(), gs.end()) - gs.begin());
+        if (nu == 1 && !fixed) continue;
+        for (int j = nu; j < 8; ++j) gs[j] = -1;
+        list.push_back({gs, (int32_t)(ix + (int64_t)g.nx * (iy + (int64_t)g.ny * iz))});
+      }
+  std::sort(list.begin(), list.end());
+  std::vector<int32_t> special(list.size());
+  std::vector<SpecialChunk> chunks;
+  for (size_t i = 0; i < list.size();) {
+    size_t j = i;
+    while (j < list.size() && list[j].first == list[i].first) ++j;
+    for (size_t c = i; c < j; c += 64) {
+      SpecialChunk ch;
+      ch.start = (int)c;
+      ch.count = (int)std::min<size_t>(64, j - c);
+      ch.ng = 0;
+      for (int q = 0; q < 8; ++q) {
+        ch.groups[q] = list[i].first[q];
+        if (ch.groups[q] >= 0) ch.ng++;
+      }
+      chunks.push_back(ch);
+    }
+    i = j;
+  }
+  for (size_t i = 0; i < list.size(); ++i) special[i] = list[i].second;
+  h->ngroups = ngroups;
+  h->nc = 6 * ngroups;
+  h->nspecial = (int)special.size();
+  h->nchunks = (int)chunks.size();
+  const int ld = round_up64(h->nc);
+  PYTO_TRY(device_alloc(h, (void**)&h->node_group, nn * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->centers, (size_t)ngroups * 3 * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->special, special.size() * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->chunks, chunks.size() * sizeof(SpecialChunk)));
+  PYTO_TRY(device_alloc(h, (void**)&h->Ec, (size_t)ld * ld * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->Einv, (size_t)ld * ld * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->cwork, ((size_t)2 * ld * ld + (size_t)ld * 64) * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->ct, (size_t)ld * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->cmu, (size_t)ld * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->cres, (size_t)ld * 8));
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->node_group, node_group, nn * 4, hipMemcpyDeviceToDevice, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->centers, centers, (size_t)ngroups * 24, hipMemcpyDeviceToDevice, s));
+  if (!special.empty()) {
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->special, special.data(), special.size() * 4,
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->chunks, chunks.data(), chunks.size() * sizeof(SpecialChunk),
+                                  hipMemcpyHostToDevice, s));
+  }
+  PYTO_CHECK_HIP(hipMemsetAsync(h->ct, 0, (size_t)ld * 8, s));
+  PYTO_CHECK_HIP(hipMemsetAsync(h->cmu, 0, (size_t)ld * 8, s));
+  PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  if (h->density_set) PYTO_TRY(coarse_setup(h, s));
+  return PYTO_OK;
+}
+
+int pyto_set_coarse_shift(pyto_handle* h, double abs_shift, double rel_shift) {
+  PYTO_REQUIRE(h, PYTO_EINVAL, "null handle");
+  PYTO_REQUIRE(abs_shift >= 0 && rel_shift >= 0, PYTO_EINVAL, "shifts must be >= 0");
+  h->shift_abs = abs_shift;
+  h->shift_rel = rel_shift;
+  return PYTO_OK;
+}
+
+int pyto_restrict_f64(pyto_handle* h, const double* v, double* t, void* stream) {
+  PYTO_REQUIRE(h && v && t, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(h->ngroups > 0, PYTO_ESTATE, "no deflation groups set");
+  return launch_restrict_full<double>(h, v, t, as_stream(stream));
+}
+
+int pyto_prolong_f64(pyto_handle* h, const double* mu, double* v, void* stream) {
+  PYTO_REQUIRE(h && mu && v, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(h->ngroups > 0, PYTO_ESTATE, "no deflation groups set");
+  return launch_prolong<double, false>(h, v, nullptr, mu, nullptr, true, nullptr,
+                                       as_stream(stream));
+}
+
+int pyto_coarse_solve_f64(pyto_handle* h, const double* t, double* mu, void* stream) {
+  PYTO_REQUIRE(h && t && mu, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(h->coarse_ready, PYTO_ESTATE, "coarse space not assembled (set groups and density)");
+  return coarse_apply(h, t, mu, nullptr, as_stream(stream));
+}
+
+int pyto_coarse_matrix_f64(pyto_handle* h, double* E, void* stream) {
+  PYTO_REQUIRE(h && E, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(h->coarse_ready, PYTO_ESTATE, "coarse space not assembled (set groups and density)");
+  const int ld = round_up64(h->nc);
+  PYTO_CHECK_HIP(hipMemcpy2DAsync(E, sizeof(double) * h->nc, h->Ec, sizeof(double) * ld,
+                                  sizeof(double) * h->nc, h->nc, hipMemcpyDeviceToDevice,
+                                  as_stream(stream)));
+  return PYTO_OK;
+}
+
+}  // extern "C"

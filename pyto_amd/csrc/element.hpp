@@ -1,0 +1,182 @@
+// Element-level arithmetic of the 8-node brick in the Hadamard ("modal") basis.
+//
+// Nodal values u[n][i] (n = reference local node 0..7, i = x,y,z component)
+// are transformed by a 3-stage butterfly into the sums c_m = sum_n s_m(n) u_n
+// for the monomials m in {x, y, z, xy, xz, yz, xyz} of the reference
+// coordinates (the constant mode is a rigid translation and drops out).  The
+// displacement gradient of each monomial gives one isotropic stress
+// evaluation; because the monomials are orthogonal over the element, KE*u is
+// the transposed butterfly of those stresses.  This reproduces KE of
+// src/hex_element_stiffness.py:231-286 (2x2x2 Gauss, exact for these
+// polynomials) to rounding.
+#pragma once
+
+#include "common.hpp"
+
+namespace pyto {
+
+template <typename T>
+struct Modal {
+  // Hadamard sums per component (index 0..2)
+  T cx[3], cy[3], cz[3], cxy[3], cxz[3], cyz[3], cxyz[3];
+};
+
+// Local node order (a,b,c): 0:(000) 1:(100) 2:(110) 3:(010) 4:(001) 5:(101)
+// 6:(111) 7:(011), src/hex_mesher.py:121-124.
+template <typename T>
+__host__ __device__ __forceinline__ void modal_forward(const T u[8][3], Modal<T>& m) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const T v000 = u[0][i], v100 = u[1][i], v110 = u[2][i], v010 = u[3][i];
+    const T v001 = u[4][i], v101 = u[5][i], v111 = u[6][i], v011 = u[7][i];
+    // x stage
+    const T sx00 = v100 + v000, dx00 = v100 - v000;
+    const T sx10 = v110 + v010, dx10 = v110 - v010;
+    const T sx01 = v101 + v001, dx01 = v101 - v001;
+    const T sx11 = v111 + v011, dx11 = v111 - v011;
+    // y stage
+    const T s0_0 = sx10 + sx00, sy_0 = sx10 - sx00;
+    const T s0_1 = sx11 + sx01, sy_1 = sx11 - sx01;
+    const T dxs_0 = dx10 + dx00, dxy_0 = dx10 - dx00;
+    const T dxs_1 = dx11 + dx01, dxy_1 = dx11 - dx01;
+    // z stage
+    m.cz[i] = s0_1 - s0_0;
+    m.cy[i] = sy_1 + sy_0;
+    m.cyz[i] = sy_1 - sy_0;
+    m.cx[i] = dxs_1 + dxs_0;
+    m.cxz[i] = dxs_1 - dxs_0;
+    m.cxy[i] = dxy_1 + dxy_0;
+    m.cxyz[i] = dxy_1 - dxy_0;
+  }
+}
+
+// Modal forces F from modal sums (linear in lam, mu which already carry E_e
+// and the V/64 scaling).  Output uses the same Modal layout.
+template <typename T>
+__host__ __device__ __forceinline__ void modal_stress(const Modal<T>& c, T gx, T gy, T gz, T lam,
+                                                      T mu, Modal<T>& F) {
+  const T two_mu = mu + mu;
+  const T third = T(1) / T(3);
+  // mode 1 (full gradient)
+  {
+    const T ax0 = gx * c.cx[0], ax1 = gx * c.cx[1], ax2 = gx * c.cx[2];
+    const T ay0 = gy * c.cy[0], ay1 = gy * c.cy[1], ay2 = gy * c.cy[2];
+    const T az0 = gz * c.cz[0], az1 = gz * c.cz[1], az2 = gz * c.cz[2];
+    const T ltr = lam * (ax0 + ay1 + az2);
+    const T s00 = ltr + two_mu * ax0, s11 = ltr + two_mu * ay1, s22 = ltr + two_mu * az2;
+    const T s01 = mu * (ay0 + ax1), s02 = mu * (az0 + ax2), s12 = mu * (az1 + ay2);
+    F.cx[0] = gx * s00; F.cx[1] = gx * s01; F.cx[2] = gx * s02;
+    F.cy[0] = gy * s01; F.cy[1] = gy * s11; F.cy[2] = gy * s12;
+    F.cz[0] = gz * s02; F.cz[1] = gz * s12; F.cz[2] = gz * s22;
+  }
+  // modes xi (columns y,z from cxy,cxz), eta (x,z from cxy,cyz),
+  // zeta (x,y from cxz,cyz)
+  const T by0 = gy * c.cxy[0], by1 = gy * c.cxy[1], by2 = gy * c.cxy[2];
+  const T bz0 = gz * c.cxz[0], bz1 = gz * c.cxz[1], bz2 = gz * c.cxz[2];
+  const T cx0 = gx * c.cxy[0], cx1 = gx * c.cxy[1], cx2 = gx * c.cxy[2];
+  const T czz0 = gz * c.cyz[0], czz1 = gz * c.cyz[1], czz2 = gz * c.cyz[2];
+  const T dx0 = gx * c.cxz[0], dx1 = gx * c.cxz[1], dx2 = gx * c.cxz[2];
+  const T dy0 = gy * c.cyz[0], dy1 = gy * c.cyz[1], dy2 = gy * c.cyz[2];
+  // S_xi[i][1], S_xi[i][2]
+  const T lxi = lam * (by1 + bz2);
+  const T sxi01 = mu * by0, sxi11 = lxi + two_mu * by1, sxi21 = mu * (by2 + bz1);
+  const T sxi02 = mu * bz0, sxi22 = lxi + two_mu * bz2;  // sxi12 == sxi21
+  // S_eta[i][0], S_eta[i][2]
+  const T leta = lam * (cx0 + czz2);
+  const T seta00 = leta + two_mu * cx0, seta10 = mu * cx1, seta20 = mu * (cx2 + czz0);
+  const T seta12 = mu * czz1, seta22 = leta + two_mu * czz2;  // seta02 == seta20
+  // S_zeta[i][0], S_zeta[i][1]
+  const T lze = lam * (dx0 + dy1);
+  const T sze00 = lze + two_mu * dx0, sze10 = mu * (dx1 + dy0), sze20 = mu * dx2;
+  const T sze11 = lze + two_mu * dy1, sze21 = mu * dy2;  // sze01 == sze10
+  F.cxy[0] = third * (gy * sxi01 + gx * seta00);
+  F.cxy[1] = third * (gy * sxi11 + gx * seta10);
+  F.cxy[2] = third * (gy * sxi21 + gx * seta20);
+  F.cxz[0] = third * (gz * sxi02 + gx * sze00);
+  F.cxz[1] = third * (gz * sxi21 + gx * sze10);
+  F.cxz[2] = third * (gz * sxi22 + gx * sze20);
+  F.cyz[0] = third * (gz * seta20 + gy * sze10);
+  F.cyz[1] = third * (gz * seta12 + gy * sze11);
+  F.cyz[2] = third * (gz * seta22 + gy * sze21);
+  // mode xyz: diagonal
+  const T gx2 = gx * gx, gy2 = gy * gy, gz2 = gz * gz;
+  const T ninth = T(1) / T(9);
+  const T base = mu * (gx2 + gy2 + gz2);
+  const T lm = lam + mu;
+  F.cxyz[0] = ninth * (base + lm * gx2) * c.cxyz[0];
+  F.cxyz[1] = ninth * (base + lm * gy2) * c.cxyz[1];
+  F.cxyz[2] = ninth * (base + lm * gz2) * c.cxyz[2];
+}
+
+// Transposed butterfly: f[n][i] = sum_m s_m(n) F_m,i (no constant mode).
+template <typename T>
+__host__ __device__ __forceinline__ void modal_backward(const Modal<T>& F, T f[8][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    // z stage
+    const T s0_0 = -F.cz[i], s0_1 = F.cz[i];
+    const T sy_0 = F.cy[i] - F.cyz[i], sy_1 = F.cy[i] + F.cyz[i];
+    const T dxs_0 = F.cx[i] - F.cxz[i], dxs_1 = F.cx[i] + F.cxz[i];
+    const T dxy_0 = F.cxy[i] - F.cxyz[i], dxy_1 = F.cxy[i] + F.cxyz[i];
+    // y stage: sx[b][c], dx[b][c]
+    const T sx00 = s0_0 - sy_0, sx10 = s0_0 + sy_0;
+    const T sx01 = s0_1 - sy_1, sx11 = s0_1 + sy_1;
+    const T dx00 = dxs_0 - dxy_0, dx10 = dxs_0 + dxy_0;
+    const T dx01 = dxs_1 - dxy_1, dx11 = dxs_1 + dxy_1;
+    // x stage
+    f[0][i] = sx00 - dx00;  // (0,0,0)
+    f[1][i] = sx00 + dx00;  // (1,0,0)
+    f[2][i] = sx10 + dx10;  // (1,1,0)
+    f[3][i] = sx10 - dx10;  // (0,1,0)
+    f[4][i] = sx01 - dx01;  // (0,0,1)
+    f[5][i] = sx01 + dx01;  // (1,0,1)
+    f[6][i] = sx11 + dx11;  // (1,1,1)
+    f[7][i] = sx11 - dx11;  // (0,1,1)
+  }
+}
+
+// f = E * KE * u for one element (E relative to E0; KE carries E0).
+template <typename T>
+__host__ __device__ __forceinline__ void element_apply(const T u[8][3], T E,
+                                                       const ModalConsts<T>& mc, T f[8][3]) {
+  Modal<T> c, F;
+  modal_forward(u, c);
+  modal_stress(c, mc.gx, mc.gy, mc.gz, E * mc.lam, E * mc.mu, F);
+  modal_backward(F, f);
+}
+
+template <typename T>
+inline ModalConsts<T> make_modal_consts(double E0, double nu, double hx, double hy, double hz) {
+  const double lam = E0 * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double mu = E0 / (2.0 * (1.0 + nu));
+  const double V = hx * hy * hz;
+  ModalConsts<T> m;
+  m.gx = T(2.0 / hx);
+  m.gy = T(2.0 / hy);
+  m.gz = T(2.0 / hz);
+  m.lam = T(lam * V / 64.0);
+  m.mu = T(mu * V / 64.0);
+  return m;
+}
+
+// Centre-point engineering strain [xx, yy, zz, xy, xz, yz] of an element,
+// src/hex_structural_fea.py:163-183: gradients at the centre are the
+// mode-1 coefficients c_x, c_y, c_z times g/8.
+template <typename T>
+__host__ __device__ __forceinline__ void centre_strain(const T u[8][3], T gx, T gy, T gz,
+                                                       T eps[6]) {
+  Modal<T> c;
+  modal_forward(u, c);
+  const T k = T(0.125);
+  const T ux = k * gx * c.cx[0], uy = k * gy * c.cy[0], uz = k * gz * c.cz[0];
+  const T vx = k * gx * c.cx[1], vy = k * gy * c.cy[1], vz = k * gz * c.cz[1];
+  const T wx = k * gx * c.cx[2], wy = k * gy * c.cy[2], wz = k * gz * c.cz[2];
+  eps[0] = ux;
+  eps[1] = vy;
+  eps[2] = wz;
+  eps[3] = uy + vx;
+  eps[4] = uz + wx;
+  eps[5] = vz + wy;
+}
+
+}  // namespace pyto

@@ -1,0 +1,475 @@
+// Grid operator: handle lifetime, SIMP densities, Jacobi diagonal and the
+// matrix-free K*u of the voxel grid (FP64 and FP32).
+//
+// K*u kernel layout.  A workgroup of 256 threads is a 16x16 tile of element
+// columns; thread (tx,ty) owns element column (15*bx-1+tx, 15*by-1+ty) and the
+// node (ex, ey) at its lower-left corner.  The tile marches up a chunk of z:
+// per element layer it stages the 17x17 node plane above in LDS, applies the
+// element operator in the Hadamard basis (element.hpp), hands the
+// contributions for the three neighbouring nodes of each plane to the
+// neighbouring threads through LDS, and carries its own top-plane sum in
+// registers to the next layer.  Tiles overlap by one element column so every
+// owned node (tx,ty >= 1) is complete without atomics; a z chunk starts one
+// element layer below its first owned node plane.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "element.hpp"
+
+namespace pyto {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int device_alloc(Handle* h, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    return PYTO_ENOMEM;
+  }
+  h->workspace_bytes += (int64_t)bytes;
+  return PYTO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Density -> E_e (relative SIMP modulus), src/topopt_material_model.py:88.
+// ---------------------------------------------------------------------------
+__global__ void k_simp(const double* __restrict__ x, double* __restrict__ E64,
+                       float* __restrict__ E32, double* __restrict__ xcopy, int64_t n,
+                       double penal, double eps) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double xi = x[i];
+    const double e = eps + pow(xi, penal) * (1.0 - eps);
+    E64[i] = e;
+    E32[i] = (float)e;
+    xcopy[i] = xi;
+  }
+}
+
+// Jacobi: d(node,c) = KEdiag_c * sum of E_e over the node's elements;
+// |d| < 1e-12 -> 1 (src/linear_solvers.py:84-85); fixed DOFs get 0 so that
+// z = M r stays zero there.
+__global__ void k_inv_diag(const double* __restrict__ E64, const uint8_t* __restrict__ mask,
+                           GridDims g, double kd0, double kd1, double kd2,
+                           double* __restrict__ invd64, float* __restrict__ invd32,
+                           double* __restrict__ diag_out) {
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < g.nnodes;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    int ix, iy, iz;
+    node_coords(n, g, ix, iy, iz);
+    double s = 0.0;
+#pragma unroll
+    for (int dz = -1; dz <= 0; ++dz)
+#pragma unroll
+      for (int dy = -1; dy <= 0; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 0; ++dx) {
+          const int ex = ix + dx, ey = iy + dy, ez = iz + dz;
+          if (ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny && ez >= 0 && ez < g.nz)
+            s += E64[ex + (int64_t)g.nx * (ey + (int64_t)g.ny * ez)];
+        }
+    const uint8_t m = mask[n];
+    const double kd[3] = {kd0, kd1, kd2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const bool fixed = (m >> c) & 1;
+      double d = kd[c] * s;
+      if (diag_out) diag_out[3 * n + c] = fixed ? 0.0 : d;
+      if (invd64) {
+        if (fabs(d) < 1e-12) d = 1.0;
+        const double inv = fixed ? 0.0 : 1.0 / d;
+        invd64[3 * n + c] = inv;
+        invd32[3 * n + c] = (float)inv;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free y = K_ff u.
+// ---------------------------------------------------------------------------
+constexpr int kTile = 16;
+constexpr int kOwn = 15;
+constexpr int kPl = 17;
+
+template <typename T>
+struct MatvecArgs {
+  const T* u;
+  T* y;
+  const T* Ee;
+  const uint8_t* mask;
+  GridDims g;
+  ModalConsts<T> mc;
+  int zchunk;
+  int tiles_x;
+  double* partials;    // dot partials (null: no dot)
+  unsigned* counter;
+  double* dot_out;     // where the finished dot goes
+  const int* done;     // skip when *done != 0 (null: never)
+  double* zero_vec;    // zeroed by block 0 (null: none)
+  int zero_n;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_plane(T (*pl)[kPl][3], const T* __restrict__ u,
+                                           const uint8_t* __restrict__ mask, const GridDims& g,
+                                           int x0, int y0, int kp) {
+  for (int e = threadIdx.x; e < kPl * kPl; e += 256) {
+    const int lx = e % kPl, ly = e / kPl;
+    const int gx = x0 + lx, gy = y0 + ly;
+    T v0 = T(0), v1 = T(0), v2 = T(0);
+    if (kp >= 0 && kp < g.sz && gx >= 0 && gx < g.sx && gy >= 0 && gy < g.sy) {
+      const int64_t n = gx + (int64_t)g.sx * (gy + (int64_t)g.sy * kp);
+      const uint8_t m = mask[n];
+      const T* p = u + 3 * n;
+      v0 = (m & 1) ? T(0) : p[0];
+      v1 = (m & 2) ? T(0) : p[1];
+      v2 = (m & 4) ? T(0) : p[2];
+    }
+    pl[ly][lx][0] = v0;
+    pl[ly][lx][1] = v1;
+    pl[ly][lx][2] = v2;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
+  if (a.done && *a.done) return;
+  __shared__ T pl[2][kPl][kPl][3];
+  __shared__ T xc[kTile][kTile][6][3];
+  __shared__ double red[4];
+  const GridDims& g = a.g;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int bxi = blockIdx.x % a.tiles_x, byi = blockIdx.x / a.tiles_x;
+  const int x0 = bxi * kOwn - 1, y0 = byi * kOwn - 1;
+  const int ex = x0 + tx, ey = y0 + ty;
+  const int kz0 = blockIdx.y * a.zchunk;
+  const int kz1 = min(kz0 + a.zchunk, g.sz);
+  const bool col_valid = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny;
+  const bool owner = tx >= 1 && ty >= 1 && ex < g.sx && ey < g.sy;
+
+  if (a.zero_vec && blockIdx.x == 0 && blockIdx.y == 0)
+    for (int i = threadIdx.x; i < a.zero_n; i += 256) a.zero_vec[i] = 0.0;
+
+  T bot[4][3], top[4][3];
+  load_plane<T>(pl[0], a.u, a.mask, g, x0, y0, kz0 - 1);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    bot[0][c] = pl[0][ty][tx][c];
+    bot[1][c] = pl[0][ty][tx + 1][c];
+    bot[2][c] = pl[0][ty + 1][tx + 1][c];
+    bot[3][c] = pl[0][ty + 1][tx][c];
+  }
+  T carry[3] = {T(0), T(0), T(0)};
+  double dot = 0.0;
+  int buf = 1;
+  for (int k = kz0 - 1; k < kz1; ++k) {
+    load_plane<T>(pl[buf], a.u, a.mask, g, x0, y0, k + 1);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      top[0][c] = pl[buf][ty][tx][c];
+      top[1][c] = pl[buf][ty][tx + 1][c];
+      top[2][c] = pl[buf][ty + 1][tx + 1][c];
+      top[3][c] = pl[buf][ty + 1][tx][c];
+    }
+    T f[8][3];
+    if (col_valid && k >= 0 && k < g.nz) {
+      T ue[8][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          ue[j][c] = bot[j][c];
+          ue[4 + j][c] = top[j][c];
+        }
+      }
+      const T E = a.Ee[ex + (int64_t)g.nx * (ey + (int64_t)g.ny * k)];
+      element_apply<T>(ue, E, a.mc, f);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[j][c] = T(0);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xc[ty][tx][0][c] = f[1][c];
+      xc[ty][tx][1][c] = f[2][c];
+      xc[ty][tx][2][c] = f[3][c];
+      xc[ty][tx][3][c] = f[5][c];
+      xc[ty][tx][4][c] = f[6][c];
+      xc[ty][tx][5][c] = f[7][c];
+    }
+    __syncthreads();
+    if (owner) {
+      T ob[3], ot[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ob[c] = f[0][c] + xc[ty][tx - 1][0][c] + xc[ty - 1][tx - 1][1][c] + xc[ty - 1][tx][2][c];
+        ot[c] = f[4][c] + xc[ty][tx - 1][3][c] + xc[ty - 1][tx - 1][4][c] + xc[ty - 1][tx][5][c];
+      }
+      if (k >= kz0) {
+        const int64_t n = ex + (int64_t)g.sx * (ey + (int64_t)g.sy * k);
+        const uint8_t m = a.mask[n];
+        T* yp = a.y + 3 * n;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const T v = ((m >> c) & 1) ? T(0) : carry[c] + ob[c];
+          yp[c] = v;
+          dot += (double)v * (double)bot[0][c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) carry[c] = ot[c];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) bot[j][c] = top[j][c];
+    buf ^= 1;
+  }
+  if (a.partials) {
+    const double bs = block_reduce_sum256(dot, red);
+    const int nb = gridDim.x * gridDim.y;
+    const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+    double* out = a.dot_out;
+    grid_reduce_finish(bs, a.partials, a.counter, nb, bid, [out](double t) { *out = t; });
+  }
+}
+
+template <typename T>
+static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& tiles_x) {
+  const GridDims& g = h->g;
+  tiles_x = (g.sx + kOwn - 1) / kOwn;
+  const int tiles_y = (g.sy + kOwn - 1) / kOwn;
+  const int tiles = tiles_x * tiles_y;
+  // aim for >= 4 workgroups per CU on 256 CUs, within the partials buffer
+  zchunk = 16;
+  while ((int64_t)tiles * ((g.sz + zchunk - 1) / zchunk) > kMaxPartials) zchunk *= 2;
+  while (zchunk > 4 && (int64_t)tiles * ((g.sz + zchunk - 1) / zchunk) < 1024) zchunk /= 2;
+  grid = dim3(tiles, (g.sz + zchunk - 1) / zchunk, 1);
+}
+
+template <typename T>
+int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalConsts<T>& mc,
+                  double* dot_out, const int* done, double* zero_vec, int zero_n,
+                  hipStream_t s) {
+  MatvecArgs<T> a;
+  a.u = u;
+  a.y = y;
+  a.Ee = Ee;
+  a.mask = h->fixmask;
+  a.g = h->g;
+  a.mc = mc;
+  dim3 grid;
+  matvec_launch_dims<T>(h, grid, a.zchunk, a.tiles_x);
+  a.partials = dot_out ? h->partials : nullptr;
+  a.counter = h->counters + C_MATVEC;
+  a.dot_out = dot_out;
+  a.done = done;
+  a.zero_vec = zero_vec;
+  a.zero_n = zero_n;
+  hipLaunchKernelGGL(k_matvec<T>, grid, dim3(256), 0, s, a);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+template int launch_matvec<double>(const Handle*, const double*, double*, const double*,
+                                   const ModalConsts<double>&, double*, const int*, double*,
+                                   int, hipStream_t);
+template int launch_matvec<float>(const Handle*, const float*, float*, const float*,
+                                  const ModalConsts<float>&, double*, const int*, double*, int,
+                                  hipStream_t);
+
+int ensure_solver_workspace(Handle* h) {
+  if (h->r64) return PYTO_OK;
+  const size_t nd = (size_t)(3 * h->g.nnodes);
+  PYTO_TRY(device_alloc(h, (void**)&h->r64, nd * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->z64, nd * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->p64, nd * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->q64, nd * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->r32, nd * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->z32, nd * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->p32, nd * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->q32, nd * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->x32, nd * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->b32, nd * 4));
+  return PYTO_OK;
+}
+
+int coarse_setup(Handle* h, hipStream_t s);  // coarse.hip
+
+static void host_ke(const Handle* h, double* ke) {
+  // Columns of KE from unit vectors through the same modal apply.
+  const ModalConsts<double>& mc = h->mc64;
+  for (int col = 0; col < 24; ++col) {
+    double u[8][3] = {};
+    u[col / 3][col % 3] = 1.0;
+    double f[8][3];
+    element_apply<double>(u, 1.0, mc, f);
+    for (int r = 0; r < 24; ++r) ke[r * 24 + col] = f[r / 3][r % 3];
+  }
+}
+
+}  // namespace pyto
+
+using namespace pyto;
+
+extern "C" {
+
+const char* pyto_last_error(void) { return g_last_error.c_str(); }
+int pyto_version(void) { return 1; }
+
+int pyto_ke_hex8(double E, double nu, double hx, double hy, double hz, double* ke_host) {
+  PYTO_REQUIRE(ke_host, PYTO_EINVAL, "ke_host is null");
+  PYTO_REQUIRE(hx > 0 && hy > 0 && hz > 0, PYTO_EINVAL, "element size must be positive");
+  PYTO_REQUIRE(nu > -1.0 && nu < 0.5, PYTO_EINVAL, "Poisson ratio must be in (-1, 0.5)");
+  Handle tmp;
+  tmp.mc64 = make_modal_consts<double>(E, nu, hx, hy, hz);
+  host_ke(&tmp, ke_host);
+  return PYTO_OK;
+}
+
+int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, double hy,
+                double hz, double E0, double nu, double penal, double eps_void,
+                const uint8_t* fixed_mask, void* stream) {
+  PYTO_REQUIRE(out, PYTO_EINVAL, "out is null");
+  *out = nullptr;
+  PYTO_REQUIRE(nelx >= 1 && nely >= 1 && nelz >= 1, PYTO_EINVAL, "grid must be >= 1x1x1");
+  PYTO_REQUIRE(hx > 0 && hy > 0 && hz > 0, PYTO_EINVAL, "element size must be positive");
+  PYTO_REQUIRE(nu > -1.0 && nu < 0.5, PYTO_EINVAL, "Poisson ratio must be in (-1, 0.5)");
+  PYTO_REQUIRE(fixed_mask, PYTO_EINVAL, "fixed_mask is null");
+  const int64_t nn = (int64_t)(nelx + 1) * (nely + 1) * (nelz + 1);
+  PYTO_REQUIRE(3 * nn < (int64_t)INT32_MAX * 4, PYTO_EINVAL, "grid too large");
+  pyto_handle* h = new pyto_handle();
+  hipGetDevice(&h->device);
+  h->g.nx = nelx; h->g.ny = nely; h->g.nz = nelz;
+  h->g.sx = nelx + 1; h->g.sy = nely + 1; h->g.sz = nelz + 1;
+  h->g.nnodes = nn;
+  h->g.nelems = (int64_t)nelx * nely * nelz;
+  h->g.hx = hx; h->g.hy = hy; h->g.hz = hz;
+  h->E0 = E0; h->nu = nu; h->penal = penal; h->eps_void = eps_void;
+  h->mc64 = make_modal_consts<double>(E0, nu, hx, hy, hz);
+  h->mc32 = make_modal_consts<float>(E0, nu, hx, hy, hz);
+  double ke[576];
+  host_ke(h, ke);
+  for (int c = 0; c < 3; ++c) h->kediag[c] = ke[c * 24 + c];
+  const int64_t ne = h->g.nelems;
+  int rc = PYTO_OK;
+  auto fail = [&](int code) { pyto_destroy(h); return code; };
+  if ((rc = device_alloc(h, (void**)&h->fixmask, nn)) ||
+      (rc = device_alloc(h, (void**)&h->Ee64, ne * 8)) ||
+      (rc = device_alloc(h, (void**)&h->Ee32, ne * 4)) ||
+      (rc = device_alloc(h, (void**)&h->x64, ne * 8)) ||
+      (rc = device_alloc(h, (void**)&h->invd64, 3 * nn * 8)) ||
+      (rc = device_alloc(h, (void**)&h->invd32, 3 * nn * 4)) ||
+      (rc = device_alloc(h, (void**)&h->partials, kMaxPartials * 8)) ||
+      (rc = device_alloc(h, (void**)&h->counters, kNumCounters * sizeof(unsigned))) ||
+      (rc = device_alloc(h, (void**)&h->scal, kNumScalars * 8)) ||
+      (rc = device_alloc(h, (void**)&h->flags, kNumFlags * sizeof(int))))
+    return fail(rc);
+  if (hipHostMalloc((void**)&h->host_scal, kNumScalars * 8, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&h->host_flags, kNumFlags * sizeof(int), hipHostMallocDefault) !=
+          hipSuccess) {
+    set_error("hipHostMalloc failed");
+    return fail(PYTO_ENOMEM);
+  }
+  hipStream_t s = as_stream(stream);
+  if (hipMemcpyAsync(h->fixmask, fixed_mask, nn, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipMemsetAsync(h->counters, 0, kNumCounters * sizeof(unsigned), s) != hipSuccess ||
+      hipMemsetAsync(h->scal, 0, kNumScalars * 8, s) != hipSuccess ||
+      hipMemsetAsync(h->flags, 0, kNumFlags * sizeof(int), s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("initialising the handle failed: %s", hipGetErrorString(hipGetLastError()));
+    return fail(PYTO_EHIP);
+  }
+  *out = h;
+  return PYTO_OK;
+}
+
+int pyto_destroy(pyto_handle* h) {
+  if (!h) return PYTO_OK;
+  void* ptrs[] = {h->fixmask, h->Ee64, h->Ee32, h->x64, h->invd64, h->invd32, h->node_group,
+                  h->centers, h->special, h->chunks, h->Ec, h->Einv, h->cwork, h->ct,
+                  h->cmu, h->cres, h->r64, h->z64, h->p64, h->q64, h->r32, h->z32, h->p32,
+                  h->q32, h->x32, h->b32, h->partials, h->counters, h->scal, h->flags, h->elem_buf};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (h->host_scal) hipHostFree(h->host_scal);
+  if (h->host_flags) hipHostFree(h->host_flags);
+  delete h;
+  return PYTO_OK;
+}
+
+int pyto_info(pyto_handle* h, int64_t* nnodes, int64_t* nelems, int64_t* ndof,
+              int64_t* workspace_bytes) {
+  PYTO_REQUIRE(h, PYTO_EINVAL, "null handle");
+  if (nnodes) *nnodes = h->g.nnodes;
+  if (nelems) *nelems = h->g.nelems;
+  if (ndof) *ndof = 3 * h->g.nnodes;
+  if (workspace_bytes) *workspace_bytes = h->workspace_bytes;
+  return PYTO_OK;
+}
+
+int pyto_set_density(pyto_handle* h, const double* x, void* stream) {
+  PYTO_REQUIRE(h && x, PYTO_EINVAL, "null argument");
+  hipStream_t s = as_stream(stream);
+  const int64_t ne = h->g.nelems;
+  int blocks = (int)std::min<int64_t>((ne + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_simp, dim3(blocks), dim3(256), 0, s, x, h->Ee64, h->Ee32, h->x64, ne,
+                     h->penal, h->eps_void);
+  PYTO_CHECK_HIP(hipGetLastError());
+  blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(256), 0, s, h->Ee64, h->fixmask, h->g,
+                     h->kediag[0], h->kediag[1], h->kediag[2], h->invd64, h->invd32,
+                     (double*)nullptr);
+  PYTO_CHECK_HIP(hipGetLastError());
+  h->density_set = true;
+  if (h->ngroups > 0) PYTO_TRY(coarse_setup(h, s));
+  return PYTO_OK;
+}
+
+int pyto_diagonal_f64(pyto_handle* h, double* d, void* stream) {
+  PYTO_REQUIRE(h && d, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(h->density_set, PYTO_ESTATE, "pyto_set_density must be called first");
+  int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(256), 0, as_stream(stream), h->Ee64,
+                     h->fixmask, h->g, h->kediag[0], h->kediag[1], h->kediag[2],
+                     (double*)nullptr, (float*)nullptr, d);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+int pyto_matvec_f64(pyto_handle* h, const double* u, double* y, void* stream) {
+  PYTO_REQUIRE(h && u && y, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(u != y, PYTO_EINVAL, "u and y must not alias");
+  PYTO_REQUIRE(h->density_set, PYTO_ESTATE, "pyto_set_density must be called first");
+  return launch_matvec<double>(h, u, y, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0,
+                               as_stream(stream));
+}
+
+int pyto_matvec_f32(pyto_handle* h, const float* u, float* y, void* stream) {
+  PYTO_REQUIRE(h && u && y, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(u != y, PYTO_EINVAL, "u and y must not alias");
+  PYTO_REQUIRE(h->density_set, PYTO_ESTATE, "pyto_set_density must be called first");
+  return launch_matvec<float>(h, u, y, h->Ee32, h->mc32, nullptr, nullptr, nullptr, 0,
+                              as_stream(stream));
+}
+
+}  // extern "C"

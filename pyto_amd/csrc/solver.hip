@@ -1,0 +1,392 @@
+// Deflated Jacobi-preconditioned CG on the device (src/deflation.py:504-584)
+// and its FP32-inner / FP64-outer iterative refinement.
+//
+// One iteration is five launches on one stream; every launch first reads the
+// device flag F_DONE and returns when it is set, so the host can enqueue a
+// batch of iterations and read the flag back once per batch while the
+// stopping rule sqrt(rz_new/rz0) <= rtol is applied on the device at exactly
+// the reference's iteration:
+//   1. q = K p, pKp = p.q (FP64 grid reduction); zero the coarse rhs
+//   2. alpha = rz/pKp; x += alpha p; r -= alpha q; z = M r; rz_new = r.z;
+//      convergence test; beta = rz_new/rz
+//   3. t = W^T K z (non-rigid elements only)
+//   4. mu = E^{-1} t (FP64 coarse inverse with one refinement step)
+//   5. p = z + beta p - W mu
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "element.hpp"
+
+namespace pyto {
+
+template <typename T>
+int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalConsts<T>& mc,
+                  double* dot_out, const int* done, double* zero_vec, int zero_n,
+                  hipStream_t s);
+template <typename T>
+int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s);
+template <typename T>
+int launch_restrict_special(Handle* h, const T* v, const T* Ee, const ModalConsts<T>& mc,
+                            double* t, const int* done, hipStream_t s);
+template <typename T, bool UPDATE>
+int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
+                   bool first, const int* done, hipStream_t s);
+int coarse_apply(Handle* h, const double* t, double* mu, const int* done, hipStream_t s);
+
+constexpr int kVecBlocks = 2048;
+
+// alpha = rz/pKp; x += alpha p; r -= alpha q; z = invd r; rz_new = r.z.
+template <typename T, typename TX>
+__global__ __launch_bounds__(256) void k_cg_update(TX* __restrict__ x, T* __restrict__ r,
+                                                   T* __restrict__ z, const T* __restrict__ p,
+                                                   const T* __restrict__ q,
+                                                   const T* __restrict__ invd, int64_t n,
+                                                   double* __restrict__ scal,
+                                                   int* __restrict__ flags,
+                                                   double* __restrict__ partials,
+                                                   unsigned* __restrict__ counter) {
+  if (flags[F_DONE]) return;
+  __shared__ double red[4];
+  const double alpha = scal[S_RZ] / scal[S_PKP];
+  const T a = (T)alpha;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] = (TX)((double)x[i] + alpha * (double)p[i]);
+    const T ri = r[i] - a * q[i];
+    const T zi = invd[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    acc += (double)ri * (double)zi;
+  }
+  const double bs = block_reduce_sum256(acc, red);
+  grid_reduce_finish(bs, partials, counter, gridDim.x, blockIdx.x, [=](double rz_new) {
+    const double rz = scal[S_RZ];
+    const double rel = sqrt(fabs(rz_new / scal[S_RZ0]));
+    const double pkp = scal[S_PKP];
+    scal[S_RZNEW] = rz_new;
+    scal[S_REL] = rel;
+    flags[F_ITERS] += 1;
+    if (!(pkp > 0.0) || !(rz_new == rz_new)) {
+      flags[F_BREAKDOWN] = 1;
+      flags[F_DONE] = 1;
+    } else if (rel <= scal[S_RTOL]) {
+      flags[F_DONE] = 1;
+    }
+    scal[S_BETA] = rz_new / rz;
+    scal[S_RZ] = rz_new;
+  });
+}
+
+// r = f - q (T from double f scaled by 1/scale), z = invd r, rz -> slots.
+template <typename T, typename TF>
+__global__ __launch_bounds__(256) void k_residual(const TF* __restrict__ f, const T* __restrict__ q,
+                                                  T* __restrict__ r, T* __restrict__ z,
+                                                  const T* __restrict__ invd, int64_t n,
+                                                  double* __restrict__ scal, int slot_a,
+                                                  int slot_b, double* __restrict__ partials,
+                                                  unsigned* __restrict__ counter) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T ri = (T)((double)f[i] - (double)q[i]);
+    const T zi = invd[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    acc += (double)ri * (double)zi;
+  }
+  const double bs = block_reduce_sum256(acc, red);
+  grid_reduce_finish(bs, partials, counter, gridDim.x, blockIdx.x, [=](double tot) {
+    scal[slot_a] = tot;
+    if (slot_b >= 0) scal[slot_b] = tot;
+  });
+}
+
+// out = (T)(in * (1/scal[slot]))  (scal[slot] > 0)
+template <typename T>
+__global__ void k_scale_convert(const double* __restrict__ in, T* __restrict__ out, int64_t n,
+                                const double* __restrict__ scal, int slot) {
+  const double inv = 1.0 / sqrt(scal[slot]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (T)(in[i] * inv);
+}
+
+// u += sqrt(scal[slot]) * d
+__global__ void k_axpy_scaled(double* __restrict__ u, const float* __restrict__ d, int64_t n,
+                              const double* __restrict__ scal, int slot) {
+  const double s = sqrt(scal[slot]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    u[i] += s * (double)d[i];
+}
+
+// scal[slot] = sum over free DOFs (free_ind != 0) of (a_i - b_i)^2 (b may be null)
+__global__ __launch_bounds__(256) void k_norm2_diff(const double* __restrict__ a,
+                                                    const double* __restrict__ b,
+                                                    const double* __restrict__ free_ind,
+                                                    int64_t n, double* __restrict__ scal, int slot,
+                                                    double* __restrict__ partials,
+                                                    unsigned* __restrict__ counter) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = a[i] - (b ? b[i] : 0.0);
+    if (free_ind[i] != 0.0) acc += d * d;
+  }
+  const double bs = block_reduce_sum256(acc, red);
+  grid_reduce_finish(bs, partials, counter, gridDim.x, blockIdx.x,
+                     [=](double tot) { scal[slot] = tot; });
+}
+
+__global__ void k_set_flags(int* flags, int done, int iters) {
+  flags[F_DONE] = done;
+  flags[F_ITERS] = iters;
+  flags[F_BREAKDOWN] = 0;
+}
+
+__global__ void k_set_scalar(double* scal, int slot, double v) { scal[slot] = v; }
+
+template <typename T>
+struct Vecs {
+  T *r, *z, *p, *q;
+  const T* invd;
+  const T* Ee;
+  ModalConsts<T> mc;
+};
+
+static Vecs<double> vecs64(Handle* h) {
+  return {h->r64, h->z64, h->p64, h->q64, h->invd64, h->Ee64, h->mc64};
+}
+static Vecs<float> vecs32(Handle* h) {
+  return {h->r32, h->z32, h->p32, h->q32, h->invd32, h->Ee32, h->mc32};
+}
+
+template <typename T>
+Vecs<T> get_vecs(Handle* h);
+template <>
+Vecs<double> get_vecs<double>(Handle* h) { return vecs64(h); }
+template <>
+Vecs<float> get_vecs<float>(Handle* h) { return vecs32(h); }
+
+struct DpcgResult {
+  int iterations = 0;
+  bool converged = false;
+  bool breakdown = false;
+  double rel = 0.0;
+  int matvecs = 0;
+};
+
+static int read_back(Handle* h, hipStream_t s) {
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->host_flags, h->flags, kNumFlags * sizeof(int),
+                                hipMemcpyDeviceToHost, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->host_scal, h->scal, kNumScalars * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  return PYTO_OK;
+}
+
+// Deflated PCG on K x = b (b and x full-length in precision T; fixed DOFs of
+// b ignored, of x zero).  The deflated start x0 = W E^-1 W^T b and the first
+// direction p0 = z0 - W E^-1 W^T K z0 follow src/deflation.py:532-551.
+template <typename T>
+static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgResult& res,
+                hipStream_t s) {
+  Vecs<T> v = get_vecs<T>(h);
+  const int64_t n = 3 * h->g.nnodes;
+  const bool defl = h->ngroups > 0;
+  if (defl) {
+    PYTO_TRY(launch_restrict_full<T>(h, b, h->ct, s));
+    PYTO_TRY(coarse_apply(h, h->ct, h->cmu, nullptr, s));
+    PYTO_TRY(launch_prolong<T, false>(h, x, nullptr, h->cmu, nullptr, true, nullptr, s));
+  } else {
+    PYTO_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(T), s));
+  }
+  PYTO_TRY(launch_matvec<T>(h, x, v.q, v.Ee, v.mc, nullptr, nullptr, nullptr, 0, s));
+  res.matvecs += 1;
+  hipLaunchKernelGGL((k_residual<T, T>), dim3(kVecBlocks), dim3(256), 0, s, b, v.q, v.r, v.z,
+                     v.invd, n, h->scal, (int)S_RZ, (int)S_RZ0, h->partials,
+                     h->counters + C_DOT);
+  if (defl) {
+    PYTO_CHECK_HIP(hipMemsetAsync(h->ct, 0, sizeof(double) * h->nc, s));
+    PYTO_TRY(launch_restrict_special<T>(h, v.z, v.Ee, v.mc, h->ct, nullptr, s));
+    PYTO_TRY(coarse_apply(h, h->ct, h->cmu, nullptr, s));
+  }
+  PYTO_TRY(launch_prolong<T, true>(h, v.p, v.z, h->cmu, nullptr, true, nullptr, s));
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RTOL, rtol);
+  hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(1), 0, s, h->flags, 0, 0);
+  PYTO_CHECK_HIP(hipGetLastError());
+  PYTO_TRY(read_back(h, s));
+  if (h->host_scal[S_RZ0] == 0.0) {
+    res.converged = true;
+    res.rel = 0.0;
+    return PYTO_OK;
+  }
+  int launched = 0;
+  int batch = 8;
+  while (launched < max_iters) {
+    const int nb = std::min(batch, max_iters - launched);
+    for (int k = 0; k < nb; ++k) {
+      PYTO_TRY(launch_matvec<T>(h, v.p, v.q, v.Ee, v.mc, h->scal + S_PKP, h->flags + F_DONE,
+                                defl ? h->ct : nullptr, h->nc, s));
+      hipLaunchKernelGGL((k_cg_update<T, T>), dim3(kVecBlocks), dim3(256), 0, s, x, v.r, v.z,
+                         v.p, v.q, v.invd, n, h->scal, h->flags, h->partials,
+                         h->counters + C_UPDATE);
+      if (defl) {
+        PYTO_TRY(launch_restrict_special<T>(h, v.z, v.Ee, v.mc, h->ct, h->flags + F_DONE, s));
+        PYTO_TRY(coarse_apply(h, h->ct, h->cmu, h->flags + F_DONE, s));
+      }
+      PYTO_TRY(launch_prolong<T, true>(h, v.p, v.z, h->cmu, h->scal + S_BETA, false,
+                                       h->flags + F_DONE, s));
+    }
+    PYTO_CHECK_HIP(hipGetLastError());
+    launched += nb;
+    PYTO_TRY(read_back(h, s));
+    if (h->host_flags[F_DONE]) break;
+    batch = std::min(batch * 2, 32);
+  }
+  res.iterations = h->host_flags[F_ITERS];
+  res.matvecs += res.iterations;
+  res.breakdown = h->host_flags[F_BREAKDOWN] != 0;
+  res.converged = h->host_flags[F_DONE] != 0 && !res.breakdown;
+  res.rel = h->host_scal[S_REL];
+  return PYTO_OK;
+}
+
+// ||f - K u|| / ||f|| over free DOFs (invd64 != 0 marks them), FP64.
+static int true_residual(Handle* h, const double* f, const double* u, double* out,
+                         hipStream_t s) {
+  const int64_t n = 3 * h->g.nnodes;
+  PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0, s));
+  hipLaunchKernelGGL(k_norm2_diff, dim3(kVecBlocks), dim3(256), 0, s, f, h->q64, h->invd64, n,
+                     h->scal, (int)S_DOT0, h->partials, h->counters + C_NORM);
+  hipLaunchKernelGGL(k_norm2_diff, dim3(kVecBlocks), dim3(256), 0, s, f, (const double*)nullptr,
+                     h->invd64, n, h->scal, (int)S_DOT1, h->partials, h->counters + C_NORM);
+  PYTO_CHECK_HIP(hipGetLastError());
+  PYTO_TRY(read_back(h, s));
+  const double nf = h->host_scal[S_DOT1];
+  *out = nf > 0 ? sqrt(h->host_scal[S_DOT0] / nf) : 0.0;
+  return PYTO_OK;
+}
+
+static int check_ready(Handle* h) {
+  PYTO_REQUIRE(h->density_set, PYTO_ESTATE, "pyto_set_density must be called first");
+  PYTO_REQUIRE(h->ngroups == 0 || h->coarse_ready, PYTO_ESTATE, "coarse space not assembled");
+  return ensure_solver_workspace(h);
+}
+
+}  // namespace pyto
+
+using namespace pyto;
+
+extern "C" {
+
+int pyto_solve_dpcg_f64(pyto_handle* h, const double* f, double* u, double rtol,
+                        int max_iters, pyto_solve_stats* stats, void* stream) {
+  PYTO_REQUIRE(h && f && u, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(f != u, PYTO_EINVAL, "f and u must not alias");
+  PYTO_REQUIRE(rtol > 0 && max_iters >= 1, PYTO_EINVAL, "rtol must be > 0 and max_iters >= 1");
+  PYTO_TRY(check_ready(h));
+  hipStream_t s = as_stream(stream);
+  DpcgResult r;
+  PYTO_TRY(dpcg<double>(h, f, u, rtol, max_iters, r, s));
+  double rel_true = 0.0;
+  PYTO_TRY(true_residual(h, f, u, &rel_true, s));
+  if (stats) {
+    stats->iterations = r.iterations;
+    stats->outer_iterations = 1;
+    stats->converged = r.converged;
+    stats->matvecs64 = r.matvecs + 1;
+    stats->matvecs32 = 0;
+    stats->rel = r.rel;
+    stats->rel_true = rel_true;
+    stats->setup_ms = 0.0;
+  }
+  if (r.breakdown) {
+    set_error("DPCG breakdown (p^T K p <= 0 or NaN) after %d iterations", r.iterations);
+    return PYTO_ENOCONV;
+  }
+  if (!r.converged) {
+    set_error("maximum iterations reached in DPCG; relative residual %.3e", r.rel);
+    return PYTO_ENOCONV;
+  }
+  return PYTO_OK;
+}
+
+int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rtol,
+                          double inner_rtol, int max_iters, int max_outer,
+                          pyto_solve_stats* stats, void* stream) {
+  PYTO_REQUIRE(h && f && u, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(f != u, PYTO_EINVAL, "f and u must not alias");
+  PYTO_REQUIRE(rtol > 0 && inner_rtol > 0 && inner_rtol < 1 && max_iters >= 1 && max_outer >= 1,
+               PYTO_EINVAL, "bad tolerances or iteration limits");
+  PYTO_TRY(check_ready(h));
+  hipStream_t s = as_stream(stream);
+  const int64_t n = 3 * h->g.nnodes;
+  const bool defl = h->ngroups > 0;
+  // FP64 deflated start and outer residual
+  if (defl) {
+    PYTO_TRY(launch_restrict_full<double>(h, f, h->ct, s));
+    PYTO_TRY(coarse_apply(h, h->ct, h->cmu, nullptr, s));
+    PYTO_TRY(launch_prolong<double, false>(h, u, nullptr, h->cmu, nullptr, true, nullptr, s));
+  } else {
+    PYTO_CHECK_HIP(hipMemsetAsync(u, 0, n * sizeof(double), s));
+  }
+  int mv64 = 0, mv32 = 0, inner_total = 0, outer = 0;
+  PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0, s));
+  mv64++;
+  hipLaunchKernelGGL((k_residual<double, double>), dim3(kVecBlocks), dim3(256), 0, s, f, h->q64,
+                     h->r64, h->z64, h->invd64, n, h->scal, (int)S_TMP1, (int)S_TMP0,
+                     h->partials, h->counters + C_DOT);
+  PYTO_CHECK_HIP(hipGetLastError());
+  PYTO_TRY(read_back(h, s));
+  const double rz0 = h->host_scal[S_TMP0];
+  double rel = rz0 > 0 ? 1.0 : 0.0;
+  bool converged = rz0 == 0.0;
+  bool inner_fail = false;
+  while (!converged && outer < max_outer && inner_total < max_iters) {
+    hipLaunchKernelGGL(k_scale_convert<float>, dim3(kVecBlocks), dim3(256), 0, s, h->r64, h->b32,
+                       n, h->scal, (int)S_TMP1);
+    DpcgResult ir;
+    PYTO_TRY(dpcg<float>(h, h->b32, h->x32, inner_rtol, max_iters - inner_total, ir, s));
+    inner_total += ir.iterations;
+    mv32 += ir.matvecs;
+    if (ir.breakdown) inner_fail = true;
+    hipLaunchKernelGGL(k_axpy_scaled, dim3(kVecBlocks), dim3(256), 0, s, u, h->x32, n, h->scal,
+                       (int)S_TMP1);
+    PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0, s));
+    mv64++;
+    hipLaunchKernelGGL((k_residual<double, double>), dim3(kVecBlocks), dim3(256), 0, s, f,
+                       h->q64, h->r64, h->z64, h->invd64, n, h->scal, (int)S_TMP1, -1,
+                       h->partials, h->counters + C_DOT);
+    PYTO_CHECK_HIP(hipGetLastError());
+    PYTO_TRY(read_back(h, s));
+    outer++;
+    rel = sqrt(fabs(h->host_scal[S_TMP1] / rz0));
+    converged = rel <= rtol;
+    if (inner_fail) break;
+  }
+  double rel_true = 0.0;
+  PYTO_TRY(true_residual(h, f, u, &rel_true, s));
+  if (stats) {
+    stats->iterations = inner_total;
+    stats->outer_iterations = outer;
+    stats->converged = converged;
+    stats->matvecs64 = mv64 + 1;
+    stats->matvecs32 = mv32;
+    stats->rel = rel;
+    stats->rel_true = rel_true;
+    stats->setup_ms = 0.0;
+  }
+  if (!converged) {
+    set_error("mixed-precision DPCG did not reach rtol %.1e (relative residual %.3e after %d "
+              "refinement steps, %d inner iterations)", rtol, rel, outer, inner_total);
+    return PYTO_ENOCONV;
+  }
+  return PYTO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,261 @@
+"""Host object around one libpytohip handle: the matrix-free stiffness operator
+of a voxel grid and everything that runs on it (solves, element kernels,
+cone filter).
+
+All vectors are torch CUDA tensors: DOF vectors are float64 of length
+3*num_nodes in the reference's AoS order, element vectors float64 of length
+num_elems.  The reference-facing modules (hex_structural_fea, linear_solvers,
+deflation, topopt_*) are thin layers over this class.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr, stream_ptr
+
+F64 = torch.float64
+F32 = torch.float32
+
+
+def fixed_mask_from_dofs(num_nodes, fixed_dofs):
+    """uint8 per node, bit c set when DOF 3*node+c is fixed."""
+    fixed = np.asarray(fixed_dofs, dtype=np.int64).ravel()
+    if fixed.size and (fixed.min() < 0 or fixed.max() >= 3 * num_nodes):
+        raise ValueError("fixed_dofs out of range")
+    m = np.zeros(num_nodes, dtype=np.uint8)
+    np.bitwise_or.at(m, fixed // 3, (1 << (fixed % 3)).astype(np.uint8))
+    return m
+
+
+class GridOperator:
+    """K(x) of a structured grid of nelx*nely*nelz brick elements.
+
+    Replaces the assembled COO/CSR ``stiff_mtrx`` of
+    ``HexStructuralFEA.solve`` (src/hex_structural_fea.py:87-102).
+    """
+
+    def __init__(self, num_elems, elem_size, E0, nu, fixed_dofs, penal=3.0, eps_void=1e-8,
+                 device=None):
+        _lib.require_gpu()
+        self.device = torch.device(device or "cuda")
+        self.grid = tuple(int(v) for v in num_elems)
+        self.elem_size = tuple(float(v) for v in elem_size)
+        nelx, nely, nelz = self.grid
+        self.num_nodes = (nelx + 1) * (nely + 1) * (nelz + 1)
+        self.num_elems = nelx * nely * nelz
+        self.num_dofs = 3 * self.num_nodes
+        self.E0, self.nu, self.penal, self.eps_void = float(E0), float(nu), float(penal), float(eps_void)
+        mask = fixed_mask_from_dofs(self.num_nodes, fixed_dofs)
+        self.fixed_mask = torch.from_numpy(mask).to(self.device)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().pyto_create(ctypes.byref(h), nelx, nely, nelz, *self.elem_size,
+                                         self.E0, self.nu, self.penal, self.eps_void,
+                                         ptr(self.fixed_mask), stream_ptr()), "pyto_create")
+        self._h = h
+        self.ngroups = 0
+        self.x = None
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().pyto_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise RuntimeError("operator is closed")
+        return self._h
+
+    def workspace_bytes(self):
+        wb = ctypes.c_int64()
+        check(_lib.lib().pyto_info(self.handle, None, None, None, ctypes.byref(wb)))
+        return wb.value
+
+    # -- state -------------------------------------------------------------
+    def set_groups(self, node_group, centers):
+        """Deflation groups (int32 per node) and centres (G x 3)."""
+        g = torch.as_tensor(np.ascontiguousarray(node_group, dtype=np.int32)).to(self.device)
+        c = torch.as_tensor(np.ascontiguousarray(centers, dtype=np.float64)).to(self.device)
+        ng = int(c.shape[0])
+        if ng == 0:
+            check(_lib.lib().pyto_set_groups(self.handle, None, 0, None, stream_ptr()))
+            self.ngroups = 0
+            return
+        check(_lib.lib().pyto_set_groups(self.handle, ptr(g, torch.int32, self.num_nodes, "node_group"),
+                                         ng, ptr(c, F64, 3 * ng, "centers"), stream_ptr()),
+              "pyto_set_groups")
+        self.ngroups = ng
+
+    def set_coarse_shift(self, abs_shift=1e-10, rel_shift=0.0):
+        """Coarse diagonal shift; (1e-10, 0) is the reference's, (0, 0) the
+        exact Galerkin matrix.  Applies from the next set_density."""
+        check(_lib.lib().pyto_set_coarse_shift(self.handle, float(abs_shift), float(rel_shift)))
+
+    def set_density(self, x):
+        """Element densities; also rebuilds Jacobi and the coarse factor."""
+        x = self.as_elem(x)
+        check(_lib.lib().pyto_set_density(self.handle, ptr(x, F64, self.num_elems, "x"),
+                                          stream_ptr()), "pyto_set_density")
+        self.x = x
+        return x
+
+    # -- conversions -------------------------------------------------------
+    def as_elem(self, v):
+        t = torch.as_tensor(v, dtype=F64)
+        return t.to(self.device).contiguous().reshape(-1)
+
+    def as_dof(self, v):
+        t = torch.as_tensor(v, dtype=F64)
+        return t.to(self.device).contiguous().reshape(-1)
+
+    # -- operator ------------------------------------------------------------
+    def matvec(self, u, out=None):
+        if u.dtype == F32:
+            out = torch.empty_like(u) if out is None else out
+            check(_lib.lib().pyto_matvec_f32(self.handle, ptr(u, F32, self.num_dofs, "u"),
+                                             ptr(out, F32, self.num_dofs, "out"), stream_ptr()))
+            return out
+        u = self.as_dof(u)
+        out = torch.empty_like(u) if out is None else out
+        check(_lib.lib().pyto_matvec_f64(self.handle, ptr(u, F64, self.num_dofs, "u"),
+                                         ptr(out, F64, self.num_dofs, "out"), stream_ptr()))
+        return out
+
+    __matmul__ = matvec
+
+    def diagonal(self):
+        d = torch.empty(self.num_dofs, dtype=F64, device=self.device)
+        check(_lib.lib().pyto_diagonal_f64(self.handle, ptr(d), stream_ptr()))
+        return d
+
+    @property
+    def shape(self):
+        return (self.num_dofs, self.num_dofs)
+
+    # -- coarse space --------------------------------------------------------
+    def restrict(self, v):
+        t = torch.empty(6 * self.ngroups, dtype=F64, device=self.device)
+        check(_lib.lib().pyto_restrict_f64(self.handle, ptr(self.as_dof(v)), ptr(t), stream_ptr()))
+        return t
+
+    def prolong(self, mu):
+        v = torch.empty(self.num_dofs, dtype=F64, device=self.device)
+        mu = torch.as_tensor(mu, dtype=F64, device=self.device).contiguous()
+        check(_lib.lib().pyto_prolong_f64(self.handle, ptr(mu, F64, 6 * self.ngroups), ptr(v),
+                                          stream_ptr()))
+        return v
+
+    def coarse_solve(self, t):
+        t = torch.as_tensor(t, dtype=F64, device=self.device).contiguous()
+        mu = torch.empty_like(t)
+        check(_lib.lib().pyto_coarse_solve_f64(self.handle, ptr(t, F64, 6 * self.ngroups),
+                                               ptr(mu), stream_ptr()))
+        return mu
+
+    def coarse_matrix(self):
+        n = 6 * self.ngroups
+        E = torch.empty((n, n), dtype=F64, device=self.device)
+        check(_lib.lib().pyto_coarse_matrix_f64(self.handle, ptr(E), stream_ptr()))
+        return E
+
+    # -- solves ----------------------------------------------------------------
+    def solve(self, f, rtol=1e-8, max_iters=2000, precision="fp64", inner_rtol=1e-4,
+              max_outer=20, out=None):
+        """u with K_ff u_f = f_f and u = 0 on fixed DOFs.
+
+        precision "fp64": the reference DPCG (src/deflation.py:504-584).
+        precision "mixed": FP32 DPCG inside FP64 iterative refinement.
+        Returns (u, stats dict).
+        """
+        f = self.as_dof(f)
+        u = torch.empty_like(f) if out is None else out
+        st = _lib.SolveStats()
+        L = _lib.lib()
+        if precision == "fp64":
+            rc = L.pyto_solve_dpcg_f64(self.handle, ptr(f, F64, self.num_dofs, "f"),
+                                       ptr(u, F64, self.num_dofs, "u"), float(rtol),
+                                       int(max_iters), ctypes.byref(st), stream_ptr())
+        elif precision == "mixed":
+            rc = L.pyto_solve_dpcg_mixed(self.handle, ptr(f, F64, self.num_dofs, "f"),
+                                         ptr(u, F64, self.num_dofs, "u"), float(rtol),
+                                         float(inner_rtol), int(max_iters), int(max_outer),
+                                         ctypes.byref(st), stream_ptr())
+        else:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        check(rc, "solve")
+        return u, st.as_dict()
+
+    # -- element kernels -------------------------------------------------------
+    def compliance(self, u, x=None, want_ce=False, want_grad=True):
+        x = self.x if x is None else self.as_elem(x)
+        u = self.as_dof(u)
+        C = torch.empty(1, dtype=F64, device=self.device)
+        ce = torch.empty(self.num_elems, dtype=F64, device=self.device) if want_ce else None
+        g = torch.empty(self.num_elems, dtype=F64, device=self.device) if want_grad else None
+        check(_lib.lib().pyto_compliance(self.handle, ptr(u, F64, self.num_dofs), ptr(x, F64, self.num_elems),
+                                         ptr(ce), ptr(g), ptr(C), stream_ptr()))
+        return C[0], g, ce
+
+    def bilinear(self, a, b):
+        out = torch.empty(self.num_elems, dtype=F64, device=self.device)
+        check(_lib.lib().pyto_element_bilinear(self.handle, ptr(self.as_dof(a), F64, self.num_dofs),
+                                               ptr(self.as_dof(b), F64, self.num_dofs), ptr(out),
+                                               stream_ptr()))
+        return out
+
+    def stress(self, u, x=None, q=0.5, pexp=6.0, want_strain=True, want_stress=True,
+               want_energy=True):
+        x = self.x if x is None else self.as_elem(x)
+        u = self.as_dof(u)
+        ne = self.num_elems
+        dev = self.device
+        strain = torch.empty((ne, 6), dtype=F64, device=dev) if want_strain else None
+        stress = torch.empty((ne, 6), dtype=F64, device=dev) if want_stress else None
+        energy = torch.empty(ne, dtype=F64, device=dev) if want_energy else None
+        vm = torch.empty(ne, dtype=F64, device=dev)
+        s = torch.empty(1, dtype=F64, device=dev)
+        check(_lib.lib().pyto_stress(self.handle, ptr(u, F64, self.num_dofs), ptr(x, F64, ne),
+                                     float(q), float(pexp), ptr(strain), ptr(stress), ptr(vm),
+                                     ptr(energy), ptr(s), stream_ptr()))
+        return {"strain": strain, "stress": stress, "vm": vm, "energy": energy, "sum_vm_p": s}
+
+    def pnorm_adjoint_rhs(self, u, x, vm, sum_vm_p, q=0.5, pexp=6.0):
+        x = self.as_elem(x)
+        g = torch.empty(self.num_dofs, dtype=F64, device=self.device)
+        T1 = torch.empty(self.num_elems, dtype=F64, device=self.device)
+        check(_lib.lib().pyto_pnorm_adjoint_rhs(self.handle, ptr(self.as_dof(u), F64, self.num_dofs),
+                                                ptr(x, F64, self.num_elems), ptr(vm), ptr(sum_vm_p, F64, 1),
+                                                float(q), float(pexp), ptr(g), ptr(T1), stream_ptr()))
+        return g, T1
+
+    # -- cone filter -------------------------------------------------------------
+    def filter(self, v, rel_radius=1.5, mode=0, aux=None):
+        v = self.as_elem(v) if v is not None else None
+        aux = self.as_elem(aux) if aux is not None else None
+        out = torch.empty(self.num_elems, dtype=F64, device=self.device)
+        check(_lib.lib().pyto_filter_apply(self.handle, float(rel_radius), int(mode),
+                                           ptr(v, F64, self.num_elems) if v is not None else None,
+                                           ptr(aux, F64, self.num_elems) if aux is not None else None,
+                                           ptr(out), stream_ptr()))
+        return out
+
+    def filter_stencil(self, rel_radius=1.5):
+        n = ctypes.c_int()
+        L = _lib.lib()
+        check(L.pyto_filter_stencil(self.handle, float(rel_radius), 0, ctypes.byref(n), None, None))
+        off = np.zeros((n.value, 3), dtype=np.int32)
+        w = np.zeros(n.value, dtype=np.float64)
+        check(L.pyto_filter_stencil(self.handle, float(rel_radius), n.value, ctypes.byref(n),
+                                    off.ctypes.data_as(ctypes.c_void_p),
+                                    w.ctypes.data_as(ctypes.c_void_p)))
+        return off, w

@@ -1,0 +1,50 @@
+"""The C ABI: the library loads, exports every symbol include/pyto_hip.h
+declares, and its host-only entry point (the element stiffness) equals the
+oracle KE.  No GPU needed."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from pyto_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "pyto_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(pyto_\w+)\s*\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert sorted(_lib.EXPORTED) == declared_symbols()
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    assert L.pyto_version() >= 1
+
+
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 0.7, 1.3)])
+@pytest.mark.parametrize("E,nu", [(1.0, 0.3), (2.1e11, 0.3), (5.0, 0.45)])
+def test_host_ke_equals_oracle(h, E, nu):
+    from oracle import ke
+    from pyto_amd.hex_element_stiffness import hex8_stiffness_matrix_structural
+    A = hex8_stiffness_matrix_structural(E, nu, h)
+    B = ke.hex8_stiffness(E, nu, h)
+    assert np.abs(A - B).max() <= 1e-14 * np.abs(B).max()
+
+
+def test_bad_arguments_raise():
+    L = _lib.lib()
+    ke = np.zeros(576)
+    rc = L.pyto_ke_hex8(1.0, 0.6, 1.0, 1.0, 1.0, ke.ctypes.data_as(ctypes.c_void_p))
+    assert rc == _lib.PYTO_EINVAL
+    assert b"Poisson" in L.pyto_last_error()
+    with pytest.raises(ValueError):
+        _lib.check(rc)

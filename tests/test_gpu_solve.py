@@ -1,0 +1,87 @@
+"""GPU deflated PCG (FP64 reference algorithm and FP32 inside FP64
+refinement) against the oracle's DPCG and a sparse direct solve."""
+import warnings
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+import torch
+
+from oracle import deflation as ODF
+from oracle import fea as OF
+from tests.helpers import Case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(c, rtol=1e-8, shift=1e-10):
+    A, b, free = OF.impose_dirichlet(c.K, c.f, c.fixed, c.gv)
+    W = ODF.deflation_matrix(c.mesh.node_xyz, c.ngn, c.cen)[free, :]
+    xd, info = ODF.deflated_pcg(A, b, W, OF.jacobi_inverse_diag(A), rtol=rtol, shift=shift)
+    u_d = OF.scatter_solution(len(c.f), c.fixed, c.gv, free, xd)
+    u_s = OF.scatter_solution(len(c.f), c.fixed, c.gv, free, spla.spsolve(A.tocsc(), b))
+    return u_d, info, u_s
+
+
+@pytest.mark.parametrize("dims,load", [((60, 20, 10), (60, 10, 5)), ((24, 12, 8), "traction")])
+def test_dpcg_f64_matches_reference_algorithm(gpu, dims, load):
+    print("dpcg f64", dims, flush=True)
+    c = Case(dims, seed=0, lo=0.01, load=load)
+    op = c.operator(gpu)
+    u_d, info, u_s = _oracle(c)
+    u, st = op.solve(torch.from_numpy(c.f).to(gpu), rtol=1e-8, precision="fp64")
+    u = u.cpu().numpy()
+    assert st["converged"]
+    assert abs(st["iterations"] - info["iterations"]) <= 2
+    C, C_d, C_s = c.f @ u, c.f @ u_d, c.f @ u_s
+    assert abs(C - C_d) <= 1e-6 * abs(C_d)
+    assert abs(C - C_s) <= 1e-6 * abs(C_s)
+    assert rel_err(u, u_s) <= 1e-5
+    assert st["rel_true"] < 1e-6
+
+
+def test_dpcg_mixed_reaches_1e8(gpu):
+    print("dpcg mixed c1", flush=True)
+    c = Case((60, 20, 10), seed=0, lo=0.001, load=(60, 10, 5))
+    op = c.operator(gpu, shift=0.0)
+    u_d, info, u_s = _oracle(c, rtol=1e-10, shift=0.0)
+    u, st = op.solve(torch.from_numpy(c.f).to(gpu), rtol=1e-8, precision="mixed")
+    u = u.cpu().numpy()
+    assert st["converged"] and st["rel"] <= 1e-8
+    assert st["matvecs32"] > 0 and st["outer_iterations"] >= 1
+    C, C_s = c.f @ u, c.f @ u_s
+    assert abs(C - C_s) <= 1e-6 * abs(C_s)
+    assert rel_err(u, u_s) <= 1e-5
+
+
+def test_exact_coarse_reaches_tight_tolerance(gpu):
+    print("dpcg f64 tight", flush=True)
+    c = Case((24, 12, 8), seed=3, lo=0.05)
+    op = c.operator(gpu, shift=0.0)
+    _, _, u_s = _oracle(c)
+    u, st = op.solve(torch.from_numpy(c.f).to(gpu), rtol=1e-12, precision="fp64")
+    assert st["converged"]
+    assert rel_err(u.cpu().numpy(), u_s) <= 1e-9
+
+
+def test_plain_pcg_without_groups(gpu):
+    print("pcg no groups", flush=True)
+    c = Case((20, 8, 6), seed=4, lo=0.2)
+    op = c.operator(gpu, groups=False)
+    _, _, u_s = _oracle(c)
+    u, st = op.solve(torch.from_numpy(c.f).to(gpu), rtol=1e-10)
+    assert st["converged"]
+    assert rel_err(u.cpu().numpy(), u_s) <= 1e-7
+
+
+def test_zero_force_and_iteration_limit(gpu):
+    print("zero force / maxit", flush=True)
+    c = Case((12, 6, 4), seed=6)
+    op = c.operator(gpu)
+    u, st = op.solve(torch.zeros(len(c.f), dtype=torch.float64, device=gpu))
+    assert st["converged"] and st["iterations"] == 0
+    assert float(u.abs().max()) == 0.0
+    from pyto_amd._lib import ConvergenceWarning
+    with pytest.warns(ConvergenceWarning):
+        u, st = op.solve(torch.from_numpy(c.f).to(gpu), rtol=1e-14, max_iters=3)
+    assert st["iterations"] == 3 and not st["converged"]
