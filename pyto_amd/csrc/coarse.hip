@@ -85,8 +85,11 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ out, const T* _
                                                  const double* __restrict__ centers,
                                                  const uint8_t* __restrict__ mask, GridDims g,
                                                  const double* __restrict__ beta_ptr,
-                                                 bool first, const int* __restrict__ done) {
+                                                 bool first, const int* __restrict__ done,
+                                                 double* __restrict__ zero_vec, int zero_n) {
   if (done && *done) return;
+  if (zero_vec && blockIdx.x == 0)
+    for (int i = threadIdx.x; i < zero_n; i += blockDim.x) zero_vec[i] = 0.0;
   const double beta = (UPDATE && !first) ? *beta_ptr : 0.0;
   for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < g.nnodes;
        n += (int64_t)gridDim.x * blockDim.x) {
@@ -147,65 +150,146 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
   }
 }
 
-// t += W^T K v over the special elements (one wave per chunk).
+// t += W^T K v over the special elements (one wave per chunk), where
+// v = invd o r + beta p (the new search direction before its coarse
+// correction) or v = r when invd is null.  The element arithmetic is FP64
+// whatever the vector precision T.  Group centres are wave-uniform per slot.
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict_special(
-    const T* __restrict__ v, const int32_t* __restrict__ special,
-    const SpecialChunk* __restrict__ chunks, int nchunks, const T* __restrict__ Ee,
+    const T* __restrict__ r, const T* __restrict__ invd, const T* __restrict__ p,
+    const double* __restrict__ beta_ptr, const int32_t* __restrict__ special,
+    const SpecialChunk* __restrict__ chunks, int nchunks, const double* __restrict__ Ee,
     const int32_t* __restrict__ grp, const double* __restrict__ centers,
-    const uint8_t* __restrict__ mask, GridDims g, ModalConsts<T> mc, double* __restrict__ t,
-    const int* __restrict__ done) {
+    const uint8_t* __restrict__ mask, GridDims g, ModalConsts<double> mc,
+    double* __restrict__ t, const int* __restrict__ done) {
   if (done && *done) return;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= nchunks) return;
   const SpecialChunk ch = chunks[wave];
   const bool active = lane < ch.count;
-  double acc[8][6];
-#pragma unroll
-  for (int s = 0; s < 8; ++s)
-#pragma unroll
-    for (int j = 0; j < 6; ++j) acc[s][j] = 0.0;
+  const double beta = (p && beta_ptr) ? *beta_ptr : 0.0;
+  double f[8][3];
+  int gid[8];
+  int ix = 0, iy = 0, iz = 0;
   if (active) {
     const int64_t e = special[ch.start + lane];
-    ElemNodes en;
-    elem_nodes(e, g, grp, centers, mask, en);
-    T u[8][3], f[8][3];
+    elem_coords(e, g, ix, iy, iz);
+    double u[8][3];
+    uint32_t mbits = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int64_t n = (ix + corner_a(j)) + (int64_t)g.sx * ((iy + corner_b(j)) +
+                                                              (int64_t)g.sy * (iz + corner_c(j)));
+      gid[j] = grp[n];
+      const uint32_t m = mask[n];
+      mbits |= m << (3 * j);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double val = (double)r[3 * n + c];
+        if (invd) val *= (double)invd[3 * n + c];
+        if (p) val += beta * (double)p[3 * n + c];
+        u[j][c] = ((m >> c) & 1) ? 0.0 : val;
+      }
+    }
+    element_apply<double>(u, Ee[e], mc, f);
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) u[j][c] = ((en.m[j] >> c) & 1) ? T(0) : v[3 * en.node[j] + c];
-    element_apply<T>(u, Ee[e], mc, f);
+      for (int c = 0; c < 3; ++c)
+        if ((mbits >> (3 * j + c)) & 1) f[j][c] = 0.0;
+  } else {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const double fx = (en.m[j] & 1) ? 0.0 : (double)f[j][0];
-      const double fy = (en.m[j] & 2) ? 0.0 : (double)f[j][1];
-      const double fz = (en.m[j] & 4) ? 0.0 : (double)f[j][2];
-      const double rx = en.r[j][0], ry = en.r[j][1], rz = en.r[j][2];
-      double w6[6];
-      w6[0] = fx; w6[1] = fy; w6[2] = fz;
-      w6[3] = -rz * fy + ry * fz;
-      w6[4] = rz * fx - rx * fz;
-      w6[5] = -ry * fx + rx * fy;
-#pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        if (s < ch.ng && en.gid[j] == ch.groups[s]) {
-#pragma unroll
-          for (int jj = 0; jj < 6; ++jj) acc[s][jj] += w6[jj];
-        }
-      }
+      gid[j] = -1;
+      f[j][0] = f[j][1] = f[j][2] = 0.0;
     }
   }
+  for (int sl = 0; sl < ch.ng; ++sl) {
+    const int gs = ch.groups[sl];
+    const double cx = centers[3 * gs], cy = centers[3 * gs + 1], cz = centers[3 * gs + 2];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    if (s < ch.ng) {
+    for (int j = 0; j < 8; ++j) {
+      if (gid[j] != gs) continue;
+      const double rx = (ix + corner_a(j)) * g.hx - cx;
+      const double ry = (iy + corner_b(j)) * g.hy - cy;
+      const double rz = (iz + corner_c(j)) * g.hz - cz;
+      const double fx = f[j][0], fy = f[j][1], fz = f[j][2];
+      acc[0] += fx;
+      acc[1] += fy;
+      acc[2] += fz;
+      acc[3] += -rz * fy + ry * fz;
+      acc[4] += rz * fx - rx * fz;
+      acc[5] += -ry * fx + rx * fy;
+    }
 #pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        const double tot = wave_reduce_sum(acc[s][j]);
-        if (lane == 0) atomicAdd(&t[6 * ch.groups[s] + j], tot);
-      }
+    for (int j = 0; j < 6; ++j) {
+      const double tot = wave_reduce_sum(acc[j]);
+      if (lane == 0) atomicAdd(&t[6 * gs + j], tot);
     }
   }
+}
+
+// Direction update of the deflated PCG, one node per thread:
+//   x += alpha p_old  (skipped on the first call),
+//   p  = invd o r + beta p_old - W mu  (fixed DOFs 0),
+// and block 0 clears the coarse right-hand side for the next iteration.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dir_update(
+    T* __restrict__ x, T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ invd,
+    const double* __restrict__ mu, const int32_t* __restrict__ grp,
+    const double* __restrict__ centers, const uint8_t* __restrict__ mask, GridDims g,
+    const double* __restrict__ scal, bool first, const int* __restrict__ done,
+    double* __restrict__ zero_vec, int zero_n) {
+  if (done && *done) return;
+  if (zero_vec && blockIdx.x == 0)
+    for (int i = threadIdx.x; i < zero_n; i += blockDim.x) zero_vec[i] = 0.0;
+  const double beta = first ? 0.0 : scal[S_BETA];
+  const double alpha = first ? 0.0 : scal[S_ALPHA];
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < g.nnodes;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    double wm[3] = {0, 0, 0};
+    if (grp) {
+      int ix, iy, iz;
+      node_coords(n, g, ix, iy, iz);
+      const int gid = grp[n];
+      const double rx = ix * g.hx - centers[3 * gid];
+      const double ry = iy * g.hy - centers[3 * gid + 1];
+      const double rz = iz * g.hz - centers[3 * gid + 2];
+      const double* m6 = mu + 6 * gid;
+      wm[0] = m6[0] + rz * m6[4] - ry * m6[5];
+      wm[1] = m6[1] - rz * m6[3] + rx * m6[5];
+      wm[2] = m6[2] + ry * m6[3] - rx * m6[4];
+    }
+    const uint8_t m = mask[n];
+    T pv[3], rv[3], dv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pv[c] = first ? T(0) : p[3 * n + c];
+      rv[c] = r[3 * n + c];
+      dv[c] = invd[3 * n + c];
+    }
+    if (!first) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x[3 * n + c] = (T)((double)x[3 * n + c] + alpha * (double)pv[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double val = (double)dv[c] * (double)rv[c] + beta * (double)pv[c] - wm[c];
+      p[3 * n + c] = ((m >> c) & 1) ? T(0) : (T)val;
+    }
+  }
+}
+
+// x += alpha p (the update the last iteration leaves pending when it stops).
+template <typename T>
+__global__ void k_axpy_alpha(T* __restrict__ x, const T* __restrict__ p, int64_t n,
+                             const double* __restrict__ scal) {
+  const double alpha = scal[S_ALPHA];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = (T)((double)x[i] + alpha * (double)p[i]);
 }
 
 // E += W_b^T K W_a over the special elements; E is nc_pad x nc_pad row-major.
@@ -398,9 +482,11 @@ __global__ void k_set_identity_rows(double* __restrict__ R, int ld, int row0, in
   }
 }
 
-// mu = Einv t (one wave per row), optional: out2 = t - E mu residual form.
+// y = b + sign * A (x1 + beta x2) (one wave per row; x2/beta optional).
 __global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ A, int ld, int n,
                                               const double* __restrict__ x,
+                                              const double* __restrict__ x2,
+                                              const double* __restrict__ beta_ptr,
                                               const double* __restrict__ b, double sign,
                                               double* __restrict__ y,
                                               const int* __restrict__ done) {
@@ -408,9 +494,13 @@ __global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ A, int 
   const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
+  const double beta = (x2 && beta_ptr) ? *beta_ptr : 0.0;
   double s = 0.0;
   const double* a = A + (int64_t)row * ld;
-  for (int j = lane; j < n; j += 64) s += a[j] * x[j];
+  if (x2)
+    for (int j = lane; j < n; j += 64) s += a[j] * (x[j] + beta * x2[j]);
+  else
+    for (int j = lane; j < n; j += 64) s += a[j] * x[j];
   s = wave_reduce_sum(s);
   if (lane == 0) y[row] = (b ? b[row] : 0.0) + sign * s;
 }
@@ -424,19 +514,26 @@ __global__ void k_axpy_small(double* __restrict__ y, const double* __restrict__ 
 
 static int round_up64(int n) { return (n + 63) / 64 * 64; }
 
-// mu = E^{-1} t with one step of refinement on the coarse residual:
-// mu0 = Einv t; r = t - E mu0; mu = mu0 + Einv r.
-int coarse_apply(Handle* h, const double* t, double* mu, const int* done, hipStream_t s) {
+// mu = E^{-1} (t1 + beta t2) with the FP64 inverse (t2/beta optional).  With
+// `refine`, one step of refinement on the coarse residual follows:
+// r = t - E mu; mu += E^{-1} r (used by the public coarse solve; the PCG
+// re-projects every direction, so it does not need it).
+int coarse_apply(Handle* h, const double* t1, const double* t2, const double* beta,
+                 double* mu, const int* done, hipStream_t s, bool refine) {
   const int n = h->nc, ld = round_up64(h->nc);
   const int blocks = (n * 64 + 255) / 256;
-  hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Einv, ld, n, t,
+  hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Einv, ld, n, t1, t2, beta,
                      (const double*)nullptr, 1.0, mu, done);
-  hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Ec, ld, n, mu, t, -1.0, h->cres,
-                     done);
-  hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Einv, ld, n, h->cres,
-                     (const double*)nullptr, 1.0, h->cwork + (int64_t)ld * ld, done);
-  hipLaunchKernelGGL(k_axpy_small, dim3((n + 255) / 256), dim3(256), 0, s, mu,
-                     h->cwork + (int64_t)ld * ld, n, done);
+  if (refine) {
+    PYTO_REQUIRE(t2 == nullptr, PYTO_EINVAL, "refined coarse solve takes one right-hand side");
+    hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Ec, ld, n, mu,
+                       (const double*)nullptr, (const double*)nullptr, t1, -1.0, h->cres, done);
+    hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Einv, ld, n, h->cres,
+                       (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
+                       1.0, h->cwork + (int64_t)ld * ld, done);
+    hipLaunchKernelGGL(k_axpy_small, dim3((n + 255) / 256), dim3(256), 0, s, mu,
+                       h->cwork + (int64_t)ld * ld, n, done);
+  }
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -452,24 +549,44 @@ int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
 }
 
 template <typename T>
-int launch_restrict_special(Handle* h, const T* v, const T* Ee, const ModalConsts<T>& mc,
-                            double* t, const int* done, hipStream_t s) {
+int launch_restrict_special(Handle* h, const T* r, const T* invd, const T* p,
+                            const double* beta, double* t, const int* done, hipStream_t s) {
   if (h->nchunks == 0) return PYTO_OK;
   const int blocks = (h->nchunks * 64 + 255) / 256;
-  hipLaunchKernelGGL(k_restrict_special<T>, dim3(blocks), dim3(256), 0, s, v, h->special,
-                     h->chunks, h->nchunks, Ee, h->node_group, h->centers, h->fixmask, h->g,
-                     mc, t, done);
+  hipLaunchKernelGGL(k_restrict_special<T>, dim3(blocks), dim3(256), 0, s, r, invd, p, beta,
+                     h->special, h->chunks, h->nchunks, h->Ee64, h->node_group, h->centers,
+                     h->fixmask, h->g, h->mc64, t, done);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+template <typename T>
+int launch_dir_update(Handle* h, T* x, T* p, const T* r, const T* invd, const double* mu,
+                      bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n) {
+  const int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), 0, s, x, p, r, invd, mu,
+                     h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
+                     h->fixmask, h->g, (const double*)h->scal, first, done, zero_vec, zero_n);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+template <typename T>
+int launch_axpy_alpha(Handle* h, T* x, const T* p, hipStream_t s) {
+  const int64_t n = 3 * h->g.nnodes;
+  hipLaunchKernelGGL(k_axpy_alpha<T>, dim3(2048), dim3(256), 0, s, x, p, n,
+                     (const double*)h->scal);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
 
 template <typename T, bool UPDATE>
 int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
-                   bool first, const int* done, hipStream_t s) {
+                   bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n) {
   const int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 8192);
   hipLaunchKernelGGL((k_prolong<T, UPDATE>), dim3(blocks), dim3(256), 0, s, out, z, mu,
                      h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
-                     h->fixmask, h->g, beta, first, done);
+                     h->fixmask, h->g, beta, first, done, zero_vec, zero_n);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -477,19 +594,30 @@ int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double
 template int launch_restrict_full<double>(Handle*, const double*, double*, hipStream_t);
 template int launch_restrict_full<float>(Handle*, const float*, double*, hipStream_t);
 template int launch_restrict_special<double>(Handle*, const double*, const double*,
-                                             const ModalConsts<double>&, double*, const int*,
+                                             const double*, const double*, double*, const int*,
                                              hipStream_t);
-template int launch_restrict_special<float>(Handle*, const float*, const float*,
-                                            const ModalConsts<float>&, double*, const int*,
-                                            hipStream_t);
+template int launch_restrict_special<float>(Handle*, const float*, const float*, const float*,
+                                            const double*, double*, const int*, hipStream_t);
+template int launch_dir_update<double>(Handle*, double*, double*, const double*, const double*,
+                                       const double*, bool, const int*, hipStream_t, double*,
+                                       int);
+template int launch_dir_update<float>(Handle*, float*, float*, const float*, const float*,
+                                      const double*, bool, const int*, hipStream_t, double*,
+                                      int);
+template int launch_axpy_alpha<double>(Handle*, double*, const double*, hipStream_t);
+template int launch_axpy_alpha<float>(Handle*, float*, const float*, hipStream_t);
 template int launch_prolong<double, false>(Handle*, double*, const double*, const double*,
-                                           const double*, bool, const int*, hipStream_t);
+                                           const double*, bool, const int*, hipStream_t,
+                                           double*, int);
 template int launch_prolong<double, true>(Handle*, double*, const double*, const double*,
-                                          const double*, bool, const int*, hipStream_t);
+                                          const double*, bool, const int*, hipStream_t,
+                                          double*, int);
 template int launch_prolong<float, false>(Handle*, float*, const float*, const double*,
-                                          const double*, bool, const int*, hipStream_t);
+                                          const double*, bool, const int*, hipStream_t,
+                                          double*, int);
 template int launch_prolong<float, true>(Handle*, float*, const float*, const double*,
-                                         const double*, bool, const int*, hipStream_t);
+                                         const double*, bool, const int*, hipStream_t,
+                                         double*, int);
 
 // Cholesky factor of Ec into cwork (lower), then Einv = L^{-T} L^{-1}.
 static int factorize_coarse(Handle* h, hipStream_t s) {
@@ -644,7 +772,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_TRY(device_alloc(h, (void**)&h->Ec, (size_t)ld * ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->Einv, (size_t)ld * ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->cwork, ((size_t)2 * ld * ld + (size_t)ld * 64) * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->ct, (size_t)ld * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->ct, (size_t)2 * ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->cmu, (size_t)ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->cres, (size_t)ld * 8));
   PYTO_CHECK_HIP(hipMemcpyAsync(h->node_group, node_group, nn * 4, hipMemcpyDeviceToDevice, s));
@@ -655,7 +783,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     PYTO_CHECK_HIP(hipMemcpyAsync(h->chunks, chunks.data(), chunks.size() * sizeof(SpecialChunk),
                                   hipMemcpyHostToDevice, s));
   }
-  PYTO_CHECK_HIP(hipMemsetAsync(h->ct, 0, (size_t)ld * 8, s));
+  PYTO_CHECK_HIP(hipMemsetAsync(h->ct, 0, (size_t)2 * ld * 8, s));
   PYTO_CHECK_HIP(hipMemsetAsync(h->cmu, 0, (size_t)ld * 8, s));
   PYTO_CHECK_HIP(hipStreamSynchronize(s));
   if (h->density_set) PYTO_TRY(coarse_setup(h, s));
@@ -680,13 +808,13 @@ int pyto_prolong_f64(pyto_handle* h, const double* mu, double* v, void* stream) 
   PYTO_REQUIRE(h && mu && v, PYTO_EINVAL, "null argument");
   PYTO_REQUIRE(h->ngroups > 0, PYTO_ESTATE, "no deflation groups set");
   return launch_prolong<double, false>(h, v, nullptr, mu, nullptr, true, nullptr,
-                                       as_stream(stream));
+                                       as_stream(stream), nullptr, 0);
 }
 
 int pyto_coarse_solve_f64(pyto_handle* h, const double* t, double* mu, void* stream) {
   PYTO_REQUIRE(h && t && mu, PYTO_EINVAL, "null argument");
   PYTO_REQUIRE(h->coarse_ready, PYTO_ESTATE, "coarse space not assembled (set groups and density)");
-  return coarse_apply(h, t, mu, nullptr, as_stream(stream));
+  return coarse_apply(h, t, nullptr, nullptr, mu, nullptr, as_stream(stream), true);
 }
 
 int pyto_coarse_matrix_f64(pyto_handle* h, double* E, void* stream) {

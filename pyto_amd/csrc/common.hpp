@@ -101,7 +101,7 @@ struct Handle {
   double* Ec = nullptr;         // coarse matrix (nc x nc), then Cholesky factor
   double* Einv = nullptr;       // coarse inverse (nc x nc)
   double* cwork = nullptr;      // nc x nc workspace (inverse computation)
-  double* ct = nullptr;         // coarse rhs (nc)
+  double* ct = nullptr;         // coarse rhs: W^T K z (ld), then W^T K p (ld) at ct + ld
   double* cmu = nullptr;        // coarse solution (nc)
   double* cres = nullptr;       // coarse residual (nc)
   bool coarse_ready = false;
@@ -109,7 +109,7 @@ struct Handle {
   double shift_rel = 0.0;
 
   // solver workspace (allocated on first use)
-  double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;
+  double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;  // z64/z32 unused
   float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr,
         *b32 = nullptr;
   double* partials = nullptr;   // block partial sums (kMaxPartials)
@@ -187,6 +187,26 @@ __device__ __forceinline__ void grid_reduce_finish(double block_sum, double* par
   if (threadIdx.x == 0) {
     fin(tot);
     *counter = 0u;
+  }
+}
+
+// Wave-level segmented flush of per-lane 6-vectors keyed by group id (gid < 0:
+// nothing): lanes sharing a group are summed and one lane adds the sum to
+// t[6*gid .. 6*gid+5].  Must be called by all lanes of the wave.
+__device__ __forceinline__ void wave_flush6(const double acc[6], int gid, double* t) {
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(gid >= 0);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lg = __shfl(gid, leader, 64);
+    const bool mine = gid == lg;
+    const uint64_t members = __ballot(mine);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double s = wave_reduce_sum(mine ? acc[j] : 0.0);
+      if (lane == leader) atomicAdd(&t[6 * lg + j], s);
+    }
+    pending &= ~members;
   }
 }
 

@@ -225,9 +225,9 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
         ot[c] = f[4][c] + xc[ty][tx - 1][3][c] + xc[ty - 1][tx - 1][4][c] + xc[ty - 1][tx][5][c];
       }
       if (k >= kz0) {
-        const int64_t n = ex + (int64_t)g.sx * (ey + (int64_t)g.sy * k);
-        const uint8_t m = a.mask[n];
-        T* yp = a.y + 3 * n;
+        const int64_t nw = ex + (int64_t)g.sx * (ey + (int64_t)g.sy * k);
+        const uint8_t m = a.mask[nw];
+        T* yp = a.y + 3 * nw;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T v = ((m >> c) & 1) ? T(0) : carry[c] + ob[c];
@@ -301,11 +301,9 @@ int ensure_solver_workspace(Handle* h) {
   if (h->r64) return PYTO_OK;
   const size_t nd = (size_t)(3 * h->g.nnodes);
   PYTO_TRY(device_alloc(h, (void**)&h->r64, nd * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->z64, nd * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->p64, nd * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->q64, nd * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->r32, nd * 4));
-  PYTO_TRY(device_alloc(h, (void**)&h->z32, nd * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->p32, nd * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->q32, nd * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->x32, nd * 4));

@@ -27,20 +27,48 @@ int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalCon
 template <typename T>
 int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s);
 template <typename T>
-int launch_restrict_special(Handle* h, const T* v, const T* Ee, const ModalConsts<T>& mc,
-                            double* t, const int* done, hipStream_t s);
+int launch_restrict_special(Handle* h, const T* r, const T* invd, const T* p,
+                            const double* beta, double* t, const int* done, hipStream_t s);
 template <typename T, bool UPDATE>
 int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
-                   bool first, const int* done, hipStream_t s);
-int coarse_apply(Handle* h, const double* t, double* mu, const int* done, hipStream_t s);
+                   bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n);
+template <typename T>
+int launch_dir_update(Handle* h, T* x, T* p, const T* r, const T* invd, const double* mu,
+                      bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n);
+template <typename T>
+int launch_axpy_alpha(Handle* h, T* x, const T* p, hipStream_t s);
+int coarse_apply(Handle* h, const double* t1, const double* t2, const double* beta,
+                 double* mu, const int* done, hipStream_t s, bool refine);
 
 constexpr int kVecBlocks = 2048;
 
-// alpha = rz/pKp; x += alpha p; r -= alpha q; z = invd r; rz_new = r.z.
-template <typename T, typename TX>
-__global__ __launch_bounds__(256) void k_cg_update(TX* __restrict__ x, T* __restrict__ r,
-                                                   T* __restrict__ z, const T* __restrict__ p,
-                                                   const T* __restrict__ q,
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+  static constexpr int width = 4;
+};
+template <>
+struct Vec4<double> {
+  using type = double2;
+  static constexpr int width = 2;
+};
+
+template <typename V, typename F>
+__device__ __forceinline__ void for_lanes(V& a, const V& b, const V& c, F fn) {
+  if constexpr (sizeof(V) / sizeof(a.x) == 4) {
+    fn(a.x, b.x, c.x); fn(a.y, b.y, c.y); fn(a.z, b.z, c.z); fn(a.w, b.w, c.w);
+  } else {
+    fn(a.x, b.x, c.x); fn(a.y, b.y, c.y);
+  }
+}
+
+// alpha = rz/pKp (kept in S_ALPHA for the direction update); r -= alpha q;
+// rz_new = r^T (invd o r); stop test; beta = rz_new/rz.  Vector loads of
+// 16 bytes; the tail (n not a multiple of the width) is handled by block 0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ r, const T* __restrict__ q,
                                                    const T* __restrict__ invd, int64_t n,
                                                    double* __restrict__ scal,
                                                    int* __restrict__ flags,
@@ -48,23 +76,38 @@ __global__ __launch_bounds__(256) void k_cg_update(TX* __restrict__ x, T* __rest
                                                    unsigned* __restrict__ counter) {
   if (flags[F_DONE]) return;
   __shared__ double red[4];
+  using V = typename Vec4<T>::type;
+  constexpr int W = Vec4<T>::width;
   const double alpha = scal[S_RZ] / scal[S_PKP];
   const T a = (T)alpha;
   double acc = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+  const int64_t nv = n / W;
+  V* rv = reinterpret_cast<V*>(r);
+  const V* qv = reinterpret_cast<const V*>(q);
+  const V* dv = reinterpret_cast<const V*>(invd);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
        i += (int64_t)gridDim.x * blockDim.x) {
-    x[i] = (TX)((double)x[i] + alpha * (double)p[i]);
-    const T ri = r[i] - a * q[i];
-    const T zi = invd[i] * ri;
-    r[i] = ri;
-    z[i] = zi;
-    acc += (double)ri * (double)zi;
+    V rr = rv[i];
+    const V qq = qv[i];
+    const V dd = dv[i];
+    for_lanes(rr, qq, dd, [&](T& ri, const T& qi, const T& di) {
+      ri = ri - a * qi;
+      acc += (double)ri * (double)di * (double)ri;
+    });
+    rv[i] = rr;
   }
+  if (blockIdx.x == 0)
+    for (int64_t i = nv * W + threadIdx.x; i < n; i += blockDim.x) {
+      const T ri = r[i] - a * q[i];
+      r[i] = ri;
+      acc += (double)ri * (double)invd[i] * (double)ri;
+    }
   const double bs = block_reduce_sum256(acc, red);
   grid_reduce_finish(bs, partials, counter, gridDim.x, blockIdx.x, [=](double rz_new) {
     const double rz = scal[S_RZ];
     const double rel = sqrt(fabs(rz_new / scal[S_RZ0]));
     const double pkp = scal[S_PKP];
+    scal[S_ALPHA] = alpha;
     scal[S_RZNEW] = rz_new;
     scal[S_REL] = rel;
     flags[F_ITERS] += 1;
@@ -79,23 +122,21 @@ __global__ __launch_bounds__(256) void k_cg_update(TX* __restrict__ x, T* __rest
   });
 }
 
-// r = f - q (T from double f scaled by 1/scale), z = invd r, rz -> slots.
+// r = f - q (in T), rz = r^T (invd o r) -> scal[slot_a] (and slot_b).
 template <typename T, typename TF>
 __global__ __launch_bounds__(256) void k_residual(const TF* __restrict__ f, const T* __restrict__ q,
-                                                  T* __restrict__ r, T* __restrict__ z,
-                                                  const T* __restrict__ invd, int64_t n,
-                                                  double* __restrict__ scal, int slot_a,
-                                                  int slot_b, double* __restrict__ partials,
+                                                  T* __restrict__ r, const T* __restrict__ invd,
+                                                  int64_t n, double* __restrict__ scal,
+                                                  int slot_a, int slot_b,
+                                                  double* __restrict__ partials,
                                                   unsigned* __restrict__ counter) {
   __shared__ double red[4];
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const T ri = (T)((double)f[i] - (double)q[i]);
-    const T zi = invd[i] * ri;
     r[i] = ri;
-    z[i] = zi;
-    acc += (double)ri * (double)zi;
+    acc += (double)ri * (double)invd[i] * (double)ri;
   }
   const double bs = block_reduce_sum256(acc, red);
   grid_reduce_finish(bs, partials, counter, gridDim.x, blockIdx.x, [=](double tot) {
@@ -152,17 +193,17 @@ __global__ void k_set_scalar(double* scal, int slot, double v) { scal[slot] = v;
 
 template <typename T>
 struct Vecs {
-  T *r, *z, *p, *q;
+  T *r, *p, *q;
   const T* invd;
   const T* Ee;
   ModalConsts<T> mc;
 };
 
 static Vecs<double> vecs64(Handle* h) {
-  return {h->r64, h->z64, h->p64, h->q64, h->invd64, h->Ee64, h->mc64};
+  return {h->r64, h->p64, h->q64, h->invd64, h->Ee64, h->mc64};
 }
 static Vecs<float> vecs32(Handle* h) {
-  return {h->r32, h->z32, h->p32, h->q32, h->invd32, h->Ee32, h->mc32};
+  return {h->r32, h->p32, h->q32, h->invd32, h->Ee32, h->mc32};
 }
 
 template <typename T>
@@ -192,30 +233,45 @@ static int read_back(Handle* h, hipStream_t s) {
 // Deflated PCG on K x = b (b and x full-length in precision T; fixed DOFs of
 // b ignored, of x zero).  The deflated start x0 = W E^-1 W^T b and the first
 // direction p0 = z0 - W E^-1 W^T K z0 follow src/deflation.py:532-551.
+//
+// Direction update.  The reference forms p = z + beta p - W E^-1 W^T K z,
+// which relies on W^T K p_old = 0 holding exactly from the previous step;
+// rounding (an FP32 operator, the coarse solve) then accumulates and the
+// iteration stalls or diverges.  Here the whole new direction is projected,
+// mu = E^-1 W^T K (z + beta p_old), so W^T K p_new = 0 is re-established
+// every step.  In exact arithmetic the two forms coincide.  z = invd o r is
+// never stored: the kernels that need it form it from r.
+//
+// Per iteration (5 launches): q = K p with p^T q; r -= alpha q with
+// r^T M r, the stop test and beta; t = W^T K (z + beta p) over the
+// non-rigid elements; mu = E^-1 t; x += alpha p, p = z + beta p - W mu.
 template <typename T>
 static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgResult& res,
                 hipStream_t s) {
   Vecs<T> v = get_vecs<T>(h);
   const int64_t n = 3 * h->g.nnodes;
   const bool defl = h->ngroups > 0;
+  const int ld = (h->nc + 63) / 64 * 64;
+  double* ct = h->ct;
   if (defl) {
-    PYTO_TRY(launch_restrict_full<T>(h, b, h->ct, s));
-    PYTO_TRY(coarse_apply(h, h->ct, h->cmu, nullptr, s));
-    PYTO_TRY(launch_prolong<T, false>(h, x, nullptr, h->cmu, nullptr, true, nullptr, s));
+    PYTO_TRY(launch_restrict_full<T>(h, b, ct, s));
+    PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
+    PYTO_TRY(launch_prolong<T, false>(h, x, nullptr, h->cmu, nullptr, true, nullptr, s, nullptr, 0));
   } else {
     PYTO_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(T), s));
   }
   PYTO_TRY(launch_matvec<T>(h, x, v.q, v.Ee, v.mc, nullptr, nullptr, nullptr, 0, s));
   res.matvecs += 1;
-  hipLaunchKernelGGL((k_residual<T, T>), dim3(kVecBlocks), dim3(256), 0, s, b, v.q, v.r, v.z,
-                     v.invd, n, h->scal, (int)S_RZ, (int)S_RZ0, h->partials,
-                     h->counters + C_DOT);
+  hipLaunchKernelGGL((k_residual<T, T>), dim3(kVecBlocks), dim3(256), 0, s, b, v.q, v.r, v.invd,
+                     n, h->scal, (int)S_RZ, (int)S_RZ0, h->partials, h->counters + C_DOT);
   if (defl) {
-    PYTO_CHECK_HIP(hipMemsetAsync(h->ct, 0, sizeof(double) * h->nc, s));
-    PYTO_TRY(launch_restrict_special<T>(h, v.z, v.Ee, v.mc, h->ct, nullptr, s));
-    PYTO_TRY(coarse_apply(h, h->ct, h->cmu, nullptr, s));
+    PYTO_CHECK_HIP(hipMemsetAsync(ct, 0, sizeof(double) * ld, s));
+    PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, (const T*)nullptr, nullptr, ct, nullptr, s));
+    PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
   }
-  PYTO_TRY(launch_prolong<T, true>(h, v.p, v.z, h->cmu, nullptr, true, nullptr, s));
+  // p0 = z0 - W mu (x untouched); clears ct for iteration 1
+  PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, true, nullptr, s,
+                                defl ? ct : nullptr, defl ? ld : 0));
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RTOL, rtol);
   hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(1), 0, s, h->flags, 0, 0);
   PYTO_CHECK_HIP(hipGetLastError());
@@ -227,20 +283,19 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   }
   int launched = 0;
   int batch = 8;
+  const int* done = h->flags + F_DONE;
   while (launched < max_iters) {
     const int nb = std::min(batch, max_iters - launched);
     for (int k = 0; k < nb; ++k) {
-      PYTO_TRY(launch_matvec<T>(h, v.p, v.q, v.Ee, v.mc, h->scal + S_PKP, h->flags + F_DONE,
-                                defl ? h->ct : nullptr, h->nc, s));
-      hipLaunchKernelGGL((k_cg_update<T, T>), dim3(kVecBlocks), dim3(256), 0, s, x, v.r, v.z,
-                         v.p, v.q, v.invd, n, h->scal, h->flags, h->partials,
-                         h->counters + C_UPDATE);
+      PYTO_TRY(launch_matvec<T>(h, v.p, v.q, v.Ee, v.mc, h->scal + S_PKP, done, nullptr, 0, s));
+      hipLaunchKernelGGL(k_cg_update<T>, dim3(kVecBlocks), dim3(256), 0, s, v.r, v.q, v.invd, n,
+                         h->scal, h->flags, h->partials, h->counters + C_UPDATE);
       if (defl) {
-        PYTO_TRY(launch_restrict_special<T>(h, v.z, v.Ee, v.mc, h->ct, h->flags + F_DONE, s));
-        PYTO_TRY(coarse_apply(h, h->ct, h->cmu, h->flags + F_DONE, s));
+        PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, v.p, h->scal + S_BETA, ct, done, s));
+        PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, done, s, false));
       }
-      PYTO_TRY(launch_prolong<T, true>(h, v.p, v.z, h->cmu, h->scal + S_BETA, false,
-                                       h->flags + F_DONE, s));
+      PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, false, done, s,
+                                    defl ? ct : nullptr, defl ? ld : 0));
     }
     PYTO_CHECK_HIP(hipGetLastError());
     launched += nb;
@@ -253,6 +308,8 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   res.breakdown = h->host_flags[F_BREAKDOWN] != 0;
   res.converged = h->host_flags[F_DONE] != 0 && !res.breakdown;
   res.rel = h->host_scal[S_REL];
+  // the stopping iteration skipped its direction update: apply its x += alpha p
+  if (res.converged) PYTO_TRY(launch_axpy_alpha<T>(h, x, v.p, s));
   return PYTO_OK;
 }
 
@@ -330,8 +387,9 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
   // FP64 deflated start and outer residual
   if (defl) {
     PYTO_TRY(launch_restrict_full<double>(h, f, h->ct, s));
-    PYTO_TRY(coarse_apply(h, h->ct, h->cmu, nullptr, s));
-    PYTO_TRY(launch_prolong<double, false>(h, u, nullptr, h->cmu, nullptr, true, nullptr, s));
+    PYTO_TRY(coarse_apply(h, h->ct, nullptr, nullptr, h->cmu, nullptr, s, false));
+    PYTO_TRY(launch_prolong<double, false>(h, u, nullptr, h->cmu, nullptr, true, nullptr, s,
+                                           nullptr, 0));
   } else {
     PYTO_CHECK_HIP(hipMemsetAsync(u, 0, n * sizeof(double), s));
   }
@@ -339,8 +397,8 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
   PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0, s));
   mv64++;
   hipLaunchKernelGGL((k_residual<double, double>), dim3(kVecBlocks), dim3(256), 0, s, f, h->q64,
-                     h->r64, h->z64, h->invd64, n, h->scal, (int)S_TMP1, (int)S_TMP0,
-                     h->partials, h->counters + C_DOT);
+                     h->r64, h->invd64, n, h->scal, (int)S_TMP1, (int)S_TMP0, h->partials,
+                     h->counters + C_DOT);
   PYTO_CHECK_HIP(hipGetLastError());
   PYTO_TRY(read_back(h, s));
   const double rz0 = h->host_scal[S_TMP0];
@@ -350,18 +408,23 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
   while (!converged && outer < max_outer && inner_total < max_iters) {
     hipLaunchKernelGGL(k_scale_convert<float>, dim3(kVecBlocks), dim3(256), 0, s, h->r64, h->b32,
                        n, h->scal, (int)S_TMP1);
+    // inner tolerance: what is left to reach rtol (with a margin), but no
+    // tighter than inner_rtol, the FP32 floor, and no looser than 0.1
+    const double want = 0.3 * rtol / rel;
+    const double tol_k = std::min(0.1, std::max(inner_rtol, want));
     DpcgResult ir;
-    PYTO_TRY(dpcg<float>(h, h->b32, h->x32, inner_rtol, max_iters - inner_total, ir, s));
+    PYTO_TRY(dpcg<float>(h, h->b32, h->x32, tol_k, max_iters - inner_total, ir, s));
     inner_total += ir.iterations;
     mv32 += ir.matvecs;
     if (ir.breakdown) inner_fail = true;
     hipLaunchKernelGGL(k_axpy_scaled, dim3(kVecBlocks), dim3(256), 0, s, u, h->x32, n, h->scal,
                        (int)S_TMP1);
-    PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0, s));
+    PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0,
+                                   s));
     mv64++;
     hipLaunchKernelGGL((k_residual<double, double>), dim3(kVecBlocks), dim3(256), 0, s, f,
-                       h->q64, h->r64, h->z64, h->invd64, n, h->scal, (int)S_TMP1, -1,
-                       h->partials, h->counters + C_DOT);
+                       h->q64, h->r64, h->invd64, n, h->scal, (int)S_TMP1, -1, h->partials,
+                       h->counters + C_DOT);
     PYTO_CHECK_HIP(hipGetLastError());
     PYTO_TRY(read_back(h, s));
     outer++;

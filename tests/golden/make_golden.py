@@ -1,0 +1,79 @@
+"""Write the golden fixtures of tests/golden/ from the oracle (CPU, FP64).
+
+c1.npz  : the 60x20x10 cantilever of config 1 (x = 0.5, unit -z point load
+          at node (60,10,5), x=0 clamped, E0 = 1, nu = 0.3, SIMP p = 3,
+          eps = 1e-8): displacement from a sparse direct solve, the
+          reference-algorithm DPCG iteration count at rtol 1e-8, compliance,
+          its gradient, p-norm (p = 6, q = 0.5), von Mises, the cone-filter
+          pattern (r = 1.5) and Hs; plus the reference-algorithm DPCG
+          iterate at rtol 1e-8 (u_dpcg, C_dpcg): the reference stops there,
+          1.5e-6 short of the direct-solve compliance.
+tiny.npz: 4x3x2 grid, densities U[0.1,1] seed 3: K (dense), KE, f, u.
+
+Provenance: the reference cannot be imported here (numba, pyvista, pyamg,
+numpy-stl absent), so these vectors come from the restatement in oracle/,
+which tests/test_oracle_*.py pin against closed forms, brute force and FD.
+Run: python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+import scipy.sparse.linalg as spla
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import deflation as ODF  # noqa: E402
+from oracle import fea as OF  # noqa: E402
+from oracle import filters as OFl  # noqa: E402
+from oracle import ke as OK  # noqa: E402
+from oracle import mesh as OM  # noqa: E402
+from oracle import sensitivities as OS  # noqa: E402
+
+
+def c1():
+    g = OM.GridMesh((60, 20, 10), (1.0, 1.0, 1.0))
+    g.createEdofMatStructural()
+    KE = OK.hex8_stiffness(1.0, 0.3, (1, 1, 1))
+    x = np.full(g.num_elems, 0.5)
+    K = OF.assemble_K(g, KE, OF.simp_scaling(x))
+    fixed, gv = OF.clamp_x0(g)
+    f = OF.point_load(g, 60, 10, 5)
+    A, b, free = OF.impose_dirichlet(K, f, fixed, gv)
+    u = OF.scatter_solution(len(f), fixed, gv, free, spla.spsolve(A.tocsc(), b))
+    ngn, cnt, cen = ODF.create_groups(g.node_xyz, ODF.recommended_groups_ocm(g.num_nodes))
+    W = ODF.deflation_matrix(g.node_xyz, ngn, cen)[free, :]
+    xd, info = ODF.deflated_pcg(A, b, W, OF.jacobi_inverse_diag(A), rtol=1e-8)
+    u_dpcg = OF.scatter_solution(len(f), fixed, gv, free, xd)
+    C_dpcg = f @ u_dpcg
+    C, grad = OS.compliance_and_gradient(u, x, g.edofMatStructural, KE)
+    post = OF.postprocess(g, u, x, 1.0, 0.3, q=0.5, pexp=6.0)
+    H, Hs = OFl.cone_filter(g, 1.5)
+    H = H.tocsr()
+    H.sort_indices()
+    np.savez_compressed(os.path.join(HERE, "c1.npz"), u=u, C=C, grad=grad,
+                        pnorm=post["pNormStress"], vm=post["vonMisesStress"],
+                        dpcg_iterations=info["iterations"], u_dpcg=u_dpcg, C_dpcg=C_dpcg,
+                        node_group=ngn, group_center=cen,
+                        H_indptr=H.indptr, H_indices=H.indices, H_data=H.data, Hs=Hs)
+
+
+def tiny():
+    g = OM.GridMesh((4, 3, 2), (1.0, 1.0, 1.0))
+    g.createEdofMatStructural()
+    KE = OK.hex8_stiffness(1.0, 0.3, (1, 1, 1))
+    x = np.random.default_rng(3).uniform(0.1, 1.0, g.num_elems)
+    K = OF.assemble_K(g, KE, OF.simp_scaling(x))
+    fixed, gv = OF.clamp_x0(g)
+    f = OF.face_traction_load(g)
+    A, b, free = OF.impose_dirichlet(K, f, fixed, gv)
+    u = OF.scatter_solution(len(f), fixed, gv, free, spla.spsolve(A.tocsc(), b))
+    np.savez_compressed(os.path.join(HERE, "tiny.npz"), K=K.toarray(), KE=KE, x=x, f=f, u=u,
+                        fixed=fixed)
+
+
+if __name__ == "__main__":
+    c1()
+    tiny()
+    print("wrote", os.listdir(HERE))

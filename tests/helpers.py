@@ -29,11 +29,10 @@ class Case:
         self.mask = np.zeros(3 * g.num_nodes)
         self.mask[self.free] = 1.0
         self.E0, self.nu, self.penal, self.eps, self.h = E0, nu, penal, eps, h
-        if groups == "reference":
+        self.ngn = None
+        if groups == "reference" and g.num_nodes >= 2 * 16:
             ng = ODF.recommended_groups_ocm(g.num_nodes)
             self.ngn, self.cnt, self.cen = ODF.create_groups(g.node_xyz, ng)
-        else:
-            self.ngn = None
 
     def Kff_full(self, u):
         """K_ff u_f embedded in full length (fixed entries 0)."""

@@ -1,0 +1,73 @@
+"""The reference call surface end to end on the 60x20x10 cantilever
+(config 1 mesh): HexStructuralFEA + DeflationSolver + linear_solvers.solve +
+postprocess + sensitivities, against the oracle; and a full-size property
+check on the 192x64x64 grid."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import deflation as ODF
+from oracle import fea as OF
+from oracle import sensitivities as OS
+from tests.helpers import Case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def test_drop_in_flow_config1(gpu):
+    from pyto_amd import deflation, hex_structural_fea, linear_solvers, problems
+    from pyto_amd import topopt_obj_cons_sensitivities as sens
+    from pyto_amd.topopt_common import FEA_MODE
+    from pyto_amd.topopt_material_model import MaterialModel
+    print("drop-in c1", flush=True)
+    mesh, mat, bc = problems.cantilever((60, 20, 10), load="point", point=(60, 10, 5))
+    x = np.full(mesh.num_elems, 0.5)
+    ds = deflation.DeflationSolver()
+    ds.create_deflation_groups(mesh, min(2000, max(10, round(3 * mesh.num_nodes / 500))))
+    ds.create_deflation_matrix(mesh)
+    fe = hex_structural_fea.HexStructuralFEA(mesh, mat, bc, linear_solvers.Solvers.DPCG, dsolver=ds,
+                                             rtol=1e-8)
+    sol = fe.solve(x)
+    assert isinstance(sol, np.ndarray) and sol.shape == (3 * mesh.num_nodes,)
+    fe.postprocess()
+    C, grad = sens.compute_compliance_and_gradient(FEA_MODE.STRUCTURAL, sol, x, fe, fe.elem_stiff[0],
+                                                   MaterialModel.SIMP)
+    c = Case((60, 20, 10), x=x, load=(60, 10, 5))
+    A, b, free = OF.impose_dirichlet(c.K, c.f, c.fixed, c.gv)
+    W = ODF.deflation_matrix(c.mesh.node_xyz, c.ngn, c.cen)[free, :]
+    xd, info = ODF.deflated_pcg(A, b, W, OF.jacobi_inverse_diag(A), rtol=1e-8)
+    u_ref = OF.scatter_solution(len(c.f), c.fixed, c.gv, free, xd)
+    C_ref, g_ref = OS.compliance_and_gradient(u_ref, x, c.mesh.edofMatStructural, c.KE)
+    assert abs(C - C_ref) <= 1e-6 * abs(C_ref)
+    assert rel_err(grad, g_ref) <= 1e-5
+    post = OF.postprocess(c.mesh, u_ref, x, 1.0, 0.3)
+    assert abs(fe.pNormStress - post["pNormStress"]) <= 1e-5 * post["pNormStress"]
+    assert rel_err(fe.vonMisesStress, post["vonMisesStress"]) <= 1e-4
+    assert abs(fe.max_deformation - np.sqrt((u_ref.reshape(-1, 3) ** 2).sum(1)).max()) <= 1e-6 * fe.max_deformation
+    assert abs(fe.last_stats["iterations"] - info["iterations"]) <= 2
+
+
+def test_full_size_solve_properties(gpu):
+    """192x64x64 (2.4M DOF), densities U[0.01,1] seed 0: FP64 and mixed
+    solves agree; C = f^T u = u^T K u; true residual small."""
+    from pyto_amd import deflation, problems
+    from pyto_amd.operator import GridOperator
+    print("full size 192x64x64", flush=True)
+    mesh, mat, bc = problems.cantilever((192, 64, 64))
+    op = GridOperator(mesh.grid, (1, 1, 1), 1.0, 0.3, bc.fixed_dofs, eps_void=1e-9)
+    op.set_coarse_shift(0.0, 0.0)
+    ds = deflation.DeflationSolver()
+    ds.create_box_groups(mesh, (4, 2, 2))
+    ds.attach(op)
+    op.set_density(problems.density_uniform(mesh.num_elems, 0.01, 1.0, 0))
+    f = torch.from_numpy(bc.force).to(gpu)
+    u64, s64 = op.solve(f, rtol=1e-8, precision="fp64", max_iters=5000)
+    um, sm = op.solve(f, rtol=1e-8, precision="mixed", max_iters=5000)
+    assert s64["converged"] and sm["converged"]
+    C64 = float(torch.dot(f, u64))
+    Cm = float(torch.dot(f, um))
+    assert abs(C64 - Cm) <= 1e-6 * abs(C64)
+    assert abs(float(torch.dot(u64, op.matvec(u64))) - C64) <= 1e-9 * abs(C64)
+    assert sm["rel_true"] < 1e-6 and s64["rel_true"] < 1e-6
+    C, g, _ = op.compliance(u64)
+    assert abs(float(C) - C64) <= 1e-9 * abs(C64)
