@@ -32,23 +32,31 @@ __device__ __forceinline__ void w_row(int comp, double rx, double ry, double rz,
   else { w[2] = 1.0; w[3] = ry; w[4] = -rx; }
 }
 
+__global__ void k_coarse_gather(const double* __restrict__ cpart, const int32_t* __restrict__ gptr,
+                                const int32_t* __restrict__ gidx, int ngroups,
+                                double* __restrict__ t, const int* __restrict__ done);
+
 // ---------------------------------------------------------------------------
-// t = W^T v over all nodes (fixed DOFs excluded).  One thread per node, wave
-// segmented reduction by group, one atomic per (wave, group, column).
+// t = W^T v over all nodes (fixed DOFs excluded): one block per run of <= 256
+// nodes of one group (nodes sorted by group on the host), a block reduction
+// into per-run partials, then the per-group gather.  Deterministic.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict_full(const T* __restrict__ v,
-                                                       const int32_t* __restrict__ grp,
+                                                       const int32_t* __restrict__ gnodes,
+                                                       const int2* __restrict__ nchunk,
+                                                       const int32_t* __restrict__ nchunk_grp,
                                                        const double* __restrict__ centers,
                                                        const uint8_t* __restrict__ mask,
-                                                       GridDims g, double* __restrict__ t) {
-  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+                                                       GridDims g, double* __restrict__ npart) {
+  __shared__ double red[4][6];
+  const int2 ch = nchunk[blockIdx.x];
+  const int gid = nchunk_grp[blockIdx.x];
   double c[6] = {0, 0, 0, 0, 0, 0};
-  int gid = -1;
-  if (n < g.nnodes) {
+  if ((int)threadIdx.x < ch.y) {
+    const int64_t n = gnodes[ch.x + threadIdx.x];
     int ix, iy, iz;
     node_coords(n, g, ix, iy, iz);
-    gid = grp[n];
     const double rx = ix * g.hx - centers[3 * gid];
     const double ry = iy * g.hy - centers[3 * gid + 1];
     const double rz = iz * g.hz - centers[3 * gid + 2];
@@ -61,19 +69,16 @@ __global__ __launch_bounds__(256) void k_restrict_full(const T* __restrict__ v,
     c[4] = rz * vx - rx * vz;
     c[5] = -ry * vx + rx * vy;
   }
-  const int lane = threadIdx.x & 63;
-  uint64_t pending = __ballot(gid >= 0);
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int lg = __shfl(gid, leader, 64);
-    const uint64_t members = __ballot(gid == lg);
-    const bool mine = gid == lg;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const double s = wave_reduce_sum(mine ? c[j] : 0.0);
-      if (lane == leader) atomicAdd(&t[6 * lg + j], s);
-    }
-    pending &= ~members;
+  for (int j = 0; j < 6; ++j) {
+    const double sj = wave_reduce_sum(c[j]);
+    if (lane == 0) red[w][j] = sj;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int j = threadIdx.x;
+    npart[(int64_t)blockIdx.x * 48 + j] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
   }
 }
 
@@ -150,7 +155,8 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
   }
 }
 
-// t += W^T K v over the special elements (one wave per chunk), where
+// Per-chunk partials of W^T K v over the special elements (one wave per
+// chunk; k_coarse_gather sums them into t), where
 // v = invd o r + beta p (the new search direction before its coarse
 // correction) or v = r when invd is null.  The element arithmetic is FP64
 // whatever the vector precision T.  Group centres are wave-uniform per slot.
@@ -161,7 +167,7 @@ __global__ __launch_bounds__(256) void k_restrict_special(
     const SpecialChunk* __restrict__ chunks, int nchunks, const double* __restrict__ Ee,
     const int32_t* __restrict__ grp, const double* __restrict__ centers,
     const uint8_t* __restrict__ mask, GridDims g, ModalConsts<double> mc,
-    double* __restrict__ t, const int* __restrict__ done) {
+    double* __restrict__ cpart, const int* __restrict__ done) {
   if (done && *done) return;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -226,8 +232,34 @@ __global__ __launch_bounds__(256) void k_restrict_special(
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
       const double tot = wave_reduce_sum(acc[j]);
-      if (lane == 0) atomicAdd(&t[6 * gs + j], tot);
+      if (lane == 0) cpart[(int64_t)wave * 48 + 6 * sl + j] = tot;
     }
+  }
+}
+
+// t[6g..6g+5] = sum of the chunk partials of group g: one wave per group,
+// lanes stride over the group's partials, then a wave reduction (fixed
+// order).  Replaces atomics: tens of thousands of FP64 atomics on a few
+// hundred addresses serialise.
+__global__ __launch_bounds__(256) void k_coarse_gather(const double* __restrict__ cpart,
+                                                       const int32_t* __restrict__ gptr,
+                                                       const int32_t* __restrict__ gidx, int ngroups,
+                                                       double* __restrict__ t,
+                                                       const int* __restrict__ done) {
+  if (done && *done) return;
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (g >= ngroups) return;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = gptr[g] + lane; k < gptr[g + 1]; k += 64) {
+    const double* c = cpart + (int64_t)gidx[k] * 6;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[j] += c[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const double s = wave_reduce_sum(acc[j]);
+    if (lane == 0) t[6 * g + j] = s;
   }
 }
 
@@ -540,22 +572,26 @@ int coarse_apply(Handle* h, const double* t1, const double* t2, const double* be
 
 template <typename T>
 int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
-  PYTO_CHECK_HIP(hipMemsetAsync(t, 0, sizeof(double) * h->nc, s));
-  const int blocks = (int)((h->g.nnodes + 255) / 256);
-  hipLaunchKernelGGL(k_restrict_full<T>, dim3(blocks), dim3(256), 0, s, v, h->node_group,
-                     h->centers, h->fixmask, h->g, t);
+  hipLaunchKernelGGL(k_restrict_full<T>, dim3(h->nnchunks), dim3(256), 0, s, v, h->gnodes,
+                     h->nchunk, h->nchunk_grp, h->centers, h->fixmask, h->g, h->npart);
+  hipLaunchKernelGGL(k_coarse_gather, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
+                     h->npart, h->nptr, h->nidx, h->ngroups, t, (const int*)nullptr);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
 
+// t = W^T K v over the special elements (overwrites t; no zeroing needed).
 template <typename T>
 int launch_restrict_special(Handle* h, const T* r, const T* invd, const T* p,
                             const double* beta, double* t, const int* done, hipStream_t s) {
-  if (h->nchunks == 0) return PYTO_OK;
-  const int blocks = (h->nchunks * 64 + 255) / 256;
-  hipLaunchKernelGGL(k_restrict_special<T>, dim3(blocks), dim3(256), 0, s, r, invd, p, beta,
-                     h->special, h->chunks, h->nchunks, h->Ee64, h->node_group, h->centers,
-                     h->fixmask, h->g, h->mc64, t, done);
+  if (h->nchunks > 0) {
+    const int blocks = (h->nchunks * 64 + 255) / 256;
+    hipLaunchKernelGGL(k_restrict_special<T>, dim3(blocks), dim3(256), 0, s, r, invd, p, beta,
+                       h->special, h->chunks, h->nchunks, h->Ee64, h->node_group, h->centers,
+                       h->fixmask, h->g, h->mc64, h->cpart, done);
+  }
+  hipLaunchKernelGGL(k_coarse_gather, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
+                     h->cpart, h->gptr, h->gidx, h->ngroups, t, done);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -701,12 +737,22 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
                "coarse space of %d groups (%d unknowns) exceeds the dense GPU limit of 2048 "
                "groups; use fewer, larger deflation groups", ngroups, 6 * ngroups);
   hipStream_t s = as_stream(stream);
+  // captured iteration graphs hold the old coarse-space pointers
+  for (auto& gc : h->graphs)
+    if (gc.exec) {
+      hipGraphExecDestroy(gc.exec);
+      gc.exec = nullptr;
+    }
   // release a previous coarse space
-  void* olds[] = {h->node_group, h->centers, h->special, h->chunks, h->Ec, h->Einv, h->cwork,
-                  h->ct, h->cmu, h->cres};
+  void* olds[] = {h->node_group, h->centers, h->special, h->chunks, h->cpart, h->gptr, h->gidx,
+                  h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
+                  h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres};
   for (void* p : olds)
     if (p) hipFree(p);
   h->node_group = nullptr; h->centers = nullptr; h->special = nullptr; h->chunks = nullptr;
+  h->cpart = nullptr; h->gptr = nullptr; h->gidx = nullptr;
+  h->gnodes = nullptr; h->nchunk = nullptr; h->nchunk_grp = nullptr; h->npart = nullptr;
+  h->nptr = nullptr; h->nidx = nullptr; h->nnchunks = 0;
   h->Ec = h->Einv = h->cwork = h->ct = h->cmu = h->cres = nullptr;
   h->ngroups = 0; h->nc = 0; h->nspecial = 0; h->nchunks = 0; h->coarse_ready = false;
   if (ngroups == 0) return PYTO_OK;
@@ -760,8 +806,49 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     i = j;
   }
   for (size_t i = 0; i < list.size(); ++i) special[i] = list[i].second;
+  // per group: the (chunk, slot) partials that belong to it
+  std::vector<int32_t> gptr(ngroups + 1, 0), gidx;
+  {
+    std::vector<std::vector<int32_t>> per(ngroups);
+    for (size_t c = 0; c < chunks.size(); ++c)
+      for (int q = 0; q < chunks[c].ng; ++q) per[chunks[c].groups[q]].push_back((int32_t)(c * 8 + q));
+    for (int gi = 0; gi < ngroups; ++gi) {
+      gptr[gi + 1] = gptr[gi] + (int32_t)per[gi].size();
+      gidx.insert(gidx.end(), per[gi].begin(), per[gi].end());
+    }
+  }
+  // nodes sorted by group, in runs of <= 256 of one group
+  std::vector<int32_t> gnodes(nn);
+  {
+    std::vector<int64_t> cnt(ngroups + 1, 0);
+    for (int64_t n = 0; n < nn; ++n) cnt[grp[n] + 1]++;
+    for (int gi = 0; gi < ngroups; ++gi) cnt[gi + 1] += cnt[gi];
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t n = 0; n < nn; ++n) gnodes[pos[grp[n]]++] = (int32_t)n;
+  }
+  std::vector<int2> nchunk;
+  std::vector<int32_t> nchunk_grp;
+  std::vector<int32_t> nptr(ngroups + 1, 0), nidx;
+  for (int64_t i = 0; i < nn;) {
+    const int gi = grp[gnodes[i]];
+    int64_t j = i;
+    while (j < nn && grp[gnodes[j]] == gi) ++j;
+    for (int64_t c = i; c < j; c += 256) {
+      nchunk.push_back(make_int2((int)c, (int)std::min<int64_t>(256, j - c)));
+      nchunk_grp.push_back(gi);
+    }
+    i = j;
+  }
+  for (size_t c = 0; c < nchunk.size(); ++c) nptr[nchunk_grp[c] + 1]++;
+  for (int gi = 0; gi < ngroups; ++gi) nptr[gi + 1] += nptr[gi];
+  nidx.resize(nchunk.size());
+  {
+    std::vector<int32_t> pos(nptr.begin(), nptr.end() - 1);
+    for (size_t c = 0; c < nchunk.size(); ++c) nidx[pos[nchunk_grp[c]]++] = (int32_t)(c * 8);
+  }
   h->ngroups = ngroups;
   h->nc = 6 * ngroups;
+  h->nnchunks = (int)nchunk.size();
   h->nspecial = (int)special.size();
   h->nchunks = (int)chunks.size();
   const int ld = round_up64(h->nc);
@@ -769,6 +856,15 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_TRY(device_alloc(h, (void**)&h->centers, (size_t)ngroups * 3 * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->special, special.size() * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->chunks, chunks.size() * sizeof(SpecialChunk)));
+  PYTO_TRY(device_alloc(h, (void**)&h->cpart, chunks.size() * 48 * sizeof(double)));
+  PYTO_TRY(device_alloc(h, (void**)&h->gnodes, nn * sizeof(int32_t)));
+  PYTO_TRY(device_alloc(h, (void**)&h->nchunk, nchunk.size() * sizeof(int2)));
+  PYTO_TRY(device_alloc(h, (void**)&h->nchunk_grp, nchunk.size() * sizeof(int32_t)));
+  PYTO_TRY(device_alloc(h, (void**)&h->npart, nchunk.size() * 48 * sizeof(double)));
+  PYTO_TRY(device_alloc(h, (void**)&h->nptr, nptr.size() * sizeof(int32_t)));
+  PYTO_TRY(device_alloc(h, (void**)&h->nidx, nidx.size() * sizeof(int32_t)));
+  PYTO_TRY(device_alloc(h, (void**)&h->gptr, gptr.size() * sizeof(int32_t)));
+  PYTO_TRY(device_alloc(h, (void**)&h->gidx, std::max<size_t>(gidx.size(), 1) * sizeof(int32_t)));
   PYTO_TRY(device_alloc(h, (void**)&h->Ec, (size_t)ld * ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->Einv, (size_t)ld * ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->cwork, ((size_t)2 * ld * ld + (size_t)ld * 64) * 8));
@@ -783,6 +879,21 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     PYTO_CHECK_HIP(hipMemcpyAsync(h->chunks, chunks.data(), chunks.size() * sizeof(SpecialChunk),
                                   hipMemcpyHostToDevice, s));
   }
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->gnodes, gnodes.data(), nn * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->nchunk, nchunk.data(), nchunk.size() * sizeof(int2),
+                                hipMemcpyHostToDevice, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->nchunk_grp, nchunk_grp.data(),
+                                nchunk_grp.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->nptr, nptr.data(), nptr.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->nidx, nidx.data(), nidx.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->gptr, gptr.data(), gptr.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
+  if (!gidx.empty())
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->gidx, gidx.data(), gidx.size() * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, s));
   PYTO_CHECK_HIP(hipMemsetAsync(h->ct, 0, (size_t)2 * ld * 8, s));
   PYTO_CHECK_HIP(hipMemsetAsync(h->cmu, 0, (size_t)ld * 8, s));
   PYTO_CHECK_HIP(hipStreamSynchronize(s));

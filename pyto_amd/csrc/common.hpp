@@ -74,6 +74,12 @@ struct SpecialChunk {
   int groups[8];  // the groups (sorted), -1 padded
 };
 
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  const void* x = nullptr;
+  int ngroups = -1, nchunks = -1;
+};
+
 struct Handle {
   int device = 0;
   GridDims g{};
@@ -98,6 +104,17 @@ struct Handle {
   int nspecial = 0, nchunks = 0;
   int32_t* special = nullptr;   // element ids sorted by group set
   SpecialChunk* chunks = nullptr;
+  double* cpart = nullptr;      // per-chunk partial W^T K v (nchunks x 8 x 6)
+  int32_t* gptr = nullptr;      // per group: range of (chunk*8+slot) in gidx
+  int32_t* gidx = nullptr;
+  // nodes sorted by group, cut into blocks of <= 256 nodes of one group
+  int nnchunks = 0;
+  int32_t* gnodes = nullptr;    // node ids, sorted by (group, id)
+  int2* nchunk = nullptr;       // (start, count) per node block
+  int32_t* nchunk_grp = nullptr;
+  double* npart = nullptr;      // per node block partials (nnchunks x 8 x 6, slot 0 used)
+  int32_t* nptr = nullptr;      // per group: range in nidx
+  int32_t* nidx = nullptr;
   double* Ec = nullptr;         // coarse matrix (nc x nc), then Cholesky factor
   double* Einv = nullptr;       // coarse inverse (nc x nc)
   double* cwork = nullptr;      // nc x nc workspace (inverse computation)
@@ -112,13 +129,16 @@ struct Handle {
   double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;  // z64/z32 unused
   float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr,
         *b32 = nullptr;
-  double* partials = nullptr;   // block partial sums (kMaxPartials)
+  double* partials = nullptr;   // block partial sums: 2 x kMaxPartials (p.q, then r.Mr)
   unsigned* counters = nullptr; // last-block counters (kNumCounters)
   double* scal = nullptr;       // device scalars (kNumScalars)
   int* flags = nullptr;         // device flags (kNumFlags)
   double* host_scal = nullptr;  // pinned mirror of scal
   int* host_flags = nullptr;    // pinned mirror of flags
   double* elem_buf = nullptr;   // N_el x 6 FP64 element workspace
+  hipStream_t gstream = nullptr;  // capture/replay stream (default priority)
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  GraphCache graphs[2];           // FP64, FP32 iteration batches
   int64_t workspace_bytes = 0;
 };
 
@@ -160,6 +180,24 @@ __device__ __forceinline__ double block_reduce_sum256(double v, double* lds4) {
   double r = 0.0;
   if (threadIdx.x == 0) r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
   return r;
+}
+
+// Sum of n per-block partials written by an earlier kernel, in a fixed order,
+// returned to every thread (blockDim.x == 256).  Kernel boundaries make the
+// partials visible, so no fence or atomic is involved; every block that
+// calls this gets bit-identical results.
+__device__ __forceinline__ double block_sum_partials(const double* __restrict__ part, int n,
+                                                     double* lds5) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  s = wave_reduce_sum(s);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds5[w] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) lds5[4] = (lds5[0] + lds5[1]) + (lds5[2] + lds5[3]);
+  __syncthreads();
+  return lds5[4];
 }
 
 // Deterministic grid reduction: every block stores its partial, the last

@@ -116,9 +116,7 @@ struct MatvecArgs {
   ModalConsts<T> mc;
   int zchunk;
   int tiles_x;
-  double* partials;    // dot partials (null: no dot)
-  unsigned* counter;
-  double* dot_out;     // where the finished dot goes
+  double* partials;    // per-block partials of u.y (null: no dot)
   const int* done;     // skip when *done != 0 (null: never)
   double* zero_vec;    // zeroed by block 0 (null: none)
   int zero_n;
@@ -245,11 +243,9 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
     buf ^= 1;
   }
   if (a.partials) {
+    // block partial of u.y; the consumer kernel sums the partials
     const double bs = block_reduce_sum256(dot, red);
-    const int nb = gridDim.x * gridDim.y;
-    const int bid = blockIdx.x + gridDim.x * blockIdx.y;
-    double* out = a.dot_out;
-    grid_reduce_finish(bs, a.partials, a.counter, nb, bid, [out](double t) { *out = t; });
+    if (threadIdx.x == 0) a.partials[blockIdx.x + gridDim.x * blockIdx.y] = bs;
   }
 }
 
@@ -266,6 +262,13 @@ static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& ti
   grid = dim3(tiles, (g.sz + zchunk - 1) / zchunk, 1);
 }
 
+int matvec_blocks(const Handle* h) {
+  dim3 grid;
+  int zc, tx;
+  matvec_launch_dims<double>(h, grid, zc, tx);
+  return (int)(grid.x * grid.y);
+}
+
 template <typename T>
 int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalConsts<T>& mc,
                   double* dot_out, const int* done, double* zero_vec, int zero_n,
@@ -279,9 +282,7 @@ int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalCon
   a.mc = mc;
   dim3 grid;
   matvec_launch_dims<T>(h, grid, a.zchunk, a.tiles_x);
-  a.partials = dot_out ? h->partials : nullptr;
-  a.counter = h->counters + C_MATVEC;
-  a.dot_out = dot_out;
+  a.partials = dot_out;
   a.done = done;
   a.zero_vec = zero_vec;
   a.zero_n = zero_n;
@@ -377,7 +378,7 @@ int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, doub
       (rc = device_alloc(h, (void**)&h->x64, ne * 8)) ||
       (rc = device_alloc(h, (void**)&h->invd64, 3 * nn * 8)) ||
       (rc = device_alloc(h, (void**)&h->invd32, 3 * nn * 4)) ||
-      (rc = device_alloc(h, (void**)&h->partials, kMaxPartials * 8)) ||
+      (rc = device_alloc(h, (void**)&h->partials, 2 * kMaxPartials * 8)) ||
       (rc = device_alloc(h, (void**)&h->counters, kNumCounters * sizeof(unsigned))) ||
       (rc = device_alloc(h, (void**)&h->scal, kNumScalars * 8)) ||
       (rc = device_alloc(h, (void**)&h->flags, kNumFlags * sizeof(int))))
@@ -404,11 +405,17 @@ int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, doub
 int pyto_destroy(pyto_handle* h) {
   if (!h) return PYTO_OK;
   void* ptrs[] = {h->fixmask, h->Ee64, h->Ee32, h->x64, h->invd64, h->invd32, h->node_group,
-                  h->centers, h->special, h->chunks, h->Ec, h->Einv, h->cwork, h->ct,
+                  h->centers, h->special, h->chunks, h->cpart, h->gptr, h->gidx, h->gnodes, h->nchunk,
+                  h->nchunk_grp, h->npart, h->nptr, h->nidx, h->Ec, h->Einv, h->cwork, h->ct,
                   h->cmu, h->cres, h->r64, h->z64, h->p64, h->q64, h->r32, h->z32, h->p32,
                   h->q32, h->x32, h->b32, h->partials, h->counters, h->scal, h->flags, h->elem_buf};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  for (auto& gc : h->graphs)
+    if (gc.exec) hipGraphExecDestroy(gc.exec);
+  if (h->ev_a) hipEventDestroy(h->ev_a);
+  if (h->ev_b) hipEventDestroy(h->ev_b);
+  if (h->gstream) hipStreamDestroy(h->gstream);
   if (h->host_scal) hipHostFree(h->host_scal);
   if (h->host_flags) hipHostFree(h->host_flags);
   delete h;

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 #include "element.hpp"
 
@@ -39,6 +40,7 @@ template <typename T>
 int launch_axpy_alpha(Handle* h, T* x, const T* p, hipStream_t s);
 int coarse_apply(Handle* h, const double* t1, const double* t2, const double* beta,
                  double* mu, const int* done, hipStream_t s, bool refine);
+int matvec_blocks(const Handle* h);
 
 constexpr int kVecBlocks = 2048;
 
@@ -64,21 +66,23 @@ __device__ __forceinline__ void for_lanes(V& a, const V& b, const V& c, F fn) {
   }
 }
 
-// alpha = rz/pKp (kept in S_ALPHA for the direction update); r -= alpha q;
-// rz_new = r^T (invd o r); stop test; beta = rz_new/rz.  Vector loads of
-// 16 bytes; the tail (n not a multiple of the width) is handled by block 0.
+// alpha = rz/pKp with pKp summed from the matvec's block partials (every
+// block sums them in the same order); r -= alpha q; per-block partials of
+// r^T (invd o r) go to part_rz for k_scalars.  Vector loads of 16 bytes; the
+// tail (n not a multiple of the width) is handled by block 0.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ r, const T* __restrict__ q,
                                                    const T* __restrict__ invd, int64_t n,
-                                                   double* __restrict__ scal,
-                                                   int* __restrict__ flags,
-                                                   double* __restrict__ partials,
-                                                   unsigned* __restrict__ counter) {
+                                                   const double* __restrict__ scal,
+                                                   const int* __restrict__ flags,
+                                                   const double* __restrict__ part_pq, int npq,
+                                                   double* __restrict__ part_rz) {
   if (flags[F_DONE]) return;
-  __shared__ double red[4];
+  __shared__ double red[5];
   using V = typename Vec4<T>::type;
   constexpr int W = Vec4<T>::width;
-  const double alpha = scal[S_RZ] / scal[S_PKP];
+  const double pkp = block_sum_partials(part_pq, npq, red);
+  const double alpha = scal[S_RZ] / pkp;
   const T a = (T)alpha;
   double acc = 0.0;
   const int64_t nv = n / W;
@@ -102,12 +106,27 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ r, const T* _
       r[i] = ri;
       acc += (double)ri * (double)invd[i] * (double)ri;
     }
-  const double bs = block_reduce_sum256(acc, red);
-  grid_reduce_finish(bs, partials, counter, gridDim.x, blockIdx.x, [=](double rz_new) {
+  __shared__ double red4[4];
+  const double bs = block_reduce_sum256(acc, red4);
+  if (threadIdx.x == 0) part_rz[blockIdx.x] = bs;
+}
+
+// One block: finishes the iteration's scalars from the partials: alpha (as
+// in k_cg_update), rz_new, the stop test sqrt(rz_new/rz0) <= rtol
+// (src/deflation.py:567), beta = rz_new/rz, the iteration count and the
+// breakdown flag (p^T K p <= 0 or NaN).
+__global__ __launch_bounds__(256) void k_scalars(double* __restrict__ scal, int* __restrict__ flags,
+                                                 const double* __restrict__ part_pq, int npq,
+                                                 const double* __restrict__ part_rz, int nrz) {
+  if (flags[F_DONE]) return;
+  __shared__ double red[5];
+  const double pkp = block_sum_partials(part_pq, npq, red);
+  const double rz_new = block_sum_partials(part_rz, nrz, red);
+  if (threadIdx.x == 0) {
     const double rz = scal[S_RZ];
     const double rel = sqrt(fabs(rz_new / scal[S_RZ0]));
-    const double pkp = scal[S_PKP];
-    scal[S_ALPHA] = alpha;
+    scal[S_PKP] = pkp;
+    scal[S_ALPHA] = rz / pkp;
     scal[S_RZNEW] = rz_new;
     scal[S_REL] = rel;
     flags[F_ITERS] += 1;
@@ -119,17 +138,26 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ r, const T* _
     }
     scal[S_BETA] = rz_new / rz;
     scal[S_RZ] = rz_new;
-  });
+  }
 }
 
-// r = f - q (in T), rz = r^T (invd o r) -> scal[slot_a] (and slot_b).
+// One block: scal[slot_a] (and slot_b) = sum of n partials.
+__global__ __launch_bounds__(256) void k_sum_to(double* __restrict__ scal,
+                                                const double* __restrict__ part, int n,
+                                                int slot_a, int slot_b) {
+  __shared__ double red[5];
+  const double v = block_sum_partials(part, n, red);
+  if (threadIdx.x == 0) {
+    scal[slot_a] = v;
+    if (slot_b >= 0) scal[slot_b] = v;
+  }
+}
+
+// r = f - q (in T); per-block partials of r^T (invd o r) (k_sum_to finishes).
 template <typename T, typename TF>
 __global__ __launch_bounds__(256) void k_residual(const TF* __restrict__ f, const T* __restrict__ q,
                                                   T* __restrict__ r, const T* __restrict__ invd,
-                                                  int64_t n, double* __restrict__ scal,
-                                                  int slot_a, int slot_b,
-                                                  double* __restrict__ partials,
-                                                  unsigned* __restrict__ counter) {
+                                                  int64_t n, double* __restrict__ part) {
   __shared__ double red[4];
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -139,10 +167,7 @@ __global__ __launch_bounds__(256) void k_residual(const TF* __restrict__ f, cons
     acc += (double)ri * (double)invd[i] * (double)ri;
   }
   const double bs = block_reduce_sum256(acc, red);
-  grid_reduce_finish(bs, partials, counter, gridDim.x, blockIdx.x, [=](double tot) {
-    scal[slot_a] = tot;
-    if (slot_b >= 0) scal[slot_b] = tot;
-  });
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 // out = (T)(in * (1/scal[slot]))  (scal[slot] > 0)
@@ -168,9 +193,7 @@ __global__ void k_axpy_scaled(double* __restrict__ u, const float* __restrict__ 
 __global__ __launch_bounds__(256) void k_norm2_diff(const double* __restrict__ a,
                                                     const double* __restrict__ b,
                                                     const double* __restrict__ free_ind,
-                                                    int64_t n, double* __restrict__ scal, int slot,
-                                                    double* __restrict__ partials,
-                                                    unsigned* __restrict__ counter) {
+                                                    int64_t n, double* __restrict__ part) {
   __shared__ double red[4];
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -179,8 +202,7 @@ __global__ __launch_bounds__(256) void k_norm2_diff(const double* __restrict__ a
     if (free_ind[i] != 0.0) acc += d * d;
   }
   const double bs = block_reduce_sum256(acc, red);
-  grid_reduce_finish(bs, partials, counter, gridDim.x, blockIdx.x,
-                     [=](double tot) { scal[slot] = tot; });
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 __global__ void k_set_flags(int* flags, int done, int iters) {
@@ -230,6 +252,45 @@ static int read_back(Handle* h, hipStream_t s) {
   return PYTO_OK;
 }
 
+constexpr int kGraphBatch = 16;
+
+// The captured batch of iterations for (precision, x) on this handle: built
+// on first use, reused while the solution buffer stays the same.
+template <typename F>
+static int iteration_graph(Handle* h, const void* x, int tsize, F&& one_iteration,
+                           hipGraphExec_t* out) {
+  if (!h->gstream) {
+    PYTO_CHECK_HIP(hipStreamCreateWithFlags(&h->gstream, hipStreamNonBlocking));
+    PYTO_CHECK_HIP(hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming));
+    PYTO_CHECK_HIP(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
+  }
+  const int slot = tsize == 8 ? 0 : 1;
+  GraphCache& gc = h->graphs[slot];
+  if (gc.exec && gc.x == x && gc.ngroups == h->ngroups && gc.nchunks == h->nchunks) {
+    *out = gc.exec;
+    return PYTO_OK;
+  }
+  if (gc.exec) {
+    hipGraphExecDestroy(gc.exec);
+    gc.exec = nullptr;
+  }
+  hipGraph_t graph;
+  PYTO_CHECK_HIP(hipStreamBeginCapture(h->gstream, hipStreamCaptureModeThreadLocal));
+  int rc = PYTO_OK;
+  for (int k = 0; k < kGraphBatch && rc == PYTO_OK; ++k) rc = one_iteration(h->gstream);
+  hipError_t e = hipStreamEndCapture(h->gstream, &graph);
+  if (rc != PYTO_OK) return rc;
+  PYTO_CHECK_HIP(e);
+  e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  PYTO_CHECK_HIP(e);
+  gc.x = x;
+  gc.ngroups = h->ngroups;
+  gc.nchunks = h->nchunks;
+  *out = gc.exec;
+  return PYTO_OK;
+}
+
 // Deflated PCG on K x = b (b and x full-length in precision T; fixed DOFs of
 // b ignored, of x zero).  The deflated start x0 = W E^-1 W^T b and the first
 // direction p0 = z0 - W E^-1 W^T K z0 follow src/deflation.py:532-551.
@@ -251,7 +312,6 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   Vecs<T> v = get_vecs<T>(h);
   const int64_t n = 3 * h->g.nnodes;
   const bool defl = h->ngroups > 0;
-  const int ld = (h->nc + 63) / 64 * 64;
   double* ct = h->ct;
   if (defl) {
     PYTO_TRY(launch_restrict_full<T>(h, b, ct, s));
@@ -260,18 +320,21 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   } else {
     PYTO_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(T), s));
   }
+  double* part_pq = h->partials;
+  double* part_rz = h->partials + kMaxPartials;
+  const int npq = matvec_blocks(h);
   PYTO_TRY(launch_matvec<T>(h, x, v.q, v.Ee, v.mc, nullptr, nullptr, nullptr, 0, s));
   res.matvecs += 1;
   hipLaunchKernelGGL((k_residual<T, T>), dim3(kVecBlocks), dim3(256), 0, s, b, v.q, v.r, v.invd,
-                     n, h->scal, (int)S_RZ, (int)S_RZ0, h->partials, h->counters + C_DOT);
+                     n, part_rz);
+  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, h->scal, (const double*)part_rz,
+                     kVecBlocks, (int)S_RZ, (int)S_RZ0);
   if (defl) {
-    PYTO_CHECK_HIP(hipMemsetAsync(ct, 0, sizeof(double) * ld, s));
     PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, (const T*)nullptr, nullptr, ct, nullptr, s));
     PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
   }
-  // p0 = z0 - W mu (x untouched); clears ct for iteration 1
-  PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, true, nullptr, s,
-                                defl ? ct : nullptr, defl ? ld : 0));
+  // p0 = z0 - W mu (x untouched)
+  PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, true, nullptr, s, nullptr, 0));
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RTOL, rtol);
   hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(1), 0, s, h->flags, 0, 0);
   PYTO_CHECK_HIP(hipGetLastError());
@@ -281,27 +344,58 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     res.rel = 0.0;
     return PYTO_OK;
   }
-  int launched = 0;
-  int batch = 8;
   const int* done = h->flags + F_DONE;
-  while (launched < max_iters) {
-    const int nb = std::min(batch, max_iters - launched);
-    for (int k = 0; k < nb; ++k) {
-      PYTO_TRY(launch_matvec<T>(h, v.p, v.q, v.Ee, v.mc, h->scal + S_PKP, done, nullptr, 0, s));
-      hipLaunchKernelGGL(k_cg_update<T>, dim3(kVecBlocks), dim3(256), 0, s, v.r, v.q, v.invd, n,
-                         h->scal, h->flags, h->partials, h->counters + C_UPDATE);
-      if (defl) {
-        PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, v.p, h->scal + S_BETA, ct, done, s));
-        PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, done, s, false));
-      }
-      PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, false, done, s,
-                                    defl ? ct : nullptr, defl ? ld : 0));
+  // one iteration's launches on stream `st`
+  auto one_iteration = [&](hipStream_t st) -> int {
+    PYTO_TRY(launch_matvec<T>(h, v.p, v.q, v.Ee, v.mc, part_pq, done, nullptr, 0, st));
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(kVecBlocks), dim3(256), 0, st, v.r, v.q, v.invd, n,
+                       (const double*)h->scal, (const int*)h->flags, (const double*)part_pq, npq,
+                       part_rz);
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, st, h->scal, h->flags,
+                       (const double*)part_pq, npq, (const double*)part_rz, kVecBlocks);
+    if (defl) {
+      PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, v.p, h->scal + S_BETA, ct, done, st));
+      PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, done, st, false));
     }
+    PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, false, done, st, nullptr, 0));
     PYTO_CHECK_HIP(hipGetLastError());
-    launched += nb;
-    PYTO_TRY(read_back(h, s));
-    if (h->host_flags[F_DONE]) break;
-    batch = std::min(batch * 2, 32);
+    return PYTO_OK;
+  };
+  int launched = 0;
+  const char* env = getenv("PYTO_NO_GRAPH");
+  const bool use_graph = !(env && env[0] == '1');
+  if (use_graph) {
+    // Batches of kGraphBatch iterations replayed from a captured hipGraph on
+    // the handle's own stream (capture needs a non-null stream), ordered after
+    // and before the caller's stream by events.
+    hipGraphExec_t exec = nullptr;
+    PYTO_TRY(iteration_graph(h, (const void*)x, sizeof(T), one_iteration, &exec));
+    PYTO_CHECK_HIP(hipEventRecord(h->ev_a, s));
+    PYTO_CHECK_HIP(hipStreamWaitEvent(h->gstream, h->ev_a, 0));
+    while (launched < max_iters) {
+      if (max_iters - launched >= kGraphBatch) {
+        PYTO_CHECK_HIP(hipGraphLaunch(exec, h->gstream));
+        launched += kGraphBatch;
+      } else {
+        const int rest = max_iters - launched;
+        for (int k = 0; k < rest; ++k) PYTO_TRY(one_iteration(h->gstream));
+        launched += rest;
+      }
+      PYTO_TRY(read_back(h, h->gstream));
+      if (h->host_flags[F_DONE]) break;
+    }
+    PYTO_CHECK_HIP(hipEventRecord(h->ev_b, h->gstream));
+    PYTO_CHECK_HIP(hipStreamWaitEvent(s, h->ev_b, 0));
+  } else {
+    int batch = 8;
+    while (launched < max_iters) {
+      const int nb = std::min(batch, max_iters - launched);
+      for (int k = 0; k < nb; ++k) PYTO_TRY(one_iteration(s));
+      launched += nb;
+      PYTO_TRY(read_back(h, s));
+      if (h->host_flags[F_DONE]) break;
+      batch = std::min(batch * 2, 32);
+    }
   }
   res.iterations = h->host_flags[F_ITERS];
   res.matvecs += res.iterations;
@@ -319,9 +413,13 @@ static int true_residual(Handle* h, const double* f, const double* u, double* ou
   const int64_t n = 3 * h->g.nnodes;
   PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0, s));
   hipLaunchKernelGGL(k_norm2_diff, dim3(kVecBlocks), dim3(256), 0, s, f, h->q64, h->invd64, n,
-                     h->scal, (int)S_DOT0, h->partials, h->counters + C_NORM);
+                     h->partials);
+  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, h->scal, (const double*)h->partials,
+                     kVecBlocks, (int)S_DOT0, -1);
   hipLaunchKernelGGL(k_norm2_diff, dim3(kVecBlocks), dim3(256), 0, s, f, (const double*)nullptr,
-                     h->invd64, n, h->scal, (int)S_DOT1, h->partials, h->counters + C_NORM);
+                     h->invd64, n, h->partials);
+  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, h->scal, (const double*)h->partials,
+                     kVecBlocks, (int)S_DOT1, -1);
   PYTO_CHECK_HIP(hipGetLastError());
   PYTO_TRY(read_back(h, s));
   const double nf = h->host_scal[S_DOT1];
@@ -397,8 +495,9 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
   PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0, s));
   mv64++;
   hipLaunchKernelGGL((k_residual<double, double>), dim3(kVecBlocks), dim3(256), 0, s, f, h->q64,
-                     h->r64, h->invd64, n, h->scal, (int)S_TMP1, (int)S_TMP0, h->partials,
-                     h->counters + C_DOT);
+                     h->r64, h->invd64, n, h->partials);
+  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, h->scal, (const double*)h->partials,
+                     kVecBlocks, (int)S_TMP1, (int)S_TMP0);
   PYTO_CHECK_HIP(hipGetLastError());
   PYTO_TRY(read_back(h, s));
   const double rz0 = h->host_scal[S_TMP0];
@@ -423,8 +522,9 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
                                    s));
     mv64++;
     hipLaunchKernelGGL((k_residual<double, double>), dim3(kVecBlocks), dim3(256), 0, s, f,
-                       h->q64, h->r64, h->invd64, n, h->scal, (int)S_TMP1, -1, h->partials,
-                       h->counters + C_DOT);
+                       h->q64, h->r64, h->invd64, n, h->partials);
+    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, h->scal, (const double*)h->partials,
+                       kVecBlocks, (int)S_TMP1, -1);
     PYTO_CHECK_HIP(hipGetLastError());
     PYTO_TRY(read_back(h, s));
     outer++;

@@ -27,7 +27,9 @@ def test_compliance_kernel(gpu, dims):
     # element energies of rigid-dominated elements cancel (u_e^T KE u_e with a
     # large rigid part), so both sides carry ~1e-12 relative rounding in C
     assert abs(float(C) - C_ref) <= 1e-10 * abs(C_ref)
-    assert rel_err(g.cpu().numpy(), g_ref) <= 1e-12
+    # same cancellation (the oracle's dense KE.u cancels the rigid part less
+    # exactly than the butterfly's differences)
+    assert rel_err(g.cpu().numpy(), g_ref) <= 1e-10
     assert rel_err(ce.cpu().numpy(), OS.element_energy(u, c.mesh.edofMatStructural, c.KE)) <= 1e-12
     # identity C = f^T u
     assert abs(float(C) - c.f @ u) <= 1e-10 * abs(C_ref)
@@ -74,8 +76,11 @@ def test_pnorm_sensitivity_end_to_end(gpu):
     ds.create_deflation_groups(mesh, 10)
     tmm.set_pNorm_exponent(8.0)
     try:
+        # FD at h = 1e-6 of a quantity first order in the solve error: solve
+        # to 1e-14 (exact coarse matrix) so solver noise / h stays ~1e-8
         fe = hex_structural_fea.HexStructuralFEA(mesh, mat_lib.Material("Unit", 1.0, 0.3), bc,
-                                                 linear_solvers.Solvers.DPCG, dsolver=ds, rtol=1e-12)
+                                                 linear_solvers.Solvers.DPCG, dsolver=ds, rtol=1e-14,
+                                                 maxIters=5000)
         fe.stiff_mtrx.set_coarse_shift(0.0, 0.0)
         u = fe.solve(c.x)
         fe.postprocess()

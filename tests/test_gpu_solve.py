@@ -85,3 +85,26 @@ def test_zero_force_and_iteration_limit(gpu):
     with pytest.warns(ConvergenceWarning):
         u, st = op.solve(torch.from_numpy(c.f).to(gpu), rtol=1e-14, max_iters=3)
     assert st["iterations"] == 3 and not st["converged"]
+
+
+def test_graph_replay_matches_eager(gpu, monkeypatch):
+    """The hipGraph-replayed iteration batches give the eager result: same
+    iteration count, same solution to rounding, over repeated solves with
+    other work in between."""
+    print("graph vs eager", flush=True)
+    c = Case((40, 12, 10), seed=9, lo=0.05)
+    op = c.operator(gpu)
+    f = torch.from_numpy(c.f).to(gpu)
+    monkeypatch.setenv("PYTO_NO_GRAPH", "1")
+    u_e, st_e = op.solve(f, rtol=1e-8)
+    ue32, st_e32 = op.solve(f, rtol=1e-8, precision="mixed")
+    monkeypatch.setenv("PYTO_NO_GRAPH", "0")
+    for rep in range(3):
+        u_g, st_g = op.solve(f, rtol=1e-8)
+        assert st_g["iterations"] == st_e["iterations"]
+        assert rel_err(u_g.cpu().numpy(), u_e.cpu().numpy()) <= 1e-12
+        _ = op.matvec(u_g)       # eager work between replays
+        torch.cuda.synchronize()
+        ug32, st_g32 = op.solve(f, rtol=1e-8, precision="mixed")
+        assert st_g32["iterations"] == st_e32["iterations"]
+        assert rel_err(ug32.cpu().numpy(), ue32.cpu().numpy()) <= 1e-12
