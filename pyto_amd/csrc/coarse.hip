@@ -163,10 +163,9 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict_special(
     const T* __restrict__ r, const T* __restrict__ invd, const T* __restrict__ p,
-    const double* __restrict__ beta_ptr, const int32_t* __restrict__ special,
+    const double* __restrict__ beta_ptr, const SpecialElem* __restrict__ sinfo,
     const SpecialChunk* __restrict__ chunks, int nchunks, const double* __restrict__ Ee,
-    const int32_t* __restrict__ grp, const double* __restrict__ centers,
-    const uint8_t* __restrict__ mask, GridDims g, ModalConsts<double> mc,
+    const double* __restrict__ centers, GridDims g, ModalConsts<double> mc,
     double* __restrict__ cpart, const int* __restrict__ done) {
   if (done && *done) return;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -176,40 +175,34 @@ __global__ __launch_bounds__(256) void k_restrict_special(
   const bool active = lane < ch.count;
   const double beta = (p && beta_ptr) ? *beta_ptr : 0.0;
   double f[8][3];
-  int gid[8];
+  uint64_t info = 0;
   int ix = 0, iy = 0, iz = 0;
   if (active) {
-    const int64_t e = special[ch.start + lane];
-    elem_coords(e, g, ix, iy, iz);
+    const SpecialElem se = sinfo[ch.start + lane];
+    info = se.info;
+    elem_coords(se.e, g, ix, iy, iz);
     double u[8][3];
-    uint32_t mbits = 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int64_t n = (ix + corner_a(j)) + (int64_t)g.sx * ((iy + corner_b(j)) +
                                                               (int64_t)g.sy * (iz + corner_c(j)));
-      gid[j] = grp[n];
-      const uint32_t m = mask[n];
-      mbits |= m << (3 * j);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         double val = (double)r[3 * n + c];
         if (invd) val *= (double)invd[3 * n + c];
         if (p) val += beta * (double)p[3 * n + c];
-        u[j][c] = ((m >> c) & 1) ? 0.0 : val;
+        u[j][c] = ((info >> (24 + 3 * j + c)) & 1) ? 0.0 : val;
       }
     }
-    element_apply<double>(u, Ee[e], mc, f);
+    element_apply<double>(u, Ee[se.e], mc, f);
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        if ((mbits >> (3 * j + c)) & 1) f[j][c] = 0.0;
+        if ((info >> (24 + 3 * j + c)) & 1) f[j][c] = 0.0;
   } else {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      gid[j] = -1;
-      f[j][0] = f[j][1] = f[j][2] = 0.0;
-    }
+    for (int j = 0; j < 8; ++j) f[j][0] = f[j][1] = f[j][2] = 0.0;
   }
   for (int sl = 0; sl < ch.ng; ++sl) {
     const int gs = ch.groups[sl];
@@ -217,7 +210,7 @@ __global__ __launch_bounds__(256) void k_restrict_special(
     double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      if (gid[j] != gs) continue;
+      if (!active || (int)((info >> (3 * j)) & 7) != sl) continue;
       const double rx = (ix + corner_a(j)) * g.hx - cx;
       const double ry = (iy + corner_b(j)) * g.hy - cy;
       const double rz = (iz + corner_c(j)) * g.hz - cz;
@@ -587,8 +580,8 @@ int launch_restrict_special(Handle* h, const T* r, const T* invd, const T* p,
   if (h->nchunks > 0) {
     const int blocks = (h->nchunks * 64 + 255) / 256;
     hipLaunchKernelGGL(k_restrict_special<T>, dim3(blocks), dim3(256), 0, s, r, invd, p, beta,
-                       h->special, h->chunks, h->nchunks, h->Ee64, h->node_group, h->centers,
-                       h->fixmask, h->g, h->mc64, h->cpart, done);
+                       h->sinfo, h->chunks, h->nchunks, h->Ee64, h->centers, h->g, h->mc64,
+                       h->cpart, done);
   }
   hipLaunchKernelGGL(k_coarse_gather, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
                      h->cpart, h->gptr, h->gidx, h->ngroups, t, done);
@@ -744,12 +737,13 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
       gc.exec = nullptr;
     }
   // release a previous coarse space
-  void* olds[] = {h->node_group, h->centers, h->special, h->chunks, h->cpart, h->gptr, h->gidx,
+  void* olds[] = {h->node_group, h->centers, h->special, h->sinfo, h->chunks, h->cpart, h->gptr, h->gidx,
                   h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
                   h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres};
   for (void* p : olds)
     if (p) hipFree(p);
   h->node_group = nullptr; h->centers = nullptr; h->special = nullptr; h->chunks = nullptr;
+  h->sinfo = nullptr;
   h->cpart = nullptr; h->gptr = nullptr; h->gidx = nullptr;
   h->gnodes = nullptr; h->nchunk = nullptr; h->nchunk_grp = nullptr; h->npart = nullptr;
   h->nptr = nullptr; h->nidx = nullptr; h->nnchunks = 0;
@@ -806,6 +800,23 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     i = j;
   }
   for (size_t i = 0; i < list.size(); ++i) special[i] = list[i].second;
+  std::vector<SpecialElem> sinfo(list.size());
+  for (size_t i = 0; i < list.size(); ++i) {
+    const int32_t e = list[i].second;
+    const int ix = (int)(e % g.nx), iy = (int)((e / g.nx) % g.ny), iz = (int)(e / ((int64_t)g.nx * g.ny));
+    uint64_t info = 0;
+    for (int j = 0; j < 8; ++j) {
+      const int64_t n = (ix + corner_a(j)) + (int64_t)g.sx * ((iy + corner_b(j)) +
+                                                              (int64_t)g.sy * (iz + corner_c(j)));
+      int slot = 0;
+      while (list[i].first[slot] != grp[n]) ++slot;
+      info |= (uint64_t)slot << (3 * j);
+      info |= (uint64_t)(mask[n] & 7) << (24 + 3 * j);
+    }
+    sinfo[i].e = e;
+    sinfo[i].pad = 0;
+    sinfo[i].info = info;
+  }
   // per group: the (chunk, slot) partials that belong to it
   std::vector<int32_t> gptr(ngroups + 1, 0), gidx;
   {
@@ -855,6 +866,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_TRY(device_alloc(h, (void**)&h->node_group, nn * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->centers, (size_t)ngroups * 3 * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->special, special.size() * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->sinfo, sinfo.size() * sizeof(SpecialElem)));
   PYTO_TRY(device_alloc(h, (void**)&h->chunks, chunks.size() * sizeof(SpecialChunk)));
   PYTO_TRY(device_alloc(h, (void**)&h->cpart, chunks.size() * 48 * sizeof(double)));
   PYTO_TRY(device_alloc(h, (void**)&h->gnodes, nn * sizeof(int32_t)));
@@ -875,6 +887,8 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_CHECK_HIP(hipMemcpyAsync(h->centers, centers, (size_t)ngroups * 24, hipMemcpyDeviceToDevice, s));
   if (!special.empty()) {
     PYTO_CHECK_HIP(hipMemcpyAsync(h->special, special.data(), special.size() * 4,
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->sinfo, sinfo.data(), sinfo.size() * sizeof(SpecialElem),
                                   hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->chunks, chunks.data(), chunks.size() * sizeof(SpecialChunk),
                                   hipMemcpyHostToDevice, s));

@@ -67,6 +67,15 @@ struct GridDims {
 
 // Work chunk of the special-element lists: up to 64 elements (one wave) that
 // share the same set of deflation groups.
+// One special element: its id and, per local node j, the slot (3 bits at
+// 3j) of its group in the chunk's group list and its fixed-DOF bits
+// (3 bits at 24+3j).
+struct SpecialElem {
+  int32_t e;
+  int32_t pad;
+  uint64_t info;
+};
+
 struct SpecialChunk {
   int start;      // offset into special element array
   int count;      // <= 64
@@ -103,6 +112,7 @@ struct Handle {
   double* centers = nullptr;
   int nspecial = 0, nchunks = 0;
   int32_t* special = nullptr;   // element ids sorted by group set
+  SpecialElem* sinfo = nullptr; // same order, with packed slots and fixed bits
   SpecialChunk* chunks = nullptr;
   double* cpart = nullptr;      // per-chunk partial W^T K v (nchunks x 8 x 6)
   int32_t* gptr = nullptr;      // per group: range of (chunk*8+slot) in gidx

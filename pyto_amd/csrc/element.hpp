@@ -108,6 +108,45 @@ __host__ __device__ __forceinline__ void modal_stress(const Modal<T>& c, T gx, T
   F.cxyz[2] = ninth * (base + lm * gz2) * c.cxyz[2];
 }
 
+// Cube elements (hx = hy = hz): every gradient coefficient carries the same
+// g = 2/h, so g^2 folds into the Lame constants and the mode weights 1/3 and
+// 1/9 into per-mode copies of them; this saves ~50 of the ~250 operations.
+// L = lam g^2 E, M = mu g^2 E (lam, mu already scaled by V/64).
+template <typename T>
+__host__ __device__ __forceinline__ void modal_stress_cube(const Modal<T>& c, T L, T M, T L3,
+                                                           T M3, T K9, Modal<T>& F) {
+  const T M2 = M + M;
+  {
+    const T ltr = L * (c.cx[0] + c.cy[1] + c.cz[2]);
+    const T s01 = M * (c.cy[0] + c.cx[1]), s02 = M * (c.cz[0] + c.cx[2]);
+    const T s12 = M * (c.cz[1] + c.cy[2]);
+    F.cx[0] = ltr + M2 * c.cx[0]; F.cx[1] = s01; F.cx[2] = s02;
+    F.cy[0] = s01; F.cy[1] = ltr + M2 * c.cy[1]; F.cy[2] = s12;
+    F.cz[0] = s02; F.cz[1] = s12; F.cz[2] = ltr + M2 * c.cz[2];
+  }
+  const T M32 = M3 + M3;
+  // b = cxy (as d/dy in mode xi), cxz (d/dz in mode xi); cxy (d/dx in mode
+  // eta), cyz (d/dz in mode eta); cxz (d/dx in mode zeta), cyz (d/dy in zeta)
+  const T lxi = L3 * (c.cxy[1] + c.cxz[2]);
+  const T leta = L3 * (c.cxy[0] + c.cyz[2]);
+  const T lze = L3 * (c.cxz[0] + c.cyz[1]);
+  const T sxi21 = M3 * (c.cxy[2] + c.cxz[1]);
+  const T seta20 = M3 * (c.cxy[2] + c.cyz[0]);
+  const T sze10 = M3 * (c.cxz[1] + c.cyz[0]);
+  F.cxy[0] = M3 * c.cxy[0] + (leta + M32 * c.cxy[0]);
+  F.cxy[1] = (lxi + M32 * c.cxy[1]) + M3 * c.cxy[1];
+  F.cxy[2] = sxi21 + seta20;
+  F.cxz[0] = M3 * c.cxz[0] + (lze + M32 * c.cxz[0]);
+  F.cxz[1] = sxi21 + sze10;
+  F.cxz[2] = (lxi + M32 * c.cxz[2]) + M3 * c.cxz[2];
+  F.cyz[0] = seta20 + sze10;
+  F.cyz[1] = M3 * c.cyz[1] + (lze + M32 * c.cyz[1]);
+  F.cyz[2] = (leta + M32 * c.cyz[2]) + M3 * c.cyz[2];
+  F.cxyz[0] = K9 * c.cxyz[0];
+  F.cxyz[1] = K9 * c.cxyz[1];
+  F.cxyz[2] = K9 * c.cxyz[2];
+}
+
 // Transposed butterfly: f[n][i] = sum_m s_m(n) F_m,i (no constant mode).
 template <typename T>
 __host__ __device__ __forceinline__ void modal_backward(const Modal<T>& F, T f[8][3]) {
@@ -142,6 +181,21 @@ __host__ __device__ __forceinline__ void element_apply(const T u[8][3], T E,
   Modal<T> c, F;
   modal_forward(u, c);
   modal_stress(c, mc.gx, mc.gy, mc.gz, E * mc.lam, E * mc.mu, F);
+  modal_backward(F, f);
+}
+
+// f = E * KE * u for a cube element (hx = hy = hz); mc as for element_apply.
+template <typename T>
+__host__ __device__ __forceinline__ void element_apply_cube(const T u[8][3], T E,
+                                                            const ModalConsts<T>& mc,
+                                                            T f[8][3]) {
+  Modal<T> c, F;
+  modal_forward(u, c);
+  const T g2 = mc.gx * mc.gx;
+  const T L = E * (mc.lam * g2), M = E * (mc.mu * g2);
+  const T L3 = L * T(1.0 / 3.0), M3 = M * T(1.0 / 3.0);
+  const T K9 = (L + T(4) * M) * T(1.0 / 9.0);
+  modal_stress_cube(c, L, M, L3, M3, K9, F);
   modal_backward(F, f);
 }
 

@@ -144,7 +144,7 @@ __device__ __forceinline__ void load_plane(T (*pl)[kPl][3], const T* __restrict_
   }
 }
 
-template <typename T>
+template <typename T, bool CUBE>
 __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
   if (a.done && *a.done) return;
   __shared__ T pl[2][kPl][kPl][3];
@@ -198,7 +198,10 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
         }
       }
       const T E = a.Ee[ex + (int64_t)g.nx * (ey + (int64_t)g.ny * k)];
-      element_apply<T>(ue, E, a.mc, f);
+      if (CUBE)
+        element_apply_cube<T>(ue, E, a.mc, f);
+      else
+        element_apply<T>(ue, E, a.mc, f);
     } else {
 #pragma unroll
       for (int j = 0; j < 8; ++j)
@@ -286,7 +289,11 @@ int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalCon
   a.done = done;
   a.zero_vec = zero_vec;
   a.zero_n = zero_n;
-  hipLaunchKernelGGL(k_matvec<T>, grid, dim3(256), 0, s, a);
+  const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
+  if (cube)
+    hipLaunchKernelGGL((k_matvec<T, true>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_matvec<T, false>), grid, dim3(256), 0, s, a);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -405,7 +412,7 @@ int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, doub
 int pyto_destroy(pyto_handle* h) {
   if (!h) return PYTO_OK;
   void* ptrs[] = {h->fixmask, h->Ee64, h->Ee32, h->x64, h->invd64, h->invd32, h->node_group,
-                  h->centers, h->special, h->chunks, h->cpart, h->gptr, h->gidx, h->gnodes, h->nchunk,
+                  h->centers, h->special, h->sinfo, h->chunks, h->cpart, h->gptr, h->gidx, h->gnodes, h->nchunk,
                   h->nchunk_grp, h->npart, h->nptr, h->nidx, h->Ec, h->Einv, h->cwork, h->ct,
                   h->cmu, h->cres, h->r64, h->z64, h->p64, h->q64, h->r32, h->z32, h->p32,
                   h->q32, h->x32, h->b32, h->partials, h->counters, h->scal, h->flags, h->elem_buf};
