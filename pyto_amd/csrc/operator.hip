@@ -122,33 +122,44 @@ struct MatvecArgs {
   int zero_n;
 };
 
+// Lane (tx-1)'s value within the 16-lane row (0 for tx = 0): DPP row_shr:1.
+__device__ __forceinline__ float dpp_shr1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double dpp_shr1(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// One node of the 17x17 staging plane: (masked) values from global memory.
 template <typename T>
-__device__ __forceinline__ void load_plane(T (*pl)[kPl][3], const T* __restrict__ u,
+__device__ __forceinline__ void fetch_node(T v[3], const T* __restrict__ u,
                                            const uint8_t* __restrict__ mask, const GridDims& g,
-                                           int x0, int y0, int kp) {
-  for (int e = threadIdx.x; e < kPl * kPl; e += 256) {
-    const int lx = e % kPl, ly = e / kPl;
-    const int gx = x0 + lx, gy = y0 + ly;
-    T v0 = T(0), v1 = T(0), v2 = T(0);
-    if (kp >= 0 && kp < g.sz && gx >= 0 && gx < g.sx && gy >= 0 && gy < g.sy) {
-      const int64_t n = gx + (int64_t)g.sx * (gy + (int64_t)g.sy * kp);
-      const uint8_t m = mask[n];
-      const T* p = u + 3 * n;
-      v0 = (m & 1) ? T(0) : p[0];
-      v1 = (m & 2) ? T(0) : p[1];
-      v2 = (m & 4) ? T(0) : p[2];
-    }
-    pl[ly][lx][0] = v0;
-    pl[ly][lx][1] = v1;
-    pl[ly][lx][2] = v2;
+                                           int gx, int gy, int kp) {
+  v[0] = v[1] = v[2] = T(0);
+  if (kp >= 0 && kp < g.sz && gx >= 0 && gx < g.sx && gy >= 0 && gy < g.sy) {
+    const int64_t n = gx + (int64_t)g.sx * (gy + (int64_t)g.sy * kp);
+    const uint8_t m = mask[n];
+    const T* p = u + 3 * n;
+    v[0] = (m & 1) ? T(0) : p[0];
+    v[1] = (m & 2) ? T(0) : p[1];
+    v[2] = (m & 4) ? T(0) : p[2];
   }
 }
 
+// Layer loop, per element layer k (nodes k and k+1):
+//   store plane k+1 (prefetched into registers one layer earlier) to LDS,
+//   issue the global loads of plane k+2, write this thread's y-exchange of
+//   layer k-1, one barrier, finish node plane k-1 (owners), then read the
+//   corners of plane k+1 and apply element (ex, ey, k).
+// x-neighbour sums use DPP within the 16-lane rows (tx is the lane's row
+// position); only the y-neighbour sums go through LDS.
 template <typename T, bool CUBE>
 __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
   if (a.done && *a.done) return;
   __shared__ T pl[2][kPl][kPl][3];
-  __shared__ T xc[kTile][kTile][6][3];
+  __shared__ T xc[2][kTile][kTile][6];
   __shared__ double red[4];
   const GridDims& g = a.g;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -163,8 +174,28 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
   if (a.zero_vec && blockIdx.x == 0 && blockIdx.y == 0)
     for (int i = threadIdx.x; i < a.zero_n; i += 256) a.zero_vec[i] = 0.0;
 
+  // staging-plane entries of this thread: e0 = tid, e1 = tid + 256 (< 289)
+  const int e0 = threadIdx.x, e1 = threadIdx.x + 256;
+  const bool has1 = e1 < kPl * kPl;
+  const int lx0 = e0 % kPl, ly0 = e0 / kPl;
+  const int lx1 = has1 ? e1 % kPl : 0, ly1 = has1 ? e1 / kPl : 0;
+  T pv0[3], pv1[3] = {T(0), T(0), T(0)};
+  auto fetch = [&](int kp) {
+    fetch_node<T>(pv0, a.u, a.mask, g, x0 + lx0, y0 + ly0, kp);
+    if (has1) fetch_node<T>(pv1, a.u, a.mask, g, x0 + lx1, y0 + ly1, kp);
+  };
+  auto store = [&](int b) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pl[b][ly0][lx0][c] = pv0[c];
+    if (has1)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pl[b][ly1][lx1][c] = pv1[c];
+  };
+
   T bot[4][3], top[4][3];
-  load_plane<T>(pl[0], a.u, a.mask, g, x0, y0, kz0 - 1);
+  fetch(kz0 - 1);
+  store(0);
+  fetch(kz0);
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -173,18 +204,53 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
     bot[2][c] = pl[0][ty + 1][tx + 1][c];
     bot[3][c] = pl[0][ty + 1][tx][c];
   }
-  T carry[3] = {T(0), T(0), T(0)};
+  T carry[3] = {T(0), T(0), T(0)};   // top-plane sum of the layer before last
+  T xb[3], xt[3], yb[3], yt[3];       // pending sums of the last layer
+  T ubot[3];                          // u at this thread's node of the pending plane
+#pragma unroll
+  for (int c = 0; c < 3; ++c) xb[c] = xt[c] = yb[c] = yt[c] = ubot[c] = T(0);
   double dot = 0.0;
-  int buf = 1;
-  for (int k = kz0 - 1; k < kz1; ++k) {
-    load_plane<T>(pl[buf], a.u, a.mask, g, x0, y0, k + 1);
-    __syncthreads();
+  for (int k = kz0 - 1; k <= kz1; ++k) {
+    const int b = (k - kz0 + 2) & 1;   // LDS plane buffer of node plane k+1
+    const int xb_i = k & 1;            // y-exchange buffer of layer k-1
+    if (k < kz1) {
+      store(b);
+      if (k + 2 <= kz1) fetch(k + 2);
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      top[0][c] = pl[buf][ty][tx][c];
-      top[1][c] = pl[buf][ty][tx + 1][c];
-      top[2][c] = pl[buf][ty + 1][tx + 1][c];
-      top[3][c] = pl[buf][ty + 1][tx][c];
+      xc[xb_i][ty][tx][c] = yb[c];
+      xc[xb_i][ty][tx][3 + c] = yt[c];
+    }
+    __syncthreads();
+    // finish node plane k-1 (bottom of layer k-1 + top of layer k-2)
+    if (k - 1 >= kz0 && owner) {
+      const int kp = k - 1;
+      const int64_t nw = ex + (int64_t)g.sx * (ey + (int64_t)g.sy * kp);
+      const uint8_t m = a.mask[nw];
+      T* yp = a.y + 3 * nw;
+      T nt[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T v = ((m >> c) & 1) ? T(0) : carry[c] + xb[c] + xc[xb_i][ty - 1][tx][c];
+        yp[c] = v;
+        dot += (double)v * (double)ubot[c];
+        nt[c] = xt[c] + xc[xb_i][ty - 1][tx][3 + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) carry[c] = nt[c];
+    } else if (owner) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) carry[c] = xt[c] + xc[xb_i][ty - 1][tx][3 + c];
+    }
+    if (k >= kz1) break;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ubot[c] = bot[0][c];
+      top[0][c] = pl[b][ty][tx][c];
+      top[1][c] = pl[b][ty][tx + 1][c];
+      top[2][c] = pl[b][ty + 1][tx + 1][c];
+      top[3][c] = pl[b][ty + 1][tx][c];
     }
     T f[8][3];
     if (col_valid && k >= 0 && k < g.nz) {
@@ -208,42 +274,18 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) f[j][c] = T(0);
     }
+    // x-neighbour sums (DPP) for the nodes (tx,ty) and (tx,ty+1) of both planes
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      xc[ty][tx][0][c] = f[1][c];
-      xc[ty][tx][1][c] = f[2][c];
-      xc[ty][tx][2][c] = f[3][c];
-      xc[ty][tx][3][c] = f[5][c];
-      xc[ty][tx][4][c] = f[6][c];
-      xc[ty][tx][5][c] = f[7][c];
-    }
-    __syncthreads();
-    if (owner) {
-      T ob[3], ot[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        ob[c] = f[0][c] + xc[ty][tx - 1][0][c] + xc[ty - 1][tx - 1][1][c] + xc[ty - 1][tx][2][c];
-        ot[c] = f[4][c] + xc[ty][tx - 1][3][c] + xc[ty - 1][tx - 1][4][c] + xc[ty - 1][tx][5][c];
-      }
-      if (k >= kz0) {
-        const int64_t nw = ex + (int64_t)g.sx * (ey + (int64_t)g.sy * k);
-        const uint8_t m = a.mask[nw];
-        T* yp = a.y + 3 * nw;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T v = ((m >> c) & 1) ? T(0) : carry[c] + ob[c];
-          yp[c] = v;
-          dot += (double)v * (double)bot[0][c];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) carry[c] = ot[c];
+      xb[c] = f[0][c] + dpp_shr1(f[1][c]);
+      yb[c] = f[3][c] + dpp_shr1(f[2][c]);
+      xt[c] = f[4][c] + dpp_shr1(f[5][c]);
+      yt[c] = f[7][c] + dpp_shr1(f[6][c]);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int c = 0; c < 3; ++c) bot[j][c] = top[j][c];
-    buf ^= 1;
   }
   if (a.partials) {
     // block partial of u.y; the consumer kernel sums the partials
@@ -252,25 +294,57 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
   }
 }
 
+// Resident workgroups of k_matvec<T, CUBE> on the whole device.
+template <typename T, bool CUBE>
+static int matvec_capacity() {
+  static int cap = -1;
+  if (cap < 0) {
+    int nb = 0, dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_matvec<T, CUBE>, 256, 0) != hipSuccess ||
+        hipGetDeviceProperties(&prop, dev) != hipSuccess || nb <= 0)
+      return 1024;
+    cap = nb * prop.multiProcessorCount;
+  }
+  return cap;
+}
+
+// z-chunk length: minimise (workgroup rounds) x (layers per workgroup,
+// including the halo layer) for the occupancy of this kernel variant.
 template <typename T>
 static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& tiles_x) {
   const GridDims& g = h->g;
   tiles_x = (g.sx + kOwn - 1) / kOwn;
   const int tiles_y = (g.sy + kOwn - 1) / kOwn;
-  const int tiles = tiles_x * tiles_y;
-  // aim for >= 4 workgroups per CU on 256 CUs, within the partials buffer
-  zchunk = 16;
-  while ((int64_t)tiles * ((g.sz + zchunk - 1) / zchunk) > kMaxPartials) zchunk *= 2;
-  while (zchunk > 4 && (int64_t)tiles * ((g.sz + zchunk - 1) / zchunk) < 1024) zchunk /= 2;
-  grid = dim3(tiles, (g.sz + zchunk - 1) / zchunk, 1);
+  const int64_t tiles = (int64_t)tiles_x * tiles_y;
+  const bool cube = g.hx == g.hy && g.hy == g.hz;
+  const int cap = cube ? matvec_capacity<T, true>() : matvec_capacity<T, false>();
+  int best = 16;
+  double best_cost = 1e300;
+  for (int zc = 4; zc <= 64; ++zc) {
+    const int64_t blocks = tiles * ((g.sz + zc - 1) / zc);
+    if (blocks > kMaxPartials) continue;
+    const int64_t rounds = (blocks + cap - 1) / cap;
+    const double cost = (double)rounds * (std::min(zc, g.sz) + 1);
+    if (cost < best_cost - 1e-9) {
+      best_cost = cost;
+      best = zc;
+    }
+  }
+  zchunk = best;
+  grid = dim3((unsigned)tiles, (g.sz + zchunk - 1) / zchunk, 1);
 }
 
+template <typename T>
 int matvec_blocks(const Handle* h) {
   dim3 grid;
   int zc, tx;
-  matvec_launch_dims<double>(h, grid, zc, tx);
+  matvec_launch_dims<T>(h, grid, zc, tx);
   return (int)(grid.x * grid.y);
 }
+template int matvec_blocks<double>(const Handle*);
+template int matvec_blocks<float>(const Handle*);
 
 template <typename T>
 int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalConsts<T>& mc,

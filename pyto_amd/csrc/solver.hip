@@ -40,6 +40,7 @@ template <typename T>
 int launch_axpy_alpha(Handle* h, T* x, const T* p, hipStream_t s);
 int coarse_apply(Handle* h, const double* t1, const double* t2, const double* beta,
                  double* mu, const int* done, hipStream_t s, bool refine);
+template <typename T>
 int matvec_blocks(const Handle* h);
 
 constexpr int kVecBlocks = 2048;
@@ -322,7 +323,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   }
   double* part_pq = h->partials;
   double* part_rz = h->partials + kMaxPartials;
-  const int npq = matvec_blocks(h);
+  const int npq = matvec_blocks<T>(h);
   PYTO_TRY(launch_matvec<T>(h, x, v.q, v.Ee, v.mc, nullptr, nullptr, nullptr, 0, s));
   res.matvecs += 1;
   hipLaunchKernelGGL((k_residual<T, T>), dim3(kVecBlocks), dim3(256), 0, s, b, v.q, v.r, v.invd,

@@ -90,21 +90,25 @@ def test_pnorm_sensitivity_end_to_end(gpu):
         assert abs(pn - pn_ref) <= 1e-9 * pn_ref
         assert abs(vmax - vmax_ref) <= 1e-9 * vmax_ref
         assert rel_err(grad, grad_ref) <= 1e-7
-        # central FD of the GPU p-norm on 20 seeded elements
-        h = 1e-6
+        # central FD of the GPU p-norm on 20 seeded elements.  At h = 1e-6
+        # FP64 differences of PN carry ~1e-7 absolute noise on the gradient
+        # for this grid even with sparse-direct solves (the oracle gives the
+        # same FD values, e.g. element 567: 2.0638e-4 vs analytic 2.0647e-4,
+        # but 2.06472e-4 at h = 1e-5), hence the absolute floor; h = 1e-5
+        # is checked with a 10x smaller floor.
+        gmax = np.abs(grad).max()
         idx = np.random.default_rng(1).choice(mesh.num_elems, 20, replace=False)
-        for e in idx:
-            vals = []
-            for s in (+1, -1):
-                xx = c.x.copy()
-                xx[e] += s * h
-                fe.solve(xx)
-                fe.postprocess()
-                vals.append(fe.pNormStress)
-            fd = (vals[0] - vals[1]) / (2 * h)
-            # the p-norm is first order in the solve error: with h = 1e-6 the
-            # FD carries ~1e-14*PN/h of solver noise, hence the small absolute term
-            assert abs(grad[e] - fd) <= 1e-5 * abs(grad[e]) + 1e-7 * np.abs(grad).max(), (e, grad[e], fd)
+        for h, floor in ((1e-6, 2e-7), (1e-5, 2e-8)):
+            for e in idx:
+                vals = []
+                for s in (+1, -1):
+                    xx = c.x.copy()
+                    xx[e] += s * h
+                    fe.solve(xx)
+                    fe.postprocess()
+                    vals.append(fe.pNormStress)
+                fd = (vals[0] - vals[1]) / (2 * h)
+                assert abs(grad[e] - fd) <= 1e-5 * abs(grad[e]) + floor * gmax, (h, e, grad[e], fd)
     finally:
         tmm.set_pNorm_exponent(6)
 
