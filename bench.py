@@ -76,6 +76,16 @@ def cpu_baseline(gpu_iters_fp64, ndof_target, budget_s=20.0):
             "est_seconds_per_solve": t_solve}
 
 
+def traffic_bytes():
+    """HBM bytes per k_matvec<float> launch at this workload, from the two
+    committed rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; see the file)."""
+    path = os.path.join(ROOT, "profiles", "k_matvec_f32_cantilever256_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("traffic_bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +131,11 @@ def main():
     for _ in range(args.warmup):
         st = step("mixed")
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.set_density(x)
+    torch.cuda.synchronize()
+    log("set_density (SIMP, Jacobi, Galerkin + Cholesky + inverse): %.2f ms"
+        % ((time.perf_counter() - t0) * 1e3))
     t0 = time.perf_counter()
     stats = []
     for _ in range(args.steps):
@@ -169,6 +184,13 @@ def main():
     t_mv64 = e0.elapsed_time(e1) / reps * 1e-3
     bytes64 = 8 * 2 * nd + 8 * ne
     gbs32 = bytes32 / t_mv32 / 1e9
+    # calibration for the PMC traffic counters: a plain device copy of one
+    # FP32 DOF vector (known bytes) under the same counters
+    cal_src = torch.randn(nd, dtype=torch.float32, device=dev)
+    cal_dst = torch.empty_like(cal_src)
+    for _ in range(3):
+        cal_dst.copy_(cal_src)
+    torch.cuda.synchronize()
     log("matvec f32 %.1f us %.0f GB/s; f64 %.1f us %.0f GB/s" %
         (t_mv32 * 1e6, gbs32, t_mv64 * 1e6, bytes64 / t_mv64 / 1e9))
 
@@ -198,7 +220,7 @@ def main():
         "speedup_vs_fp64": t_fp64 / t_mixed,
         "error_vs_fp64": {"displacement_max_rel": err_u, "compliance_rel": err_C},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": traffic_bytes(),
                      "kernel": "k_matvec<float>", "kernel_us": t_mv32 * 1e6,
                      "algorithmic_bytes": bytes32,
                      "fp64_kernel_us": t_mv64 * 1e6, "fp64_GBs": bytes64 / t_mv64 / 1e9},

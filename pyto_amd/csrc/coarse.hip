@@ -160,39 +160,59 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
 // v = invd o r + beta p (the new search direction before its coarse
 // correction) or v = r when invd is null.  The element arithmetic is FP64
 // whatever the vector precision T.  Group centres are wave-uniform per slot.
+// A workgroup serves one SpecialBlock: it first stages v at the block's
+// sorted corner nodes in LDS (an element's corners would otherwise be 24
+// scattered loads, each wave instruction touching up to 64 cache lines on
+// interfaces normal to x), then each wave applies the element operator of
+// one chunk reading its corners from LDS.
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict_special(
     const T* __restrict__ r, const T* __restrict__ invd, const T* __restrict__ p,
     const double* __restrict__ beta_ptr, const SpecialElem* __restrict__ sinfo,
-    const SpecialChunk* __restrict__ chunks, int nchunks, const double* __restrict__ Ee,
-    const double* __restrict__ centers, GridDims g, ModalConsts<double> mc,
-    double* __restrict__ cpart, const int* __restrict__ done) {
+    const uint4* __restrict__ sidx, const SpecialChunk* __restrict__ chunks,
+    const SpecialBlock* __restrict__ sblocks, const int32_t* __restrict__ sbnodes,
+    const double* __restrict__ Ee, const double* __restrict__ centers, GridDims g,
+    ModalConsts<double> mc, double* __restrict__ cpart, const int* __restrict__ done) {
   if (done && *done) return;
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= nchunks) return;
-  const SpecialChunk ch = chunks[wave];
-  const bool active = lane < ch.count;
+  __shared__ double vs[3 * kSpecialBlockNodes];
+  const SpecialBlock sb = sblocks[xcd_logical_block(blockIdx.x, gridDim.x)];
   const double beta = (p && beta_ptr) ? *beta_ptr : 0.0;
+  // stage v at the block's corner nodes: consecutive lanes read consecutive
+  // DOFs wherever the nodes are consecutive
+  for (int i = threadIdx.x; i < 3 * sb.nnodes; i += 256) {
+    const int q = i / 3;
+    const int64_t d = 3 * (int64_t)sbnodes[sb.node0 + q] + (i - 3 * q);
+    double val = (double)r[d];
+    if (invd) val *= (double)invd[d];
+    if (p) val += beta * (double)p[d];
+    vs[i] = val;
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  if (w >= sb.nch) return;
+  const int wave = sb.chunk0 + w;
+  const int lane = threadIdx.x & 63;
+  // count and ng by value; the group list is read in place (a local copy of
+  // it indexed by the loop variable would live in scratch memory)
+  const int ch_start = chunks[wave].start, ch_count = chunks[wave].count;
+  const int ch_ng = chunks[wave].ng;
+  const bool active = lane < ch_count;
   double f[8][3];
   uint64_t info = 0;
   int ix = 0, iy = 0, iz = 0;
   if (active) {
-    const SpecialElem se = sinfo[ch.start + lane];
+    const SpecialElem se = sinfo[ch_start + lane];
+    const uint4 li = sidx[ch_start + lane];
+    const uint32_t lw[4] = {li.x, li.y, li.z, li.w};
     info = se.info;
     elem_coords(se.e, g, ix, iy, iz);
     double u[8][3];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int64_t n = (ix + corner_a(j)) + (int64_t)g.sx * ((iy + corner_b(j)) +
-                                                              (int64_t)g.sy * (iz + corner_c(j)));
+      const int l = (int)((lw[j >> 1] >> (16 * (j & 1))) & 0xffffu);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        double val = (double)r[3 * n + c];
-        if (invd) val *= (double)invd[3 * n + c];
-        if (p) val += beta * (double)p[3 * n + c];
-        u[j][c] = ((info >> (24 + 3 * j + c)) & 1) ? 0.0 : val;
-      }
+      for (int c = 0; c < 3; ++c)
+        u[j][c] = ((info >> (24 + 3 * j + c)) & 1) ? 0.0 : vs[3 * l + c];
     }
     element_apply<double>(u, Ee[se.e], mc, f);
 #pragma unroll
@@ -204,8 +224,8 @@ __global__ __launch_bounds__(256) void k_restrict_special(
 #pragma unroll
     for (int j = 0; j < 8; ++j) f[j][0] = f[j][1] = f[j][2] = 0.0;
   }
-  for (int sl = 0; sl < ch.ng; ++sl) {
-    const int gs = ch.groups[sl];
+  for (int sl = 0; sl < ch_ng; ++sl) {
+    const int gs = chunks[wave].groups[sl];
     const double cx = centers[3 * gs], cy = centers[3 * gs + 1], cz = centers[3 * gs + 2];
     double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -577,12 +597,10 @@ int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
 template <typename T>
 int launch_restrict_special(Handle* h, const T* r, const T* invd, const T* p,
                             const double* beta, double* t, const int* done, hipStream_t s) {
-  if (h->nchunks > 0) {
-    const int blocks = (h->nchunks * 64 + 255) / 256;
-    hipLaunchKernelGGL(k_restrict_special<T>, dim3(blocks), dim3(256), 0, s, r, invd, p, beta,
-                       h->sinfo, h->chunks, h->nchunks, h->Ee64, h->centers, h->g, h->mc64,
-                       h->cpart, done);
-  }
+  if (h->nsblocks > 0)
+    hipLaunchKernelGGL(k_restrict_special<T>, dim3(h->nsblocks), dim3(256), 0, s, r, invd, p,
+                       beta, h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes, h->Ee64,
+                       h->centers, h->g, h->mc64, h->cpart, done);
   hipLaunchKernelGGL(k_coarse_gather, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
                      h->cpart, h->gptr, h->gidx, h->ngroups, t, done);
   PYTO_CHECK_HIP(hipGetLastError());
@@ -737,7 +755,8 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
       gc.exec = nullptr;
     }
   // release a previous coarse space
-  void* olds[] = {h->node_group, h->centers, h->special, h->sinfo, h->chunks, h->cpart, h->gptr, h->gidx,
+  void* olds[] = {h->node_group, h->centers, h->special, h->sinfo, h->chunks, h->cpart,
+                  h->sblocks, h->sbnodes, h->sidx, h->gptr, h->gidx,
                   h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
                   h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres};
   for (void* p : olds)
@@ -745,6 +764,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   h->node_group = nullptr; h->centers = nullptr; h->special = nullptr; h->chunks = nullptr;
   h->sinfo = nullptr;
   h->cpart = nullptr; h->gptr = nullptr; h->gidx = nullptr;
+  h->sblocks = nullptr; h->sbnodes = nullptr; h->sidx = nullptr; h->nsblocks = 0;
   h->gnodes = nullptr; h->nchunk = nullptr; h->nchunk_grp = nullptr; h->npart = nullptr;
   h->nptr = nullptr; h->nidx = nullptr; h->nnchunks = 0;
   h->Ec = h->Einv = h->cwork = h->ct = h->cmu = h->cres = nullptr;
@@ -817,6 +837,54 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     sinfo[i].pad = 0;
     sinfo[i].info = info;
   }
+  // LDS staging blocks: up to kSpecialBlockChunks consecutive chunks whose
+  // sorted unique corner nodes fit kSpecialBlockNodes, and per element the
+  // block-local index of each corner
+  std::vector<SpecialBlock> sblocks;
+  std::vector<int32_t> sbnodes;
+  std::vector<uint16_t> sidx(list.size() * 8);
+  {
+    auto corner_node = [&](int32_t e, int j) {
+      const int ix = (int)(e % g.nx), iy = (int)((e / g.nx) % g.ny);
+      const int iz = (int)(e / ((int64_t)g.nx * g.ny));
+      return (int32_t)((ix + corner_a(j)) + (int64_t)g.sx * ((iy + corner_b(j)) +
+                                                           (int64_t)g.sy * (iz + corner_c(j))));
+    };
+    auto chunk_nodes = [&](const SpecialChunk& ch, std::vector<int32_t>& out) {
+      for (int i = ch.start; i < ch.start + ch.count; ++i)
+        for (int j = 0; j < 8; ++j) out.push_back(corner_node(list[i].second, j));
+    };
+    std::vector<int32_t> cur, trial;
+    size_t c = 0;
+    while (c < chunks.size()) {
+      SpecialBlock sb;
+      sb.chunk0 = (int)c;
+      sb.nch = 0;
+      cur.clear();
+      while (c < chunks.size() && sb.nch < kSpecialBlockChunks) {
+        trial = cur;
+        chunk_nodes(chunks[c], trial);
+        std::sort(trial.begin(), trial.end());
+        trial.erase(std::unique(trial.begin(), trial.end()), trial.end());
+        if (sb.nch > 0 && (int)trial.size() > kSpecialBlockNodes) break;
+        cur.swap(trial);
+        ++sb.nch;
+        ++c;
+      }
+      sb.node0 = (int)sbnodes.size();
+      sb.nnodes = (int)cur.size();
+      for (int q = 0; q < sb.nch; ++q) {
+        const SpecialChunk& ch = chunks[sb.chunk0 + q];
+        for (int i = ch.start; i < ch.start + ch.count; ++i)
+          for (int j = 0; j < 8; ++j)
+            sidx[(size_t)i * 8 + j] = (uint16_t)(
+                std::lower_bound(cur.begin(), cur.end(), corner_node(list[i].second, j)) -
+                cur.begin());
+      }
+      sbnodes.insert(sbnodes.end(), cur.begin(), cur.end());
+      sblocks.push_back(sb);
+    }
+  }
   // per group: the (chunk, slot) partials that belong to it
   std::vector<int32_t> gptr(ngroups + 1, 0), gidx;
   {
@@ -862,6 +930,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   h->nnchunks = (int)nchunk.size();
   h->nspecial = (int)special.size();
   h->nchunks = (int)chunks.size();
+  h->nsblocks = (int)sblocks.size();
   const int ld = round_up64(h->nc);
   PYTO_TRY(device_alloc(h, (void**)&h->node_group, nn * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->centers, (size_t)ngroups * 3 * 8));
@@ -869,6 +938,9 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_TRY(device_alloc(h, (void**)&h->sinfo, sinfo.size() * sizeof(SpecialElem)));
   PYTO_TRY(device_alloc(h, (void**)&h->chunks, chunks.size() * sizeof(SpecialChunk)));
   PYTO_TRY(device_alloc(h, (void**)&h->cpart, chunks.size() * 48 * sizeof(double)));
+  PYTO_TRY(device_alloc(h, (void**)&h->sblocks, std::max<size_t>(sblocks.size(), 1) * sizeof(SpecialBlock)));
+  PYTO_TRY(device_alloc(h, (void**)&h->sbnodes, std::max<size_t>(sbnodes.size(), 1) * sizeof(int32_t)));
+  PYTO_TRY(device_alloc(h, (void**)&h->sidx, std::max<size_t>(sidx.size(), 8) * sizeof(uint16_t)));
   PYTO_TRY(device_alloc(h, (void**)&h->gnodes, nn * sizeof(int32_t)));
   PYTO_TRY(device_alloc(h, (void**)&h->nchunk, nchunk.size() * sizeof(int2)));
   PYTO_TRY(device_alloc(h, (void**)&h->nchunk_grp, nchunk.size() * sizeof(int32_t)));
@@ -891,6 +963,12 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     PYTO_CHECK_HIP(hipMemcpyAsync(h->sinfo, sinfo.data(), sinfo.size() * sizeof(SpecialElem),
                                   hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->chunks, chunks.data(), chunks.size() * sizeof(SpecialChunk),
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->sblocks, sblocks.data(), sblocks.size() * sizeof(SpecialBlock),
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->sbnodes, sbnodes.data(), sbnodes.size() * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->sidx, sidx.data(), sidx.size() * sizeof(uint16_t),
                                   hipMemcpyHostToDevice, s));
   }
   PYTO_CHECK_HIP(hipMemcpyAsync(h->gnodes, gnodes.data(), nn * sizeof(int32_t),

@@ -83,6 +83,18 @@ struct SpecialChunk {
   int groups[8];  // the groups (sorted), -1 padded
 };
 
+// A workgroup of the special-element restriction: up to kSpecialBlockChunks
+// consecutive chunks whose corner nodes (sorted, unique, at most
+// kSpecialBlockNodes) are staged together in LDS.
+struct SpecialBlock {
+  int chunk0;     // first chunk
+  int nch;        // chunks in the block
+  int node0;      // offset into the block node list
+  int nnodes;     // unique corner nodes of the block
+};
+constexpr int kSpecialBlockChunks = 4;
+constexpr int kSpecialBlockNodes = 1024;
+
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   const void* x = nullptr;
@@ -99,6 +111,7 @@ struct Handle {
 
   // constants / per-density state
   uint8_t* fixmask = nullptr;   // per node
+  uint8_t* tileflags = nullptr; // per (matvec tile, node plane): any fixed node in its window
   double* Ee64 = nullptr;       // per element E_e/E0 (SIMP)
   float* Ee32 = nullptr;
   double* x64 = nullptr;        // copy of densities
@@ -115,6 +128,10 @@ struct Handle {
   SpecialElem* sinfo = nullptr; // same order, with packed slots and fixed bits
   SpecialChunk* chunks = nullptr;
   double* cpart = nullptr;      // per-chunk partial W^T K v (nchunks x 8 x 6)
+  int nsblocks = 0;
+  SpecialBlock* sblocks = nullptr;
+  int32_t* sbnodes = nullptr;   // per block: its sorted unique corner nodes
+  uint4* sidx = nullptr;        // per special element: 8 block-local corner indices (u16)
   int32_t* gptr = nullptr;      // per group: range of (chunk*8+slot) in gidx
   int32_t* gidx = nullptr;
   // nodes sorted by group, cut into blocks of <= 256 nodes of one group
@@ -136,8 +153,8 @@ struct Handle {
   double shift_rel = 0.0;
 
   // solver workspace (allocated on first use)
-  double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;  // z64/z32 unused
-  float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr,
+  double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr;
+  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr,
         *b32 = nullptr;
   double* partials = nullptr;   // block partial sums: 2 x kMaxPartials (p.q, then r.Mr)
   unsigned* counters = nullptr; // last-block counters (kNumCounters)
@@ -175,6 +192,17 @@ int device_alloc(Handle* h, void** p, size_t bytes);
 // ---------------------------------------------------------------------------
 // Device helpers
 // ---------------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b mod 8), and each XCD has its own L2.  Map hardware block b to a logical
+// block so that every XCD receives one contiguous range of logical blocks:
+// neighbouring tiles then share halo cache lines in one L2 instead of
+// fetching them once per XCD.  A bijection on [0, nb) for any nb.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_logical_block(int b, int nb) {
+  const int q = nb / kNumXcd, rem = nb % kNumXcd;
+  const int x = b % kNumXcd, i = b / kNumXcd;
+  return x * q + min(x, rem) + i;
+}
 __device__ __forceinline__ double wave_reduce_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -114,8 +114,10 @@ struct MatvecArgs {
   const uint8_t* mask;
   GridDims g;
   ModalConsts<T> mc;
+  const uint8_t* tileflags;  // per (tile, node plane): window holds a fixed node
   int zchunk;
   int tiles_x;
+  int ntiles;
   double* partials;    // per-block partials of u.y (null: no dot)
   const int* done;     // skip when *done != 0 (null: never)
   double* zero_vec;    // zeroed by block 0 (null: none)
@@ -132,20 +134,26 @@ __device__ __forceinline__ double dpp_shr1(double v) {
   return __hiloint2double(hi, lo);
 }
 
-// One node of the 17x17 staging plane: (masked) values from global memory.
+// Issue the loads of one node of the 17x17 staging plane without consuming
+// them: raw values in a[], the node's fixed bits (pending load) in ml, and
+// 7 in mo when the node is outside the grid (the address is then clamped
+// into the grid).  The masking happens when the plane is stored to LDS, so
+// the loads stay in flight across a whole layer of element work.
 template <typename T>
-__device__ __forceinline__ void fetch_node(T v[3], const T* __restrict__ u,
+__device__ __forceinline__ void issue_node(T a[3], int& mo, int& ml, const T* __restrict__ u,
                                            const uint8_t* __restrict__ mask, const GridDims& g,
-                                           int gx, int gy, int kp) {
-  v[0] = v[1] = v[2] = T(0);
-  if (kp >= 0 && kp < g.sz && gx >= 0 && gx < g.sx && gy >= 0 && gy < g.sy) {
-    const int64_t n = gx + (int64_t)g.sx * (gy + (int64_t)g.sy * kp);
-    const uint8_t m = mask[n];
-    const T* p = u + 3 * n;
-    v[0] = (m & 1) ? T(0) : p[0];
-    v[1] = (m & 2) ? T(0) : p[1];
-    v[2] = (m & 4) ? T(0) : p[2];
-  }
+                                           int gx, int gy, int kp, bool any_fixed) {
+  const bool inb = kp >= 0 && kp < g.sz && gx >= 0 && gx < g.sx && gy >= 0 && gy < g.sy;
+  const int cx = min(max(gx, 0), g.sx - 1), cy = min(max(gy, 0), g.sy - 1);
+  const int cz = min(max(kp, 0), g.sz - 1);
+  const int64_t n = cx + (int64_t)g.sx * (cy + (int64_t)g.sy * cz);
+  const T* p = u + 3 * n;
+  a[0] = p[0];
+  a[1] = p[1];
+  a[2] = p[2];
+  mo = inb ? 0 : 7;
+  ml = 0;
+  if (any_fixed) ml = mask[n];
 }
 
 // Layer loop, per element layer k (nodes k and k+1):
@@ -160,18 +168,32 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
   if (a.done && *a.done) return;
   __shared__ T pl[2][kPl][kPl][3];
   __shared__ T xc[2][kTile][kTile][6];
+  __shared__ T es[2][kTile][kTile];   // E of the element layer below plane b
+  __shared__ uint8_t pm[2][kPl][kPl];  // fixed bits of the staged nodes
   __shared__ double red[4];
   const GridDims& g = a.g;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int bxi = blockIdx.x % a.tiles_x, byi = blockIdx.x / a.tiles_x;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tile = lb % a.ntiles;
+  const int bxi = tile % a.tiles_x, byi = tile / a.tiles_x;
   const int x0 = bxi * kOwn - 1, y0 = byi * kOwn - 1;
   const int ex = x0 + tx, ey = y0 + ty;
-  const int kz0 = blockIdx.y * a.zchunk;
+  const int kz0 = (lb / a.ntiles) * a.zchunk;
   const int kz1 = min(kz0 + a.zchunk, g.sz);
+  // fixed-node flags of the chunk's node planes kz0-1 .. kz1 (<= 64 planes,
+  // zchunk <= 62) as one wave-uniform bit mask: no memory access in the loop
+  uint64_t pfix;
+  {
+    const int kp = kz0 - 1 + (int)(threadIdx.x & 63);
+    const bool f = kp >= 0 && kp <= kz1 && kp < g.sz &&
+                   a.tileflags[(int64_t)tile * g.sz + kp] != 0;
+    pfix = __ballot(f);
+  }
+  auto plane_fixed = [&](int kp) { return ((pfix >> (kp - kz0 + 1)) & 1) != 0; };
   const bool col_valid = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny;
   const bool owner = tx >= 1 && ty >= 1 && ex < g.sx && ey < g.sy;
 
-  if (a.zero_vec && blockIdx.x == 0 && blockIdx.y == 0)
+  if (a.zero_vec && blockIdx.x == 0)
     for (int i = threadIdx.x; i < a.zero_n; i += 256) a.zero_vec[i] = 0.0;
 
   // staging-plane entries of this thread: e0 = tid, e1 = tid + 256 (< 289)
@@ -180,22 +202,37 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
   const int lx0 = e0 % kPl, ly0 = e0 / kPl;
   const int lx1 = has1 ? e1 % kPl : 0, ly1 = has1 ? e1 / kPl : 0;
   T pv0[3], pv1[3] = {T(0), T(0), T(0)};
+  int mo0 = 0, ml0 = 0, mo1 = 7, ml1 = 0;
+  // E of element layer kp-1, loaded with node plane kp and staged with it
+  const int64_t ecol = min(max(ex, 0), g.nx - 1) + (int64_t)g.nx * min(max(ey, 0), g.ny - 1);
+  T enext = T(0);
+  // fixed bits of this thread's own node, read from the staged plane when
+  // it becomes the top plane: mq byte 0 = newest plane
+  uint32_t mq = 0;
   auto fetch = [&](int kp) {
-    fetch_node<T>(pv0, a.u, a.mask, g, x0 + lx0, y0 + ly0, kp);
-    if (has1) fetch_node<T>(pv1, a.u, a.mask, g, x0 + lx1, y0 + ly1, kp);
+    const bool fx = plane_fixed(kp);
+    issue_node<T>(pv0, mo0, ml0, a.u, a.mask, g, x0 + lx0, y0 + ly0, kp, fx);
+    if (has1) issue_node<T>(pv1, mo1, ml1, a.u, a.mask, g, x0 + lx1, y0 + ly1, kp, fx);
+    const int kl = min(max(kp - 1, 0), g.nz - 1);
+    enext = a.Ee[ecol + (int64_t)g.nx * g.ny * kl];
   };
   auto store = [&](int b) {
+    const int m0 = mo0 | ml0, m1 = mo1 | ml1;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) pl[b][ly0][lx0][c] = pv0[c];
-    if (has1)
+    for (int c = 0; c < 3; ++c) pl[b][ly0][lx0][c] = ((m0 >> c) & 1) ? T(0) : pv0[c];
+    pm[b][ly0][lx0] = (uint8_t)m0;
+    if (has1) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) pl[b][ly1][lx1][c] = pv1[c];
+      for (int c = 0; c < 3; ++c) pl[b][ly1][lx1][c] = ((m1 >> c) & 1) ? T(0) : pv1[c];
+      pm[b][ly1][lx1] = (uint8_t)m1;
+    }
+    es[b][ty][tx] = enext;
   };
 
   T bot[4][3], top[4][3];
   fetch(kz0 - 1);
   store(0);
-  fetch(kz0);
+  fetch(kz0);   // also E of layer kz0-1
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -204,6 +241,7 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
     bot[2][c] = pl[0][ty + 1][tx + 1][c];
     bot[3][c] = pl[0][ty + 1][tx][c];
   }
+  mq = pm[0][ty][tx];
   T carry[3] = {T(0), T(0), T(0)};   // top-plane sum of the layer before last
   T xb[3], xt[3], yb[3], yt[3];       // pending sums of the last layer
   T ubot[3];                          // u at this thread's node of the pending plane
@@ -213,10 +251,10 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
   for (int k = kz0 - 1; k <= kz1; ++k) {
     const int b = (k - kz0 + 2) & 1;   // LDS plane buffer of node plane k+1
     const int xb_i = k & 1;            // y-exchange buffer of layer k-1
-    if (k < kz1) {
-      store(b);
-      if (k + 2 <= kz1) fetch(k + 2);
-    }
+    if (k < kz1) store(b);
+    // unconditional (clamped to plane kz1): a plane past kz1 is never stored,
+    // and no load result is carried through a conditional path
+    fetch(min(k + 2, kz1));
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       xc[xb_i][ty][tx][c] = yb[c];
@@ -224,24 +262,28 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
     }
     __syncthreads();
     // finish node plane k-1 (bottom of layer k-1 + top of layer k-2)
-    if (k - 1 >= kz0 && owner) {
-      const int kp = k - 1;
-      const int64_t nw = ex + (int64_t)g.sx * (ey + (int64_t)g.sy * kp);
-      const uint8_t m = a.mask[nw];
-      T* yp = a.y + 3 * nw;
-      T nt[3];
+    if (owner) {
+      // all six exchange values are read unconditionally (branch-free)
+      T sb[3], st[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T v = ((m >> c) & 1) ? T(0) : carry[c] + xb[c] + xc[xb_i][ty - 1][tx][c];
-        yp[c] = v;
-        dot += (double)v * (double)ubot[c];
-        nt[c] = xt[c] + xc[xb_i][ty - 1][tx][3 + c];
+        sb[c] = carry[c] + xb[c] + xc[xb_i][ty - 1][tx][c];
+        st[c] = xt[c] + xc[xb_i][ty - 1][tx][3 + c];
+      }
+      if (k - 1 >= kz0) {
+        const int kp = k - 1;
+        const int64_t nw = ex + (int64_t)g.sx * (ey + (int64_t)g.sy * kp);
+        const uint32_t m = (mq >> 8) & 0xffu;   // plane k-1 (byte 0: plane k)
+        T* yp = a.y + 3 * nw;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const T v = ((m >> c) & 1u) ? T(0) : sb[c];
+          yp[c] = v;
+          dot += (double)v * (double)ubot[c];
+        }
       }
 #pragma unroll
-      for (int c = 0; c < 3; ++c) carry[c] = nt[c];
-    } else if (owner) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) carry[c] = xt[c] + xc[xb_i][ty - 1][tx][3 + c];
+      for (int c = 0; c < 3; ++c) carry[c] = st[c];
     }
     if (k >= kz1) break;
 #pragma unroll
@@ -252,6 +294,7 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
       top[2][c] = pl[b][ty + 1][tx + 1][c];
       top[3][c] = pl[b][ty + 1][tx][c];
     }
+    mq = (mq << 8) | (uint32_t)pm[b][ty][tx];
     T f[8][3];
     if (col_valid && k >= 0 && k < g.nz) {
       T ue[8][3];
@@ -263,7 +306,7 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
           ue[4 + j][c] = top[j][c];
         }
       }
-      const T E = a.Ee[ex + (int64_t)g.nx * (ey + (int64_t)g.ny * k)];
+      const T E = es[b][ty][tx];   // E of layer k, staged with plane k+1
       if (CUBE)
         element_apply_cube<T>(ue, E, a.mc, f);
       else
@@ -290,8 +333,32 @@ __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
   if (a.partials) {
     // block partial of u.y; the consumer kernel sums the partials
     const double bs = block_reduce_sum256(dot, red);
-    if (threadIdx.x == 0) a.partials[blockIdx.x + gridDim.x * blockIdx.y] = bs;
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
   }
+}
+
+// tileflags[tile * sz + kp] = 1 when node plane kp of the tile's 17x17
+// staging window holds a node with a fixed DOF.  Tile bxi stages nodes
+// x in [15 bxi - 1, 15 bxi + 15] (same in y).  Racing stores write the same
+// byte value.
+__global__ __launch_bounds__(256) void k_tile_flags(const uint8_t* __restrict__ mask, GridDims g,
+                                                    int tiles_x, int tiles_y,
+                                                    uint8_t* __restrict__ flags) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= g.nnodes || mask[n] == 0) return;
+  const int gx = (int)(n % g.sx), gy = (int)((n / g.sx) % g.sy);
+  const int kp = (int)(n / ((int64_t)g.sx * g.sy));
+  const int bx0 = max(0, (gx + kOwn - 1) / kOwn - 1), bx1 = min(tiles_x - 1, (gx + 1) / kOwn);
+  const int by0 = max(0, (gy + kOwn - 1) / kOwn - 1), by1 = min(tiles_y - 1, (gy + 1) / kOwn);
+  for (int by = by0; by <= by1; ++by)
+    for (int bx = bx0; bx <= bx1; ++bx) flags[((int64_t)by * tiles_x + bx) * g.sz + kp] = 1;
+}
+
+static hipError_t launch_tile_flags(Handle* h, hipStream_t s) {
+  const int tx = (h->g.sx + kOwn - 1) / kOwn, ty = (h->g.sy + kOwn - 1) / kOwn;
+  hipLaunchKernelGGL(k_tile_flags, dim3((unsigned)((h->g.nnodes + 255) / 256)), dim3(256), 0, s,
+                     h->fixmask, h->g, tx, ty, h->tileflags);
+  return hipGetLastError();
 }
 
 // Resident workgroups of k_matvec<T, CUBE> on the whole device.
@@ -313,7 +380,8 @@ static int matvec_capacity() {
 // z-chunk length: minimise (workgroup rounds) x (layers per workgroup,
 // including the halo layer) for the occupancy of this kernel variant.
 template <typename T>
-static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& tiles_x) {
+static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& tiles_x,
+                               int& ntiles) {
   const GridDims& g = h->g;
   tiles_x = (g.sx + kOwn - 1) / kOwn;
   const int tiles_y = (g.sy + kOwn - 1) / kOwn;
@@ -322,7 +390,7 @@ static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& ti
   const int cap = cube ? matvec_capacity<T, true>() : matvec_capacity<T, false>();
   int best = 16;
   double best_cost = 1e300;
-  for (int zc = 4; zc <= 64; ++zc) {
+  for (int zc = 4; zc <= 62; ++zc) {   // planes kz0-1 .. kz1 fit one 64-bit mask
     const int64_t blocks = tiles * ((g.sz + zc - 1) / zc);
     if (blocks > kMaxPartials) continue;
     const int64_t rounds = (blocks + cap - 1) / cap;
@@ -333,15 +401,16 @@ static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& ti
     }
   }
   zchunk = best;
-  grid = dim3((unsigned)tiles, (g.sz + zchunk - 1) / zchunk, 1);
+  ntiles = (int)tiles;
+  grid = dim3((unsigned)(tiles * ((g.sz + zchunk - 1) / zchunk)), 1, 1);
 }
 
 template <typename T>
 int matvec_blocks(const Handle* h) {
   dim3 grid;
-  int zc, tx;
-  matvec_launch_dims<T>(h, grid, zc, tx);
-  return (int)(grid.x * grid.y);
+  int zc, tx, nt;
+  matvec_launch_dims<T>(h, grid, zc, tx, nt);
+  return (int)grid.x;
 }
 template int matvec_blocks<double>(const Handle*);
 template int matvec_blocks<float>(const Handle*);
@@ -355,10 +424,11 @@ int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalCon
   a.y = y;
   a.Ee = Ee;
   a.mask = h->fixmask;
+  a.tileflags = h->tileflags;
   a.g = h->g;
   a.mc = mc;
   dim3 grid;
-  matvec_launch_dims<T>(h, grid, a.zchunk, a.tiles_x);
+  matvec_launch_dims<T>(h, grid, a.zchunk, a.tiles_x, a.ntiles);
   a.partials = dot_out;
   a.done = done;
   a.zero_vec = zero_vec;
@@ -453,7 +523,9 @@ int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, doub
   const int64_t ne = h->g.nelems;
   int rc = PYTO_OK;
   auto fail = [&](int code) { pyto_destroy(h); return code; };
+  const int64_t ntiles = (int64_t)((h->g.sx + kOwn - 1) / kOwn) * ((h->g.sy + kOwn - 1) / kOwn);
   if ((rc = device_alloc(h, (void**)&h->fixmask, nn)) ||
+      (rc = device_alloc(h, (void**)&h->tileflags, ntiles * h->g.sz)) ||
       (rc = device_alloc(h, (void**)&h->Ee64, ne * 8)) ||
       (rc = device_alloc(h, (void**)&h->Ee32, ne * 4)) ||
       (rc = device_alloc(h, (void**)&h->x64, ne * 8)) ||
@@ -472,6 +544,8 @@ int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, doub
   }
   hipStream_t s = as_stream(stream);
   if (hipMemcpyAsync(h->fixmask, fixed_mask, nn, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipMemsetAsync(h->tileflags, 0, ntiles * h->g.sz, s) != hipSuccess ||
+      launch_tile_flags(h, s) != hipSuccess ||
       hipMemsetAsync(h->counters, 0, kNumCounters * sizeof(unsigned), s) != hipSuccess ||
       hipMemsetAsync(h->scal, 0, kNumScalars * 8, s) != hipSuccess ||
       hipMemsetAsync(h->flags, 0, kNumFlags * sizeof(int), s) != hipSuccess ||
@@ -485,11 +559,12 @@ int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, doub
 
 int pyto_destroy(pyto_handle* h) {
   if (!h) return PYTO_OK;
-  void* ptrs[] = {h->fixmask, h->Ee64, h->Ee32, h->x64, h->invd64, h->invd32, h->node_group,
-                  h->centers, h->special, h->sinfo, h->chunks, h->cpart, h->gptr, h->gidx, h->gnodes, h->nchunk,
+  void* ptrs[] = {h->fixmask, h->tileflags, h->Ee64, h->Ee32, h->x64, h->invd64, h->invd32,
+                  h->node_group, h->centers, h->special, h->sinfo, h->chunks, h->cpart,
+                  h->sblocks, h->sbnodes, h->sidx, h->gptr, h->gidx, h->gnodes, h->nchunk,
                   h->nchunk_grp, h->npart, h->nptr, h->nidx, h->Ec, h->Einv, h->cwork, h->ct,
-                  h->cmu, h->cres, h->r64, h->z64, h->p64, h->q64, h->r32, h->z32, h->p32,
-                  h->q32, h->x32, h->b32, h->partials, h->counters, h->scal, h->flags, h->elem_buf};
+                  h->cmu, h->cres, h->r64, h->p64, h->q64, h->r32, h->p32, h->q32, h->x32,
+                  h->b32, h->partials, h->counters, h->scal, h->flags, h->elem_buf};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& gc : h->graphs)
