@@ -37,6 +37,7 @@ extern "C" {
 #define PYTO_ENOMEM -3     /* device allocation failed */
 #define PYTO_ESTATE -4     /* call out of order (e.g. solve before density) */
 #define PYTO_ENOCONV -5    /* solver stopped at maxIters (result still written) */
+#define PYTO_ENUMERIC -6   /* coarse matrix W^T K W not positive definite */
 
 typedef struct pyto_handle pyto_handle;
 

@@ -19,6 +19,7 @@ PYTO_EHIP = -2
 PYTO_ENOMEM = -3
 PYTO_ESTATE = -4
 PYTO_ENOCONV = -5
+PYTO_ENUMERIC = -6
 
 
 class ConvergenceWarning(UserWarning):
@@ -98,6 +99,9 @@ def check(rc, what=""):
     if rc == PYTO_ENOCONV:
         warnings.warn(msg, ConvergenceWarning, stacklevel=3)
         return rc
+    if rc == PYTO_ENUMERIC:
+        import numpy.linalg
+        raise numpy.linalg.LinAlgError(msg)
     if rc == PYTO_ESTATE:
         raise RuntimeError("invalid call order: " + msg)
     if rc == PYTO_ENOMEM:

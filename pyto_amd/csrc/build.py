@@ -17,6 +17,7 @@ LIB = os.path.join(PKG, "libpytohip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PYTO_ARCH", "gfx950")
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 
 FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
@@ -47,7 +48,7 @@ def build(verbose=False):
     objs = [os.path.join(HERE, "_build", s.replace(".hip", ".o")) for s in SOURCES]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-Wl,--no-undefined", *objs,
-           "-o", tmp]
+           "-L" + ROCM_LIB, "-Wl,-rpath," + ROCM_LIB, "-lrocsolver", "-lrocblas", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stdout + r.stderr))

@@ -1,7 +1,7 @@
 // Deflation coarse space on the GPU: implicit rigid-body W, restriction and
-// prolongation, the Galerkin matrix sym(W^T K W) + 1e-10 I assembled from the
-// elements that are not rigid under W, and its FP64 Cholesky factor and
-// inverse (blocked, 64x64 tiles).
+// prolongation, the Galerkin matrix E = sym(W^T K W) + shift assembled from
+// the elements that are not rigid under W, and E^{-1} (rocSOLVER Cholesky and
+// inverse) applied as a dense FP64 GEMV inside the iteration.
 //
 // W (src/deflation.py:386-458): node n of group g contributes rows 3n+c and
 // columns 6g..6g+5 with the rigid-body modes about the group centre, column
@@ -11,12 +11,16 @@
 // "Special" elements: an element whose 8 nodes are in one group and carry no
 // fixed DOF sees W as a rigid motion, so KE * (W restricted to it) = 0.  Only
 // the other elements contribute to W^T K v and to W^T K W; they are listed
-// once per set of groups, sorted by that set, and processed one wave per
-// chunk of up to 64 elements with a wave reduction before the FP64 atomics.
+// once per set of groups, sorted by that set, cut into chunks of up to 64
+// elements (one wave each), and the per-chunk partial sums are gathered per
+// group in a fixed order (no atomics in the iteration).
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <vector>
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include "element.hpp"
 
@@ -422,110 +426,6 @@ __global__ void k_symmetrize(double* __restrict__ E, int nc, int ld, double sa, 
 // ---------------------------------------------------------------------------
 // Dense FP64 blocks (n multiple of 64, row-major, leading dimension ld).
 // ---------------------------------------------------------------------------
-constexpr int NB = 64;
-
-// In-place Cholesky of the 64x64 diagonal block at (k0,k0), lower; also writes
-// its triangular inverse to Dinv (64x64).  flag[0] set on a non-positive pivot.
-__global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, int ld, int k0,
-                                                    double* __restrict__ Dinv,
-                                                    int* __restrict__ flag) {
-  __shared__ double L[NB][NB + 1];
-  __shared__ double X[NB][NB + 1];
-  for (int e = threadIdx.x; e < NB * NB; e += 256) {
-    const int i = e / NB, j = e % NB;
-    L[i][j] = (j <= i) ? A[(int64_t)(k0 + i) * ld + (k0 + j)] : 0.0;
-  }
-  __syncthreads();
-  for (int j = 0; j < NB; ++j) {
-    if (threadIdx.x == 0) {
-      double d = L[j][j];
-      if (!(d > 0.0)) {
-        flag[0] = 1;
-        d = 1e-300;
-      }
-      L[j][j] = sqrt(d);
-    }
-    __syncthreads();
-    const double ljj = L[j][j];
-    for (int i = j + 1 + threadIdx.x; i < NB; i += 256) L[i][j] /= ljj;
-    __syncthreads();
-    for (int e = threadIdx.x; e < NB * NB; e += 256) {
-      const int i = e / NB, l = e % NB;
-      if (i > j && l > j && l <= i) L[i][l] -= L[i][j] * L[l][j];
-    }
-    __syncthreads();
-  }
-  // triangular inverse X = L^{-1}: column c solved by thread c (forward sub)
-  if (threadIdx.x < NB) {
-    const int c = threadIdx.x;
-    for (int i = 0; i < NB; ++i) {
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int l = c; l < i; ++l) s -= L[i][l] * X[l][c];
-      X[i][c] = (i < c) ? 0.0 : s / L[i][i];
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < NB * NB; e += 256) {
-    const int i = e / NB, j = e % NB;
-    if (j <= i) A[(int64_t)(k0 + i) * ld + (k0 + j)] = L[i][j];
-    Dinv[(int64_t)i * NB + j] = X[i][j];
-  }
-}
-
-// C (MxN) = alpha * op(A) op(B) + beta * C, all dims multiples of 64.
-// op(A)[i][k] = TA ? A[k*lda+i] : A[i*lda+k]; likewise B.  LOWER: only tiles
-// with tile_row >= tile_col.
-template <bool TA, bool TB, bool LOWER>
-__global__ __launch_bounds__(256) void k_gemm64(int M, int N, int K, double alpha,
-                                                const double* __restrict__ A, int lda,
-                                                const double* __restrict__ B, int ldb,
-                                                double beta, double* __restrict__ C, int ldc) {
-  const int tr = blockIdx.y, tc = blockIdx.x;
-  if (LOWER && tr < tc) return;
-  __shared__ double As[16][NB + 1];
-  __shared__ double Bs[16][NB + 1];
-  const int ti = threadIdx.x / 16, tj = threadIdx.x % 16;
-  double acc[4][4] = {};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    for (int e = threadIdx.x; e < 16 * NB; e += 256) {
-      const int kk = e / NB, ii = e % NB;
-      const int gi = tr * NB + ii, gk = k0 + kk;
-      As[kk][ii] = TA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
-      const int gj = tc * NB + ii;
-      Bs[kk][ii] = TB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      double a[4], b[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[kk][ti + 16 * r];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) b[q] = Bs[kk][tj + 16 * q];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[r][q] += a[r] * b[q];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int gi = tr * NB + ti + 16 * r, gj = tc * NB + tj + 16 * q;
-      double* cp = C + (int64_t)gi * ldc + gj;
-      *cp = alpha * acc[r][q] + (beta == 0.0 ? 0.0 : beta * *cp);
-    }
-}
-
-// Y block row i: Y_i = Dinv_i * R where R holds I_i - L_{i,<i} Y_{<i}.
-__global__ void k_set_identity_rows(double* __restrict__ R, int ld, int row0, int ncols) {
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < NB * ncols; e += gridDim.x * blockDim.x) {
-    const int i = e / ncols, j = e % ncols;
-    R[(int64_t)i * ld + j] = (row0 + i == j) ? 1.0 : 0.0;
-  }
-}
 
 // y = b + sign * A (x1 + beta x2) (one wave per row; x2/beta optional).
 __global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ A, int ld, int n,
@@ -575,9 +475,9 @@ int coarse_apply(Handle* h, const double* t1, const double* t2, const double* be
                        (const double*)nullptr, (const double*)nullptr, t1, -1.0, h->cres, done);
     hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Einv, ld, n, h->cres,
                        (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                       1.0, h->cwork + (int64_t)ld * ld, done);
-    hipLaunchKernelGGL(k_axpy_small, dim3((n + 255) / 256), dim3(256), 0, s, mu,
-                       h->cwork + (int64_t)ld * ld, n, done);
+                       1.0, h->cwork, done);
+    hipLaunchKernelGGL(k_axpy_small, dim3((n + 255) / 256), dim3(256), 0, s, mu, h->cwork, n,
+                       done);
   }
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
@@ -666,53 +566,48 @@ template int launch_prolong<float, true>(Handle*, float*, const float*, const do
                                          const double*, bool, const int*, hipStream_t,
                                          double*, int);
 
-// Cholesky factor of Ec into cwork (lower), then Einv = L^{-T} L^{-1}.
+// Full symmetric matrix from the column-major lower triangle that the
+// factorisation leaves (the row-major upper triangle): A[i][j] = A[j][i], i > j.
+__global__ void k_fill_symmetric(double* __restrict__ A, int n, int ld) {
+  const int i = blockIdx.y;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < i; j += gridDim.x * blockDim.x)
+    A[(int64_t)i * ld + j] = A[(int64_t)j * ld + i];
+}
+
+// Einv = Ec^{-1} by rocSOLVER: Cholesky (dpotrf) then the inverse from the
+// factor (dpotri), both in place on a copy of Ec, then the mirror fill.  Ec
+// is symmetric, so its row-major storage is read as column-major.
 static int factorize_coarse(Handle* h, hipStream_t s) {
-  const int ld = round_up64(h->nc);
-  const int nb = ld / NB;
-  double* L = h->cwork;                       // ld x ld factor
-  double* Y = h->Einv;                        // L^{-1} then overwritten
-  double* Dinv = h->cwork + 2 * (int64_t)ld * ld;   // nb blocks of 64x64
-  double* Tmp = h->cwork + (int64_t)ld * ld;        // ld x ld scratch (first ld*64 used)
-  PYTO_CHECK_HIP(hipMemcpyAsync(L, h->Ec, sizeof(double) * ld * ld, hipMemcpyDeviceToDevice, s));
-  for (int kb = 0; kb < nb; ++kb) {
-    const int k0 = kb * NB;
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, s, L, ld, k0,
-                       Dinv + (int64_t)kb * NB * NB, h->flags + F_COARSE_PIVOT);
-    const int m = ld - k0 - NB;
-    if (m > 0) {
-      double* P = L + (int64_t)(k0 + NB) * ld + k0;
-      // P <- P * Dinv^T (in place is safe: each 64-row tile reads only itself, K = 64)
-      hipLaunchKernelGGL((k_gemm64<false, true, false>), dim3(1, m / NB), dim3(256), 0, s, m,
-                         NB, NB, 1.0, P, ld, Dinv + (int64_t)kb * NB * NB, NB, 0.0, P, ld);
-      double* Tr = L + (int64_t)(k0 + NB) * ld + (k0 + NB);
-      hipLaunchKernelGGL((k_gemm64<false, true, true>), dim3(m / NB, m / NB), dim3(256), 0, s,
-                         m, m, NB, -1.0, P, ld, P, ld, 1.0, Tr, ld);
-    }
+  const int n = h->nc, ld = round_up64(h->nc);
+  if (!h->blas) {
+    rocblas_handle bh = nullptr;
+    PYTO_REQUIRE(rocblas_create_handle(&bh) == rocblas_status_success, PYTO_EHIP,
+                 "rocblas_create_handle failed");
+    h->blas = bh;
   }
-  // Y = L^{-1}, block row by block row: Y_i = Dinv_i (I_i - L_{i,<i} Y_{<i})
-  for (int ib = 0; ib < nb; ++ib) {
-    const int r0 = ib * NB;
-    const int ncols = r0 + NB;
-    hipLaunchKernelGGL(k_set_identity_rows, dim3(64), dim3(256), 0, s, Tmp, ld, r0, ncols);
-    if (ib > 0)
-      hipLaunchKernelGGL((k_gemm64<false, false, false>), dim3(ncols / NB, 1), dim3(256), 0, s,
-                         NB, ncols, r0, -1.0, L + (int64_t)r0 * ld, ld, Y, ld, 1.0, Tmp, ld);
-    hipLaunchKernelGGL((k_gemm64<false, false, false>), dim3(ncols / NB, 1), dim3(256), 0, s,
-                       NB, ncols, NB, 1.0, Dinv + (int64_t)ib * NB * NB, NB, Tmp, ld, 0.0,
-                       Y + (int64_t)r0 * ld, ld);
-    // zero the strictly upper part of this block row (columns >= ncols)
-    if (ncols < ld)
-      hipLaunchKernelGGL(k_set_identity_rows, dim3(64), dim3(256), 0, s,
-                         Y + (int64_t)r0 * ld + ncols, ld, -1000000, ld - ncols);
-  }
-  // Einv = Y^T Y  (into Tmp region, then copy)
-  double* Out = h->cwork + (int64_t)ld * ld;
-  hipLaunchKernelGGL((k_gemm64<true, false, false>), dim3(nb, nb), dim3(256), 0, s, ld, ld, ld,
-                     1.0, Y, ld, Y, ld, 0.0, Out, ld);
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->Einv, Out, sizeof(double) * ld * ld, hipMemcpyDeviceToDevice, s));
+  rocblas_handle bh = (rocblas_handle)h->blas;
+  PYTO_REQUIRE(rocblas_set_stream(bh, s) == rocblas_status_success, PYTO_EHIP,
+               "rocblas_set_stream failed");
+  int* dinfo = h->flags + F_COARSE_PIVOT;
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->Einv, h->Ec, sizeof(double) * ld * ld, hipMemcpyDeviceToDevice, s));
+  PYTO_REQUIRE(rocsolver_dpotrf(bh, rocblas_fill_lower, n, h->Einv, ld, dinfo) ==
+                   rocblas_status_success, PYTO_EHIP, "rocsolver_dpotrf failed");
+  int info = 0;
+  PYTO_CHECK_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  PYTO_REQUIRE(info == 0, PYTO_ENUMERIC,
+               "coarse matrix W^T K W is not positive definite (leading minor %d of %d); a "
+               "coarse diagonal shift (pyto_set_coarse_shift) regularises it", info, n);
+  PYTO_REQUIRE(rocsolver_dpotri(bh, rocblas_fill_lower, n, h->Einv, ld, dinfo) ==
+                   rocblas_status_success, PYTO_EHIP, "rocsolver_dpotri failed");
+  hipLaunchKernelGGL(k_fill_symmetric, dim3((n + 255) / 256, n), dim3(256), 0, s, h->Einv, n, ld);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
+}
+
+void release_blas(Handle* h) {
+  if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
+  h->blas = nullptr;
 }
 
 int coarse_setup(Handle* h, hipStream_t s) {
@@ -951,7 +846,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_TRY(device_alloc(h, (void**)&h->gidx, std::max<size_t>(gidx.size(), 1) * sizeof(int32_t)));
   PYTO_TRY(device_alloc(h, (void**)&h->Ec, (size_t)ld * ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->Einv, (size_t)ld * ld * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->cwork, ((size_t)2 * ld * ld + (size_t)ld * 64) * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->cwork, (size_t)ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->ct, (size_t)2 * ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->cmu, (size_t)ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->cres, (size_t)ld * 8));

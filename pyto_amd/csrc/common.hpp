@@ -144,7 +144,8 @@ struct Handle {
   int32_t* nidx = nullptr;
   double* Ec = nullptr;         // coarse matrix (nc x nc), then Cholesky factor
   double* Einv = nullptr;       // coarse inverse (nc x nc)
-  double* cwork = nullptr;      // nc x nc workspace (inverse computation)
+  double* cwork = nullptr;      // ld workspace (coarse refinement correction)
+  void* blas = nullptr;         // rocblas_handle of the dense coarse factorisation
   double* ct = nullptr;         // coarse rhs: W^T K z (ld), then W^T K p (ld) at ct + ld
   double* cmu = nullptr;        // coarse solution (nc)
   double* cres = nullptr;       // coarse residual (nc)

@@ -464,6 +464,7 @@ int ensure_solver_workspace(Handle* h) {
 }
 
 int coarse_setup(Handle* h, hipStream_t s);  // coarse.hip
+void release_blas(Handle* h);                // coarse.hip
 
 static void host_ke(const Handle* h, double* ke) {
   // Columns of KE from unit vectors through the same modal apply.
@@ -569,6 +570,7 @@ int pyto_destroy(pyto_handle* h) {
     if (p) hipFree(p);
   for (auto& gc : h->graphs)
     if (gc.exec) hipGraphExecDestroy(gc.exec);
+  release_blas(h);
   if (h->ev_a) hipEventDestroy(h->ev_a);
   if (h->ev_b) hipEventDestroy(h->ev_b);
   if (h->gstream) hipStreamDestroy(h->gstream);
