@@ -162,14 +162,14 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
 // Per-chunk partials of W^T K v over the special elements (one wave per
 // chunk; k_coarse_gather sums them into t), where
 // v = invd o r + beta p (the new search direction before its coarse
-// correction) or v = r when invd is null.  The element arithmetic is FP64
+// correction; HAS_P) or v = invd o r.  The element arithmetic is FP64
 // whatever the vector precision T.  Group centres are wave-uniform per slot.
 // A workgroup serves one SpecialBlock: it first stages v at the block's
 // sorted corner nodes in LDS (an element's corners would otherwise be 24
 // scattered loads, each wave instruction touching up to 64 cache lines on
 // interfaces normal to x), then each wave applies the element operator of
 // one chunk reading its corners from LDS.
-template <typename T>
+template <typename T, bool HAS_P>
 __global__ __launch_bounds__(256) void k_restrict_special(
     const T* __restrict__ r, const T* __restrict__ invd, const T* __restrict__ p,
     const double* __restrict__ beta_ptr, const SpecialElem* __restrict__ sinfo,
@@ -178,35 +178,57 @@ __global__ __launch_bounds__(256) void k_restrict_special(
     const double* __restrict__ Ee, const double* __restrict__ centers, GridDims g,
     ModalConsts<double> mc, double* __restrict__ cpart, const int* __restrict__ done) {
   if (done && *done) return;
+  constexpr int kStage = (3 * kSpecialBlockNodes + 255) / 256;   // staged DOFs per thread
   __shared__ double vs[3 * kSpecialBlockNodes];
   const SpecialBlock sb = sblocks[xcd_logical_block(blockIdx.x, gridDim.x)];
-  const double beta = (p && beta_ptr) ? *beta_ptr : 0.0;
-  // stage v at the block's corner nodes: consecutive lanes read consecutive
-  // DOFs wherever the nodes are consecutive
-  for (int i = threadIdx.x; i < 3 * sb.nnodes; i += 256) {
-    const int q = i / 3;
-    const int64_t d = 3 * (int64_t)sbnodes[sb.node0 + q] + (i - 3 * q);
-    double val = (double)r[d];
-    if (invd) val *= (double)invd[d];
-    if (p) val += beta * (double)p[d];
-    vs[i] = val;
-  }
-  __syncthreads();
+  const double beta = HAS_P ? *beta_ptr : 0.0;
   const int w = threadIdx.x >> 6;
-  if (w >= sb.nch) return;
-  const int wave = sb.chunk0 + w;
+  const int wave = sb.chunk0 + min(w, sb.nch - 1);
   const int lane = threadIdx.x & 63;
   // count and ng by value; the group list is read in place (a local copy of
   // it indexed by the loop variable would live in scratch memory)
   const int ch_start = chunks[wave].start, ch_count = chunks[wave].count;
   const int ch_ng = chunks[wave].ng;
-  const bool active = lane < ch_count;
+  const bool active = w < sb.nch && lane < ch_count;
+  // this lane's element record, issued before the staging loads
+  SpecialElem se{0, 0, 0};
+  uint4 li = make_uint4(0, 0, 0, 0);
+  double Ev = 0.0;
+  if (active) {
+    se = sinfo[ch_start + lane];
+    li = sidx[ch_start + lane];
+    Ev = Ee[se.e];
+  }
+  // stage v at the block's corner nodes: consecutive lanes read consecutive
+  // DOFs wherever the nodes are consecutive.  Branch-free (indices clamped
+  // into the block's list) so that all node-id loads, then all value loads,
+  // are in flight together before the first LDS store.
+  const int nst = 3 * sb.nnodes;
+  int64_t dof[kStage];
+#pragma unroll
+  for (int t = 0; t < kStage; ++t) {
+    const int i = min((int)threadIdx.x + 256 * t, nst - 1);
+    const int q = i / 3;
+    dof[t] = 3 * (int64_t)sbnodes[sb.node0 + q] + (i - 3 * q);
+  }
+  T ra[kStage], da[kStage], pa[kStage];
+#pragma unroll
+  for (int t = 0; t < kStage; ++t) {
+    ra[t] = r[dof[t]];
+    da[t] = invd[dof[t]];
+    pa[t] = HAS_P ? p[dof[t]] : T(0);
+  }
+#pragma unroll
+  for (int t = 0; t < kStage; ++t) {
+    const int i = threadIdx.x + 256 * t;
+    if (i < nst) vs[i] = (double)ra[t] * (double)da[t] + (HAS_P ? beta * (double)pa[t] : 0.0);
+  }
+  __syncthreads();
+  if (w >= sb.nch) return;
   double f[8][3];
   uint64_t info = 0;
   int ix = 0, iy = 0, iz = 0;
   if (active) {
-    const SpecialElem se = sinfo[ch_start + lane];
-    const uint4 li = sidx[ch_start + lane];
     const uint32_t lw[4] = {li.x, li.y, li.z, li.w};
     info = se.info;
     elem_coords(se.e, g, ix, iy, iz);
@@ -218,7 +240,7 @@ __global__ __launch_bounds__(256) void k_restrict_special(
       for (int c = 0; c < 3; ++c)
         u[j][c] = ((info >> (24 + 3 * j + c)) & 1) ? 0.0 : vs[3 * l + c];
     }
-    element_apply<double>(u, Ee[se.e], mc, f);
+    element_apply<double>(u, Ev, mc, f);
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
@@ -497,10 +519,18 @@ int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
 template <typename T>
 int launch_restrict_special(Handle* h, const T* r, const T* invd, const T* p,
                             const double* beta, double* t, const int* done, hipStream_t s) {
-  if (h->nsblocks > 0)
-    hipLaunchKernelGGL(k_restrict_special<T>, dim3(h->nsblocks), dim3(256), 0, s, r, invd, p,
-                       beta, h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes, h->Ee64,
-                       h->centers, h->g, h->mc64, h->cpart, done);
+  PYTO_REQUIRE(r && invd && (p == nullptr || beta != nullptr), PYTO_EINVAL,
+               "special restriction needs r, invd and (with p) beta");
+  if (h->nsblocks > 0) {
+    if (p)
+      hipLaunchKernelGGL((k_restrict_special<T, true>), dim3(h->nsblocks), dim3(256), 0, s, r,
+                         invd, p, beta, h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes,
+                         h->Ee64, h->centers, h->g, h->mc64, h->cpart, done);
+    else
+      hipLaunchKernelGGL((k_restrict_special<T, false>), dim3(h->nsblocks), dim3(256), 0, s, r,
+                         invd, p, beta, h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes,
+                         h->Ee64, h->centers, h->g, h->mc64, h->cpart, done);
+  }
   hipLaunchKernelGGL(k_coarse_gather, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
                      h->cpart, h->gptr, h->gidx, h->ngroups, t, done);
   PYTO_CHECK_HIP(hipGetLastError());
