@@ -175,6 +175,21 @@ int pyto_filter_apply(pyto_handle* h, double rel_radius, int mode, const double*
 int pyto_filter_stencil(pyto_handle* h, double rel_radius, int capacity, int* count,
                         int32_t* offsets_host, double* weights_host);
 
+/* ---- optimality-criteria update ----------------------------------------- */
+/* OC density update with the volume bisection of src/topopt_ocm.py:247-269,
+ * run on the device: l1 = 0, l2 = 1e12; while (l2-l1)/((l1+l2)/2+1e-10) > 1e-7:
+ * lmid = (l1+l2)/2, D = clip(-dc/(dv*lmid), 0.1, 10),
+ * xnew = max(xmin, max(x-move, min(xmax, min(x+move, x*sqrt(D))))),
+ * l1 = lmid if mean(xnew)/volfrac - 1 > 0 else l2 = lmid.  Writes the xnew of
+ * the last lmid evaluated.  x, dc, xnew: device arrays of n values; dv is the
+ * (uniform) volume-constraint gradient 1/(volfrac*n).  At most max_steps
+ * bisection steps are enqueued; nothing is read back unless lmid_host or
+ * steps_host is given (then the stream is synchronised; *steps_host < 0 means
+ * max_steps ran out before the stopping rule held). */
+int pyto_oc_update(pyto_handle* h, const double* x, const double* dc, int64_t n, double dv,
+                   double volfrac, double move, double xmin, double xmax, int max_steps,
+                   double* xnew, double* lmid_host, int* steps_host, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,12 +77,16 @@ def density_filter_chain(H, Hs, dphys):
     return H @ (dphys / Hs)
 
 
-def oc_update(x, grad, volfrac, move=0.2, xmin=0.001, xmax=1.0):
-    """Bisection OC update, src/topopt_ocm.py:253-269."""
+def oc_update(x, grad, volfrac, move=0.2, xmin=0.001, xmax=1.0, return_info=False):
+    """Bisection OC update, src/topopt_ocm.py:253-269.  With return_info,
+    also the last lmid evaluated and the number of bisection steps."""
     dv = np.ones_like(x) / volfrac / x.size
     l1, l2 = 0.0, 1e12
+    lmid = 0.5 * (l2 + l1)
+    steps = 0
     xnew = x.copy()
     while (l2 - l1) / ((l1 + l2) / 2 + 1e-10) > 1e-7:
+        steps += 1
         lmid = 0.5 * (l2 + l1)
         D = np.clip(-grad / (dv * lmid), 0.1, 10)
         xnew = np.maximum(xmin, np.maximum(x - move, np.minimum(xmax, np.minimum(x + move, x * np.sqrt(D)))))
@@ -91,4 +95,6 @@ def oc_update(x, grad, volfrac, move=0.2, xmin=0.001, xmax=1.0):
             l1 = lmid
         else:
             l2 = lmid
+    if return_info:
+        return xnew, lmid, steps
     return xnew

@@ -67,6 +67,8 @@ _SIGS = {
     "pyto_pnorm_adjoint_rhs": ([_vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp, _vp], _i),
     "pyto_filter_apply": ([_vp, _d, _i, _vp, _vp, _vp, _vp], _i),
     "pyto_filter_stencil": ([_vp, _d, _i, ctypes.POINTER(_i), _vp, _vp], _i),
+    "pyto_oc_update": ([_vp, _vp, _vp, ctypes.c_int64, _d, _d, _d, _d, _d, _i, _vp,
+                        ctypes.POINTER(_d), ctypes.POINTER(_i), _vp], _i),
 }
 
 EXPORTED = tuple(_SIGS)

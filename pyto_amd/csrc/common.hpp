@@ -179,10 +179,25 @@ constexpr int kNumFlags = 8;
 enum ScalarSlot {
   S_RZ = 0, S_RZ0 = 1, S_PKP = 2, S_ALPHA = 3, S_BETA = 4, S_RZNEW = 5, S_REL = 6,
   S_TMP0 = 7, S_TMP1 = 8, S_TMP2 = 9, S_RTOL = 10, S_DOT0 = 11, S_DOT1 = 12,
-  S_SUMVMP = 13, S_COMPLIANCE = 14, S_SCALE = 15
+  S_SUMVMP = 13, S_COMPLIANCE = 14, S_SCALE = 15,
+  // OC bisection: two (l1, l2, lmid, done) states alternating by step
+  // parity, and the last multiplier at which the volume was evaluated
+  S_OC_STATE = 16, S_OC_LAST = 24,
+  S_RELMIN = 25   // best relative residual of the running solve (stagnation test)
 };
 // Device flag slots.
-enum FlagSlot { F_DONE = 0, F_ITERS = 1, F_BREAKDOWN = 2, F_COARSE_PIVOT = 3 };
+// F_BREAKDOWN: 1 = p^T K p <= 0 or NaN, 2 = stagnation (see k_scalars)
+enum FlagSlot { F_DONE = 0, F_ITERS = 1, F_BREAKDOWN = 2, F_COARSE_PIVOT = 3, F_OC_DONE = 4,
+                F_OC_STEPS = 5, F_STALL = 6 };
+// Past its attainable accuracy the deflated iteration's recursive residual
+// stops falling and then grows geometrically (rounding leaves coarse-space
+// components in r that no K-orthogonal direction removes; p^T K p shrinks,
+// alpha and the iterate grow).  A solve therefore stops as stagnated when
+// its relative residual exceeds kDivergeFactor times its best value, or has
+// not improved on it by 1% for max(kStallIters, k/2) iterations (k =
+// iterations so far).
+constexpr int kStallIters = 200;
+constexpr double kDivergeFactor = 1e3;
 // Last-block counter slots (one per reduction site in flight).
 enum CounterSlot { C_MATVEC = 0, C_UPDATE = 1, C_DOT = 2, C_ELEM = 3, C_NORM = 4 };
 

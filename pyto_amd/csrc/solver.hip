@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ r, const T* _
 // One block: finishes the iteration's scalars from the partials: alpha (as
 // in k_cg_update), rz_new, the stop test sqrt(rz_new/rz0) <= rtol
 // (src/deflation.py:567), beta = rz_new/rz, the iteration count and the
-// breakdown flag (p^T K p <= 0 or NaN).
+// breakdown flag (p^T K p <= 0 or NaN) and the stagnation test.
 __global__ __launch_bounds__(256) void k_scalars(double* __restrict__ scal, int* __restrict__ flags,
                                                  const double* __restrict__ part_pq, int npq,
                                                  const double* __restrict__ part_rz, int nrz) {
@@ -135,6 +135,13 @@ __global__ __launch_bounds__(256) void k_scalars(double* __restrict__ scal, int*
       flags[F_BREAKDOWN] = 1;
       flags[F_DONE] = 1;
     } else if (rel <= scal[S_RTOL]) {
+      flags[F_DONE] = 1;
+    } else if (rel < 0.99 * scal[S_RELMIN]) {
+      scal[S_RELMIN] = rel;
+      flags[F_STALL] = 0;
+    } else if (rel > kDivergeFactor * scal[S_RELMIN] ||
+               ++flags[F_STALL] >= max(kStallIters, flags[F_ITERS] / 2)) {
+      flags[F_BREAKDOWN] = 2;
       flags[F_DONE] = 1;
     }
     scal[S_BETA] = rz_new / rz;
@@ -210,6 +217,7 @@ __global__ void k_set_flags(int* flags, int done, int iters) {
   flags[F_DONE] = done;
   flags[F_ITERS] = iters;
   flags[F_BREAKDOWN] = 0;
+  flags[F_STALL] = 0;
 }
 
 __global__ void k_set_scalar(double* scal, int slot, double v) { scal[slot] = v; }
@@ -240,6 +248,7 @@ struct DpcgResult {
   int iterations = 0;
   bool converged = false;
   bool breakdown = false;
+  bool stagnated = false;
   double rel = 0.0;
   int matvecs = 0;
 };
@@ -337,6 +346,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   // p0 = z0 - W mu (x untouched)
   PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, true, nullptr, s, nullptr, 0));
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RTOL, rtol);
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RELMIN, 1e300);
   hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(1), 0, s, h->flags, 0, 0);
   PYTO_CHECK_HIP(hipGetLastError());
   PYTO_TRY(read_back(h, s));
@@ -401,6 +411,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   res.iterations = h->host_flags[F_ITERS];
   res.matvecs += res.iterations;
   res.breakdown = h->host_flags[F_BREAKDOWN] != 0;
+  res.stagnated = h->host_flags[F_BREAKDOWN] == 2;
   res.converged = h->host_flags[F_DONE] != 0 && !res.breakdown;
   res.rel = h->host_scal[S_REL];
   // the stopping iteration skipped its direction update: apply its x += alpha p
@@ -461,6 +472,13 @@ int pyto_solve_dpcg_f64(pyto_handle* h, const double* f, double* u, double rtol,
     stats->rel_true = rel_true;
     stats->setup_ms = 0.0;
   }
+  if (r.stagnated) {
+    set_error("DPCG stagnated at relative residual %.3e after %d iterations (best %.3e; "
+              "no 1%% improvement in max(%d, k/2) iterations, or growth by %.0fx): the "
+              "attainable accuracy is above rtol", r.rel, r.iterations, h->host_scal[S_RELMIN],
+              kStallIters, kDivergeFactor);
+    return PYTO_ENOCONV;
+  }
   if (r.breakdown) {
     set_error("DPCG breakdown (p^T K p <= 0 or NaN) after %d iterations", r.iterations);
     return PYTO_ENOCONV;
@@ -516,7 +534,7 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
     PYTO_TRY(dpcg<float>(h, h->b32, h->x32, tol_k, max_iters - inner_total, ir, s));
     inner_total += ir.iterations;
     mv32 += ir.matvecs;
-    if (ir.breakdown) inner_fail = true;
+    if (ir.breakdown && !ir.stagnated) inner_fail = true;
     hipLaunchKernelGGL(k_axpy_scaled, dim3(kVecBlocks), dim3(256), 0, s, u, h->x32, n, h->scal,
                        (int)S_TMP1);
     PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0,

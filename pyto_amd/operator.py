@@ -238,6 +238,31 @@ class GridOperator:
                                                 float(q), float(pexp), ptr(g), ptr(T1), stream_ptr()))
         return g, T1
 
+    # -- optimality-criteria update ----------------------------------------------
+    def oc_update(self, x, dc, volfrac, move=0.2, xmin=0.001, xmax=1.0, max_steps=160,
+                  out=None, read_back=False):
+        """OC update with the reference's volume bisection run on the device
+        (src/topopt_ocm.py:247-269).  Returns xnew, or (xnew, lmid, steps)
+        with read_back (which synchronises)."""
+        x = self.as_elem(x)
+        dc = self.as_elem(dc)
+        n = x.numel()
+        if dc.numel() != n:
+            raise ValueError("x and dc differ in length")
+        xnew = torch.empty_like(x) if out is None else out
+        dv = 1.0 / float(volfrac) / n
+        lm = ctypes.c_double()
+        st = ctypes.c_int()
+        check(_lib.lib().pyto_oc_update(self.handle, ptr(x, F64, n), ptr(dc, F64, n), n, dv,
+                                        float(volfrac), float(move), float(xmin), float(xmax),
+                                        int(max_steps), ptr(xnew, F64, n),
+                                        ctypes.byref(lm) if read_back else None,
+                                        ctypes.byref(st) if read_back else None, stream_ptr()),
+              "pyto_oc_update")
+        if read_back:
+            return xnew, lm.value, st.value
+        return xnew
+
     # -- cone filter -------------------------------------------------------------
     def filter(self, v, rel_radius=1.5, mode=0, aux=None):
         v = self.as_elem(v) if v is not None else None

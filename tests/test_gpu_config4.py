@@ -1,0 +1,115 @@
+"""The differentiation workload at full size: 128x64x64 hexes (1.64M DOF),
+densities U[0.2, 1] seed 1, p-norm p = 8, q = 0.5, plus compliance, through
+the autograd wrapper: central finite differences on 20 seeded elements and
+30 OC iterations end to end."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fea as OF
+from oracle import mesh as OM
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (128, 64, 64)
+
+
+def _fe(rtol):
+    from pyto_amd import bound_cond, deflation, hex_mesher, hex_structural_fea, linear_solvers, mat_lib
+    g = OM.GridMesh(DIMS, (1, 1, 1))
+    fixed, gv = OF.clamp_x0(g)
+    f = OF.face_traction_load(g)
+    mesh = hex_mesher.HexMesher().grid_mesh(DIMS, (1, 1, 1))
+    bc = bound_cond.BC(force=f, fixed_dofs=fixed, dirichlet_values=gv)
+    ds = deflation.DeflationSolver()
+    ds.create_box_groups(mesh, (8, 4, 4))
+    fe = hex_structural_fea.HexStructuralFEA(mesh, mat_lib.Material("Unit", 1.0, 0.3), bc,
+                                             linear_solvers.Solvers.DPCG, dsolver=ds, rtol=rtol,
+                                             maxIters=20000)
+    fe.stiff_mtrx.set_coarse_shift(0.0, 0.0)
+    return mesh, fe
+
+
+@pytest.fixture
+def pnorm8():
+    from pyto_amd import topopt_material_model as tmm
+    tmm.set_pNorm_exponent(8.0)
+    tmm.set_EVOID_RELATIVE(1e-9)
+    yield
+    tmm.set_pNorm_exponent(6)
+    tmm.set_EVOID_RELATIVE(1e-8)
+
+
+def test_config4_gradient_fd(gpu, pnorm8):
+    """The FP64 DPCG reaches rtol 1e-12 reliably at this size (its true
+    residual settles near 4e-11; 1e-13 sits at the attainable limit)."""
+    from pyto_amd import autograd as AG
+    print("config4 FD", flush=True)
+    rtol = 1e-12
+    mesh, fe = _fe(rtol)
+    n = mesh.num_elems
+    x = np.random.default_rng(1).uniform(0.2, 1.0, n)
+    xt = torch.from_numpy(x).to(gpu)
+    with torch.no_grad():
+        C0, P0 = (float(v) for v in AG.responses(xt, fe))
+    assert np.isfinite(C0) and np.isfinite(P0) and C0 > 0 and P0 > 0
+    assert fe.last_stats["converged"] == 1
+
+    def J(v):
+        C, PN = AG.responses(v, fe)
+        return C / C0 + PN / P0
+
+    xr = xt.clone().requires_grad_(True)
+    J0 = J(xr)
+    (g,) = torch.autograd.grad(J0, xr)
+    g = g.cpu().numpy()
+    assert np.all(np.isfinite(g)) and np.abs(g).max() > 0
+    idx = np.random.default_rng(1).choice(n, 20, replace=False)
+    # A solve to relative residual rtol leaves a first-order error ~rtol |J|
+    # in PN, so a central difference carries noise ~rtol |J| / h (allowed
+    # 4x).  At h = 1e-6 that floor dominates the small gradients of a
+    # 524k-element sum; h = 1e-4 (truncation ~h^2 / x^2 < 1e-7 relative for
+    # x >= 0.2) checks every element at 1e-5 relative.
+    bad = []
+    with torch.no_grad():
+        for h in (1e-6, 1e-4):
+            floor = 4 * rtol * abs(float(J0)) / h
+            for e in idx:
+                vals = []
+                for s in (+1, -1):
+                    xx = xt.clone()
+                    xx[e] += s * h
+                    vals.append(float(J(xx)))
+                fd = (vals[0] - vals[1]) / (2 * h)
+                print("h %.0e elem %d adjoint %.8e fd %.8e" % (h, e, g[e], fd), flush=True)
+                if not (abs(g[e] - fd) <= 1e-5 * abs(g[e]) + floor):
+                    bad.append((h, int(e), g[e], fd))
+    assert not bad, bad
+
+
+def test_config4_oc_30_iterations(gpu, pnorm8):
+    from pyto_amd import autograd as AG
+    from pyto_amd import topopt_common, topopt_ocm
+    print("config4 OC x30", flush=True)
+    mesh, fe = _fe(1e-8)
+    n = mesh.num_elems
+    x0 = np.random.default_rng(1).uniform(0.2, 1.0, n)
+    volfrac = 0.6
+    scale = {}
+
+    def objective(v, fe_):
+        C, PN = AG.responses(v, fe_)
+        if not scale:
+            scale["C"], scale["P"] = float(C), float(PN)
+        return C / scale["C"] + PN / scale["P"]
+
+    params = topopt_common.TOParams(Constraints=[(topopt_common.TO_QOI.VOLUME_FRACTION, None, volfrac)])
+    sol, hist, ok, msg, nfea = topopt_ocm.topopt_optimality_criteria(
+        topopt_common.FEA_MODE.STRUCTURAL, fe, params, maxIterations=30, rel_conv_tol=0.0,
+        print_progress=True, binarize_topology=False, objective=objective, x0=x0)
+    J = np.array(hist["objective"][:30])
+    print("J:", np.array2string(J, precision=4), flush=True)
+    assert nfea == 30
+    assert np.all(np.isfinite(J))
+    assert J[-1] < 0.8 * J[0]
+    assert abs(hist["volfrac"][29] - volfrac) <= 1e-5

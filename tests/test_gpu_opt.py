@@ -1,0 +1,191 @@
+"""Device OC update, the autograd adjoints and the device-resident OC loop
+against the oracle, plus central finite differences."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fea as OF
+from oracle import filters as OFL
+from oracle import sensitivities as OS
+from tests.helpers import Case, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,volfrac,seed", [(12000, 0.5, 0), (777, 0.3, 1), (50000, 0.62, 2)])
+def test_oc_update_matches_oracle(gpu, n, volfrac, seed):
+    print("oc update", n, volfrac, flush=True)
+    from pyto_amd.operator import GridOperator
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.001, 1.0, n)
+    g = -rng.uniform(0.0, 3.0, n)
+    g[rng.choice(n, n // 20, replace=False)] *= -0.1     # some positive: D clipped at 0.1
+    op = GridOperator((4, 3, 2), (1, 1, 1), 1.0, 0.3, [])
+    xt, gt = torch.from_numpy(x).to(gpu), torch.from_numpy(g).to(gpu)
+    xn, lmid, steps = op.oc_update(xt, gt, volfrac, read_back=True)
+    xr, lref, sref = OFL.oc_update(x, g, volfrac, return_info=True)
+    assert steps == sref
+    assert lmid == lref
+    assert np.abs(xn.cpu().numpy() - xr).max() <= 1e-15
+    # a second call reuses the device state from scratch
+    xn2 = op.oc_update(xt, gt, volfrac)
+    assert torch.equal(xn2, xn)
+
+
+def _fe(dims, x, seed_groups=10, rtol=1e-12, load="traction", boxes=None):
+    from pyto_amd import bound_cond, deflation, hex_mesher, hex_structural_fea, linear_solvers, mat_lib
+    c = Case(dims, seed=0, lo=0.2, x=x, load=load)
+    mesh = hex_mesher.HexMesher().grid_mesh(dims, (1, 1, 1))
+    bc = bound_cond.BC(force=c.f, fixed_dofs=c.fixed, dirichlet_values=np.zeros(len(c.fixed)))
+    ds = deflation.DeflationSolver()
+    if boxes is None:
+        ds.create_deflation_groups(mesh, seed_groups)
+    else:
+        ds.create_box_groups(mesh, boxes)
+    fe = hex_structural_fea.HexStructuralFEA(mesh, mat_lib.Material("Unit", 1.0, 0.3), bc,
+                                             linear_solvers.Solvers.DPCG, dsolver=ds, rtol=rtol,
+                                             maxIters=5000)
+    fe.stiff_mtrx.set_coarse_shift(0.0, 0.0)
+    return c, mesh, fe
+
+
+def test_autograd_matches_module_sensitivities(gpu):
+    """responses() backward == compute_compliance_and_gradient and
+    compute_pnorm_stress_and_sensitivity on the same x (same kernels, same
+    adjoint solve tolerance)."""
+    from pyto_amd import autograd as AG
+    from pyto_amd import topopt_material_model as tmm
+    from pyto_amd import topopt_obj_cons_sensitivities as sens
+    print("autograd vs module", flush=True)
+    dims = (16, 8, 6)
+    x = np.random.default_rng(4).uniform(0.2, 1.0, np.prod(dims))
+    tmm.set_pNorm_exponent(8.0)
+    try:
+        c, mesh, fe = _fe(dims, x, rtol=1e-13)
+        xt = torch.from_numpy(x).to(gpu).requires_grad_(True)
+        C, PN = AG.responses(xt, fe)
+        gC, = torch.autograd.grad(C, xt, retain_graph=True)
+        gP, = torch.autograd.grad(PN, xt)
+        u = fe.solve(torch.from_numpy(x).to(gpu))
+        C_ref, g_ref = sens.compute_compliance_and_gradient(sens.FEA_MODE.STRUCTURAL, u,
+                                                            torch.from_numpy(x).to(gpu), fe, None,
+                                                            tmm.MaterialModel.SIMP)
+        pn_ref, gp_ref, _ = sens.compute_pnorm_stress_and_sensitivity(
+            u, torch.from_numpy(x).to(gpu), fe, None, tmm.MaterialModel.SIMP)
+        assert abs(float(C) - C_ref) <= 1e-12 * abs(C_ref)
+        assert rel_err(gC.cpu().numpy(), g_ref.cpu().numpy()) <= 1e-12
+        assert abs(float(PN) - pn_ref) <= 1e-12 * pn_ref
+        assert rel_err(gP.cpu().numpy(), gp_ref.cpu().numpy()) <= 1e-9
+        # and the oracle with sparse direct solves
+        solve, _ = OS.direct_solver(c.mesh, c.KE, x, c.fixed, c.gv, 3.0, 1e-8)
+        u_ref = solve(c.f)
+        pn_o, grad_o, _, _ = OS.pnorm_and_sensitivity(u_ref, x, c.mesh, c.KE, 1.0, 0.3, solve, 3.0,
+                                                      1e-8, q=0.5, pexp=8.0)
+        C_o, gC_o = OS.compliance_and_gradient(u_ref, x, c.mesh.edofMatStructural, c.KE, 3.0, 1e-8)
+        assert abs(float(C) - C_o) <= 1e-9 * abs(C_o)
+        assert rel_err(gC.cpu().numpy(), gC_o) <= 1e-8
+        assert rel_err(gP.cpu().numpy(), grad_o) <= 1e-7
+    finally:
+        tmm.set_pNorm_exponent(6)
+
+
+def test_autograd_displacement_adjoint(gpu):
+    """d(f.u)/dx through the generic displacement adjoint equals the
+    self-adjoint compliance gradient."""
+    from pyto_amd import autograd as AG
+    print("autograd displacement", flush=True)
+    dims = (20, 8, 6)
+    x = np.random.default_rng(5).uniform(0.2, 1.0, np.prod(dims))
+    c, mesh, fe = _fe(dims, x, rtol=1e-13)
+    xt = torch.from_numpy(x).to(gpu).requires_grad_(True)
+    f = torch.from_numpy(c.f).to(gpu)
+    u = AG.displacement(xt, fe)
+    (g1,) = torch.autograd.grad(torch.dot(f, u), xt)
+    xt2 = torch.from_numpy(x).to(gpu).requires_grad_(True)
+    (g2,) = torch.autograd.grad(AG.compliance(xt2, fe), xt2)
+    assert rel_err(g1.cpu().numpy(), g2.cpu().numpy()) <= 1e-9
+
+
+def test_autograd_combined_objective_fd(gpu):
+    """J = C/C0 + PN/PN0 (p = 8, q = 0.5): backward vs central FD on 20 seeded
+    elements.  A solve to relative residual rtol leaves a first-order error
+    ~rtol |J| in PN, so the central difference carries noise ~rtol |J| / h
+    (1e-7 at h = 1e-6 for J ~ 2, rtol 1e-13); the comparison allows 4x that
+    beside the 1e-5 relative target."""
+    from pyto_amd import autograd as AG
+    from pyto_amd import topopt_material_model as tmm
+    print("autograd combined FD", flush=True)
+    dims = (16, 8, 6)
+    x = np.random.default_rng(6).uniform(0.2, 1.0, np.prod(dims))
+    tmm.set_pNorm_exponent(8.0)
+    try:
+        c, mesh, fe = _fe(dims, x, rtol=1e-13)
+        xt = torch.from_numpy(x).to(gpu).requires_grad_(True)
+        C0, P0 = (float(v) for v in AG.responses(xt.detach(), fe))
+
+        def J(v):
+            C, PN = AG.responses(v, fe)
+            return C / C0 + PN / P0
+
+        J0 = J(xt)
+        (g,) = torch.autograd.grad(J0, xt)
+        g = g.cpu().numpy()
+        rtol = 1e-13
+        idx = np.random.default_rng(1).choice(mesh.num_elems, 20, replace=False)
+        with torch.no_grad():
+            for h in (1e-6, 1e-5):
+                floor = 4 * rtol * abs(float(J0)) / h
+                for e in idx:
+                    vals = []
+                    for s in (+1, -1):
+                        xx = x.copy()
+                        xx[e] += s * h
+                        vals.append(float(J(torch.from_numpy(xx).to(gpu))))
+                    fd = (vals[0] - vals[1]) / (2 * h)
+                    assert abs(g[e] - fd) <= 1e-5 * abs(g[e]) + floor, (h, e, g[e], fd)
+    finally:
+        tmm.set_pNorm_exponent(6)
+
+
+def _oracle_oc_loop(c, H, Hs, volfrac, iters, force_elems):
+    x = np.full(c.mesh.num_elems, volfrac)
+    hist = []
+    scale = None
+    for _ in range(iters):
+        solve, _ = OS.direct_solver(c.mesh, c.KE, x, c.fixed, c.gv, 3.0, 1e-9)
+        u = solve(c.f)
+        C, g = OS.compliance_and_gradient(u, x, c.mesh.edofMatStructural, c.KE, 3.0, 1e-9)
+        scale = abs(C) if scale is None else scale
+        g = g / scale
+        g = OFL.sensitivity_filter(H, Hs, x, g)
+        g[force_elems] = g.min()
+        x = OFL.oc_update(x, g, volfrac)
+        hist.append(C)
+    return x, hist
+
+
+def test_oc_loop_matches_oracle(gpu):
+    """Five device-resident OC iterations of the 60x20x10 point-load
+    cantilever (x0 = 0.5, r = 1.5, volfrac 0.5) vs the oracle loop with sparse
+    direct solves."""
+    from pyto_amd import topopt_common, topopt_material_model as tmm, topopt_ocm
+    print("oc loop vs oracle", flush=True)
+    dims = (60, 20, 10)
+    tmm.set_EVOID_RELATIVE(1e-9)
+    try:
+        c, mesh, fe = _fe(dims, np.full(np.prod(dims), 0.5), rtol=1e-12, load=(60, 10, 5),
+                          boxes=(6, 2, 1))
+        c = Case(dims, x=np.full(np.prod(dims), 0.5), load=(60, 10, 5), eps=1e-9)
+        H, Hs = OFL.cone_filter(c.mesh, 1.5)
+        fel = topopt_common.find_elements_with_forces(mesh, c.f, 3)
+        x_ref, hist_ref = _oracle_oc_loop(c, H, Hs, 0.5, 5, fel)
+        params = topopt_common.TOParams()
+        sol, hist, ok, msg, n = topopt_ocm.topopt_optimality_criteria(
+            topopt_common.FEA_MODE.STRUCTURAL, fe, params, maxIterations=5, print_progress=False,
+            binarize_topology=False)
+        assert n == 5
+        assert np.allclose(hist["objective"][:5], hist_ref, rtol=1e-7, atol=0)
+        x_dev = fe.stiff_mtrx.x.cpu().numpy()
+        assert np.abs(x_dev - x_ref).max() <= 1e-6
+    finally:
+        tmm.set_EVOID_RELATIVE(1e-8)

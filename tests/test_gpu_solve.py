@@ -87,6 +87,26 @@ def test_zero_force_and_iteration_limit(gpu):
     assert st["iterations"] == 3 and not st["converged"]
 
 
+def test_unattainable_tolerance_stops_as_stagnated(gpu):
+    """rtol below the attainable accuracy: the recursive residual bottoms out
+    (~2e-14 here) and then grows geometrically; the solve stops once it is
+    1e3x above its best, with the iterate still near the attainable accuracy,
+    instead of iterating until p^T K p breaks down."""
+    print("stagnation stop", flush=True)
+    import scipy.sparse.linalg as spla
+    from pyto_amd._lib import ConvergenceWarning
+    c = Case((24, 8, 6), seed=11, lo=0.05)
+    op = c.operator(gpu, shift=0.0)
+    with pytest.warns(ConvergenceWarning, match="stagnated"):
+        u, st = op.solve(torch.from_numpy(c.f).to(gpu), rtol=1e-18, max_iters=20000)
+    assert not st["converged"] and st["iterations"] < 20000
+    un = u.cpu().numpy()
+    assert np.all(np.isfinite(un))
+    A, b, free = OF.impose_dirichlet(c.K, c.f, c.fixed, c.gv)
+    ud = spla.spsolve(A.tocsc(), b)
+    assert rel_err(un[free], ud) <= 1e-8
+
+
 def test_graph_replay_matches_eager(gpu, monkeypatch):
     """The hipGraph-replayed iteration batches give the eager result: same
     iteration count, same solution to rounding, over repeated solves with

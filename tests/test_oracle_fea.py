@@ -184,3 +184,26 @@ def test_oc_update_meets_volume():
     assert abs(xn.mean() - 0.4) < 1e-5
     assert np.all(np.abs(xn - x) <= 0.2 + 1e-15)
     assert xn.min() >= 0.001 and xn.max() <= 1.0
+
+
+def test_oc_update_closed_form_multiplier():
+    """Uniform gradient -c and move limits inactive: x_new = x sqrt(c/(dv l)),
+    so the volume constraint fixes l* = c mean(x)^2 / (dv volfrac^2); the
+    bisection must land within its 1e-7 relative stopping width."""
+    rng = np.random.default_rng(3)
+    n, volfrac, c = 4000, 0.45, 0.7
+    x = rng.uniform(0.4, 0.5, n)
+    xn, lmid, steps = filters.oc_update(x, np.full(n, -c), volfrac, move=0.5, return_info=True)
+    dv = 1.0 / volfrac / n
+    lstar = c * x.mean() ** 2 / (dv * volfrac ** 2)
+    assert abs(lmid - lstar) <= 2e-7 * lstar
+    assert abs(xn.mean() - volfrac) <= 1e-6
+    # every step halves the bracket [0, 1e12] until its width relative to the
+    # midpoint is <= 1e-7: the count follows from l* alone
+    width, expect = 1e12, 0
+    lo, hi = 0.0, 1e12
+    while (hi - lo) / ((lo + hi) / 2 + 1e-10) > 1e-7:
+        expect += 1
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if mid < lstar else (lo, mid)
+    assert steps == expect
