@@ -78,12 +78,16 @@ def cpu_baseline(gpu_iters_fp64, ndof_target, budget_s=20.0):
 
 def traffic_bytes():
     """HBM bytes per k_matvec<float> launch at this workload, from the two
-    committed rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; see the file)."""
-    path = os.path.join(ROOT, "profiles", "k_matvec_f32_cantilever256_traffic.json")
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in the profile
+    file, each scaled by the same counter's reading for a copy of known size."""
+    path = os.path.join(ROOT, "profiles", "k_matvec_f32_cantilever_256x128x128.md")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("traffic_bytes")
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def main():
@@ -96,6 +100,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--tuned-boxes", default="12,6,6",
+                    help="second aggregate layout timed beside the named one ('' to skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -120,7 +126,8 @@ def main():
     x = torch.from_numpy(x_np).to(dev)
     f = torch.from_numpy(bc.force).to(dev)
     u = torch.empty_like(f)
-    log("groups", op.ngroups, "workspace MB", op.workspace_bytes() / 1e6)
+    ngroups = op.ngroups
+    log("groups", ngroups, "workspace MB", op.workspace_bytes() / 1e6)
 
     def step(precision):
         op.set_density(x)
@@ -194,6 +201,31 @@ def main():
     log("matvec f32 %.1f us %.0f GB/s; f64 %.1f us %.0f GB/s" %
         (t_mv32 * 1e6, gbs32, t_mv64 * 1e6, bytes64 / t_mv64 / 1e9))
 
+    # the same solves with a tuned aggregate layout (reported beside `value`,
+    # which stays on the named 64-aggregate configuration)
+    tuned = None
+    if args.tuned_boxes:
+        ds2 = deflation.DeflationSolver()
+        ds2.create_box_groups(mesh, tuple(int(v) for v in args.tuned_boxes.split(",")))
+        ds2.attach(op)
+        step("mixed")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st2 = step("mixed")
+        torch.cuda.synchronize()
+        t2m = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        st2d = step("fp64")
+        torch.cuda.synchronize()
+        t2d = time.perf_counter() - t0
+        tuned = {"boxes": args.tuned_boxes, "groups": op.ngroups,
+                 "fp32_refined_solves_per_s": 1.0 / t2m, "inner_iterations": st2["iterations"],
+                 "fp64_plain_solves_per_s": 1.0 / t2d, "fp64_iterations": st2d["iterations"],
+                 "speedup_vs_fp64": t2d / t2m}
+        log("tuned aggregates %s: mixed %.3f ms, fp64 %.3f ms" % (args.tuned_boxes, t2m * 1e3,
+                                                                   t2d * 1e3))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(st64["iterations"], nd)
@@ -208,7 +240,7 @@ def main():
         "config": {"workload": "voxel cantilever %dx%dx%d (%d DOF), x=0 clamped, unit -z traction "
                                "on x=max, SIMP p=3 Emin=1e-9, 16-bump density seed 2; %d box "
                                "deflation groups; FP32 DPCG in FP64 refinement to rel res 1e-8"
-                               % (grid + (nd, op.ngroups)),
+                               % (grid + (nd, ngroups)),
                    "grid": list(grid), "dof": nd},
         "fp32_refined": {"solves_per_s": 1.0 / t_mixed, "inner_iterations": s["iterations"],
                          "outer_iterations": s["outer_iterations"], "rel": s["rel"],
@@ -224,6 +256,7 @@ def main():
                      "kernel": "k_matvec<float>", "kernel_us": t_mv32 * 1e6,
                      "algorithmic_bytes": bytes32,
                      "fp64_kernel_us": t_mv64 * 1e6, "fp64_GBs": bytes64 / t_mv64 / 1e9},
+        "tuned_aggregates": tuned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
