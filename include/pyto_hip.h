@@ -36,7 +36,7 @@ extern "C" {
 #define PYTO_EHIP -2       /* HIP runtime error */
 #define PYTO_ENOMEM -3     /* device allocation failed */
 #define PYTO_ESTATE -4     /* call out of order (e.g. solve before density) */
-#define PYTO_ENOCONV -5    /* solver stopped at maxIters (result still written) */
+#define PYTO_ENOCONV -5    /* solver stopped at maxIters or stagnated (result still written) */
 #define PYTO_ENUMERIC -6   /* coarse matrix W^T K W not positive definite */
 
 typedef struct pyto_handle pyto_handle;
@@ -77,7 +77,8 @@ int pyto_info(pyto_handle* h, int64_t* nnodes, int64_t* nelems, int64_t* ndof,
 
 /* Densities x (FP64, per element) -> E_e (FP64 and FP32 copies) and the
  * Jacobi inverse diagonal; if deflation groups are set, also assembles the
- * coarse Galerkin matrix sym(W^T K W) + 1e-10 I and factorises it (FP64).
+ * coarse Galerkin matrix sym(W^T K W) + shift (default 1e-10 I) and inverts
+ * it in FP64 (Cholesky; PYTO_ENUMERIC if it is not positive definite).
  * Replaces the per-solve part of HexStructuralFEA.solve
  * (src/hex_structural_fea.py:87-102), _jacobi_preconditioner
  * (src/linear_solvers.py:71-88) and the set-up of deflatedPCG_CPU
@@ -122,7 +123,10 @@ int pyto_coarse_matrix_f64(pyto_handle* h, double* E, void* stream);
  * (src/deflation.py:504-584) with the stopping rule sqrt(rz/rz0) <= rtol and
  * maxIters; f and u are full-length DOF vectors (u[fixed] = 0), i.e.
  * linear_solvers.solve(..., Solvers.DPCG, ...) (src/linear_solvers.py:177-202).
- * Synchronises `stream`; returns PYTO_ENOCONV (u still written) at maxIters. */
+ * Synchronises `stream`; returns PYTO_ENOCONV (u still written) at maxIters,
+ * or when the iteration stagnates past its attainable accuracy (the relative
+ * residual grows 1e3x above its best, or gains < 1% for max(200, k/2)
+ * iterations), which the reference loop does not detect. */
 int pyto_solve_dpcg_f64(pyto_handle* h, const double* f, double* u, double rtol,
                         int max_iters, pyto_solve_stats* stats, void* stream);
 
