@@ -124,9 +124,10 @@ int pyto_coarse_matrix_f64(pyto_handle* h, double* E, void* stream);
  * maxIters; f and u are full-length DOF vectors (u[fixed] = 0), i.e.
  * linear_solvers.solve(..., Solvers.DPCG, ...) (src/linear_solvers.py:177-202).
  * Synchronises `stream`; returns PYTO_ENOCONV (u still written) at maxIters,
- * or when the iteration stagnates past its attainable accuracy (the relative
- * residual grows 1e3x above its best, or gains < 1% for max(200, k/2)
- * iterations), which the reference loop does not detect. */
+ * or when the iteration stagnates past its attainable accuracy (once the
+ * best relative residual is below 1e-11: it grows 1e3x above that best, or
+ * gains < 1% for max(200, k/2) iterations), which the reference loop does
+ * not detect. */
 int pyto_solve_dpcg_f64(pyto_handle* h, const double* f, double* u, double rtol,
                         int max_iters, pyto_solve_stats* stats, void* stream);
 
@@ -193,6 +194,34 @@ int pyto_filter_stencil(pyto_handle* h, double rel_radius, int capacity, int* co
 int pyto_oc_update(pyto_handle* h, const double* x, const double* dc, int64_t n, double dv,
                    double volfrac, double move, double xmin, double xmax, int max_steps,
                    double* xnew, double* lmid_host, int* steps_host, void* stream);
+
+/* ---- batches of independent systems ------------------------------------- */
+/* A batch solves many independent systems K_i u_i = f_i at once: several
+ * designs (handles, each with its own grid, densities and BCs) and several
+ * load cases per design (the same handle listed once per load case).  Every
+ * kernel of the iteration is one launch for the whole batch; each system has
+ * its own scalars and stopping test.  Per system the algorithm is Jacobi PCG
+ * (the reference's PCG branch, src/linear_solvers.py:157-176, no deflation)
+ * in FP32 inside FP64 iterative refinement, as pyto_solve_dpcg_mixed.  No
+ * reference counterpart: the reference solves one system per call.
+ * All handles must use cube elements, or all bricks. */
+typedef struct pyto_batch pyto_batch;
+int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, void* stream);
+int pyto_batch_destroy(pyto_batch* b);
+/* Node planes per workgroup of the batched operator (the tile depth; the
+ * autotuner times candidates with pyto_batch_time_matvec). */
+int pyto_batch_set_zchunk(pyto_batch* b, int zchunk, void* stream);
+int pyto_batch_info(pyto_batch* b, int* zchunk, int* matvec_blocks, int* vector_blocks);
+/* Average duration (ms) of `reps` launches of the batched FP32 operator. */
+int pyto_batch_time_matvec(pyto_batch* b, int reps, float* ms_per_launch, void* stream);
+/* Solve all systems: f[i], u[i] are device FP64 vectors of system i's DOFs
+ * (u overwritten).  Densities must be set on the handles.  stats (nsys
+ * entries, may be null) per system.  Synchronises `stream`; PYTO_ENOCONV if
+ * any system misses rtol within max_iters inner iterations / max_outer
+ * refinement steps. */
+int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, double rtol,
+                     double inner_rtol, int max_iters, int max_outer, pyto_solve_stats* stats,
+                     void* stream);
 
 #ifdef __cplusplus
 }

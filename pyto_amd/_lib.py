@@ -67,6 +67,12 @@ _SIGS = {
     "pyto_pnorm_adjoint_rhs": ([_vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp, _vp], _i),
     "pyto_filter_apply": ([_vp, _d, _i, _vp, _vp, _vp, _vp], _i),
     "pyto_filter_stencil": ([_vp, _d, _i, ctypes.POINTER(_i), _vp, _vp], _i),
+    "pyto_batch_create": ([ctypes.POINTER(_vp), _i, _vp, _vp], _i),
+    "pyto_batch_destroy": ([_vp], _i),
+    "pyto_batch_set_zchunk": ([_vp, _i, _vp], _i),
+    "pyto_batch_info": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)], _i),
+    "pyto_batch_time_matvec": ([_vp, _i, ctypes.POINTER(ctypes.c_float), _vp], _i),
+    "pyto_batch_solve": ([_vp, _vp, _vp, _d, _d, _i, _i, _vp, _vp], _i),
     "pyto_oc_update": ([_vp, _vp, _vp, ctypes.c_int64, _d, _d, _d, _d, _d, _i, _vp,
                         ctypes.POINTER(_d), ctypes.POINTER(_i), _vp], _i),
 }

@@ -1,0 +1,98 @@
+"""Many independent designs and load cases solved together on one GPU.
+
+``BatchSolver(operators, loads)`` takes one ``GridOperator`` per design
+(each with its own grid, BCs and densities) and, per design, a list of load
+vectors.  Every (design, load case) pair is one system; all systems iterate
+in lockstep through batched kernels (one launch per kernel per iteration for
+the whole batch, captured as a hipGraph), each with its own scalars and
+stopping test.  Per system: Jacobi PCG in FP32 inside FP64 iterative
+refinement.  ``autotune()`` picks the operator's tile depth (node planes per
+workgroup) for this batch by timing the batched FP32 operator.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import SolveStats, check, stream_ptr
+
+F64 = torch.float64
+
+
+class BatchSolver:
+    def __init__(self, operators, loads):
+        if len(operators) != len(loads):
+            raise ValueError("one list of loads per operator")
+        self.ops = list(operators)
+        self.systems = []          # (design index, load index)
+        self.forces = []
+        for d, (op, fl) in enumerate(zip(self.ops, loads)):
+            if len(fl) == 0:
+                raise ValueError("design %d has no load case" % d)
+            for k, f in enumerate(fl):
+                ft = op.as_dof(torch.as_tensor(f, dtype=F64))
+                if ft.numel() != op.num_dofs:
+                    raise ValueError("load %d of design %d has %d DOFs, expected %d"
+                                     % (k, d, ft.numel(), op.num_dofs))
+                self.systems.append((d, k))
+                self.forces.append(ft)
+        self.nsys = len(self.systems)
+        self.device = self.ops[0].device
+        hs = (ctypes.c_void_p * self.nsys)(*[self.ops[d].handle for d, _ in self.systems])
+        self._b = ctypes.c_void_p()
+        check(_lib.lib().pyto_batch_create(ctypes.byref(self._b), self.nsys, hs, stream_ptr()),
+              "pyto_batch_create")
+        self.tuning = None
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            _lib.lib().pyto_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        z, m, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(_lib.lib().pyto_batch_info(self._b, ctypes.byref(z), ctypes.byref(m), ctypes.byref(v)))
+        return {"zchunk": z.value, "matvec_blocks": m.value, "vector_blocks": v.value}
+
+    def set_zchunk(self, z):
+        check(_lib.lib().pyto_batch_set_zchunk(self._b, int(z), stream_ptr()), "pyto_batch_set_zchunk")
+
+    def time_matvec(self, reps=10):
+        ms = ctypes.c_float()
+        check(_lib.lib().pyto_batch_time_matvec(self._b, int(reps), ctypes.byref(ms), stream_ptr()))
+        return ms.value
+
+    def autotune(self, candidates=(4, 6, 8, 12, 16, 24, 32, 48, 62), reps=10):
+        """Time the batched FP32 operator for each tile depth; keep the fastest.
+        Densities must be set (the operator reads E)."""
+        times = {}
+        for z in candidates:
+            self.set_zchunk(z)
+            self.time_matvec(2)
+            times[z] = self.time_matvec(reps)
+        best = min(times, key=times.get)
+        self.set_zchunk(best)
+        self.tuning = {"zchunk": best, "ms_per_matvec": times}
+        return self.tuning
+
+    def solve(self, rtol=1e-8, inner_rtol=1e-4, max_iters=200000, max_outer=60, out=None):
+        """Returns (list of solutions per system in system order, list of
+        stats dicts).  Systems are ordered design by design, load case by
+        load case (``self.systems``)."""
+        us = out if out is not None else [torch.empty(self.ops[d].num_dofs, dtype=F64,
+                                                      device=self.device)
+                                          for d, _ in self.systems]
+        fp = (ctypes.c_void_p * self.nsys)(*[f.data_ptr() for f in self.forces])
+        up = (ctypes.c_void_p * self.nsys)(*[u.data_ptr() for u in us])
+        st = (SolveStats * self.nsys)()
+        rc = _lib.lib().pyto_batch_solve(self._b, fp, up, float(rtol), float(inner_rtol),
+                                         int(max_iters), int(max_outer), st, stream_ptr())
+        check(rc, "batch solve")
+        return us, [st[i].as_dict() for i in range(self.nsys)]

@@ -1,0 +1,777 @@
+// Batched Jacobi PCG over many independent systems (designs x load cases)
+// in FP32 inside FP64 iterative refinement.
+//
+// Every kernel is ONE launch for all systems of the batch: a device table of
+// systems (grid, E_e, Jacobi diagonal and fixed mask of the system's operator,
+// the system's vectors) and prefix sums of per-system block counts map each
+// workgroup to its system (binary search) and to its share of that system.
+// Each system has its own scalars and flags, so each stops on its own test;
+// launches of finished systems return at once.  The inner iteration
+// (operator, update, scalars, direction) is captured once as a hipGraph of
+// kBatchGraphIters iterations and replayed until every system has stopped.
+//
+// Per system the algorithm is the plain Jacobi PCG of the reference's PCG
+// branch (src/linear_solvers.py:157-176 with _jacobi_preconditioner,
+// :71-88), with the FP64 refinement of the single-system mixed solver:
+// r = f - K u in FP64, inner FP32 PCG on K d = r / sqrt(r^T M r) to
+// min(0.1, max(inner_rtol, 0.3 rtol / rel)), u += sqrt(r^T M r) d, until the
+// FP64 sqrt(r^T M r / r0^T M r0) <= rtol.
+#include <algorithm>
+#include <vector>
+
+#include "matvec.hpp"
+
+namespace pyto {
+
+constexpr int kBatchGraphIters = 16;   // inner iterations per captured graph
+
+struct BatchSys {
+  GridDims g;
+  ModalConsts<float> mc32;
+  ModalConsts<double> mc64;
+  const float* E32;
+  const double* E64;
+  const float* invd32;
+  const double* invd64;
+  const uint8_t* mask;
+  const uint8_t* tileflags;
+  float *r, *p, *q, *x, *b;
+  double *r64, *q64;
+  const double* f;
+  double* u;
+  int64_t ndof;
+  int tiles_x, ntiles, zchunk;
+};
+
+// Outer-refinement flag: the system has met rtol (or has a zero load).
+constexpr int F_ODONE = 7;
+
+__device__ __forceinline__ int find_sys(const int* __restrict__ pre, int nsys, int b) {
+  int lo = 0, hi = nsys;   // pre[lo] <= b < pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---- operator --------------------------------------------------------------
+template <bool CUBE>
+__global__ __launch_bounds__(256) void k_bmatvec_inner(const BatchSys* __restrict__ sys,
+                                                       const int* __restrict__ pre, int nsys,
+                                                       const int* __restrict__ flags,
+                                                       double* __restrict__ partials) {
+  const int gb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int s = find_sys(pre, nsys, gb);
+  const BatchSys& S = sys[s];
+  MatvecArgs<float> a;
+  a.u = S.p;
+  a.y = S.q;
+  a.Ee = S.E32;
+  a.mask = S.mask;
+  a.tileflags = S.tileflags;
+  a.g = S.g;
+  a.mc = S.mc32;
+  a.zchunk = S.zchunk;
+  a.tiles_x = S.tiles_x;
+  a.ntiles = S.ntiles;
+  a.partials = partials;
+  a.done = flags + s * kNumFlags + F_DONE;
+  a.zero_vec = nullptr;
+  a.zero_n = 0;
+  matvec_tile<float, CUBE>(a, gb - pre[s], gb);
+}
+
+template <bool CUBE>
+__global__ __launch_bounds__(256) void k_bmatvec_outer(const BatchSys* __restrict__ sys,
+                                                       const int* __restrict__ pre, int nsys,
+                                                       const int* __restrict__ flags) {
+  const int gb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int s = find_sys(pre, nsys, gb);
+  const BatchSys& S = sys[s];
+  MatvecArgs<double> a;
+  a.u = S.u;
+  a.y = S.q64;
+  a.Ee = S.E64;
+  a.mask = S.mask;
+  a.tileflags = S.tileflags;
+  a.g = S.g;
+  a.mc = S.mc64;
+  a.zchunk = S.zchunk;
+  a.tiles_x = S.tiles_x;
+  a.ntiles = S.ntiles;
+  a.partials = nullptr;
+  a.done = flags + s * kNumFlags + F_ODONE;
+  a.zero_vec = nullptr;
+  a.zero_n = 0;
+  matvec_tile<double, CUBE>(a, gb - pre[s], gb);
+}
+
+// ---- vector kernels: a block works on a contiguous share of one system ----
+struct VecShare {
+  int s;        // system
+  int lb, nb;   // block within the system, blocks of the system
+};
+__device__ __forceinline__ VecShare vec_share(const int* __restrict__ vpre, int nsys) {
+  VecShare v;
+  v.s = find_sys(vpre, nsys, blockIdx.x);
+  v.lb = blockIdx.x - vpre[v.s];
+  v.nb = vpre[v.s + 1] - vpre[v.s];
+  return v;
+}
+
+// x = 0, r = b, partials of r^T M r (systems still refining).
+__global__ __launch_bounds__(256) void k_binit(const BatchSys* __restrict__ sys,
+                                               const int* __restrict__ vpre, int nsys,
+                                               const int* __restrict__ flags,
+                                               double* __restrict__ part) {
+  const VecShare v = vec_share(vpre, nsys);
+  __shared__ double red[4];
+  if (flags[v.s * kNumFlags + F_ODONE]) return;
+  const BatchSys& S = sys[v.s];
+  const int64_t nv = S.ndof / 4;
+  const float4* bv = reinterpret_cast<const float4*>(S.b);
+  const float4* dv = reinterpret_cast<const float4*>(S.invd32);
+  float4* rv = reinterpret_cast<float4*>(S.r);
+  float4* xv = reinterpret_cast<float4*>(S.x);
+  double acc = 0.0;
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < nv; i += (int64_t)v.nb * 256) {
+    const float4 bb = bv[i], dd = dv[i];
+    rv[i] = bb;
+    xv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    acc += (double)bb.x * dd.x * bb.x + (double)bb.y * dd.y * bb.y +
+           (double)bb.z * dd.z * bb.z + (double)bb.w * dd.w * bb.w;
+  }
+  if (v.lb == 0)
+    for (int64_t i = nv * 4 + threadIdx.x; i < S.ndof; i += 256) {
+      const float bb = S.b[i];
+      S.r[i] = bb;
+      S.x[i] = 0.f;
+      acc += (double)bb * S.invd32[i] * bb;
+    }
+  const double bs = block_reduce_sum256(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// One block per system: rz0, inner tolerance and flags of a new inner solve.
+__global__ __launch_bounds__(256) void k_bsetup(const int* __restrict__ vpre, int nsys,
+                                                double* __restrict__ scal,
+                                                int* __restrict__ flags,
+                                                const double* __restrict__ part, double rtol,
+                                                double inner_rtol) {
+  __shared__ double red[5];
+  const int s = blockIdx.x;
+  int* fl = flags + s * kNumFlags;
+  double* sc = scal + s * kNumScalars;
+  if (fl[F_ODONE]) {
+    if (threadIdx.x == 0) {
+      fl[F_DONE] = 1;
+      fl[F_ITERS] = 0;
+      fl[F_BREAKDOWN] = 0;
+      fl[F_STALL] = 0;
+    }
+    return;
+  }
+  const double rz = block_sum_partials(part + vpre[s], vpre[s + 1] - vpre[s], red);
+  if (threadIdx.x == 0) {
+    sc[S_RZ] = rz;
+    sc[S_RZ0] = rz;
+    const double want = 0.3 * rtol / fmax(sc[S_TMP2], 1e-300);
+    sc[S_RTOL] = fmin(0.1, fmax(inner_rtol, want));
+    sc[S_RELMIN] = 1e300;
+    fl[F_DONE] = rz == 0.0 ? 1 : 0;
+    fl[F_ITERS] = 0;
+    fl[F_BREAKDOWN] = 0;
+    fl[F_STALL] = 0;
+  }
+}
+
+// alpha = rz / p^T q (system's matvec partials), r -= alpha q, r^T M r partials.
+__global__ __launch_bounds__(256) void k_bcg_update(const BatchSys* __restrict__ sys,
+                                                    const int* __restrict__ vpre,
+                                                    const int* __restrict__ mpre, int nsys,
+                                                    const double* __restrict__ scal,
+                                                    const int* __restrict__ flags,
+                                                    const double* __restrict__ part_mv,
+                                                    double* __restrict__ part_v) {
+  const VecShare v = vec_share(vpre, nsys);
+  __shared__ double red[5];
+  __shared__ double red4[4];
+  if (flags[v.s * kNumFlags + F_DONE]) return;
+  const BatchSys& S = sys[v.s];
+  const double pkp = block_sum_partials(part_mv + mpre[v.s], mpre[v.s + 1] - mpre[v.s], red);
+  const float a = (float)(scal[v.s * kNumScalars + S_RZ] / pkp);
+  const int64_t nv = S.ndof / 4;
+  float4* rv = reinterpret_cast<float4*>(S.r);
+  const float4* qv = reinterpret_cast<const float4*>(S.q);
+  const float4* dv = reinterpret_cast<const float4*>(S.invd32);
+  double acc = 0.0;
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < nv; i += (int64_t)v.nb * 256) {
+    float4 rr = rv[i];
+    const float4 qq = qv[i], dd = dv[i];
+    rr.x -= a * qq.x;
+    rr.y -= a * qq.y;
+    rr.z -= a * qq.z;
+    rr.w -= a * qq.w;
+    rv[i] = rr;
+    acc += (double)rr.x * dd.x * rr.x + (double)rr.y * dd.y * rr.y +
+           (double)rr.z * dd.z * rr.z + (double)rr.w * dd.w * rr.w;
+  }
+  if (v.lb == 0)
+    for (int64_t i = nv * 4 + threadIdx.x; i < S.ndof; i += 256) {
+      const float ri = S.r[i] - a * S.q[i];
+      S.r[i] = ri;
+      acc += (double)ri * S.invd32[i] * ri;
+    }
+  const double bs = block_reduce_sum256(acc, red4);
+  if (threadIdx.x == 0) part_v[blockIdx.x] = bs;
+}
+
+// One block per system: the iteration's scalars, stop and stagnation tests
+// (as the single-system solver's k_scalars).
+__global__ __launch_bounds__(256) void k_bscalars(const int* __restrict__ vpre,
+                                                  const int* __restrict__ mpre, int nsys,
+                                                  double* __restrict__ scal,
+                                                  int* __restrict__ flags,
+                                                  const double* __restrict__ part_mv,
+                                                  const double* __restrict__ part_v) {
+  __shared__ double red[5];
+  const int s = blockIdx.x;
+  int* fl = flags + s * kNumFlags;
+  if (fl[F_DONE]) return;
+  double* sc = scal + s * kNumScalars;
+  const double pkp = block_sum_partials(part_mv + mpre[s], mpre[s + 1] - mpre[s], red);
+  const double rz_new = block_sum_partials(part_v + vpre[s], vpre[s + 1] - vpre[s], red);
+  if (threadIdx.x == 0) {
+    const double rz = sc[S_RZ];
+    const double rel = sqrt(fabs(rz_new / sc[S_RZ0]));
+    sc[S_ALPHA] = rz / pkp;
+    sc[S_REL] = rel;
+    fl[F_ITERS] += 1;
+    if (!(pkp > 0.0) || !(rz_new == rz_new)) {
+      fl[F_BREAKDOWN] = 1;
+      fl[F_DONE] = 1;
+    } else if (rel <= sc[S_RTOL]) {
+      fl[F_DONE] = 1;
+    } else if (rel < 0.99 * sc[S_RELMIN]) {
+      sc[S_RELMIN] = rel;
+      fl[F_STALL] = 0;
+    } else if (sc[S_RELMIN] < kFloor32 &&
+               (rel > kDivergeFactor * sc[S_RELMIN] ||
+                ++fl[F_STALL] >= max(kStallIters, fl[F_ITERS] / 2))) {
+      fl[F_BREAKDOWN] = 2;
+      fl[F_DONE] = 1;
+    }
+    sc[S_BETA] = rz_new / rz;
+    sc[S_RZ] = rz_new;
+  }
+}
+
+// x += alpha p (unless FIRST), p = invd o r + beta p (FIRST: p = invd o r).
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_bdir(const BatchSys* __restrict__ sys,
+                                              const int* __restrict__ vpre, int nsys,
+                                              const double* __restrict__ scal,
+                                              const int* __restrict__ flags) {
+  const VecShare v = vec_share(vpre, nsys);
+  if (flags[v.s * kNumFlags + F_DONE]) return;
+  const BatchSys& S = sys[v.s];
+  const double* sc = scal + v.s * kNumScalars;
+  const float al = FIRST ? 0.f : (float)sc[S_ALPHA];
+  const float be = FIRST ? 0.f : (float)sc[S_BETA];
+  const int64_t nv = S.ndof / 4;
+  float4* pv = reinterpret_cast<float4*>(S.p);
+  float4* xv = reinterpret_cast<float4*>(S.x);
+  const float4* rv = reinterpret_cast<const float4*>(S.r);
+  const float4* dv = reinterpret_cast<const float4*>(S.invd32);
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < nv; i += (int64_t)v.nb * 256) {
+    const float4 rr = rv[i], dd = dv[i];
+    float4 pp = FIRST ? make_float4(0.f, 0.f, 0.f, 0.f) : pv[i];
+    if (!FIRST) {
+      float4 xx = xv[i];
+      xx.x += al * pp.x;
+      xx.y += al * pp.y;
+      xx.z += al * pp.z;
+      xx.w += al * pp.w;
+      xv[i] = xx;
+    }
+    pp.x = dd.x * rr.x + be * pp.x;
+    pp.y = dd.y * rr.y + be * pp.y;
+    pp.z = dd.z * rr.z + be * pp.z;
+    pp.w = dd.w * rr.w + be * pp.w;
+    pv[i] = pp;
+  }
+  if (v.lb == 0)
+    for (int64_t i = nv * 4 + threadIdx.x; i < S.ndof; i += 256) {
+      const float pp = FIRST ? 0.f : S.p[i];
+      if (!FIRST) S.x[i] += al * pp;
+      S.p[i] = S.invd32[i] * S.r[i] + be * pp;
+    }
+}
+
+// The x += alpha p that the stopping iteration leaves pending (converged
+// systems that iterated in this inner solve only).
+__global__ __launch_bounds__(256) void k_bfinal(const BatchSys* __restrict__ sys,
+                                                const int* __restrict__ vpre, int nsys,
+                                                const double* __restrict__ scal,
+                                                const int* __restrict__ flags) {
+  const VecShare v = vec_share(vpre, nsys);
+  const int* fl = flags + v.s * kNumFlags;
+  if (fl[F_ODONE] || !fl[F_DONE] || fl[F_BREAKDOWN] || fl[F_ITERS] == 0) return;
+  const BatchSys& S = sys[v.s];
+  const float al = (float)scal[v.s * kNumScalars + S_ALPHA];
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < S.ndof; i += (int64_t)v.nb * 256)
+    S.x[i] += al * S.p[i];
+}
+
+// ---- FP64 outer refinement --------------------------------------------------
+// u = 0 (start of a solve).
+__global__ __launch_bounds__(256) void k_bzero_u(const BatchSys* __restrict__ sys,
+                                                 const int* __restrict__ vpre, int nsys) {
+  const VecShare v = vec_share(vpre, nsys);
+  const BatchSys& S = sys[v.s];
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < S.ndof; i += (int64_t)v.nb * 256)
+    S.u[i] = 0.0;
+}
+
+// r64 = f - q64 (fixed DOFs carry invd = 0 and drop out of r^T M r), partials.
+__global__ __launch_bounds__(256) void k_bresid64(const BatchSys* __restrict__ sys,
+                                                  const int* __restrict__ vpre, int nsys,
+                                                  const int* __restrict__ flags, bool first,
+                                                  double* __restrict__ part) {
+  const VecShare v = vec_share(vpre, nsys);
+  __shared__ double red[4];
+  if (flags[v.s * kNumFlags + F_ODONE]) return;
+  const BatchSys& S = sys[v.s];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < S.ndof; i += (int64_t)v.nb * 256) {
+    const double ri = S.f[i] - (first ? 0.0 : S.q64[i]);
+    S.r64[i] = ri;
+    acc += ri * S.invd64[i] * ri;
+  }
+  const double bs = block_reduce_sum256(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// One block per system: outer relative residual, stop test, scale.
+__global__ __launch_bounds__(256) void k_bouter(const int* __restrict__ vpre, int nsys,
+                                                double* __restrict__ scal,
+                                                int* __restrict__ flags,
+                                                const double* __restrict__ part, bool first,
+                                                double rtol) {
+  __shared__ double red[5];
+  const int s = blockIdx.x;
+  int* fl = flags + s * kNumFlags;
+  if (fl[F_ODONE]) return;
+  double* sc = scal + s * kNumScalars;
+  const double rmr = block_sum_partials(part + vpre[s], vpre[s + 1] - vpre[s], red);
+  if (threadIdx.x == 0) {
+    if (first) sc[S_TMP1] = rmr;
+    const double rel = sc[S_TMP1] > 0.0 ? sqrt(rmr / sc[S_TMP1]) : 0.0;
+    sc[S_TMP2] = rel;
+    sc[S_TMP0] = rmr;
+    if (rel <= rtol) fl[F_ODONE] = 1;
+    else sc[S_SCALE] = sqrt(rmr);
+  }
+}
+
+// b32 = r64 / scale.
+__global__ __launch_bounds__(256) void k_bscale(const BatchSys* __restrict__ sys,
+                                                const int* __restrict__ vpre, int nsys,
+                                                const double* __restrict__ scal,
+                                                const int* __restrict__ flags) {
+  const VecShare v = vec_share(vpre, nsys);
+  if (flags[v.s * kNumFlags + F_ODONE]) return;
+  const BatchSys& S = sys[v.s];
+  const double inv = 1.0 / scal[v.s * kNumScalars + S_SCALE];
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < S.ndof; i += (int64_t)v.nb * 256)
+    S.b[i] = (float)(S.r64[i] * inv);
+}
+
+// u += scale * x32.
+__global__ __launch_bounds__(256) void k_baxpy(const BatchSys* __restrict__ sys,
+                                               const int* __restrict__ vpre, int nsys,
+                                               const double* __restrict__ scal,
+                                               const int* __restrict__ flags) {
+  const VecShare v = vec_share(vpre, nsys);
+  if (flags[v.s * kNumFlags + F_ODONE]) return;
+  const BatchSys& S = sys[v.s];
+  const double sc = scal[v.s * kNumScalars + S_SCALE];
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < S.ndof; i += (int64_t)v.nb * 256)
+    S.u[i] += sc * (double)S.x[i];
+}
+
+// All flags of the batch to F_DONE (timing runs of the operator use DONE = 0).
+__global__ void k_bset_flag(int* flags, int nsys, int slot, int value) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nsys; s += gridDim.x * blockDim.x)
+    flags[s * kNumFlags + slot] = value;
+}
+
+// ---- host side ----------------------------------------------------------------
+struct Batch {
+  int device = 0;
+  int nsys = 0;
+  bool cube = true;
+  std::vector<pyto_handle*> handles;
+  std::vector<BatchSys> hs;        // host copy of the table
+  BatchSys* sys = nullptr;
+  std::vector<int> h_mpre, h_vpre;
+  int* mpre = nullptr;
+  int* vpre = nullptr;
+  double* scal = nullptr;
+  int* flags = nullptr;
+  double* host_scal = nullptr;
+  int* host_flags = nullptr;
+  double* part_mv = nullptr;
+  double* part_v = nullptr;
+  int cap_mv = 0;
+  std::vector<void*> bufs;
+  hipStream_t gstream = nullptr;
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int zchunk = 16;
+};
+
+}  // namespace pyto
+
+struct pyto_batch : public pyto::Batch {};
+
+namespace pyto {
+
+static int batch_geometry(Batch* b, int zc, hipStream_t s) {
+  b->h_mpre.assign(b->nsys + 1, 0);
+  b->h_vpre.assign(b->nsys + 1, 0);
+  for (int i = 0; i < b->nsys; ++i) {
+    BatchSys& S = b->hs[i];
+    const int z = std::max(1, std::min({zc, 62, S.g.sz}));
+    S.zchunk = z;
+    const int mb = S.ntiles * ((S.g.sz + z - 1) / z);
+    b->h_mpre[i + 1] = b->h_mpre[i] + mb;
+    const int64_t nv = (S.ndof / 4 + 1023) / 1024;   // ~4 float4 per thread
+    b->h_vpre[i + 1] = b->h_vpre[i] + (int)std::max<int64_t>(1, std::min<int64_t>(nv, 2048));
+  }
+  b->zchunk = zc;
+  const int nm = b->h_mpre[b->nsys];
+  if (nm > b->cap_mv) {
+    if (b->part_mv) hipFree(b->part_mv);
+    b->part_mv = nullptr;
+    PYTO_CHECK_HIP(hipMalloc((void**)&b->part_mv, (size_t)nm * sizeof(double)));
+    b->cap_mv = nm;
+  }
+  PYTO_CHECK_HIP(hipMemcpyAsync(b->sys, b->hs.data(), b->hs.size() * sizeof(BatchSys),
+                                hipMemcpyHostToDevice, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(b->mpre, b->h_mpre.data(), (b->nsys + 1) * sizeof(int),
+                                hipMemcpyHostToDevice, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(b->vpre, b->h_vpre.data(), (b->nsys + 1) * sizeof(int),
+                                hipMemcpyHostToDevice, s));
+  PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  if (b->graph) {
+    hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return PYTO_OK;
+}
+
+static void launch_bmatvec_inner(Batch* b, hipStream_t s) {
+  const int nm = b->h_mpre[b->nsys];
+  if (b->cube)
+    hipLaunchKernelGGL(k_bmatvec_inner<true>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
+                       b->nsys, (const int*)b->flags, b->part_mv);
+  else
+    hipLaunchKernelGGL(k_bmatvec_inner<false>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
+                       b->nsys, (const int*)b->flags, b->part_mv);
+}
+
+static int batch_iteration(Batch* b, hipStream_t s) {
+  const int nv = b->h_vpre[b->nsys];
+  launch_bmatvec_inner(b, s);
+  hipLaunchKernelGGL(k_bcg_update, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, b->mpre, b->nsys,
+                     (const double*)b->scal, (const int*)b->flags, (const double*)b->part_mv,
+                     b->part_v);
+  hipLaunchKernelGGL(k_bscalars, dim3(b->nsys), dim3(256), 0, s, b->vpre, b->mpre, b->nsys,
+                     b->scal, b->flags, (const double*)b->part_mv, (const double*)b->part_v);
+  hipLaunchKernelGGL(k_bdir<false>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, b->nsys,
+                     (const double*)b->scal, (const int*)b->flags);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+static int batch_read(Batch* b, hipStream_t s) {
+  PYTO_CHECK_HIP(hipMemcpyAsync(b->host_flags, b->flags, (size_t)b->nsys * kNumFlags * sizeof(int),
+                                hipMemcpyDeviceToHost, s));
+  PYTO_CHECK_HIP(hipMemcpyAsync(b->host_scal, b->scal,
+                                (size_t)b->nsys * kNumScalars * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  return PYTO_OK;
+}
+
+static bool all_flag(const Batch* b, int slot) {
+  for (int i = 0; i < b->nsys; ++i)
+    if (!b->host_flags[i * kNumFlags + slot]) return false;
+  return true;
+}
+
+}  // namespace pyto
+
+using namespace pyto;
+
+extern "C" {
+
+int pyto_batch_destroy(pyto_batch* b);
+
+int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, void* stream) {
+  PYTO_REQUIRE(out && handles && nsys > 0, PYTO_EINVAL, "null argument or nsys <= 0");
+  pyto_batch* b = new pyto_batch();
+  *out = nullptr;
+  hipGetDevice(&b->device);
+  b->nsys = nsys;
+  b->handles.assign(handles, handles + nsys);
+  auto fail = [&](int rc) { pyto_batch_destroy(b); return rc; };
+  hipStream_t s = as_stream(stream);
+  for (int i = 0; i < nsys; ++i) {
+    const Handle* h = handles[i];
+    if (!h) {
+      set_error("handle %d is null", i);
+      return fail(PYTO_EINVAL);
+    }
+    const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
+    if (i == 0) b->cube = cube;
+    if (cube != b->cube) {
+      set_error("a batch holds either cube or brick elements, not both (system %d)", i);
+      return fail(PYTO_EINVAL);
+    }
+    BatchSys S{};
+    S.g = h->g;
+    S.mc32 = h->mc32;
+    S.mc64 = h->mc64;
+    S.E32 = h->Ee32;
+    S.E64 = h->Ee64;
+    S.invd32 = h->invd32;
+    S.invd64 = h->invd64;
+    S.mask = h->fixmask;
+    S.tileflags = h->tileflags;
+    S.ndof = 3 * h->g.nnodes;
+    S.tiles_x = (h->g.sx + kOwn - 1) / kOwn;
+    S.ntiles = S.tiles_x * ((h->g.sy + kOwn - 1) / kOwn);
+    const size_t n32 = (size_t)S.ndof * 4, n64 = (size_t)S.ndof * 8;
+    float* f32[5];
+    for (int k = 0; k < 5; ++k) {
+      void* p = nullptr;
+      if (hipMalloc(&p, n32) != hipSuccess) {
+        set_error("device allocation of %zu bytes failed", n32);
+        return fail(PYTO_ENOMEM);
+      }
+      b->bufs.push_back(p);
+      hipMemsetAsync(p, 0, n32, s);
+      f32[k] = (float*)p;
+    }
+    S.r = f32[0];
+    S.p = f32[1];
+    S.q = f32[2];
+    S.x = f32[3];
+    S.b = f32[4];
+    double* f64[2];
+    for (int k = 0; k < 2; ++k) {
+      void* p = nullptr;
+      if (hipMalloc(&p, n64) != hipSuccess) {
+        set_error("device allocation of %zu bytes failed", n64);
+        return fail(PYTO_ENOMEM);
+      }
+      b->bufs.push_back(p);
+      hipMemsetAsync(p, 0, n64, s);
+      f64[k] = (double*)p;
+    }
+    S.r64 = f64[0];
+    S.q64 = f64[1];
+    b->hs.push_back(S);
+  }
+  if (hipMalloc((void**)&b->sys, nsys * sizeof(BatchSys)) != hipSuccess ||
+      hipMalloc((void**)&b->mpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&b->vpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&b->scal, (size_t)nsys * kNumScalars * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&b->flags, (size_t)nsys * kNumFlags * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void**)&b->host_scal, (size_t)nsys * kNumScalars * sizeof(double),
+                    hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&b->host_flags, (size_t)nsys * kNumFlags * sizeof(int),
+                    hipHostMallocDefault) != hipSuccess) {
+    set_error("batch allocation failed");
+    return fail(PYTO_ENOMEM);
+  }
+  hipMemsetAsync(b->scal, 0, (size_t)nsys * kNumScalars * sizeof(double), s);
+  hipMemsetAsync(b->flags, 0, (size_t)nsys * kNumFlags * sizeof(int), s);
+  int rc = batch_geometry(b, 16, s);
+  if (rc) return fail(rc);
+  int64_t nvtot = b->h_vpre[nsys];
+  if (hipMalloc((void**)&b->part_v, (size_t)nvtot * sizeof(double)) != hipSuccess) {
+    set_error("batch allocation failed");
+    return fail(PYTO_ENOMEM);
+  }
+  if (hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_a, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_b, hipEventDisableTiming) != hipSuccess) {
+    set_error("stream/event creation failed");
+    return fail(PYTO_EHIP);
+  }
+  *out = b;
+  return PYTO_OK;
+}
+
+int pyto_batch_destroy(pyto_batch* b) {
+  if (!b) return PYTO_OK;
+  for (void* p : b->bufs) hipFree(p);
+  void* ptrs[] = {b->sys, b->mpre, b->vpre, b->scal, b->flags, b->part_mv, b->part_v};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  if (b->ev_a) hipEventDestroy(b->ev_a);
+  if (b->ev_b) hipEventDestroy(b->ev_b);
+  if (b->gstream) hipStreamDestroy(b->gstream);
+  if (b->host_scal) hipHostFree(b->host_scal);
+  if (b->host_flags) hipHostFree(b->host_flags);
+  delete b;
+  return PYTO_OK;
+}
+
+int pyto_batch_set_zchunk(pyto_batch* b, int zchunk, void* stream) {
+  PYTO_REQUIRE(b && zchunk >= 1, PYTO_EINVAL, "bad arguments");
+  return batch_geometry(b, zchunk, as_stream(stream));
+}
+
+int pyto_batch_info(pyto_batch* b, int* zchunk, int* matvec_blocks, int* vector_blocks) {
+  PYTO_REQUIRE(b, PYTO_EINVAL, "null batch");
+  if (zchunk) *zchunk = b->zchunk;
+  if (matvec_blocks) *matvec_blocks = b->h_mpre[b->nsys];
+  if (vector_blocks) *vector_blocks = b->h_vpre[b->nsys];
+  return PYTO_OK;
+}
+
+int pyto_batch_time_matvec(pyto_batch* b, int reps, float* ms_per_launch, void* stream) {
+  PYTO_REQUIRE(b && reps > 0 && ms_per_launch, PYTO_EINVAL, "bad arguments");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_bset_flag, dim3(1), dim3(256), 0, s, b->flags, b->nsys, (int)F_DONE, 0);
+  launch_bmatvec_inner(b, s);
+  hipEvent_t e0, e1;
+  PYTO_CHECK_HIP(hipEventCreate(&e0));
+  PYTO_CHECK_HIP(hipEventCreate(&e1));
+  hipEventRecord(e0, s);
+  for (int k = 0; k < reps; ++k) launch_bmatvec_inner(b, s);
+  hipEventRecord(e1, s);
+  PYTO_CHECK_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *ms_per_launch = ms / reps;
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, double rtol,
+                     double inner_rtol, int max_iters, int max_outer, pyto_solve_stats* stats,
+                     void* stream) {
+  PYTO_REQUIRE(b && f && u, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(rtol > 0 && inner_rtol > 0 && max_iters > 0 && max_outer > 0, PYTO_EINVAL,
+               "rtol, inner_rtol, max_iters and max_outer must be positive");
+  hipStream_t s = as_stream(stream);
+  for (int i = 0; i < b->nsys; ++i) {
+    PYTO_REQUIRE(f[i] && u[i], PYTO_EINVAL, "null load or solution of system %d", i);
+    PYTO_REQUIRE(b->handles[i]->density_set, PYTO_ESTATE, "system %d: density not set", i);
+    b->hs[i].f = f[i];
+    b->hs[i].u = u[i];
+  }
+  PYTO_CHECK_HIP(hipMemcpyAsync(b->sys, b->hs.data(), b->hs.size() * sizeof(BatchSys),
+                                hipMemcpyHostToDevice, s));
+  const int nsys = b->nsys;
+  const int nv = b->h_vpre[nsys];
+  const int nm = b->h_mpre[nsys];
+  std::vector<int> inner_iters(nsys, 0), outer_iters(nsys, 0);
+  hipLaunchKernelGGL(k_bset_flag, dim3(1), dim3(256), 0, s, b->flags, nsys, (int)F_ODONE, 0);
+  hipLaunchKernelGGL(k_bzero_u, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys);
+  int launched_total = 0;
+  for (int outer = 0; outer < max_outer; ++outer) {
+    const bool first = outer == 0;
+    if (!first) {
+      if (b->cube)
+        hipLaunchKernelGGL(k_bmatvec_outer<true>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
+                           nsys, (const int*)b->flags);
+      else
+        hipLaunchKernelGGL(k_bmatvec_outer<false>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
+                           nsys, (const int*)b->flags);
+    }
+    hipLaunchKernelGGL(k_bresid64, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                       (const int*)b->flags, first, b->part_v);
+    hipLaunchKernelGGL(k_bouter, dim3(nsys), dim3(256), 0, s, b->vpre, nsys, b->scal, b->flags,
+                       (const double*)b->part_v, first, rtol);
+    PYTO_CHECK_HIP(hipGetLastError());
+    PYTO_TRY(batch_read(b, s));
+    if (all_flag(b, F_ODONE) || launched_total >= max_iters) break;
+    for (int i = 0; i < nsys; ++i)
+      if (!b->host_flags[i * kNumFlags + F_ODONE]) outer_iters[i]++;
+    hipLaunchKernelGGL(k_bscale, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                       (const double*)b->scal, (const int*)b->flags);
+    hipLaunchKernelGGL(k_binit, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                       (const int*)b->flags, b->part_v);
+    hipLaunchKernelGGL(k_bsetup, dim3(nsys), dim3(256), 0, s, b->vpre, nsys, b->scal, b->flags,
+                       (const double*)b->part_v, rtol, inner_rtol);
+    hipLaunchKernelGGL(k_bdir<true>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                       (const double*)b->scal, (const int*)b->flags);
+    PYTO_CHECK_HIP(hipGetLastError());
+    // inner iterations: replays of the captured batch of kGraphBatch
+    if (!b->graph) {
+      hipGraph_t g;
+      PYTO_CHECK_HIP(hipStreamBeginCapture(b->gstream, hipStreamCaptureModeThreadLocal));
+      int rc = PYTO_OK;
+      for (int k = 0; k < kBatchGraphIters && rc == PYTO_OK; ++k) rc = batch_iteration(b, b->gstream);
+      hipError_t e = hipStreamEndCapture(b->gstream, &g);
+      if (rc != PYTO_OK) return rc;
+      PYTO_CHECK_HIP(e);
+      e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      PYTO_CHECK_HIP(e);
+    }
+    PYTO_CHECK_HIP(hipEventRecord(b->ev_a, s));
+    PYTO_CHECK_HIP(hipStreamWaitEvent(b->gstream, b->ev_a, 0));
+    int launched = 0;
+    while (launched_total + launched < max_iters) {
+      PYTO_CHECK_HIP(hipGraphLaunch(b->graph, b->gstream));
+      launched += kBatchGraphIters;
+      PYTO_TRY(batch_read(b, b->gstream));
+      if (all_flag(b, F_DONE)) break;
+    }
+    launched_total += launched;
+    PYTO_CHECK_HIP(hipEventRecord(b->ev_b, b->gstream));
+    PYTO_CHECK_HIP(hipStreamWaitEvent(s, b->ev_b, 0));
+    for (int i = 0; i < nsys; ++i) inner_iters[i] += b->host_flags[i * kNumFlags + F_ITERS];
+    hipLaunchKernelGGL(k_bfinal, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                       (const double*)b->scal, (const int*)b->flags);
+    hipLaunchKernelGGL(k_baxpy, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                       (const double*)b->scal, (const int*)b->flags);
+    PYTO_CHECK_HIP(hipGetLastError());
+  }
+  PYTO_TRY(batch_read(b, s));
+  bool all = true;
+  for (int i = 0; i < nsys; ++i) {
+    const bool conv = b->host_flags[i * kNumFlags + F_ODONE] != 0;
+    all = all && conv;
+    if (stats) {
+      pyto_solve_stats& st = stats[i];
+      st.iterations = inner_iters[i];
+      st.outer_iterations = outer_iters[i];
+      st.converged = conv;
+      st.matvecs64 = outer_iters[i] + 1;
+      st.matvecs32 = inner_iters[i];
+      st.rel = b->host_scal[i * kNumScalars + S_TMP2];
+      st.rel_true = -1.0;
+      st.setup_ms = 0.0;
+    }
+  }
+  if (!all) {
+    set_error("batch: not every system met rtol within max_iters / max_outer");
+    return PYTO_ENOCONV;
+  }
+  return PYTO_OK;
+}
+
+}  // extern "C"

@@ -302,55 +302,161 @@ __global__ __launch_bounds__(256) void k_coarse_gather(const double* __restrict_
   }
 }
 
-// Direction update of the deflated PCG, one node per thread:
+// Direction update of the deflated PCG:
 //   x += alpha p_old  (skipped on the first call),
 //   p  = invd o r + beta p_old - W mu  (fixed DOFs 0),
 // and block 0 clears the coarse right-hand side for the next iteration.
+// Vectorised over DOFs like the CG update: a thread owns one 16-byte vector
+// (4 FP32 / 2 FP64 DOFs), which spans at most two nodes; the node
+// coordinates are computed once and stepped to the next node.  Fixed DOFs
+// are those with invd = 0 (the Jacobi diagonal marks them), so the fixed mask
+// is not read.
+template <typename T>
+struct DirVec;
+template <>
+struct DirVec<float> {
+  using V = float4;
+  static constexpr int W = 4;
+  __device__ static float get(const V& v, int j) { return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w; }
+  __device__ static void set(V& v, int j, float a) {
+    if (j == 0) v.x = a; else if (j == 1) v.y = a; else if (j == 2) v.z = a; else v.w = a;
+  }
+};
+template <>
+struct DirVec<double> {
+  using V = double2;
+  static constexpr int W = 2;
+  __device__ static double get(const V& v, int j) { return j == 0 ? v.x : v.y; }
+  __device__ static void set(V& v, int j, double a) {
+    if (j == 0) v.x = a; else v.y = a;
+  }
+};
+
+// Component c of W mu at node (ix,iy,iz) of group gid.
+__device__ __forceinline__ double w_mu(int c, int ix, int iy, int iz, int gid,
+                                       const double* __restrict__ mu,
+                                       const double* __restrict__ centers, const GridDims& g) {
+  const double rx = ix * g.hx - centers[3 * gid];
+  const double ry = iy * g.hy - centers[3 * gid + 1];
+  const double rz = iz * g.hz - centers[3 * gid + 2];
+  const double* m6 = mu + 6 * gid;
+  return c == 0 ? m6[0] + rz * m6[4] - ry * m6[5]
+       : c == 1 ? m6[1] - rz * m6[3] + rx * m6[5]
+                : m6[2] + ry * m6[3] - rx * m6[4];
+}
+
+// Groups whose rigid map is staged in LDS by k_dir_update (more: read on the fly).
+constexpr int kLdsGroups = 512;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_dir_update(
     T* __restrict__ x, T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ invd,
     const double* __restrict__ mu, const int32_t* __restrict__ grp,
-    const double* __restrict__ centers, const uint8_t* __restrict__ mask, GridDims g,
+    const double* __restrict__ centers, int ngroups, GridDims g,
     const double* __restrict__ scal, bool first, const int* __restrict__ done,
     double* __restrict__ zero_vec, int zero_n) {
   if (done && *done) return;
   if (zero_vec && blockIdx.x == 0)
     for (int i = threadIdx.x; i < zero_n; i += blockDim.x) zero_vec[i] = 0.0;
+  // W mu of group q at position X is a_q + w_q x X with w = (mu3, mu4, mu5)
+  // and a = (mu0, mu1, mu2) - w x c_q: 6 values per group staged in LDS
+  __shared__ double aw[kLdsGroups * 6];
+  const bool lds = grp && ngroups <= kLdsGroups;
+  if (lds) {
+    for (int q = threadIdx.x; q < ngroups; q += blockDim.x) {
+      const double* m6 = mu + 6 * q;
+      const double wx = m6[3], wy = m6[4], wz = m6[5];
+      const double cx = centers[3 * q], cy = centers[3 * q + 1], cz = centers[3 * q + 2];
+      aw[6 * q + 0] = m6[0] - (wy * cz - wz * cy);
+      aw[6 * q + 1] = m6[1] - (wz * cx - wx * cz);
+      aw[6 * q + 2] = m6[2] - (wx * cy - wy * cx);
+      aw[6 * q + 3] = wx;
+      aw[6 * q + 4] = wy;
+      aw[6 * q + 5] = wz;
+    }
+    __syncthreads();
+  }
+  auto wmu = [&](int c, int px, int py, int pz, int q) -> double {
+    if (!lds) return w_mu(c, px, py, pz, q, mu, centers, g);
+    const double X = px * g.hx, Y = py * g.hy, Z = pz * g.hz;
+    const double* e = aw + 6 * q;
+    return c == 0 ? e[0] + e[4] * Z - e[5] * Y
+         : c == 1 ? e[1] + e[5] * X - e[3] * Z
+                  : e[2] + e[3] * Y - e[4] * X;
+  };
+  using DV = DirVec<T>;
+  using V = typename DV::V;
+  constexpr int W = DV::W;
   const double beta = first ? 0.0 : scal[S_BETA];
   const double alpha = first ? 0.0 : scal[S_ALPHA];
-  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < g.nnodes;
-       n += (int64_t)gridDim.x * blockDim.x) {
-    double wm[3] = {0, 0, 0};
+  const int64_t n = 3 * g.nnodes;
+  const int64_t nv = n / W;
+  V* pv = reinterpret_cast<V*>(p);
+  V* xv = reinterpret_cast<V*>(x);
+  const V* rv = reinterpret_cast<const V*>(r);
+  const V* dv = reinterpret_cast<const V*>(invd);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d0 = i * W;
+    const int64_t n0 = d0 / 3;
+    const int c0 = (int)(d0 - 3 * n0);
+    int ix = 0, iy = 0, iz = 0, jx = 0, jy = 0, jz = 0, g0 = 0, g1 = 0;
     if (grp) {
-      int ix, iy, iz;
-      node_coords(n, g, ix, iy, iz);
-      const int gid = grp[n];
-      const double rx = ix * g.hx - centers[3 * gid];
-      const double ry = iy * g.hy - centers[3 * gid + 1];
-      const double rz = iz * g.hz - centers[3 * gid + 2];
-      const double* m6 = mu + 6 * gid;
-      wm[0] = m6[0] + rz * m6[4] - ry * m6[5];
-      wm[1] = m6[1] - rz * m6[3] + rx * m6[5];
-      wm[2] = m6[2] + ry * m6[3] - rx * m6[4];
+      node_coords(n0, g, ix, iy, iz);
+      jx = ix + 1;
+      jy = iy;
+      jz = iz;
+      if (jx == g.sx) {
+        jx = 0;
+        if (++jy == g.sy) {
+          jy = 0;
+          ++jz;
+        }
+      }
+      g0 = grp[n0];
+      g1 = (c0 + W > 3 && n0 + 1 < g.nnodes) ? grp[n0 + 1] : g0;
     }
-    const uint8_t m = mask[n];
-    T pv[3], rv[3], dv[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      pv[c] = first ? T(0) : p[3 * n + c];
-      rv[c] = r[3 * n + c];
-      dv[c] = invd[3 * n + c];
-    }
+    V pp = pv[i];
+    const V rr = rv[i], dd = dv[i];
+    if (first) pp = V{};
     if (!first) {
+      V xx = xv[i];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) x[3 * n + c] = (T)((double)x[3 * n + c] + alpha * (double)pv[c]);
+      for (int j = 0; j < W; ++j)
+        DV::set(xx, j, (T)((double)DV::get(xx, j) + alpha * (double)DV::get(pp, j)));
+      xv[i] = xx;
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double val = (double)dv[c] * (double)rv[c] + beta * (double)pv[c] - wm[c];
-      p[3 * n + c] = ((m >> c) & 1) ? T(0) : (T)val;
+    for (int j = 0; j < W; ++j) {
+      const int cj = c0 + j;
+      const bool next = cj >= 3;
+      const int c = next ? cj - 3 : cj;
+      const double wm = grp ? wmu(c, next ? jx : ix, next ? jy : iy, next ? jz : iz,
+                                  next ? g1 : g0)
+                            : 0.0;
+      const T di = DV::get(dd, j);
+      const double val = (double)di * (double)DV::get(rr, j) + beta * (double)DV::get(pp, j) - wm;
+      DV::set(pp, j, di == T(0) ? T(0) : (T)val);
     }
+    pv[i] = pp;
   }
+  // tail: DOFs past the last whole vector, one node at a time by block 0
+  if (blockIdx.x == 0)
+    for (int64_t d = nv * W + threadIdx.x; d < n; d += blockDim.x) {
+      const int64_t nd = d / 3;
+      const int c = (int)(d - 3 * nd);
+      double wm = 0.0;
+      if (grp) {
+        int ix, iy, iz;
+        node_coords(nd, g, ix, iy, iz);
+        wm = wmu(c, ix, iy, iz, grp[nd]);
+      }
+      const T pd = first ? T(0) : p[d];
+      if (!first) x[d] = (T)((double)x[d] + alpha * (double)pd);
+      const T di = invd[d];
+      const double val = (double)di * (double)r[d] + beta * (double)pd - wm;
+      p[d] = di == T(0) ? T(0) : (T)val;
+    }
 }
 
 // x += alpha p (the update the last iteration leaves pending when it stops).
@@ -540,10 +646,13 @@ int launch_restrict_special(Handle* h, const T* r, const T* invd, const T* p,
 template <typename T>
 int launch_dir_update(Handle* h, T* x, T* p, const T* r, const T* invd, const double* mu,
                       bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n) {
-  const int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 8192);
+  // 2048 workgroups (8 per CU) striding over the vectors: each stages the
+  // group maps once, so fewer, longer-lived blocks keep those L2 reads small
+  const int64_t nvec = 3 * h->g.nnodes / (16 / (int64_t)sizeof(T));
+  const int blocks = (int)std::min<int64_t>((nvec + 255) / 256, 2048);
   hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), 0, s, x, p, r, invd, mu,
                      h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
-                     h->fixmask, h->g, (const double*)h->scal, first, done, zero_vec, zero_n);
+                     h->ngroups, h->g, (const double*)h->scal, first, done, zero_vec, zero_n);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }

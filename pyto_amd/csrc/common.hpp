@@ -183,7 +183,8 @@ enum ScalarSlot {
   // OC bisection: two (l1, l2, lmid, done) states alternating by step
   // parity, and the last multiplier at which the volume was evaluated
   S_OC_STATE = 16, S_OC_LAST = 24,
-  S_RELMIN = 25   // best relative residual of the running solve (stagnation test)
+  S_RELMIN = 25,  // best relative residual of the running solve (stagnation test)
+  S_FLOOR = 26    // the stagnation test applies once S_RELMIN is below this
 };
 // Device flag slots.
 // F_BREAKDOWN: 1 = p^T K p <= 0 or NaN, 2 = stagnation (see k_scalars)
@@ -192,12 +193,17 @@ enum FlagSlot { F_DONE = 0, F_ITERS = 1, F_BREAKDOWN = 2, F_COARSE_PIVOT = 3, F_
 // Past its attainable accuracy the deflated iteration's recursive residual
 // stops falling and then grows geometrically (rounding leaves coarse-space
 // components in r that no K-orthogonal direction removes; p^T K p shrinks,
-// alpha and the iterate grow).  A solve therefore stops as stagnated when
-// its relative residual exceeds kDivergeFactor times its best value, or has
-// not improved on it by 1% for max(kStallIters, k/2) iterations (k =
-// iterations so far).
+// alpha and the iterate grow).  Once its best relative residual is below a
+// floor near that accuracy (kFloor64 / kFloor32), a solve therefore stops as
+// stagnated when its relative residual exceeds kDivergeFactor times its best
+// value, or has not improved on it by 1% for max(kStallIters, k/2)
+// iterations (k = iterations so far).  Above the floor nothing is tested:
+// CG residuals of hard problems plateau for hundreds of iterations and rise
+// by large factors on the way down.
 constexpr int kStallIters = 200;
 constexpr double kDivergeFactor = 1e3;
+constexpr double kFloor64 = 1e-11;
+constexpr double kFloor32 = 1e-5;
 // Last-block counter slots (one per reduction site in flight).
 enum CounterSlot { C_MATVEC = 0, C_UPDATE = 1, C_DOT = 2, C_ELEM = 3, C_NORM = 4 };
 

@@ -16,7 +16,7 @@
 #include <cstring>
 #include <vector>
 
-#include "element.hpp"
+#include "matvec.hpp"
 
 namespace pyto {
 
@@ -99,242 +99,9 @@ __global__ void k_inv_diag(const double* __restrict__ E64, const uint8_t* __rest
   }
 }
 
-// ---------------------------------------------------------------------------
-// Matrix-free y = K_ff u.
-// ---------------------------------------------------------------------------
-constexpr int kTile = 16;
-constexpr int kOwn = 15;
-constexpr int kPl = 17;
-
-template <typename T>
-struct MatvecArgs {
-  const T* u;
-  T* y;
-  const T* Ee;
-  const uint8_t* mask;
-  GridDims g;
-  ModalConsts<T> mc;
-  const uint8_t* tileflags;  // per (tile, node plane): window holds a fixed node
-  int zchunk;
-  int tiles_x;
-  int ntiles;
-  double* partials;    // per-block partials of u.y (null: no dot)
-  const int* done;     // skip when *done != 0 (null: never)
-  double* zero_vec;    // zeroed by block 0 (null: none)
-  int zero_n;
-};
-
-// Lane (tx-1)'s value within the 16-lane row (0 for tx = 0): DPP row_shr:1.
-__device__ __forceinline__ float dpp_shr1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
-}
-__device__ __forceinline__ double dpp_shr1(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
-// Issue the loads of one node of the 17x17 staging plane without consuming
-// them: raw values in a[], the node's fixed bits (pending load) in ml, and
-// 7 in mo when the node is outside the grid (the address is then clamped
-// into the grid).  The masking happens when the plane is stored to LDS, so
-// the loads stay in flight across a whole layer of element work.
-template <typename T>
-__device__ __forceinline__ void issue_node(T a[3], int& mo, int& ml, const T* __restrict__ u,
-                                           const uint8_t* __restrict__ mask, const GridDims& g,
-                                           int gx, int gy, int kp, bool any_fixed) {
-  const bool inb = kp >= 0 && kp < g.sz && gx >= 0 && gx < g.sx && gy >= 0 && gy < g.sy;
-  const int cx = min(max(gx, 0), g.sx - 1), cy = min(max(gy, 0), g.sy - 1);
-  const int cz = min(max(kp, 0), g.sz - 1);
-  const int64_t n = cx + (int64_t)g.sx * (cy + (int64_t)g.sy * cz);
-  const T* p = u + 3 * n;
-  a[0] = p[0];
-  a[1] = p[1];
-  a[2] = p[2];
-  mo = inb ? 0 : 7;
-  ml = 0;
-  if (any_fixed) ml = mask[n];
-}
-
-// Layer loop, per element layer k (nodes k and k+1):
-//   store plane k+1 (prefetched into registers one layer earlier) to LDS,
-//   issue the global loads of plane k+2, write this thread's y-exchange of
-//   layer k-1, one barrier, finish node plane k-1 (owners), then read the
-//   corners of plane k+1 and apply element (ex, ey, k).
-// x-neighbour sums use DPP within the 16-lane rows (tx is the lane's row
-// position); only the y-neighbour sums go through LDS.
 template <typename T, bool CUBE>
 __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
-  if (a.done && *a.done) return;
-  __shared__ T pl[2][kPl][kPl][3];
-  __shared__ T xc[2][kTile][kTile][6];
-  __shared__ T es[2][kTile][kTile];   // E of the element layer below plane b
-  __shared__ uint8_t pm[2][kPl][kPl];  // fixed bits of the staged nodes
-  __shared__ double red[4];
-  const GridDims& g = a.g;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int tile = lb % a.ntiles;
-  const int bxi = tile % a.tiles_x, byi = tile / a.tiles_x;
-  const int x0 = bxi * kOwn - 1, y0 = byi * kOwn - 1;
-  const int ex = x0 + tx, ey = y0 + ty;
-  const int kz0 = (lb / a.ntiles) * a.zchunk;
-  const int kz1 = min(kz0 + a.zchunk, g.sz);
-  // fixed-node flags of the chunk's node planes kz0-1 .. kz1 (<= 64 planes,
-  // zchunk <= 62) as one wave-uniform bit mask: no memory access in the loop
-  uint64_t pfix;
-  {
-    const int kp = kz0 - 1 + (int)(threadIdx.x & 63);
-    const bool f = kp >= 0 && kp <= kz1 && kp < g.sz &&
-                   a.tileflags[(int64_t)tile * g.sz + kp] != 0;
-    pfix = __ballot(f);
-  }
-  auto plane_fixed = [&](int kp) { return ((pfix >> (kp - kz0 + 1)) & 1) != 0; };
-  const bool col_valid = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny;
-  const bool owner = tx >= 1 && ty >= 1 && ex < g.sx && ey < g.sy;
-
-  if (a.zero_vec && blockIdx.x == 0)
-    for (int i = threadIdx.x; i < a.zero_n; i += 256) a.zero_vec[i] = 0.0;
-
-  // staging-plane entries of this thread: e0 = tid, e1 = tid + 256 (< 289)
-  const int e0 = threadIdx.x, e1 = threadIdx.x + 256;
-  const bool has1 = e1 < kPl * kPl;
-  const int lx0 = e0 % kPl, ly0 = e0 / kPl;
-  const int lx1 = has1 ? e1 % kPl : 0, ly1 = has1 ? e1 / kPl : 0;
-  T pv0[3], pv1[3] = {T(0), T(0), T(0)};
-  int mo0 = 0, ml0 = 0, mo1 = 7, ml1 = 0;
-  // E of element layer kp-1, loaded with node plane kp and staged with it
-  const int64_t ecol = min(max(ex, 0), g.nx - 1) + (int64_t)g.nx * min(max(ey, 0), g.ny - 1);
-  T enext = T(0);
-  // fixed bits of this thread's own node, read from the staged plane when
-  // it becomes the top plane: mq byte 0 = newest plane
-  uint32_t mq = 0;
-  auto fetch = [&](int kp) {
-    const bool fx = plane_fixed(kp);
-    issue_node<T>(pv0, mo0, ml0, a.u, a.mask, g, x0 + lx0, y0 + ly0, kp, fx);
-    if (has1) issue_node<T>(pv1, mo1, ml1, a.u, a.mask, g, x0 + lx1, y0 + ly1, kp, fx);
-    const int kl = min(max(kp - 1, 0), g.nz - 1);
-    enext = a.Ee[ecol + (int64_t)g.nx * g.ny * kl];
-  };
-  auto store = [&](int b) {
-    const int m0 = mo0 | ml0, m1 = mo1 | ml1;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) pl[b][ly0][lx0][c] = ((m0 >> c) & 1) ? T(0) : pv0[c];
-    pm[b][ly0][lx0] = (uint8_t)m0;
-    if (has1) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) pl[b][ly1][lx1][c] = ((m1 >> c) & 1) ? T(0) : pv1[c];
-      pm[b][ly1][lx1] = (uint8_t)m1;
-    }
-    es[b][ty][tx] = enext;
-  };
-
-  T bot[4][3], top[4][3];
-  fetch(kz0 - 1);
-  store(0);
-  fetch(kz0);   // also E of layer kz0-1
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    bot[0][c] = pl[0][ty][tx][c];
-    bot[1][c] = pl[0][ty][tx + 1][c];
-    bot[2][c] = pl[0][ty + 1][tx + 1][c];
-    bot[3][c] = pl[0][ty + 1][tx][c];
-  }
-  mq = pm[0][ty][tx];
-  T carry[3] = {T(0), T(0), T(0)};   // top-plane sum of the layer before last
-  T xb[3], xt[3], yb[3], yt[3];       // pending sums of the last layer
-  T ubot[3];                          // u at this thread's node of the pending plane
-#pragma unroll
-  for (int c = 0; c < 3; ++c) xb[c] = xt[c] = yb[c] = yt[c] = ubot[c] = T(0);
-  double dot = 0.0;
-  for (int k = kz0 - 1; k <= kz1; ++k) {
-    const int b = (k - kz0 + 2) & 1;   // LDS plane buffer of node plane k+1
-    const int xb_i = k & 1;            // y-exchange buffer of layer k-1
-    if (k < kz1) store(b);
-    // unconditional (clamped to plane kz1): a plane past kz1 is never stored,
-    // and no load result is carried through a conditional path
-    fetch(min(k + 2, kz1));
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      xc[xb_i][ty][tx][c] = yb[c];
-      xc[xb_i][ty][tx][3 + c] = yt[c];
-    }
-    __syncthreads();
-    // finish node plane k-1 (bottom of layer k-1 + top of layer k-2)
-    if (owner) {
-      // all six exchange values are read unconditionally (branch-free)
-      T sb[3], st[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        sb[c] = carry[c] + xb[c] + xc[xb_i][ty - 1][tx][c];
-        st[c] = xt[c] + xc[xb_i][ty - 1][tx][3 + c];
-      }
-      if (k - 1 >= kz0) {
-        const int kp = k - 1;
-        const int64_t nw = ex + (int64_t)g.sx * (ey + (int64_t)g.sy * kp);
-        const uint32_t m = (mq >> 8) & 0xffu;   // plane k-1 (byte 0: plane k)
-        T* yp = a.y + 3 * nw;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T v = ((m >> c) & 1u) ? T(0) : sb[c];
-          yp[c] = v;
-          dot += (double)v * (double)ubot[c];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) carry[c] = st[c];
-    }
-    if (k >= kz1) break;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      ubot[c] = bot[0][c];
-      top[0][c] = pl[b][ty][tx][c];
-      top[1][c] = pl[b][ty][tx + 1][c];
-      top[2][c] = pl[b][ty + 1][tx + 1][c];
-      top[3][c] = pl[b][ty + 1][tx][c];
-    }
-    mq = (mq << 8) | (uint32_t)pm[b][ty][tx];
-    T f[8][3];
-    if (col_valid && k >= 0 && k < g.nz) {
-      T ue[8][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          ue[j][c] = bot[j][c];
-          ue[4 + j][c] = top[j][c];
-        }
-      }
-      const T E = es[b][ty][tx];   // E of layer k, staged with plane k+1
-      if (CUBE)
-        element_apply_cube<T>(ue, E, a.mc, f);
-      else
-        element_apply<T>(ue, E, a.mc, f);
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) f[j][c] = T(0);
-    }
-    // x-neighbour sums (DPP) for the nodes (tx,ty) and (tx,ty+1) of both planes
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      xb[c] = f[0][c] + dpp_shr1(f[1][c]);
-      yb[c] = f[3][c] + dpp_shr1(f[2][c]);
-      xt[c] = f[4][c] + dpp_shr1(f[5][c]);
-      yt[c] = f[7][c] + dpp_shr1(f[6][c]);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) bot[j][c] = top[j][c];
-  }
-  if (a.partials) {
-    // block partial of u.y; the consumer kernel sums the partials
-    const double bs = block_reduce_sum256(dot, red);
-    if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
-  }
+  matvec_tile<T, CUBE>(a, xcd_logical_block(blockIdx.x, gridDim.x), blockIdx.x);
 }
 
 // tileflags[tile * sz + kp] = 1 when node plane kp of the tile's 17x17

@@ -139,8 +139,9 @@ __global__ __launch_bounds__(256) void k_scalars(double* __restrict__ scal, int*
     } else if (rel < 0.99 * scal[S_RELMIN]) {
       scal[S_RELMIN] = rel;
       flags[F_STALL] = 0;
-    } else if (rel > kDivergeFactor * scal[S_RELMIN] ||
-               ++flags[F_STALL] >= max(kStallIters, flags[F_ITERS] / 2)) {
+    } else if (scal[S_RELMIN] < scal[S_FLOOR] &&
+               (rel > kDivergeFactor * scal[S_RELMIN] ||
+                ++flags[F_STALL] >= max(kStallIters, flags[F_ITERS] / 2))) {
       flags[F_BREAKDOWN] = 2;
       flags[F_DONE] = 1;
     }
@@ -347,6 +348,8 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, true, nullptr, s, nullptr, 0));
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RTOL, rtol);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RELMIN, 1e300);
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_FLOOR,
+                     sizeof(T) == 8 ? kFloor64 : kFloor32);
   hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(1), 0, s, h->flags, 0, 0);
   PYTO_CHECK_HIP(hipGetLastError());
   PYTO_TRY(read_back(h, s));
@@ -473,10 +476,10 @@ int pyto_solve_dpcg_f64(pyto_handle* h, const double* f, double* u, double rtol,
     stats->setup_ms = 0.0;
   }
   if (r.stagnated) {
-    set_error("DPCG stagnated at relative residual %.3e after %d iterations (best %.3e; "
-              "no 1%% improvement in max(%d, k/2) iterations, or growth by %.0fx): the "
-              "attainable accuracy is above rtol", r.rel, r.iterations, h->host_scal[S_RELMIN],
-              kStallIters, kDivergeFactor);
+    set_error("DPCG stagnated at relative residual %.3e after %d iterations (best %.3e "
+              "below %.0e; then no 1%% improvement in max(%d, k/2) iterations, or growth by "
+              "%.0fx): the attainable accuracy is above rtol", r.rel, r.iterations,
+              h->host_scal[S_RELMIN], kFloor64, kStallIters, kDivergeFactor);
     return PYTO_ENOCONV;
   }
   if (r.breakdown) {

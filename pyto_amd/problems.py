@@ -86,3 +86,33 @@ def config_density(name, grid):
     if name == "c4":
         return density_uniform(ne, 0.2, 1.0, 1)
     raise KeyError(name)
+
+
+def batch_designs(n_designs=32, seed=5, nx_choices=(32, 48, 64, 80, 96, 112, 128),
+                  nz_choices=(16, 32), max_loads=6):
+    """The batching workload: n_designs cantilevers with grids (nx, nx/2, nz),
+    nx and nz drawn from the choices, each with 1..max_loads unit -z point
+    loads on distinct random nodes of the x = max face, densities U[0.01, 1].
+    One generator (PCG64, ``seed``) draws, per design in order: nx, nz, the
+    number of load cases, the load nodes, then the densities.  Returns a list
+    of dicts (grid, mesh, bc, x, loads)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_designs):
+        nx = int(rng.choice(nx_choices))
+        nz = int(rng.choice(nz_choices))
+        grid = (nx, nx // 2, nz)
+        mesh, mat, bc = cantilever(grid, load="point")
+        nl = int(rng.integers(1, max_loads + 1))
+        sx, sy = grid[0] + 1, grid[1] + 1
+        face = np.arange(sy * (grid[2] + 1))
+        pick = rng.choice(face, nl, replace=False)
+        loads = []
+        for fn in pick:
+            iy, iz = fn % sy, fn // sy
+            f = np.zeros(3 * mesh.num_nodes)
+            f[3 * (grid[0] + sx * (iy + sy * iz)) + 2] = -1.0
+            loads.append(f)
+        x = rng.uniform(0.01, 1.0, mesh.num_elems)
+        out.append(dict(grid=grid, mesh=mesh, mat=mat, bc=bc, x=x, loads=loads))
+    return out

@@ -1,0 +1,97 @@
+"""Batched solves of many designs and load cases against single-system solves."""
+import numpy as np
+import pytest
+import torch
+
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _design(grid, seed, nloads, gpu):
+    from pyto_amd import problems
+    from pyto_amd.operator import GridOperator
+    mesh, mat, bc = problems.cantilever(grid, load="point")
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.01, 1.0, mesh.num_elems)
+    sx, sy = grid[0] + 1, grid[1] + 1
+    loads = []
+    for fn in rng.choice(sy * (grid[2] + 1), nloads, replace=False):
+        f = np.zeros(3 * mesh.num_nodes)
+        f[3 * (grid[0] + sx * (fn % sy + sy * (fn // sy))) + 2] = -1.0
+        loads.append(f)
+    op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, bc.fixed_dofs, penal=3.0, eps_void=1e-8)
+    op.set_density(torch.from_numpy(x).to(gpu))
+    return op, loads
+
+
+def _check_against_single(op, f, u, tol_u=1e-6):
+    ft = torch.from_numpy(f).to(u.device)
+    # true residual of the batched solution
+    r = ft - op.matvec(u)
+    free = op.diagonal() != 0
+    res = float(torch.linalg.norm(r[free]) / torch.linalg.norm(ft[free]))
+    ref, st = op.solve(ft, rtol=1e-11, max_iters=20000)
+    C, C_ref = float(torch.dot(ft, u)), float(torch.dot(ft, ref))
+    return res, rel_err(u.cpu().numpy(), ref.cpu().numpy()), abs(C - C_ref) / abs(C_ref)
+
+
+def test_batch_matches_single_solves(gpu):
+    from pyto_amd.batch import BatchSolver
+    print("batch vs single", flush=True)
+    specs = [((16, 8, 8), 1, 2), ((24, 12, 8), 2, 1), ((32, 16, 16), 3, 3), ((12, 6, 4), 4, 2),
+             ((20, 10, 12), 5, 1)]
+    ops, loads = zip(*[_design(g, s, n, gpu) for g, s, n in specs])
+    bs = BatchSolver(ops, loads)
+    us, stats = bs.solve(rtol=1e-8)
+    assert len(us) == sum(n for _, _, n in specs) == bs.nsys
+    for (d, k), u, st in zip(bs.systems, us, stats):
+        assert st["converged"] and st["rel"] <= 1e-8
+        res, eu, eC = _check_against_single(ops[d], loads[d][k], u)
+        print(d, k, st["iterations"], st["outer_iterations"], "res %.1e err u %.1e C %.1e" % (res, eu, eC),
+              flush=True)
+        assert res <= 1e-6 and eu <= 1e-5 and eC <= 1e-7
+
+
+def test_batch_autotune_and_resolve(gpu):
+    """The autotuner picks a tile depth among the candidates; solutions after
+    re-tuning (new graph) and a repeated solve (graph reuse) agree."""
+    from pyto_amd.batch import BatchSolver
+    print("batch autotune", flush=True)
+    specs = [((32, 16, 16), 7, 2), ((48, 24, 16), 8, 1)]
+    ops, loads = zip(*[_design(g, s, n, gpu) for g, s, n in specs])
+    bs = BatchSolver(ops, loads)
+    us0, _ = bs.solve(rtol=1e-9)
+    tune = bs.autotune(candidates=(4, 8, 16, 32))
+    assert tune["zchunk"] in (4, 8, 16, 32) and bs.info()["zchunk"] == tune["zchunk"]
+    us1, st1 = bs.solve(rtol=1e-9)
+    us2, st2 = bs.solve(rtol=1e-9)
+    for a, b, c in zip(us0, us1, us2):
+        assert rel_err(b.cpu().numpy(), a.cpu().numpy()) <= 1e-7
+        assert torch.equal(b, c)
+    assert [s["iterations"] for s in st1] == [s["iterations"] for s in st2]
+
+
+def test_batch_32_designs(gpu):
+    """The batching workload: 32 designs (seed 5) with 1-6 point-load cases
+    each, all systems to rtol 1e-8 in one batch; a sample checked against
+    single-system solves."""
+    from pyto_amd import problems
+    from pyto_amd.batch import BatchSolver
+    from pyto_amd.operator import GridOperator
+    print("batch 32 designs", flush=True)
+    designs = problems.batch_designs()
+    ops = []
+    for d in designs:
+        op = GridOperator(d["grid"], (1, 1, 1), 1.0, 0.3, d["bc"].fixed_dofs, penal=3.0, eps_void=1e-8)
+        op.set_density(torch.from_numpy(d["x"]).to(gpu))
+        ops.append(op)
+    bs = BatchSolver(ops, [d["loads"] for d in designs])
+    bs.autotune()
+    us, stats = bs.solve(rtol=1e-8, max_iters=200000, max_outer=60)
+    assert all(s["converged"] for s in stats), [(i, s) for i, s in enumerate(stats) if not s["converged"]][:3]
+    print("systems", bs.nsys, "max inner iterations", max(s["iterations"] for s in stats), flush=True)
+    for i in (0, 17, 60, bs.nsys - 1):
+        d, k = bs.systems[i]
+        res, eu, eC = _check_against_single(ops[d], designs[d]["loads"][k], us[i])
+        assert res <= 1e-6 and eu <= 1e-5 and eC <= 1e-7, (i, res, eu, eC)
