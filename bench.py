@@ -90,6 +90,38 @@ def traffic_bytes():
     return None
 
 
+def batch_line(torch, dev):
+    from pyto_amd import problems
+    from pyto_amd.batch import BatchSolver
+    from pyto_amd.operator import GridOperator
+    designs = problems.batch_designs()
+    ops = []
+    for d in designs:
+        op = GridOperator(d["grid"], (1, 1, 1), 1.0, 0.3, d["bc"].fixed_dofs, penal=3.0,
+                          eps_void=1e-8)
+        op.set_density(torch.from_numpy(d["x"]).to(dev))
+        ops.append(op)
+    bs = BatchSolver(ops, [d["loads"] for d in designs])
+    tune = bs.autotune()
+    bs.solve(rtol=1e-8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, stats = bs.solve(rtol=1e-8)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    it = [s["iterations"] for s in stats]
+    ndof = sum(3 * d["mesh"].num_nodes * len(d["loads"]) for d in designs)
+    out = {"designs": len(designs), "systems": bs.nsys, "dof_total": ndof,
+           "seconds": t, "systems_per_s": bs.nsys / t,
+           "all_converged": all(s["converged"] for s in stats),
+           "inner_iterations_max": max(it), "inner_iterations_median": float(np.median(it)),
+           "zchunk": tune["zchunk"]}
+    bs.close()
+    for op in ops:
+        op.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +134,8 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
     ap.add_argument("--tuned-boxes", default="12,6,6",
                     help="second aggregate layout timed beside the named one ('' to skip)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched 32-design line reported beside `value`")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -226,6 +260,15 @@ def main():
         log("tuned aggregates %s: mixed %.3f ms, fp64 %.3f ms" % (args.tuned_boxes, t2m * 1e3,
                                                                    t2d * 1e3))
 
+    # the batching workload, reported beside `value`: 32 designs (seed 5)
+    # with 1-6 point-load cases each, all systems to rtol 1e-8 in one batch
+    batch = None
+    if not args.no_batch:
+        del op
+        torch.cuda.empty_cache()
+        batch = batch_line(torch, dev)
+        log("batch", batch)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(st64["iterations"], nd)
@@ -257,6 +300,7 @@ def main():
                      "algorithmic_bytes": bytes32,
                      "fp64_kernel_us": t_mv64 * 1e6, "fp64_GBs": bytes64 / t_mv64 / 1e9},
         "tuned_aggregates": tuned,
+        "batch_32_designs": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -243,29 +243,7 @@ __global__ __launch_bounds__(256) void k_bscalars(const int* __restrict__ vpre,
   double* sc = scal + s * kNumScalars;
   const double pkp = block_sum_partials(part_mv + mpre[s], mpre[s + 1] - mpre[s], red);
   const double rz_new = block_sum_partials(part_v + vpre[s], vpre[s + 1] - vpre[s], red);
-  if (threadIdx.x == 0) {
-    const double rz = sc[S_RZ];
-    const double rel = sqrt(fabs(rz_new / sc[S_RZ0]));
-    sc[S_ALPHA] = rz / pkp;
-    sc[S_REL] = rel;
-    fl[F_ITERS] += 1;
-    if (!(pkp > 0.0) || !(rz_new == rz_new)) {
-      fl[F_BREAKDOWN] = 1;
-      fl[F_DONE] = 1;
-    } else if (rel <= sc[S_RTOL]) {
-      fl[F_DONE] = 1;
-    } else if (rel < 0.99 * sc[S_RELMIN]) {
-      sc[S_RELMIN] = rel;
-      fl[F_STALL] = 0;
-    } else if (sc[S_RELMIN] < kFloor32 &&
-               (rel > kDivergeFactor * sc[S_RELMIN] ||
-                ++fl[F_STALL] >= max(kStallIters, fl[F_ITERS] / 2))) {
-      fl[F_BREAKDOWN] = 2;
-      fl[F_DONE] = 1;
-    }
-    sc[S_BETA] = rz_new / rz;
-    sc[S_RZ] = rz_new;
-  }
+  if (threadIdx.x == 0) pcg_scalar_step(sc, fl, pkp, rz_new, kFloor32);
 }
 
 // x += alpha p (unless FIRST), p = invd o r + beta p (FIRST: p = invd o r).

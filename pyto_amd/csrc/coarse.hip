@@ -162,7 +162,8 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
 // Per-chunk partials of W^T K v over the special elements (one wave per
 // chunk; k_coarse_gather sums them into t), where
 // v = invd o r + beta p (the new search direction before its coarse
-// correction; HAS_P) or v = invd o r.  The element arithmetic is FP64
+// correction; HAS_P, with beta = rz_new / rz formed from the update kernel's
+// block partials) or v = invd o r.  The element arithmetic is FP64
 // whatever the vector precision T.  Group centres are wave-uniform per slot.
 // A workgroup serves one SpecialBlock: it first stages v at the block's
 // sorted corner nodes in LDS (an element's corners would otherwise be 24
@@ -172,7 +173,8 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
 template <typename T, bool HAS_P>
 __global__ __launch_bounds__(256) void k_restrict_special(
     const T* __restrict__ r, const T* __restrict__ invd, const T* __restrict__ p,
-    const double* __restrict__ beta_ptr, const SpecialElem* __restrict__ sinfo,
+    const double* __restrict__ part_rz, int nrz, const double* __restrict__ rz_old,
+    const SpecialElem* __restrict__ sinfo,
     const uint4* __restrict__ sidx, const SpecialChunk* __restrict__ chunks,
     const SpecialBlock* __restrict__ sblocks, const int32_t* __restrict__ sbnodes,
     const double* __restrict__ Ee, const double* __restrict__ centers, GridDims g,
@@ -181,7 +183,10 @@ __global__ __launch_bounds__(256) void k_restrict_special(
   constexpr int kStage = (3 * kSpecialBlockNodes + 255) / 256;   // staged DOFs per thread
   __shared__ double vs[3 * kSpecialBlockNodes];
   const SpecialBlock sb = sblocks[xcd_logical_block(blockIdx.x, gridDim.x)];
-  const double beta = HAS_P ? *beta_ptr : 0.0;
+  // beta = rz_new / rz from the update kernel's partials, summed here in the
+  // same order as by the scalar logic (no scalar kernel needed before this)
+  __shared__ double redb[5];
+  const double beta = HAS_P ? block_sum_partials(part_rz, nrz, redb) / *rz_old : 0.0;
   const int w = threadIdx.x >> 6;
   const int wave = sb.chunk0 + min(w, sb.nch - 1);
   const int lane = threadIdx.x & 63;
@@ -300,6 +305,56 @@ __global__ __launch_bounds__(256) void k_coarse_gather(const double* __restrict_
     const double s = wave_reduce_sum(acc[j]);
     if (lane == 0) t[6 * g + j] = s;
   }
+}
+
+static int round_up64(int n) { return (n + 63) / 64 * 64; }
+
+// t = per-group sums of the chunk partials (one wave per group, as
+// k_coarse_gather) and, in one extra workgroup running beside them, the
+// iteration's scalars (pcg_scalar_step) from the operator's and the update's
+// block partials.
+__global__ __launch_bounds__(256) void k_gather_scalars(
+    const double* __restrict__ cpart, const int32_t* __restrict__ gptr,
+    const int32_t* __restrict__ gidx, int ngroups, double* __restrict__ t,
+    const double* __restrict__ part_pq, int npq, const double* __restrict__ part_rz, int nrz,
+    double* __restrict__ scal, int* __restrict__ flags) {
+  if (blockIdx.x == gridDim.x - 1) {
+    __shared__ int done_s;
+    __shared__ double red[10];
+    if (threadIdx.x == 0) done_s = flags[F_DONE];
+    __syncthreads();
+    if (done_s) return;
+    double pkp, rz_new;
+    block_sum_partials2(part_pq, npq, part_rz, nrz, red, pkp, rz_new);
+    if (threadIdx.x == 0) pcg_scalar_step(scal, flags, pkp, rz_new, scal[S_FLOOR]);
+    return;
+  }
+  // F_DONE may be set by the scalar workgroup meanwhile: a wave that sees it
+  // skips (t is then unused)
+  if (flags[F_DONE]) return;
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (g >= ngroups) return;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = gptr[g] + lane; k < gptr[g + 1]; k += 64) {
+    const double* c = cpart + (int64_t)gidx[k] * 6;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[j] += c[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const double sj = wave_reduce_sum(acc[j]);
+    if (lane == 0) t[6 * g + j] = sj;
+  }
+}
+
+int launch_gather_scalars(Handle* h, double* t, const double* part_pq, int npq,
+                          const double* part_rz, int nrz, hipStream_t s) {
+  const int gb = (h->ngroups * 64 + 255) / 256;
+  hipLaunchKernelGGL(k_gather_scalars, dim3(gb + 1), dim3(256), 0, s, h->cpart, h->gptr, h->gidx,
+                     h->ngroups, t, part_pq, npq, part_rz, nrz, h->scal, h->flags);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
 }
 
 // Direction update of the deflated PCG:
@@ -585,7 +640,6 @@ __global__ void k_axpy_small(double* __restrict__ y, const double* __restrict__ 
   if (i < n) y[i] += x[i];
 }
 
-static int round_up64(int n) { return (n + 63) / 64 * 64; }
 
 // mu = E^{-1} (t1 + beta t2) with the FP64 inverse (t2/beta optional).  With
 // `refine`, one step of refinement on the coarse residual follows:
@@ -622,25 +676,44 @@ int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
 }
 
 // t = W^T K v over the special elements (overwrites t; no zeroing needed).
+// Chunk partials of W^T K v over the special elements (no gather).  With p:
+// v = invd o r + beta p, beta from part_rz (nrz partials) and *rz_old.
 template <typename T>
-int launch_restrict_special(Handle* h, const T* r, const T* invd, const T* p,
-                            const double* beta, double* t, const int* done, hipStream_t s) {
-  PYTO_REQUIRE(r && invd && (p == nullptr || beta != nullptr), PYTO_EINVAL,
-               "special restriction needs r, invd and (with p) beta");
+int launch_restrict_special_parts(Handle* h, const T* r, const T* invd, const T* p,
+                                  const double* part_rz, int nrz, const double* rz_old,
+                                  const int* done, hipStream_t s) {
+  PYTO_REQUIRE(r && invd && (p == nullptr || (part_rz != nullptr && rz_old != nullptr)),
+               PYTO_EINVAL, "special restriction needs r, invd and (with p) the rz partials");
   if (h->nsblocks > 0) {
     if (p)
       hipLaunchKernelGGL((k_restrict_special<T, true>), dim3(h->nsblocks), dim3(256), 0, s, r,
-                         invd, p, beta, h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes,
-                         h->Ee64, h->centers, h->g, h->mc64, h->cpart, done);
+                         invd, p, part_rz, nrz, rz_old, h->sinfo, h->sidx, h->chunks,
+                         h->sblocks, h->sbnodes, h->Ee64, h->centers, h->g, h->mc64, h->cpart,
+                         done);
     else
       hipLaunchKernelGGL((k_restrict_special<T, false>), dim3(h->nsblocks), dim3(256), 0, s, r,
-                         invd, p, beta, h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes,
-                         h->Ee64, h->centers, h->g, h->mc64, h->cpart, done);
+                         invd, p, part_rz, nrz, rz_old, h->sinfo, h->sidx, h->chunks,
+                         h->sblocks, h->sbnodes, h->Ee64, h->centers, h->g, h->mc64, h->cpart,
+                         done);
   }
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+int launch_coarse_gather(Handle* h, double* t, const int* done, hipStream_t s) {
   hipLaunchKernelGGL(k_coarse_gather, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
                      h->cpart, h->gptr, h->gidx, h->ngroups, t, done);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
+}
+
+// t = W^T K (invd o r) over the special elements (overwrites t).
+template <typename T>
+int launch_restrict_special(Handle* h, const T* r, const T* invd, double* t, const int* done,
+                            hipStream_t s) {
+  PYTO_TRY(launch_restrict_special_parts<T>(h, r, invd, (const T*)nullptr, nullptr, 0, nullptr,
+                                            done, s));
+  return launch_coarse_gather(h, t, done, s);
 }
 
 template <typename T>
@@ -679,11 +752,16 @@ int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double
 
 template int launch_restrict_full<double>(Handle*, const double*, double*, hipStream_t);
 template int launch_restrict_full<float>(Handle*, const float*, double*, hipStream_t);
-template int launch_restrict_special<double>(Handle*, const double*, const double*,
-                                             const double*, const double*, double*, const int*,
-                                             hipStream_t);
-template int launch_restrict_special<float>(Handle*, const float*, const float*, const float*,
-                                            const double*, double*, const int*, hipStream_t);
+template int launch_restrict_special<double>(Handle*, const double*, const double*, double*,
+                                             const int*, hipStream_t);
+template int launch_restrict_special<float>(Handle*, const float*, const float*, double*,
+                                            const int*, hipStream_t);
+template int launch_restrict_special_parts<double>(Handle*, const double*, const double*,
+                                                   const double*, const double*, int,
+                                                   const double*, const int*, hipStream_t);
+template int launch_restrict_special_parts<float>(Handle*, const float*, const float*,
+                                                  const float*, const double*, int,
+                                                  const double*, const int*, hipStream_t);
 template int launch_dir_update<double>(Handle*, double*, double*, const double*, const double*,
                                        const double*, bool, const int*, hipStream_t, double*,
                                        int);

@@ -260,6 +260,63 @@ __device__ __forceinline__ double block_sum_partials(const double* __restrict__ 
   return lds5[4];
 }
 
+// The scalar step of one PCG iteration, run by one thread: alpha, the
+// relative residual sqrt(rz_new/rz0) with the stop test (src/deflation.py:567),
+// the breakdown test (p^T K p <= 0 or NaN), the stagnation test (see
+// kFloor64), beta and rz.  sc/fl are the system's scalar and flag slots.
+__device__ __forceinline__ void pcg_scalar_step(double* sc, int* fl, double pkp, double rz_new,
+                                                double floor) {
+  const double rz = sc[S_RZ];
+  const double rel = sqrt(fabs(rz_new / sc[S_RZ0]));
+  sc[S_PKP] = pkp;
+  sc[S_ALPHA] = rz / pkp;
+  sc[S_RZNEW] = rz_new;
+  sc[S_REL] = rel;
+  fl[F_ITERS] += 1;
+  if (!(pkp > 0.0) || !(rz_new == rz_new)) {
+    fl[F_BREAKDOWN] = 1;
+    fl[F_DONE] = 1;
+  } else if (rel <= sc[S_RTOL]) {
+    fl[F_DONE] = 1;
+  } else if (rel < 0.99 * sc[S_RELMIN]) {
+    sc[S_RELMIN] = rel;
+    fl[F_STALL] = 0;
+  } else if (sc[S_RELMIN] < floor &&
+             (rel > kDivergeFactor * sc[S_RELMIN] ||
+              ++fl[F_STALL] >= max(kStallIters, fl[F_ITERS] / 2))) {
+    fl[F_BREAKDOWN] = 2;
+    fl[F_DONE] = 1;
+  }
+  sc[S_BETA] = rz_new / rz;
+  sc[S_RZ] = rz_new;
+}
+
+// Two sums of block partials in one pass (as block_sum_partials, fixed
+// order), returned to every thread; lds10 holds 10 doubles.
+__device__ __forceinline__ void block_sum_partials2(const double* __restrict__ pa, int na,
+                                                    const double* __restrict__ pb, int nb,
+                                                    double* lds10, double& sa, double& sb) {
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < na; i += 256) a += pa[i];
+  for (int i = threadIdx.x; i < nb; i += 256) b += pb[i];
+  a = wave_reduce_sum(a);
+  b = wave_reduce_sum(b);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    lds10[w] = a;
+    lds10[4 + w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    lds10[8] = (lds10[0] + lds10[1]) + (lds10[2] + lds10[3]);
+    lds10[9] = (lds10[4] + lds10[5]) + (lds10[6] + lds10[7]);
+  }
+  __syncthreads();
+  sa = lds10[8];
+  sb = lds10[9];
+}
+
 // Deterministic grid reduction: every block stores its partial, the last
 // block to finish sums the partials in index order and calls `fin` with the
 // total.  `counter` returns to zero, so the pattern can be replayed.

@@ -28,8 +28,15 @@ int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalCon
 template <typename T>
 int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s);
 template <typename T>
-int launch_restrict_special(Handle* h, const T* r, const T* invd, const T* p,
-                            const double* beta, double* t, const int* done, hipStream_t s);
+int launch_restrict_special(Handle* h, const T* r, const T* invd, double* t, const int* done,
+                            hipStream_t s);
+template <typename T>
+int launch_restrict_special_parts(Handle* h, const T* r, const T* invd, const T* p,
+                                  const double* part_rz, int nrz, const double* rz_old,
+                                  const int* done, hipStream_t s);
+int launch_coarse_gather(Handle* h, double* t, const int* done, hipStream_t s);
+int launch_gather_scalars(Handle* h, double* t, const double* part_pq, int npq,
+                          const double* part_rz, int nrz, hipStream_t s);
 template <typename T, bool UPDATE>
 int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
                    bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n);
@@ -120,34 +127,10 @@ __global__ __launch_bounds__(256) void k_scalars(double* __restrict__ scal, int*
                                                  const double* __restrict__ part_pq, int npq,
                                                  const double* __restrict__ part_rz, int nrz) {
   if (flags[F_DONE]) return;
-  __shared__ double red[5];
-  const double pkp = block_sum_partials(part_pq, npq, red);
-  const double rz_new = block_sum_partials(part_rz, nrz, red);
-  if (threadIdx.x == 0) {
-    const double rz = scal[S_RZ];
-    const double rel = sqrt(fabs(rz_new / scal[S_RZ0]));
-    scal[S_PKP] = pkp;
-    scal[S_ALPHA] = rz / pkp;
-    scal[S_RZNEW] = rz_new;
-    scal[S_REL] = rel;
-    flags[F_ITERS] += 1;
-    if (!(pkp > 0.0) || !(rz_new == rz_new)) {
-      flags[F_BREAKDOWN] = 1;
-      flags[F_DONE] = 1;
-    } else if (rel <= scal[S_RTOL]) {
-      flags[F_DONE] = 1;
-    } else if (rel < 0.99 * scal[S_RELMIN]) {
-      scal[S_RELMIN] = rel;
-      flags[F_STALL] = 0;
-    } else if (scal[S_RELMIN] < scal[S_FLOOR] &&
-               (rel > kDivergeFactor * scal[S_RELMIN] ||
-                ++flags[F_STALL] >= max(kStallIters, flags[F_ITERS] / 2))) {
-      flags[F_BREAKDOWN] = 2;
-      flags[F_DONE] = 1;
-    }
-    scal[S_BETA] = rz_new / rz;
-    scal[S_RZ] = rz_new;
-  }
+  __shared__ double red[10];
+  double pkp, rz_new;
+  block_sum_partials2(part_pq, npq, part_rz, nrz, red, pkp, rz_new);
+  if (threadIdx.x == 0) pcg_scalar_step(scal, flags, pkp, rz_new, scal[S_FLOOR]);
 }
 
 // One block: scal[slot_a] (and slot_b) = sum of n partials.
@@ -314,9 +297,10 @@ static int iteration_graph(Handle* h, const void* x, int tsize, F&& one_iteratio
 // every step.  In exact arithmetic the two forms coincide.  z = invd o r is
 // never stored: the kernels that need it form it from r.
 //
-// Per iteration (5 launches): q = K p with p^T q; r -= alpha q with
-// r^T M r, the stop test and beta; t = W^T K (z + beta p) over the
-// non-rigid elements; mu = E^-1 t; x += alpha p, p = z + beta p - W mu.
+// Per iteration: q = K p with p^T q partials; r -= alpha q with r^T M r
+// partials; chunk partials of W^T K (z + beta p) over the non-rigid elements
+// (beta formed there); the per-group gather with the scalars and stop test
+// beside it; mu = E^-1 t; x += alpha p, p = z + beta p - W mu.
 template <typename T>
 static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgResult& res,
                 hipStream_t s) {
@@ -341,7 +325,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, h->scal, (const double*)part_rz,
                      kVecBlocks, (int)S_RZ, (int)S_RZ0);
   if (defl) {
-    PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, (const T*)nullptr, nullptr, ct, nullptr, s));
+    PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, ct, nullptr, s));
     PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
   }
   // p0 = z0 - W mu (x untouched)
@@ -365,11 +349,16 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     hipLaunchKernelGGL(k_cg_update<T>, dim3(kVecBlocks), dim3(256), 0, st, v.r, v.q, v.invd, n,
                        (const double*)h->scal, (const int*)h->flags, (const double*)part_pq, npq,
                        part_rz);
-    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, st, h->scal, h->flags,
-                       (const double*)part_pq, npq, (const double*)part_rz, kVecBlocks);
     if (defl) {
-      PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, v.p, h->scal + S_BETA, ct, done, st));
+      // beta is formed inside the restriction from the update's partials, so
+      // the scalar logic runs beside the per-group gather in one launch
+      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, v.invd, v.p, part_rz, kVecBlocks,
+                                                h->scal + S_RZ, done, st));
+      PYTO_TRY(launch_gather_scalars(h, ct, part_pq, npq, part_rz, kVecBlocks, st));
       PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, done, st, false));
+    } else {
+      hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, st, h->scal, h->flags,
+                         (const double*)part_pq, npq, (const double*)part_rz, kVecBlocks);
     }
     PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, false, done, st, nullptr, 0));
     PYTO_CHECK_HIP(hipGetLastError());
