@@ -199,6 +199,64 @@ __host__ __device__ __forceinline__ void element_apply_cube(const T u[8][3], T E
   modal_backward(F, f);
 }
 
+// x and y butterfly stages of one face of the element (corners (0,0), (1,0),
+// (1,1), (0,1) of that face).  Adjacent element layers share a face, so an
+// operator marching in z computes each face once: the top face of one layer
+// is the bottom face of the next.
+template <typename T>
+struct Face {
+  T s[3], sy[3], dx[3], dxy[3];
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ void face_xy(const T c0[3], const T c1[3], const T c2[3],
+                                                 const T c3[3], Face<T>& f) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const T sx0 = c1[i] + c0[i], dx0 = c1[i] - c0[i];
+    const T sx1 = c2[i] + c3[i], dx1 = c2[i] - c3[i];
+    f.s[i] = sx1 + sx0;
+    f.sy[i] = sx1 - sx0;
+    f.dx[i] = dx1 + dx0;
+    f.dxy[i] = dx1 - dx0;
+  }
+}
+
+// The z stage of modal_forward from the two faces' x/y stages.
+template <typename T>
+__host__ __device__ __forceinline__ void modal_from_faces(const Face<T>& b, const Face<T>& t,
+                                                          Modal<T>& m) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    m.cz[i] = t.s[i] - b.s[i];
+    m.cy[i] = t.sy[i] + b.sy[i];
+    m.cyz[i] = t.sy[i] - b.sy[i];
+    m.cx[i] = t.dx[i] + b.dx[i];
+    m.cxz[i] = t.dx[i] - b.dx[i];
+    m.cxy[i] = t.dxy[i] + b.dxy[i];
+    m.cxyz[i] = t.dxy[i] - b.dxy[i];
+  }
+}
+
+// element_apply / element_apply_cube from the faces' butterfly stages.
+template <typename T, bool CUBE>
+__host__ __device__ __forceinline__ void element_apply_faces(const Face<T>& b, const Face<T>& t,
+                                                             T E, const ModalConsts<T>& mc,
+                                                             T f[8][3]) {
+  Modal<T> c, F;
+  modal_from_faces(b, t, c);
+  if (CUBE) {
+    const T g2 = mc.gx * mc.gx;
+    const T L = E * (mc.lam * g2), M = E * (mc.mu * g2);
+    const T L3 = L * T(1.0 / 3.0), M3 = M * T(1.0 / 3.0);
+    const T K9 = (L + T(4) * M) * T(1.0 / 9.0);
+    modal_stress_cube(c, L, M, L3, M3, K9, F);
+  } else {
+    modal_stress(c, mc.gx, mc.gy, mc.gz, E * mc.lam, E * mc.mu, F);
+  }
+  modal_backward(F, f);
+}
+
 template <typename T>
 inline ModalConsts<T> make_modal_consts(double E0, double nu, double hx, double hy, double hz) {
   const double lam = E0 * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));

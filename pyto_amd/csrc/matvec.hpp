@@ -49,12 +49,14 @@ __device__ __forceinline__ double dpp_shr1(double v) {
 // the loads stay in flight across a whole layer of element work.
 template <typename T>
 __device__ __forceinline__ void issue_node(T a[3], int& mo, int& ml, const T* __restrict__ u,
-                                           const uint8_t* __restrict__ mask, const GridDims& g,
-                                           int gx, int gy, int kp, bool any_fixed) {
-  const bool inb = kp >= 0 && kp < g.sz && gx >= 0 && gx < g.sx && gy >= 0 && gy < g.sy;
-  const int cx = min(max(gx, 0), g.sx - 1), cy = min(max(gy, 0), g.sy - 1);
-  const int cz = min(max(kp, 0), g.sz - 1);
-  const int64_t n = cx + (int64_t)g.sx * (cy + (int64_t)g.sy * cz);
+                                           const uint8_t* __restrict__ mask, int64_t nxy,
+                                           bool inxy, int64_t plane, int kp, int sz,
+                                           bool any_fixed) {
+  // nxy: the node's (clamped) index within its plane; inxy: x, y inside the
+  // grid.  Only the plane offset changes as the workgroup marches in z.
+  const bool inb = inxy && kp >= 0 && kp < sz;
+  const int cz = min(max(kp, 0), sz - 1);
+  const int64_t n = nxy + plane * cz;
   const T* p = u + 3 * n;
   a[0] = p[0];
   a[1] = p[1];
@@ -119,10 +121,17 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
   // fixed bits of this thread's own node, read from the staged plane when
   // it becomes the top plane: mq byte 0 = newest plane
   uint32_t mq = 0;
+  // in-plane parts of the two staged nodes' indices (constant over z)
+  const int64_t plane = (int64_t)g.sx * g.sy;
+  const int gx0 = x0 + lx0, gy0 = y0 + ly0, gx1 = x0 + lx1, gy1 = y0 + ly1;
+  const bool inxy0 = gx0 >= 0 && gx0 < g.sx && gy0 >= 0 && gy0 < g.sy;
+  const bool inxy1 = gx1 >= 0 && gx1 < g.sx && gy1 >= 0 && gy1 < g.sy;
+  const int64_t nxy0 = min(max(gx0, 0), g.sx - 1) + (int64_t)g.sx * min(max(gy0, 0), g.sy - 1);
+  const int64_t nxy1 = min(max(gx1, 0), g.sx - 1) + (int64_t)g.sx * min(max(gy1, 0), g.sy - 1);
   auto fetch = [&](int kp) {
     const bool fx = plane_fixed(kp);
-    issue_node<T>(pv0, mo0, ml0, a.u, a.mask, g, x0 + lx0, y0 + ly0, kp, fx);
-    if (has1) issue_node<T>(pv1, mo1, ml1, a.u, a.mask, g, x0 + lx1, y0 + ly1, kp, fx);
+    issue_node<T>(pv0, mo0, ml0, a.u, a.mask, nxy0, inxy0, plane, kp, g.sz, fx);
+    if (has1) issue_node<T>(pv1, mo1, ml1, a.u, a.mask, nxy1, inxy1, plane, kp, g.sz, fx);
     const int kl = min(max(kp - 1, 0), g.nz - 1);
     enext = a.Ee[ecol + (int64_t)g.nx * g.ny * kl];
   };
@@ -139,17 +148,33 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
     es[b][ty][tx] = enext;
   };
 
-  T bot[4][3], top[4][3];
+  // FP32: x/y butterfly stages of the element's bottom face, carried from
+  // the layer below (its top face), and u at this thread's node of that face.
+  // FP64 keeps the corner values instead: carrying the face stages costs it
+  // 8 VGPRs more and one wave per SIMD.
+  constexpr bool kFaces = sizeof(T) == 4;
+  Face<T> bface;
+  T bot[4][3];
+  T ucur[3];
   fetch(kz0 - 1);
   store(0);
   fetch(kz0);   // also E of layer kz0-1
   __syncthreads();
+  {
+    T c0[3], c1[3], c2[3], c3[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    bot[0][c] = pl[0][ty][tx][c];
-    bot[1][c] = pl[0][ty][tx + 1][c];
-    bot[2][c] = pl[0][ty + 1][tx + 1][c];
-    bot[3][c] = pl[0][ty + 1][tx][c];
+    for (int c = 0; c < 3; ++c) {
+      c0[c] = pl[0][ty][tx][c];
+      c1[c] = pl[0][ty][tx + 1][c];
+      c2[c] = pl[0][ty + 1][tx + 1][c];
+      c3[c] = pl[0][ty + 1][tx][c];
+      ucur[c] = c0[c];
+      bot[0][c] = c0[c];
+      bot[1][c] = c1[c];
+      bot[2][c] = c2[c];
+      bot[3][c] = c3[c];
+    }
+    if (kFaces) face_xy<T>(c0, c1, c2, c3, bface);
   }
   mq = pm[0][ty][tx];
   T carry[3] = {T(0), T(0), T(0)};   // top-plane sum of the layer before last
@@ -158,7 +183,13 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
 #pragma unroll
   for (int c = 0; c < 3; ++c) xb[c] = xt[c] = yb[c] = yt[c] = ubot[c] = T(0);
   double dot = 0.0;
-  for (int k = kz0 - 1; k <= kz1; ++k) {
+  // this thread's own node in its plane (owners only write it)
+  const int64_t own_xy = (int64_t)ex + (int64_t)g.sx * ey;
+  // one element layer k (see the comment above the kernel); false once the
+  // last node plane of the chunk is finished.  Called twice per loop trip so
+  // that the values carried from layer to layer alternate between register
+  // sets instead of being copied.
+  auto layer = [&](int k) -> bool {
     const int b = (k - kz0 + 2) & 1;   // LDS plane buffer of node plane k+1
     const int xb_i = k & 1;            // y-exchange buffer of layer k-1
     if (k < kz1) store(b);
@@ -182,7 +213,7 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
       }
       if (k - 1 >= kz0) {
         const int kp = k - 1;
-        const int64_t nw = ex + (int64_t)g.sx * (ey + (int64_t)g.sy * kp);
+        const int64_t nw = own_xy + plane * kp;
         const uint32_t m = (mq >> 8) & 0xffu;   // plane k-1 (byte 0: plane k)
         T* yp = a.y + 3 * nw;
 #pragma unroll
@@ -195,32 +226,39 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
 #pragma unroll
       for (int c = 0; c < 3; ++c) carry[c] = st[c];
     }
-    if (k >= kz1) break;
+    if (k >= kz1) return false;
+    Face<T> tface;
+    T top[4][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      ubot[c] = bot[0][c];
+      ubot[c] = ucur[c];
       top[0][c] = pl[b][ty][tx][c];
       top[1][c] = pl[b][ty][tx + 1][c];
       top[2][c] = pl[b][ty + 1][tx + 1][c];
       top[3][c] = pl[b][ty + 1][tx][c];
+      ucur[c] = top[0][c];
     }
+    if (kFaces) face_xy<T>(top[0], top[1], top[2], top[3], tface);
     mq = (mq << 8) | (uint32_t)pm[b][ty][tx];
     T f[8][3];
     if (col_valid && k >= 0 && k < g.nz) {
-      T ue[8][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          ue[j][c] = bot[j][c];
-          ue[4 + j][c] = top[j][c];
-        }
-      }
       const T E = es[b][ty][tx];   // E of layer k, staged with plane k+1
-      if (CUBE)
-        element_apply_cube<T>(ue, E, a.mc, f);
-      else
-        element_apply<T>(ue, E, a.mc, f);
+      if (kFaces) {
+        element_apply_faces<T, CUBE>(bface, tface, E, a.mc, f);
+      } else {
+        T ue[8][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            ue[j][c] = bot[j][c];
+            ue[4 + j][c] = top[j][c];
+          }
+        if (CUBE)
+          element_apply_cube<T>(ue, E, a.mc, f);
+        else
+          element_apply<T>(ue, E, a.mc, f);
+      }
     } else {
 #pragma unroll
       for (int j = 0; j < 8; ++j)
@@ -235,10 +273,19 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
       xt[c] = f[4][c] + dpp_shr1(f[5][c]);
       yt[c] = f[7][c] + dpp_shr1(f[6][c]);
     }
+    if (kFaces) {
+      bface = tface;
+    } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) bot[j][c] = top[j][c];
+        for (int c = 0; c < 3; ++c) bot[j][c] = top[j][c];
+    }
+    return true;
+  };
+  for (int k = kz0 - 1;; k += 2) {
+    if (!layer(k)) break;
+    if (!layer(k + 1)) break;
   }
   if (a.partials) {
     // block partial of u.y; the consumer kernel sums the partials
