@@ -134,8 +134,13 @@ int pyto_solve_dpcg_f64(pyto_handle* h, const double* f, double* u, double rtol,
 /* FP32 deflated PCG inside FP64 iterative refinement: the outer residual
  * f - K u is recomputed in FP64, inner products accumulate in FP64, the coarse
  * factor is FP64; stops when the FP64 sqrt(r^T M r / r0^T M r0) <= rtol
- * (the reference's rule on the true residual).  No reference counterpart:
- * this is the precision form of the same solve. */
+ * (the reference's rule on the true residual).  Each refinement step runs
+ * the FP32 iteration until its residual has dropped by max(inner_rtol,
+ * 0.3 rtol/rel) (at most 0.1), then replaces it by the FP64 residual and
+ * continues the same search direction (reliable update) unless the two
+ * residuals differ by more than 1.25x in norm, where it restarts.  inner_rtol
+ * around 0.1 keeps the FP32 drift below the residual.  No reference
+ * counterpart: this is the precision form of the same solve. */
 int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rtol,
                           double inner_rtol, int max_iters, int max_outer,
                           pyto_solve_stats* stats, void* stream);

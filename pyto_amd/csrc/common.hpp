@@ -184,7 +184,8 @@ enum ScalarSlot {
   // parity, and the last multiplier at which the volume was evaluated
   S_OC_STATE = 16, S_OC_LAST = 24,
   S_RELMIN = 25,  // best relative residual of the running solve (stagnation test)
-  S_FLOOR = 26    // the stagnation test applies once S_RELMIN is below this
+  S_FLOOR = 26,   // the stagnation test applies once S_RELMIN is below this
+  S_RZPREV = 27   // r^T M r before the last iteration's update (resumed directions)
 };
 // Device flag slots.
 // F_BREAKDOWN: 1 = p^T K p <= 0 or NaN, 2 = stagnation (see k_scalars)
@@ -288,6 +289,7 @@ __device__ __forceinline__ void pcg_scalar_step(double* sc, int* fl, double pkp,
     fl[F_DONE] = 1;
   }
   sc[S_BETA] = rz_new / rz;
+  sc[S_RZPREV] = rz;
   sc[S_RZ] = rz_new;
 }
 

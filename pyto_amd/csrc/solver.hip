@@ -206,6 +206,11 @@ __global__ void k_set_flags(int* flags, int done, int iters) {
 
 __global__ void k_set_scalar(double* scal, int slot, double v) { scal[slot] = v; }
 
+__global__ void k_resume_scalars(double* scal) {
+  scal[S_BETA] = scal[S_RZ] / scal[S_RZPREV];
+  scal[S_ALPHA] = 0.0;
+}
+
 template <typename T>
 struct Vecs {
   T *r, *p, *q;
@@ -247,6 +252,9 @@ static int read_back(Handle* h, hipStream_t s) {
 }
 
 constexpr int kGraphBatch = 16;
+// largest ratio of true to recurrence residual norm at which a mixed
+// refinement step continues the previous inner run's search direction
+constexpr double kResumeDrift = 1.25;
 
 // The captured batch of iterations for (precision, x) on this handle: built
 // on first use, reused while the solution buffer stays the same.
@@ -301,9 +309,17 @@ static int iteration_graph(Handle* h, const void* x, int tsize, F&& one_iteratio
 // partials; chunk partials of W^T K (z + beta p) over the non-rigid elements
 // (beta formed there); the per-group gather with the scalars and stop test
 // beside it; mu = E^-1 t; x += alpha p, p = z + beta p - W mu.
+//
+// resume: b is the new residual of a system whose previous DPCG run on this
+// handle (same precision) stopped; the deflated start is formed for b as
+// usual, but the first direction continues the previous run's instead of
+// restarting: beta = r^T M r / rz_prev (rz_prev: r^T M r before that run's
+// last update) and p = z + beta p_prev - W mu, projected like every later
+// direction.  With b the true residual this is a reliable update (residual
+// replacement) that keeps the Krylov information across refinement steps.
 template <typename T>
 static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgResult& res,
-                hipStream_t s) {
+                hipStream_t s, bool resume = false) {
   Vecs<T> v = get_vecs<T>(h);
   const int64_t n = 3 * h->g.nnodes;
   const bool defl = h->ngroups > 0;
@@ -324,12 +340,22 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
                      n, part_rz);
   hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, h->scal, (const double*)part_rz,
                      kVecBlocks, (int)S_RZ, (int)S_RZ0);
-  if (defl) {
+  if (resume) {
+    // beta = rz / rz_prev, alpha = 0 (x keeps its deflated start)
+    hipLaunchKernelGGL(k_resume_scalars, dim3(1), dim3(1), 0, s, h->scal);
+    if (defl) {
+      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, v.invd, v.p, part_rz, kVecBlocks,
+                                                h->scal + S_RZPREV, nullptr, s));
+      PYTO_TRY(launch_coarse_gather(h, ct, nullptr, s));
+      PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
+    }
+  } else if (defl) {
     PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, ct, nullptr, s));
     PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
   }
-  // p0 = z0 - W mu (x untouched)
-  PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, true, nullptr, s, nullptr, 0));
+  // p0 = z0 - W mu, or (resume) z0 + beta p_prev - W mu; x untouched
+  PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, !resume, nullptr, s, nullptr,
+                                0));
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RTOL, rtol);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RELMIN, 1e300);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_FLOOR,
@@ -514,21 +540,29 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
   const double rz0 = h->host_scal[S_TMP0];
   double rel = rz0 > 0 ? 1.0 : 0.0;
   bool converged = rz0 == 0.0;
-  bool inner_fail = false;
+  bool inner_fail = false, resume = false;
+  // The correction systems K e = r_k are all scaled by the same factor
+  // 1/sqrt(r0^T M r0) (slot S_TMP0), so that each inner run can continue the
+  // previous run's search direction (reliable update): the FP32 iteration
+  // keeps its Krylov space while its residual is replaced by the true FP64
+  // one at every refinement step.  After a breakdown or stagnation the next
+  // inner run restarts from z - W mu.
   while (!converged && outer < max_outer && inner_total < max_iters) {
     hipLaunchKernelGGL(k_scale_convert<float>, dim3(kVecBlocks), dim3(256), 0, s, h->r64, h->b32,
-                       n, h->scal, (int)S_TMP1);
+                       n, h->scal, (int)S_TMP0);
     // inner tolerance: what is left to reach rtol (with a margin), but no
     // tighter than inner_rtol, the FP32 floor, and no looser than 0.1
     const double want = 0.3 * rtol / rel;
     const double tol_k = std::min(0.1, std::max(inner_rtol, want));
     DpcgResult ir;
-    PYTO_TRY(dpcg<float>(h, h->b32, h->x32, tol_k, max_iters - inner_total, ir, s));
+    PYTO_TRY(dpcg<float>(h, h->b32, h->x32, tol_k, max_iters - inner_total, ir, s, resume));
     inner_total += ir.iterations;
     mv32 += ir.matvecs;
     if (ir.breakdown && !ir.stagnated) inner_fail = true;
+    // recurrence r^T M r at the stop, in the scaled units of the inner run
+    const double rz_rec = h->host_scal[S_RZ];
     hipLaunchKernelGGL(k_axpy_scaled, dim3(kVecBlocks), dim3(256), 0, s, u, h->x32, n, h->scal,
-                       (int)S_TMP1);
+                       (int)S_TMP0);
     PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0,
                                    s));
     mv64++;
@@ -541,6 +575,12 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
     outer++;
     rel = sqrt(fabs(h->host_scal[S_TMP1] / rz0));
     converged = rel <= rtol;
+    // Continue the search direction only while the FP32 recurrence still
+    // tracks the true residual (within a factor 1.25 in norm); once rounding
+    // drift dominates, the old direction is no longer conjugate to the
+    // replaced residual and the next run restarts.
+    resume = !ir.breakdown && ir.converged &&
+             h->host_scal[S_TMP1] / rz0 <= kResumeDrift * kResumeDrift * rz_rec;
     if (inner_fail) break;
   }
   double rel_true = 0.0;

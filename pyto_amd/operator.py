@@ -169,13 +169,15 @@ class GridOperator:
         return E
 
     # -- solves ----------------------------------------------------------------
-    def solve(self, f, rtol=1e-8, max_iters=2000, precision="fp64", inner_rtol=1e-4,
-              max_outer=20, out=None):
+    def solve(self, f, rtol=1e-8, max_iters=2000, precision="fp64", inner_rtol=0.1,
+              max_outer=40, out=None):
         """u with K_ff u_f = f_f and u = 0 on fixed DOFs.
 
         precision "fp64": the reference DPCG (src/deflation.py:504-584).
-        precision "mixed": FP32 DPCG inside FP64 iterative refinement.
-        Returns (u, stats dict).
+        precision "mixed": FP32 DPCG inside FP64 iterative refinement; each
+        refinement step lowers the residual by about inner_rtol and then
+        replaces it by the FP64 one, continuing the FP32 search direction
+        (reliable update).  Returns (u, stats dict).
         """
         f = self.as_dof(f)
         u = torch.empty_like(f) if out is None else out
