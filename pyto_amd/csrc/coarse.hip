@@ -163,7 +163,7 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
 // chunk; k_coarse_gather sums them into t), where
 // v = invd o r + beta p (the new search direction before its coarse
 // correction; HAS_P, with beta = rz_new / rz formed from the update kernel's
-// block partials) or v = invd o r.  The element arithmetic is FP64
+// block partials) or v = invd o r.  invd null: r already holds z = invd o r.  The element arithmetic is FP64
 // whatever the vector precision T.  Group centres are wave-uniform per slot.
 // A workgroup serves one SpecialBlock: it first stages v at the block's
 // sorted corner nodes in LDS (an element's corners would otherwise be 24
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_restrict_special(
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
     ra[t] = r[dof[t]];
-    da[t] = invd[dof[t]];
+    da[t] = invd ? invd[dof[t]] : T(1);
     pa[t] = HAS_P ? p[dof[t]] : T(0);
   }
 #pragma unroll
@@ -357,15 +357,18 @@ int launch_gather_scalars(Handle* h, double* t, const double* part_pq, int npq,
   return PYTO_OK;
 }
 
-// Direction update of the deflated PCG:
-//   x += alpha p_old  (skipped on the first call),
-//   p  = invd o r + beta p_old - W mu  (fixed DOFs 0),
-// and block 0 clears the coarse right-hand side for the next iteration.
-// Vectorised over DOFs like the CG update: a thread owns one 16-byte vector
-// (4 FP32 / 2 FP64 DOFs), which spans at most two nodes; the node
-// coordinates are computed once and stepped to the next node.  Fixed DOFs
-// are those with invd = 0 (the Jacobi diagonal marks them), so the fixed mask
-// is not read.
+// Direction update of the deflated PCG (z-form):
+//   x += alpha p_old  (alpha = scal[S_ALPHA_X], published by the CG update;
+//                      skipped on the first call or when alpha is 0),
+//   p  = z + beta p_old - W mu  (first: z - W mu; fixed DOFs 0),
+// with z = M^-1 r as stored by the CG update.  z and p_old are 0 on fixed
+// DOFs; the fixed mask drops W mu there.  Vectorised over DOFs: a thread owns
+// one 16-byte vector (4 FP32 / 2 FP64 DOFs), which spans at most two nodes;
+// W mu is formed for both nodes (all three components) and each DOF selects
+// its own, so the loop has no per-DOF branches.  W mu of group q at position
+// X is a_q + w_q x X with w = (mu3, mu4, mu5) and a = (mu0, mu1, mu2) - w x c_q,
+// staged in LDS (6 values per group) for up to kLdsGroups groups and formed
+// from mu and the centre otherwise.
 template <typename T>
 struct DirVec;
 template <>
@@ -387,34 +390,15 @@ struct DirVec<double> {
   }
 };
 
-// Component c of W mu at node (ix,iy,iz) of group gid.
-__device__ __forceinline__ double w_mu(int c, int ix, int iy, int iz, int gid,
-                                       const double* __restrict__ mu,
-                                       const double* __restrict__ centers, const GridDims& g) {
-  const double rx = ix * g.hx - centers[3 * gid];
-  const double ry = iy * g.hy - centers[3 * gid + 1];
-  const double rz = iz * g.hz - centers[3 * gid + 2];
-  const double* m6 = mu + 6 * gid;
-  return c == 0 ? m6[0] + rz * m6[4] - ry * m6[5]
-       : c == 1 ? m6[1] - rz * m6[3] + rx * m6[5]
-                : m6[2] + ry * m6[3] - rx * m6[4];
-}
-
-// Groups whose rigid map is staged in LDS by k_dir_update (more: read on the fly).
 constexpr int kLdsGroups = 512;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_dir_update(
-    T* __restrict__ x, T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ invd,
+    T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
     const double* __restrict__ mu, const int32_t* __restrict__ grp,
-    const double* __restrict__ centers, int ngroups, GridDims g,
-    const double* __restrict__ scal, bool first, const int* __restrict__ done,
-    double* __restrict__ zero_vec, int zero_n) {
+    const double* __restrict__ centers, int ngroups, const uint8_t* __restrict__ mask,
+    GridDims g, const double* __restrict__ scal, bool first, const int* __restrict__ done) {
   if (done && *done) return;
-  if (zero_vec && blockIdx.x == 0)
-    for (int i = threadIdx.x; i < zero_n; i += blockDim.x) zero_vec[i] = 0.0;
-  // W mu of group q at position X is a_q + w_q x X with w = (mu3, mu4, mu5)
-  // and a = (mu0, mu1, mu2) - w x c_q: 6 values per group staged in LDS
   __shared__ double aw[kLdsGroups * 6];
   const bool lds = grp && ngroups <= kLdsGroups;
   if (lds) {
@@ -431,50 +415,65 @@ __global__ __launch_bounds__(256) void k_dir_update(
     }
     __syncthreads();
   }
-  auto wmu = [&](int c, int px, int py, int pz, int q) -> double {
-    if (!lds) return w_mu(c, px, py, pz, q, mu, centers, g);
+  // the three components of W mu at node (px, py, pz) of group q
+  auto wmu3 = [&](int px, int py, int pz, int q, double* w) {
     const double X = px * g.hx, Y = py * g.hy, Z = pz * g.hz;
-    const double* e = aw + 6 * q;
-    return c == 0 ? e[0] + e[4] * Z - e[5] * Y
-         : c == 1 ? e[1] + e[5] * X - e[3] * Z
-                  : e[2] + e[3] * Y - e[4] * X;
+    double e[6];
+    if (lds) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) e[k] = aw[6 * q + k];
+    } else {
+      const double* m6 = mu + 6 * q;
+      const double wx = m6[3], wy = m6[4], wz = m6[5];
+      const double cx = centers[3 * q], cy = centers[3 * q + 1], cz = centers[3 * q + 2];
+      e[0] = m6[0] - (wy * cz - wz * cy);
+      e[1] = m6[1] - (wz * cx - wx * cz);
+      e[2] = m6[2] - (wx * cy - wy * cx);
+      e[3] = wx;
+      e[4] = wy;
+      e[5] = wz;
+    }
+    w[0] = e[0] + e[4] * Z - e[5] * Y;
+    w[1] = e[1] + e[5] * X - e[3] * Z;
+    w[2] = e[2] + e[3] * Y - e[4] * X;
   };
   using DV = DirVec<T>;
   using V = typename DV::V;
   constexpr int W = DV::W;
   const double beta = first ? 0.0 : scal[S_BETA];
-  const double alpha = first ? 0.0 : scal[S_ALPHA];
+  const double alpha = first ? 0.0 : scal[S_ALPHA_X];
   const int64_t n = 3 * g.nnodes;
   const int64_t nv = n / W;
   V* pv = reinterpret_cast<V*>(p);
   V* xv = reinterpret_cast<V*>(x);
-  const V* rv = reinterpret_cast<const V*>(r);
-  const V* dv = reinterpret_cast<const V*>(invd);
+  const V* zv = reinterpret_cast<const V*>(z);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t d0 = i * W;
     const int64_t n0 = d0 / 3;
     const int c0 = (int)(d0 - 3 * n0);
-    int ix = 0, iy = 0, iz = 0, jx = 0, jy = 0, jz = 0, g0 = 0, g1 = 0;
+    // W mu of the vector's first node (0..2) and of the next node (3..5);
+    // fixed bits likewise (bit c of the first node, 3 + c of the next)
+    double w6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    uint32_t fx = 0;
     if (grp) {
+      int ix, iy, iz;
       node_coords(n0, g, ix, iy, iz);
-      jx = ix + 1;
-      jy = iy;
-      jz = iz;
-      if (jx == g.sx) {
-        jx = 0;
-        if (++jy == g.sy) {
-          jy = 0;
-          ++jz;
-        }
-      }
-      g0 = grp[n0];
-      g1 = (c0 + W > 3 && n0 + 1 < g.nnodes) ? grp[n0 + 1] : g0;
+      const bool wx = ix + 1 == g.sx;
+      const int jx = wx ? 0 : ix + 1;
+      const bool wy = wx && iy + 1 == g.sy;
+      const int jy = wx ? (wy ? 0 : iy + 1) : iy;
+      const int jz = wy ? iz + 1 : iz;
+      const bool two = c0 + W > 3 && n0 + 1 < g.nnodes;
+      const int g0 = grp[n0];
+      const int g1 = two ? grp[n0 + 1] : g0;
+      fx = (uint32_t)mask[n0] | (two ? (uint32_t)mask[n0 + 1] << 3 : 0u);
+      wmu3(ix, iy, iz, g0, w6);
+      wmu3(jx, jy, jz, g1, w6 + 3);
     }
-    V pp = pv[i];
-    const V rr = rv[i], dd = dv[i];
-    if (first) pp = V{};
-    if (!first) {
+    V pp = first ? V{} : pv[i];
+    const V zz = zv[i];
+    if (alpha != 0.0) {
       V xx = xv[i];
 #pragma unroll
       for (int j = 0; j < W; ++j)
@@ -483,46 +482,50 @@ __global__ __launch_bounds__(256) void k_dir_update(
     }
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-      const int cj = c0 + j;
-      const bool next = cj >= 3;
-      const int c = next ? cj - 3 : cj;
-      const double wm = grp ? wmu(c, next ? jx : ix, next ? jy : iy, next ? jz : iz,
-                                  next ? g1 : g0)
-                            : 0.0;
-      const T di = DV::get(dd, j);
-      const double val = (double)di * (double)DV::get(rr, j) + beta * (double)DV::get(pp, j) - wm;
-      DV::set(pp, j, di == T(0) ? T(0) : (T)val);
+      // component c0 + j of the pair of nodes; c0 is 0, 1 or 2
+      const double w = c0 == 0 ? w6[j] : (c0 == 1 ? w6[j + 1] : w6[j + 2]);
+      const double wm = ((fx >> (c0 + j)) & 1u) ? 0.0 : w;
+      DV::set(pp, j, (T)((double)DV::get(zz, j) + beta * (double)DV::get(pp, j) - wm));
     }
     pv[i] = pp;
   }
-  // tail: DOFs past the last whole vector, one node at a time by block 0
+  // tail: DOFs past the last whole vector, one at a time by block 0
   if (blockIdx.x == 0)
     for (int64_t d = nv * W + threadIdx.x; d < n; d += blockDim.x) {
       const int64_t nd = d / 3;
       const int c = (int)(d - 3 * nd);
       double wm = 0.0;
-      if (grp) {
+      if (grp && !((mask[nd] >> c) & 1u)) {
         int ix, iy, iz;
         node_coords(nd, g, ix, iy, iz);
-        wm = wmu(c, ix, iy, iz, grp[nd]);
+        double w3[3];
+        wmu3(ix, iy, iz, grp[nd], w3);
+        wm = c == 0 ? w3[0] : (c == 1 ? w3[1] : w3[2]);
       }
       const T pd = first ? T(0) : p[d];
-      if (!first) x[d] = (T)((double)x[d] + alpha * (double)pd);
-      const T di = invd[d];
-      const double val = (double)di * (double)r[d] + beta * (double)pd - wm;
-      p[d] = di == T(0) ? T(0) : (T)val;
+      if (alpha != 0.0) x[d] = (T)((double)x[d] + alpha * (double)pd);
+      p[d] = (T)((double)z[d] + beta * (double)pd - wm);
     }
 }
 
-// x += alpha p (the update the last iteration leaves pending when it stops).
 template <typename T>
-__global__ void k_axpy_alpha(T* __restrict__ x, const T* __restrict__ p, int64_t n,
-                             const double* __restrict__ scal) {
-  const double alpha = scal[S_ALPHA];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* done,
+                      hipStream_t s) {
+  // 2048 workgroups (8 per CU) striding over the vectors: each stages the
+  // group maps once, so fewer, longer-lived blocks keep those L2 reads small
+  const int64_t nvec = 3 * h->g.nnodes / (16 / (int64_t)sizeof(T));
+  const int blocks = (int)std::min<int64_t>((nvec + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), 0, s, x, p, z,
+                     (const double*)h->cmu,
+                     h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
+                     h->ngroups, h->fixmask, h->g, (const double*)h->scal, first, done);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
 }
+template int launch_dir_update<double>(Handle*, double*, double*, const double*, bool,
+                                       const int*, hipStream_t);
+template int launch_dir_update<float>(Handle*, float*, float*, const float*, bool, const int*,
+                                      hipStream_t);
 
 // E += W_b^T K W_a over the special elements; E is nc_pad x nc_pad row-major.
 __global__ __launch_bounds__(256) void k_galerkin(
@@ -682,8 +685,8 @@ template <typename T>
 int launch_restrict_special_parts(Handle* h, const T* r, const T* invd, const T* p,
                                   const double* part_rz, int nrz, const double* rz_old,
                                   const int* done, hipStream_t s) {
-  PYTO_REQUIRE(r && invd && (p == nullptr || (part_rz != nullptr && rz_old != nullptr)),
-               PYTO_EINVAL, "special restriction needs r, invd and (with p) the rz partials");
+  PYTO_REQUIRE(r && (p == nullptr || (part_rz != nullptr && rz_old != nullptr)),
+               PYTO_EINVAL, "special restriction needs r and (with p) the rz partials");
   if (h->nsblocks > 0) {
     if (p)
       hipLaunchKernelGGL((k_restrict_special<T, true>), dim3(h->nsblocks), dim3(256), 0, s, r,
@@ -716,29 +719,6 @@ int launch_restrict_special(Handle* h, const T* r, const T* invd, double* t, con
   return launch_coarse_gather(h, t, done, s);
 }
 
-template <typename T>
-int launch_dir_update(Handle* h, T* x, T* p, const T* r, const T* invd, const double* mu,
-                      bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n) {
-  // 2048 workgroups (8 per CU) striding over the vectors: each stages the
-  // group maps once, so fewer, longer-lived blocks keep those L2 reads small
-  const int64_t nvec = 3 * h->g.nnodes / (16 / (int64_t)sizeof(T));
-  const int blocks = (int)std::min<int64_t>((nvec + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), 0, s, x, p, r, invd, mu,
-                     h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
-                     h->ngroups, h->g, (const double*)h->scal, first, done, zero_vec, zero_n);
-  PYTO_CHECK_HIP(hipGetLastError());
-  return PYTO_OK;
-}
-
-template <typename T>
-int launch_axpy_alpha(Handle* h, T* x, const T* p, hipStream_t s) {
-  const int64_t n = 3 * h->g.nnodes;
-  hipLaunchKernelGGL(k_axpy_alpha<T>, dim3(2048), dim3(256), 0, s, x, p, n,
-                     (const double*)h->scal);
-  PYTO_CHECK_HIP(hipGetLastError());
-  return PYTO_OK;
-}
-
 template <typename T, bool UPDATE>
 int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
                    bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n) {
@@ -762,14 +742,6 @@ template int launch_restrict_special_parts<double>(Handle*, const double*, const
 template int launch_restrict_special_parts<float>(Handle*, const float*, const float*,
                                                   const float*, const double*, int,
                                                   const double*, const int*, hipStream_t);
-template int launch_dir_update<double>(Handle*, double*, double*, const double*, const double*,
-                                       const double*, bool, const int*, hipStream_t, double*,
-                                       int);
-template int launch_dir_update<float>(Handle*, float*, float*, const float*, const float*,
-                                      const double*, bool, const int*, hipStream_t, double*,
-                                      int);
-template int launch_axpy_alpha<double>(Handle*, double*, const double*, hipStream_t);
-template int launch_axpy_alpha<float>(Handle*, float*, const float*, hipStream_t);
 template int launch_prolong<double, false>(Handle*, double*, const double*, const double*,
                                            const double*, bool, const int*, hipStream_t,
                                            double*, int);

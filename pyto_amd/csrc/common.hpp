@@ -153,7 +153,8 @@ struct Handle {
   double shift_abs = 1e-10;     // coarse diagonal shift (reference default)
   double shift_rel = 0.0;
 
-  // solver workspace (allocated on first use)
+  // solver workspace (allocated on first use); in the DPCG, r holds
+  // z = M^-1 r
   double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr;
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr,
         *b32 = nullptr;
@@ -185,7 +186,8 @@ enum ScalarSlot {
   S_OC_STATE = 16, S_OC_LAST = 24,
   S_RELMIN = 25,  // best relative residual of the running solve (stagnation test)
   S_FLOOR = 26,   // the stagnation test applies once S_RELMIN is below this
-  S_RZPREV = 27   // r^T M r before the last iteration's update (resumed directions)
+  S_RZPREV = 27,  // r^T M r before the last iteration's update (resumed directions)
+  S_ALPHA_X = 28  // alpha of the running iteration for x += alpha p
 };
 // Device flag slots.
 // F_BREAKDOWN: 1 = p^T K p <= 0 or NaN, 2 = stagnation (see k_scalars)

@@ -1,17 +1,18 @@
 // Deflated Jacobi-preconditioned CG on the device (src/deflation.py:504-584)
 // and its FP32-inner / FP64-outer iterative refinement.
 //
-// One iteration is five launches on one stream; every launch first reads the
+// One iteration is six launches on one stream; every launch first reads the
 // device flag F_DONE and returns when it is set, so the host can enqueue a
 // batch of iterations and read the flag back once per batch while the
 // stopping rule sqrt(rz_new/rz0) <= rtol is applied on the device at exactly
 // the reference's iteration:
-//   1. q = K p, pKp = p.q (FP64 grid reduction); zero the coarse rhs
-//   2. alpha = rz/pKp; x += alpha p; r -= alpha q; z = M r; rz_new = r.z;
-//      convergence test; beta = rz_new/rz
-//   3. t = W^T K z (non-rigid elements only)
-//   4. mu = E^{-1} t (FP64 coarse inverse with one refinement step)
-//   5. p = z + beta p - W mu
+//   1. q = K p, p.q block partials
+//   2. alpha = rz/pKp; z -= alpha M^-1 q (z = M^-1 r is stored); r^T M^-1 r
+//      partials; alpha published for step 6
+//   3. beta = rz_new/rz; partials of W^T K (z + beta p) (non-rigid elements)
+//   4. per-group gather of those partials; scalars and convergence test
+//   5. mu = E^-1 t (FP64 coarse inverse as a dense GEMV)
+//   6. x += alpha p; p = z + beta p - W mu
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -40,15 +41,13 @@ int launch_gather_scalars(Handle* h, double* t, const double* part_pq, int npq,
 template <typename T, bool UPDATE>
 int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
                    bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n);
-template <typename T>
-int launch_dir_update(Handle* h, T* x, T* p, const T* r, const T* invd, const double* mu,
-                      bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n);
-template <typename T>
-int launch_axpy_alpha(Handle* h, T* x, const T* p, hipStream_t s);
 int coarse_apply(Handle* h, const double* t1, const double* t2, const double* beta,
                  double* mu, const int* done, hipStream_t s, bool refine);
 template <typename T>
 int matvec_blocks(const Handle* h);
+template <typename T>
+int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* done,
+                      hipStream_t s);
 
 constexpr int kVecBlocks = 2048;
 
@@ -65,58 +64,104 @@ struct Vec4<double> {
   static constexpr int width = 2;
 };
 
-template <typename V, typename F>
-__device__ __forceinline__ void for_lanes(V& a, const V& b, const V& c, F fn) {
-  if constexpr (sizeof(V) / sizeof(a.x) == 4) {
-    fn(a.x, b.x, c.x); fn(a.y, b.y, c.y); fn(a.z, b.z, c.z); fn(a.w, b.w, c.w);
-  } else {
-    fn(a.x, b.x, c.x); fn(a.y, b.y, c.y);
-  }
-}
-
-// alpha = rz/pKp with pKp summed from the matvec's block partials (every
-// block sums them in the same order); r -= alpha q; per-block partials of
-// r^T (invd o r) go to part_rz for k_scalars.  Vector loads of 16 bytes; the
-// tail (n not a multiple of the width) is handled by block 0.
+// z-form update: alpha = rz/pKp with pKp summed from the operator's block
+// partials (every block sums them in the same order); z -= alpha invd o q
+// (formed in FP64, stored in T); per-block partials of
+// r^T M^-1 r = sum z_i^2 / invd_i over free DOFs (invd_i > 0) to part_rz.
+// Block 0 publishes alpha in scal[S_ALPHA_X] for the x update, which the
+// direction update applies (or, for the iteration that stops the solve, the
+// host after the loop).  With pKp <= 0 or NaN the block leaves z untouched
+// and alpha is published as 0 (the scalar step flags the breakdown).  Vector
+// loads of 16 bytes; block 0 takes the tail.
 template <typename T>
-__global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ r, const T* __restrict__ q,
-                                                   const T* __restrict__ invd, int64_t n,
-                                                   const double* __restrict__ scal,
-                                                   const int* __restrict__ flags,
-                                                   const double* __restrict__ part_pq, int npq,
-                                                   double* __restrict__ part_rz) {
+__global__ __launch_bounds__(256) void k_cg_update_z(
+    T* __restrict__ z, const T* __restrict__ q, const T* __restrict__ invd, int64_t n,
+    double* __restrict__ scal, const int* __restrict__ flags,
+    const double* __restrict__ part_pq, int npq, double* __restrict__ part_rz) {
   if (flags[F_DONE]) return;
   __shared__ double red[5];
-  using V = typename Vec4<T>::type;
-  constexpr int W = Vec4<T>::width;
   const double pkp = block_sum_partials(part_pq, npq, red);
-  const double alpha = scal[S_RZ] / pkp;
-  const T a = (T)alpha;
   double acc = 0.0;
-  const int64_t nv = n / W;
-  V* rv = reinterpret_cast<V*>(r);
-  const V* qv = reinterpret_cast<const V*>(q);
-  const V* dv = reinterpret_cast<const V*>(invd);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    V rr = rv[i];
-    const V qq = qv[i];
-    const V dd = dv[i];
-    for_lanes(rr, qq, dd, [&](T& ri, const T& qi, const T& di) {
-      ri = ri - a * qi;
-      acc += (double)ri * (double)di * (double)ri;
-    });
-    rv[i] = rr;
-  }
-  if (blockIdx.x == 0)
-    for (int64_t i = nv * W + threadIdx.x; i < n; i += blockDim.x) {
-      const T ri = r[i] - a * q[i];
-      r[i] = ri;
-      acc += (double)ri * (double)invd[i] * (double)ri;
+  const bool ok = pkp > 0.0;   // false for NaN
+  const double alpha = ok ? scal[S_RZ] / pkp : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_ALPHA_X] = alpha;
+  if (ok) {
+    auto lane = [&](T& zi, const T qi, const T di) {
+      zi = (T)((double)zi - alpha * ((double)di * (double)qi));
+      if (di > T(0)) acc += (double)zi * (double)(zi / di);
+    };
+    using V = typename Vec4<T>::type;
+    constexpr int W = Vec4<T>::width;
+    const int64_t nv = n / W;
+    V* zv = reinterpret_cast<V*>(z);
+    const V* qv = reinterpret_cast<const V*>(q);
+    const V* dv = reinterpret_cast<const V*>(invd);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      V zz = zv[i];
+      const V qq = qv[i], dd = dv[i];
+      lane(zz.x, qq.x, dd.x);
+      lane(zz.y, qq.y, dd.y);
+      if constexpr (W == 4) {
+        lane(zz.z, qq.z, dd.z);
+        lane(zz.w, qq.w, dd.w);
+      }
+      zv[i] = zz;
     }
+    if (blockIdx.x == 0)
+      for (int64_t i = nv * W + threadIdx.x; i < n; i += blockDim.x) lane(z[i], q[i], invd[i]);
+  }
   __shared__ double red4[4];
   const double bs = block_reduce_sum256(acc, red4);
   if (threadIdx.x == 0) part_rz[blockIdx.x] = bs;
+}
+
+// x += scal[S_ALPHA_X] p (in FP64, stored in T): the update of the iteration
+// that stopped the solve, whose direction update was skipped.
+template <typename T>
+__global__ __launch_bounds__(256) void k_axpy_x(T* __restrict__ x, const T* __restrict__ p,
+                                               int64_t n, const double* __restrict__ scal) {
+  const double alpha = scal[S_ALPHA_X];
+  if (alpha == 0.0) return;
+  using V = typename Vec4<T>::type;
+  constexpr int W = Vec4<T>::width;
+  const int64_t nv = n / W;
+  V* xv = reinterpret_cast<V*>(x);
+  const V* pv = reinterpret_cast<const V*>(p);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    V xx = xv[i];
+    const V pp = pv[i];
+    xx.x = (T)((double)xx.x + alpha * (double)pp.x);
+    xx.y = (T)((double)xx.y + alpha * (double)pp.y);
+    if constexpr (W == 4) {
+      xx.z = (T)((double)xx.z + alpha * (double)pp.z);
+      xx.w = (T)((double)xx.w + alpha * (double)pp.w);
+    }
+    xv[i] = xx;
+  }
+  if (blockIdx.x == 0)
+    for (int64_t i = nv * W + threadIdx.x; i < n; i += blockDim.x)
+      x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+}
+
+// z = invd o (f - q) (in T); per-block partials of r^T (invd o r).
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual_z(const T* __restrict__ f,
+                                                    const T* __restrict__ q, T* __restrict__ z,
+                                                    const T* __restrict__ invd, int64_t n,
+                                                    double* __restrict__ part) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double ri = (double)f[i] - (double)q[i];
+    const double di = (double)invd[i];
+    z[i] = (T)(di * ri);
+    acc += ri * di * ri;
+  }
+  const double bs = block_reduce_sum256(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 // One block: finishes the iteration's scalars from the partials: alpha (as
@@ -208,7 +253,7 @@ __global__ void k_set_scalar(double* scal, int slot, double v) { scal[slot] = v;
 
 __global__ void k_resume_scalars(double* scal) {
   scal[S_BETA] = scal[S_RZ] / scal[S_RZPREV];
-  scal[S_ALPHA] = 0.0;
+  scal[S_ALPHA_X] = 0.0;
 }
 
 template <typename T>
@@ -252,6 +297,17 @@ static int read_back(Handle* h, hipStream_t s) {
 }
 
 constexpr int kGraphBatch = 16;
+
+// The handle's capture/replay stream (default priority) and its events,
+// created on first use.
+static int ensure_streams(Handle* h) {
+  if (!h->gstream) {
+    PYTO_CHECK_HIP(hipStreamCreateWithFlags(&h->gstream, hipStreamNonBlocking));
+    PYTO_CHECK_HIP(hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming));
+    PYTO_CHECK_HIP(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
+  }
+  return PYTO_OK;
+}
 // largest ratio of true to recurrence residual norm at which a mixed
 // refinement step continues the previous inner run's search direction
 constexpr double kResumeDrift = 1.25;
@@ -261,11 +317,7 @@ constexpr double kResumeDrift = 1.25;
 template <typename F>
 static int iteration_graph(Handle* h, const void* x, int tsize, F&& one_iteration,
                            hipGraphExec_t* out) {
-  if (!h->gstream) {
-    PYTO_CHECK_HIP(hipStreamCreateWithFlags(&h->gstream, hipStreamNonBlocking));
-    PYTO_CHECK_HIP(hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming));
-    PYTO_CHECK_HIP(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
-  }
+  PYTO_TRY(ensure_streams(h));
   const int slot = tsize == 8 ? 0 : 1;
   GraphCache& gc = h->graphs[slot];
   if (gc.exec && gc.x == x && gc.ngroups == h->ngroups && gc.nchunks == h->nchunks) {
@@ -302,13 +354,15 @@ static int iteration_graph(Handle* h, const void* x, int tsize, F&& one_iteratio
 // rounding (an FP32 operator, the coarse solve) then accumulates and the
 // iteration stalls or diverges.  Here the whole new direction is projected,
 // mu = E^-1 W^T K (z + beta p_old), so W^T K p_new = 0 is re-established
-// every step.  In exact arithmetic the two forms coincide.  z = invd o r is
-// never stored: the kernels that need it form it from r.
+// every step.  In exact arithmetic the two forms coincide.
 //
-// Per iteration: q = K p with p^T q partials; r -= alpha q with r^T M r
-// partials; chunk partials of W^T K (z + beta p) over the non-rigid elements
-// (beta formed there); the per-group gather with the scalars and stop test
-// beside it; mu = E^-1 t; x += alpha p, p = z + beta p - W mu.
+// The preconditioned residual z = M^-1 r is stored instead of r (r = z/invd
+// on free DOFs), so neither the restriction nor the direction update reads
+// the diagonal.  Per iteration: q = K p with p^T q partials; z -= alpha M^-1 q
+// with r^T M^-1 r partials; chunk partials of W^T K (z + beta p) over the
+// non-rigid elements (beta formed there); the per-group gather with the
+// scalars and stop test beside it; mu = E^-1 t; x += alpha p,
+// p = z + beta p - W mu.
 //
 // resume: b is the new residual of a system whose previous DPCG run on this
 // handle (same precision) stopped; the deflated start is formed for b as
@@ -324,6 +378,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   const int64_t n = 3 * h->g.nnodes;
   const bool defl = h->ngroups > 0;
   double* ct = h->ct;
+  PYTO_TRY(ensure_streams(h));
   if (defl) {
     PYTO_TRY(launch_restrict_full<T>(h, b, ct, s));
     PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
@@ -336,26 +391,26 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   const int npq = matvec_blocks<T>(h);
   PYTO_TRY(launch_matvec<T>(h, x, v.q, v.Ee, v.mc, nullptr, nullptr, nullptr, 0, s));
   res.matvecs += 1;
-  hipLaunchKernelGGL((k_residual<T, T>), dim3(kVecBlocks), dim3(256), 0, s, b, v.q, v.r, v.invd,
+  // z0 = M^-1 (b - K x0)
+  hipLaunchKernelGGL((k_residual_z<T>), dim3(kVecBlocks), dim3(256), 0, s, b, v.q, v.r, v.invd,
                      n, part_rz);
   hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, h->scal, (const double*)part_rz,
                      kVecBlocks, (int)S_RZ, (int)S_RZ0);
   if (resume) {
-    // beta = rz / rz_prev, alpha = 0 (x keeps its deflated start)
+    // beta = rz / rz_prev; no x update
     hipLaunchKernelGGL(k_resume_scalars, dim3(1), dim3(1), 0, s, h->scal);
     if (defl) {
-      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, v.invd, v.p, part_rz, kVecBlocks,
-                                                h->scal + S_RZPREV, nullptr, s));
+      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, (const T*)nullptr, v.p, part_rz,
+                                                kVecBlocks, h->scal + S_RZPREV, nullptr, s));
       PYTO_TRY(launch_coarse_gather(h, ct, nullptr, s));
       PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
     }
   } else if (defl) {
-    PYTO_TRY(launch_restrict_special<T>(h, v.r, v.invd, ct, nullptr, s));
+    PYTO_TRY(launch_restrict_special<T>(h, v.r, (const T*)nullptr, ct, nullptr, s));
     PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
   }
-  // p0 = z0 - W mu, or (resume) z0 + beta p_prev - W mu; x untouched
-  PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, !resume, nullptr, s, nullptr,
-                                0));
+  // p0 = z0 - W mu, or (resume) z0 + beta p_prev - W mu
+  PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, !resume, nullptr, s));
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RTOL, rtol);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RELMIN, 1e300);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_FLOOR,
@@ -372,21 +427,20 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   // one iteration's launches on stream `st`
   auto one_iteration = [&](hipStream_t st) -> int {
     PYTO_TRY(launch_matvec<T>(h, v.p, v.q, v.Ee, v.mc, part_pq, done, nullptr, 0, st));
-    hipLaunchKernelGGL(k_cg_update<T>, dim3(kVecBlocks), dim3(256), 0, st, v.r, v.q, v.invd, n,
-                       (const double*)h->scal, (const int*)h->flags, (const double*)part_pq, npq,
-                       part_rz);
+    hipLaunchKernelGGL(k_cg_update_z<T>, dim3(kVecBlocks), dim3(256), 0, st, v.r, v.q, v.invd,
+                       n, h->scal, (const int*)h->flags, (const double*)part_pq, npq, part_rz);
     if (defl) {
       // beta is formed inside the restriction from the update's partials, so
       // the scalar logic runs beside the per-group gather in one launch
-      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, v.invd, v.p, part_rz, kVecBlocks,
-                                                h->scal + S_RZ, done, st));
+      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, (const T*)nullptr, v.p, part_rz,
+                                                kVecBlocks, h->scal + S_RZ, done, st));
       PYTO_TRY(launch_gather_scalars(h, ct, part_pq, npq, part_rz, kVecBlocks, st));
       PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, done, st, false));
     } else {
       hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, st, h->scal, h->flags,
                          (const double*)part_pq, npq, (const double*)part_rz, kVecBlocks);
     }
-    PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, v.invd, h->cmu, false, done, st, nullptr, 0));
+    PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, false, done, st));
     PYTO_CHECK_HIP(hipGetLastError());
     return PYTO_OK;
   };
@@ -432,8 +486,12 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   res.stagnated = h->host_flags[F_BREAKDOWN] == 2;
   res.converged = h->host_flags[F_DONE] != 0 && !res.breakdown;
   res.rel = h->host_scal[S_REL];
-  // the stopping iteration skipped its direction update: apply its x += alpha p
-  if (res.converged) PYTO_TRY(launch_axpy_alpha<T>(h, x, v.p, s));
+  // the iteration that set the stop flag skipped its direction update, and
+  // with it x += alpha p (alpha 0 after a breakdown)
+  if (h->host_flags[F_DONE])
+    hipLaunchKernelGGL(k_axpy_x<T>, dim3(kVecBlocks), dim3(256), 0, s, x, (const T*)v.p, n,
+                       (const double*)h->scal);
+  PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
 
