@@ -60,6 +60,11 @@ int pyto_version(void);
  * Replaces hex8_stiffness_matrix_structural, src/hex_element_stiffness.py:231-286. */
 int pyto_ke_hex8(double E, double nu, double hx, double hy, double hz, double* ke_host);
 
+/* Thermal element matrix KE_T (8x8, row-major FP64) of one brick of
+ * conductivity k0.  Replaces hex8_stiffness_matrix_thermal,
+ * src/hex_element_stiffness.py:339-385. */
+int pyto_ke_hex8_thermal(double k0, double hx, double hy, double hz, double* ke_host);
+
 /* Create the grid operator on the current device.
  * nelx/nely/nelz, element size and material replace HexMesher.grid_mesh +
  * HexStructuralFEA.__init__ (src/hex_mesher.py:72-169,
@@ -71,6 +76,19 @@ int pyto_create(pyto_handle** out, int nelx, int nely, int nelz,
                 double hx, double hy, double hz, double E0, double nu,
                 double penal, double eps_void, const uint8_t* fixed_mask,
                 void* stream);
+
+/* Create a thermal grid operator (one temperature per node) on the current
+ * device: the 8x8 KE_T of conductivity k0 applied matrix-free, replacing the
+ * COO assembly of HexThermalFEA.solve (src/hex_thermal_fea.py:74-117).
+ * fixed_mask: uint8 per node, bit 0 set if the node's temperature is fixed
+ * (zero Dirichlet value).  penal/k_void parametrise the conductivity law
+ * k_e = k0*(k_void + x^penal (1-k_void)) (src/topopt_material_model.py:98-109).
+ * The same matvec, diagonal, set_density, solve and compliance entry points
+ * then act on vectors of one value per node.  Deflation groups are not
+ * supported on a thermal handle (the solves are Jacobi PCG). */
+int pyto_create_thermal(pyto_handle** out, int nelx, int nely, int nelz, double hx, double hy,
+                        double hz, double k0, double penal, double k_void,
+                        const uint8_t* fixed_mask, void* stream);
 int pyto_destroy(pyto_handle* h);
 int pyto_info(pyto_handle* h, int64_t* nnodes, int64_t* nelems, int64_t* ndof,
               int64_t* workspace_bytes);
@@ -98,6 +116,12 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
  * (src/deflation.py:540,558,573; src/hex_structural_fea.py:101). */
 int pyto_matvec_f64(pyto_handle* h, const double* u, double* y, void* stream);
 int pyto_matvec_f32(pyto_handle* h, const float* u, float* y, void* stream);
+
+/* y = K_fc g: the operator with g's fixed entries kept and y's fixed entries
+ * zeroed (FP64).  f_f - K_fc g is the right-hand side with the non-zero
+ * Dirichlet values g eliminated: b' = b - A[:, fixed] g of impose_dirichlet_bc
+ * (src/bound_cond.py:57-66). */
+int pyto_matvec_lift_f64(pyto_handle* h, const double* g, double* y, void* stream);
 
 /* diag(K_ff) (fixed DOFs 0); replaces A.diagonal() in
  * _jacobi_preconditioner (src/linear_solvers.py:84). */
@@ -151,6 +175,11 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
  * ce and grad may be null. */
 int pyto_compliance(pyto_handle* h, const double* u, const double* x, double* ce,
                     double* grad, double* C, void* stream);
+
+/* Centre temperature gradient in reference coordinates of a thermal handle:
+ * grad (3 x N_el, FP64) = gradN @ T[edofMatThermal].T with gradN the +-1/8
+ * table of HexThermalFEA.postprocess (src/hex_thermal_fea.py:250-270). */
+int pyto_thermal_gradient(pyto_handle* h, const double* T, double* grad, void* stream);
 
 /* Bilinear element term be_e = a_e^T KE b_e (FP64); with a = adjoint and
  * b = u it is the T2 factor of src/topopt_obj_cons_sensitivities.py:279-282. */

@@ -12,7 +12,10 @@ Restates:
   * Jacobi diag^{-1} with |d| < 1e-12 -> 1, src/linear_solvers.py:71-88;
   * centre-point stresses, von Mises and p-norm,
     src/hex_structural_fea.py:152-234 (relaxation q at
-    src/topopt_material_model.py:15,199-201; p at :26).
+    src/topopt_material_model.py:15,199-201; p at :26);
+  * the thermal conductivity law, K_T assembly and centre temperature
+    gradient, src/topopt_material_model.py:98-157,
+    src/hex_thermal_fea.py:47-50,85-112,256-270.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -28,6 +31,36 @@ def simp_scaling(x, penal=3.0, eps=1e-8):
 def simp_sensitivity(x, penal=3.0, eps=1e-8):
     """src/topopt_material_model.py:131."""
     return penal * (x ** (penal - 1.0)) * (1.0 - eps)
+
+
+def thermal_scaling(x, penal=1.0, kvoid=1e-8):
+    """src/topopt_material_model.py:106-109 (SIMP; _KVOID_RELATIVE at :19,
+    _SIMP_THERMAL_PENALTY at :14)."""
+    return kvoid + (x ** penal) * (1.0 - kvoid)
+
+
+def thermal_sensitivity(x, penal=1.0, kvoid=1e-8):
+    """src/topopt_material_model.py:150-151."""
+    return penal * (x ** (penal - 1.0)) * (1.0 - kvoid)
+
+
+def assemble_KT(mesh, KE_T, k_e):
+    """Global K_T as CSR; src/hex_thermal_fea.py:47-50,85-112 + .tocsr()."""
+    edof = mesh.elemArray.astype(np.int64)
+    nn = mesh.num_nodes
+    iK = np.kron(edof, np.ones((8, 1), dtype=np.int64)).ravel()
+    jK = np.kron(edof, np.ones((1, 8), dtype=np.int64)).ravel()
+    vals = np.einsum("ij,e->eij", KE_T, k_e).ravel(order="C")
+    return sp.coo_matrix((vals, (iK, jK)), shape=(nn, nn)).tocsr()
+
+
+def thermal_gradient(mesh, T):
+    """Centre gradient in reference coordinates, gradN @ T[edof].T with
+    gradN = (1/8) [+-1 signs]; src/hex_thermal_fea.py:256-270."""
+    gradN = 0.125 * np.array([[-1, 1, 1, -1, -1, 1, 1, -1],
+                              [-1, -1, 1, 1, -1, -1, 1, 1],
+                              [-1, -1, -1, -1, 1, 1, 1, 1]], dtype=np.float64)
+    return gradN @ T[mesh.elemArray].T
 
 
 def assemble_K(mesh, KE, E_e):

@@ -5,7 +5,9 @@ Restates ``src/hex_element_stiffness.py``:
   * node ordering (bottom face CCW 0-1-2-3, top 4-7) ``:63-71,103-105``;
   * shape-function gradients ``:126-140``; Jacobian ``:159-186``;
   * physical gradients ``:205-218``; isotropic D ``:220-228``;
-  * B rows [xx, yy, zz, xy, yz, xz] and KE = sum B^T D B detJ w ``:254-286``.
+  * B rows [xx, yy, zz, xy, yz, xz] and KE = sum B^T D B detJ w ``:254-286``;
+  * thermal KE_T = sum k B^T B detJ w ``:339-385``;
+  * the thermo-elastic coupling matrix of ``src/hex_thermal_fea.py:119-196``.
 """
 from fractions import Fraction
 
@@ -154,6 +156,88 @@ def hex8_stiffness_exact(E, nu, elem_size):
                     val += mu * I[n, m, k, i] / (h[k] * h[i])
                     ke[3 * n + i, 3 * m + k] = V * val
     return ke
+
+
+def hex8_stiffness_thermal(k, elem_size):
+    """8x8 KE_T by 2x2x2 Gauss; src/hex_element_stiffness.py:339-385:
+    b = dN/dx (3 x 8) at each point, KE_T = sum k b^T b detJ w."""
+    dx, dy, dz = elem_size
+    nodes = np.array([[0, dx, dx, 0, 0, dx, dx, 0],
+                      [0, 0, dy, dy, 0, 0, dy, dy],
+                      [0, 0, 0, 0, dz, dz, dz, dz]], dtype=np.float64).T
+    ke = np.zeros((8, 8))
+    pts, wts = gauss_points_3d()
+    for p, w in zip(pts, wts):
+        diso = _iso_gradients(p)
+        jac = np.stack([diso[:, j] @ nodes for j in range(3)], axis=1)
+        jinv = np.linalg.inv(jac)
+        b = np.array([jinv @ diso[n] for n in range(8)]).T   # (3, 8)
+        ke += k * b.T @ b * np.linalg.det(jac) * w
+    return ke
+
+
+def _corner_polys():
+    """Shape functions on [0,1]^3 and their reference gradients as exact
+    polynomials: N_n = X(a) Y(b) Z(c), X(0) = 1-x, X(1) = x."""
+    corner = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+              (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+
+    def lin(flag, axis):
+        e1 = [0, 0, 0]
+        e1[axis] = 1
+        if flag:
+            return {tuple(e1): Fraction(1)}
+        return {(0, 0, 0): Fraction(1), tuple(e1): Fraction(-1)}
+
+    def dlin(flag):
+        return {(0, 0, 0): Fraction(1 if flag else -1)}
+
+    N, grads = [], []
+    for (a, b, c) in corner:
+        N.append(_poly_mul(_poly_mul(lin(a, 0), lin(b, 1)), lin(c, 2)))
+        grads.append([_poly_mul(_poly_mul(dlin(a), lin(b, 1)), lin(c, 2)),
+                      _poly_mul(_poly_mul(lin(a, 0), dlin(b)), lin(c, 2)),
+                      _poly_mul(_poly_mul(lin(a, 0), lin(b, 1)), dlin(c))])
+    return N, grads
+
+
+def hex8_stiffness_thermal_exact(k, elem_size):
+    """Exact KE_T = k int grad N_n . grad N_m dV (rational integration on
+    the unit cube, scaled to the brick)."""
+    h = np.asarray(elem_size, dtype=np.float64)
+    V = float(np.prod(h))
+    _, grads = _corner_polys()
+    ke = np.zeros((8, 8))
+    for n in range(8):
+        for m in range(8):
+            ke[n, m] = k * V * sum(float(_poly_integrate_unit_cube(
+                _poly_mul(grads[n][j], grads[m][j]))) / (h[j] * h[j]) for j in range(3))
+    return ke
+
+
+def thermoelastic_H(elem_size, nu, layout="reference"):
+    """24 x 8 coupling matrix H with f_e = E0 alpha E(x) H (T_e - T_ref):
+    H = (1/(1-2nu)) int B^T m N dV, m = [1,1,1,0,0,0] (the thermal strain
+    alpha dT m through the isotropic D, D m = E/(1-2nu) m), i.e.
+    H[(c, j), k] = (1/(1-2nu)) int dN_j/dx_c N_k dV (exact integration).
+
+    layout "physical": row 3j + c (the node-major DOF order of edofMat).
+    layout "reference": row 8c + j, the row order the reference's getHMatrix
+    returns (src/hex_thermal_fea.py:119-196, reshaped column-major), which
+    the reference then indexes as 3j + c (:224-237, and
+    src/topopt_obj_cons_sensitivities.py:358-366,439-447).
+    """
+    h = np.asarray(elem_size, dtype=np.float64)
+    V = float(np.prod(h))
+    N, grads = _corner_polys()
+    H = np.zeros((24, 8))
+    for j in range(8):
+        for c in range(3):
+            row = 3 * j + c if layout == "physical" else 8 * c + j
+            for k in range(8):
+                H[row, k] = V / h[c] * float(_poly_integrate_unit_cube(
+                    _poly_mul(grads[j][c], N[k]))) / (1.0 - 2.0 * nu)
+    return H
 
 
 def rigid_body_modes(elem_size):

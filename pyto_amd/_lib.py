@@ -47,6 +47,11 @@ _SIGS = {
     "pyto_version": ([], _i),
     "pyto_ke_hex8": ([_d, _d, _d, _d, _d, _vp], _i),
     "pyto_create": ([ctypes.POINTER(_vp), _i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _vp, _vp], _i),
+    "pyto_ke_hex8_thermal": ([_d, _d, _d, _d, _vp], _i),
+    "pyto_matvec_lift_f64": ([_vp, _vp, _vp, _vp], _i),
+    "pyto_thermal_gradient": ([_vp, _vp, _vp, _vp], _i),
+    "pyto_create_thermal": ([ctypes.POINTER(_vp), _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp, _vp],
+                            _i),
     "pyto_destroy": ([_vp], _i),
     "pyto_info": ([_vp, _vp, _vp, _vp, _vp], _i),
     "pyto_set_density": ([_vp, _vp, _vp], _i),

@@ -3,7 +3,9 @@
 ``BC`` keeps the reference's fields.  On the GPU the Dirichlet elimination of
 ``impose_dirichlet_bc`` is a per-node fixed-DOF mask applied inside the
 operator (fixed entries of inputs ignored, of outputs zeroed), which equals
-``A[free][:, free]`` for zero Dirichlet values.
+``A[free][:, free]``.  Non-zero Dirichlet values g are lifted: the right-hand
+side becomes b - A[:, fixed] g (one operator pass that keeps g's fixed
+entries) and g is written into the fixed entries of the solution.
 """
 import dataclasses
 
@@ -32,21 +34,32 @@ class BC:
 
 
 def check_supported(bc):
-    """The GPU path eliminates zero Dirichlet values only (all the reference's
-    structural recipes use zeros, e.g. src/hex_structural_examples.py:356)."""
+    """Lagrange-multiplier constraints are not part of the GPU path."""
     if bc.constraint_matrix is not None:
         raise NotImplementedError("constraint_matrix BCs are not supported "
                                   "(the reference solves them with PARDISO only, src/linear_solvers.py:204-225)")
-    dv = np.asarray(bc.dirichlet_values, dtype=np.float64)
-    if dv.size and np.any(dv != 0.0):
-        raise NotImplementedError("non-zero Dirichlet values are not supported on the GPU path")
+
+
+def dirichlet_vector(bc, num_dofs):
+    """g at the fixed DOFs and 0 elsewhere, or None when every value is 0."""
+    dv = np.asarray(bc.dirichlet_values, dtype=np.float64).ravel()
+    if dv.size == 0 or not np.any(dv != 0.0):
+        return None
+    fixed = np.asarray(bc.fixed_dofs, dtype=np.int64).ravel()
+    g = np.zeros(num_dofs, dtype=np.float64)
+    g[fixed] = np.broadcast_to(dv, fixed.shape) if dv.size in (1, fixed.size) else dv
+    return g
 
 
 def impose_dirichlet_bc(A, b, bc):
     """src/bound_cond.py:40-71 for the matrix-free operator: returns the
-    operator (which applies the fixed-DOF mask itself) and b restricted to
-    the free DOFs."""
+    operator (which applies the fixed-DOF mask itself) and
+    b - A[:, fixed] g restricted to the free DOFs."""
     if bc.constraint_matrix is not None:
         raise NotImplementedError("impose_dirichlet_bc not implemented for constraint_matrix BCs")
     check_supported(bc)
-    return A, np.asarray(b)[bc.free_dofs]
+    b = np.asarray(b, dtype=np.float64)
+    g = dirichlet_vector(bc, b.shape[0])
+    if g is not None:
+        b = b - A.lift(g).cpu().numpy()
+    return A, b[bc.free_dofs]

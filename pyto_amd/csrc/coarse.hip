@@ -397,7 +397,8 @@ __global__ __launch_bounds__(256) void k_dir_update(
     T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
     const double* __restrict__ mu, const int32_t* __restrict__ grp,
     const double* __restrict__ centers, int ngroups, const uint8_t* __restrict__ mask,
-    GridDims g, const double* __restrict__ scal, bool first, const int* __restrict__ done) {
+    GridDims g, int nc, const double* __restrict__ scal, bool first,
+    const int* __restrict__ done) {
   if (done && *done) return;
   __shared__ double aw[kLdsGroups * 6];
   const bool lds = grp && ngroups <= kLdsGroups;
@@ -442,7 +443,7 @@ __global__ __launch_bounds__(256) void k_dir_update(
   constexpr int W = DV::W;
   const double beta = first ? 0.0 : scal[S_BETA];
   const double alpha = first ? 0.0 : scal[S_ALPHA_X];
-  const int64_t n = 3 * g.nnodes;
+  const int64_t n = nc * g.nnodes;   // groups (grp) only with nc = 3
   const int64_t nv = n / W;
   V* pv = reinterpret_cast<V*>(p);
   V* xv = reinterpret_cast<V*>(x);
@@ -513,12 +514,13 @@ int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* 
                       hipStream_t s) {
   // 2048 workgroups (8 per CU) striding over the vectors: each stages the
   // group maps once, so fewer, longer-lived blocks keep those L2 reads small
-  const int64_t nvec = 3 * h->g.nnodes / (16 / (int64_t)sizeof(T));
+  const int64_t nvec = h->ncomp * h->g.nnodes / (16 / (int64_t)sizeof(T));
   const int blocks = (int)std::min<int64_t>((nvec + 255) / 256, 2048);
   hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), 0, s, x, p, z,
                      (const double*)h->cmu,
                      h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
-                     h->ngroups, h->fixmask, h->g, (const double*)h->scal, first, done);
+                     h->ngroups, h->fixmask, h->g, h->ncomp, (const double*)h->scal, first,
+                     done);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -828,6 +830,8 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
                     const double* centers, void* stream) {
   PYTO_REQUIRE(h, PYTO_EINVAL, "null handle");
   PYTO_REQUIRE(ngroups >= 0, PYTO_EINVAL, "ngroups must be >= 0");
+  PYTO_REQUIRE(ngroups == 0 || h->ncomp == 3, PYTO_EINVAL,
+               "rigid-body deflation groups need a structural handle");
   PYTO_REQUIRE(6 * (int64_t)ngroups <= 12288, PYTO_EINVAL,
                "coarse space of %d groups (%d unknowns) exceeds the dense GPU limit of 2048 "
                "groups; use fewer, larger deflation groups", ngroups, 6 * ngroups);

@@ -56,6 +56,7 @@ template <typename T>
 struct ModalConsts {
   T gx, gy, gz;   // 2/h per axis
   T lam, mu;      // Lame constants of E0 times V/64 (Hadamard scaling folded)
+  T tw[7];        // thermal: k0 V/64 times the modal weights (element.hpp)
 };
 
 struct GridDims {
@@ -108,6 +109,7 @@ struct Handle {
   ModalConsts<double> mc64{};
   ModalConsts<float> mc32{};
   double kediag[3] = {0, 0, 0};
+  int ncomp = 3;                // values per node: 3 (structural) or 1 (thermal)
 
   // constants / per-density state
   uint8_t* fixmask = nullptr;   // per node

@@ -15,18 +15,20 @@
 
 namespace pyto {
 
-template <typename T>
-struct Modal {
-  // Hadamard sums per component (index 0..2)
-  T cx[3], cy[3], cz[3], cxy[3], cxz[3], cyz[3], cxyz[3];
+// Hadamard sums per component (NC = 3: displacement, 1: temperature).
+template <typename T, int NC>
+struct ModalN {
+  T cx[NC], cy[NC], cz[NC], cxy[NC], cxz[NC], cyz[NC], cxyz[NC];
 };
+template <typename T>
+using Modal = ModalN<T, 3>;
 
 // Local node order (a,b,c): 0:(000) 1:(100) 2:(110) 3:(010) 4:(001) 5:(101)
 // 6:(111) 7:(011), src/hex_mesher.py:121-124.
-template <typename T>
-__host__ __device__ __forceinline__ void modal_forward(const T u[8][3], Modal<T>& m) {
+template <typename T, int NC>
+__host__ __device__ __forceinline__ void modal_forward(const T u[8][NC], ModalN<T, NC>& m) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < NC; ++i) {
     const T v000 = u[0][i], v100 = u[1][i], v110 = u[2][i], v010 = u[3][i];
     const T v001 = u[4][i], v101 = u[5][i], v111 = u[6][i], v011 = u[7][i];
     // x stage
@@ -148,10 +150,10 @@ __host__ __device__ __forceinline__ void modal_stress_cube(const Modal<T>& c, T 
 }
 
 // Transposed butterfly: f[n][i] = sum_m s_m(n) F_m,i (no constant mode).
-template <typename T>
-__host__ __device__ __forceinline__ void modal_backward(const Modal<T>& F, T f[8][3]) {
+template <typename T, int NC>
+__host__ __device__ __forceinline__ void modal_backward(const ModalN<T, NC>& F, T f[8][NC]) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < NC; ++i) {
     // z stage
     const T s0_0 = -F.cz[i], s0_1 = F.cz[i];
     const T sy_0 = F.cy[i] - F.cyz[i], sy_1 = F.cy[i] + F.cyz[i];
@@ -203,16 +205,16 @@ __host__ __device__ __forceinline__ void element_apply_cube(const T u[8][3], T E
 // (1,1), (0,1) of that face).  Adjacent element layers share a face, so an
 // operator marching in z computes each face once: the top face of one layer
 // is the bottom face of the next.
-template <typename T>
+template <typename T, int NC = 3>
 struct Face {
-  T s[3], sy[3], dx[3], dxy[3];
+  T s[NC], sy[NC], dx[NC], dxy[NC];
 };
 
-template <typename T>
-__host__ __device__ __forceinline__ void face_xy(const T c0[3], const T c1[3], const T c2[3],
-                                                 const T c3[3], Face<T>& f) {
+template <typename T, int NC>
+__host__ __device__ __forceinline__ void face_xy(const T c0[NC], const T c1[NC], const T c2[NC],
+                                                 const T c3[NC], Face<T, NC>& f) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < NC; ++i) {
     const T sx0 = c1[i] + c0[i], dx0 = c1[i] - c0[i];
     const T sx1 = c2[i] + c3[i], dx1 = c2[i] - c3[i];
     f.s[i] = sx1 + sx0;
@@ -223,11 +225,12 @@ __host__ __device__ __forceinline__ void face_xy(const T c0[3], const T c1[3], c
 }
 
 // The z stage of modal_forward from the two faces' x/y stages.
-template <typename T>
-__host__ __device__ __forceinline__ void modal_from_faces(const Face<T>& b, const Face<T>& t,
-                                                          Modal<T>& m) {
+template <typename T, int NC>
+__host__ __device__ __forceinline__ void modal_from_faces(const Face<T, NC>& b,
+                                                          const Face<T, NC>& t,
+                                                          ModalN<T, NC>& m) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < NC; ++i) {
     m.cz[i] = t.s[i] - b.s[i];
     m.cy[i] = t.sy[i] + b.sy[i];
     m.cyz[i] = t.sy[i] - b.sy[i];
@@ -257,12 +260,72 @@ __host__ __device__ __forceinline__ void element_apply_faces(const Face<T>& b, c
   modal_backward(F, f);
 }
 
+// Thermal brick (one temperature per node): the conduction energy of each
+// monomial is a single gradient norm, so KE_T is diagonal in the modal basis,
+// F_m = k w_m c_m with w = (gx^2, gy^2, gz^2, (gx^2+gy^2)/3, (gx^2+gz^2)/3,
+// (gy^2+gz^2)/3, (gx^2+gy^2+gz^2)/9) times k0 V/64 (2x2x2 Gauss is exact for
+// these products).  Reproduces hex8_stiffness_matrix_thermal,
+// src/hex_element_stiffness.py:339-385.
+template <typename T>
+__host__ __device__ __forceinline__ void thermal_modal(const ModalN<T, 1>& c, T k,
+                                                       const ModalConsts<T>& mc,
+                                                       ModalN<T, 1>& F) {
+  F.cx[0] = (k * mc.tw[0]) * c.cx[0];
+  F.cy[0] = (k * mc.tw[1]) * c.cy[0];
+  F.cz[0] = (k * mc.tw[2]) * c.cz[0];
+  F.cxy[0] = (k * mc.tw[3]) * c.cxy[0];
+  F.cxz[0] = (k * mc.tw[4]) * c.cxz[0];
+  F.cyz[0] = (k * mc.tw[5]) * c.cyz[0];
+  F.cxyz[0] = (k * mc.tw[6]) * c.cxyz[0];
+}
+
+// f = k * KE_T * T_e for one thermal element (k relative to k0).
+template <typename T>
+__host__ __device__ __forceinline__ void element_apply_thermal(const T u[8][1], T k,
+                                                               const ModalConsts<T>& mc,
+                                                               T f[8][1]) {
+  ModalN<T, 1> c, F;
+  modal_forward(u, c);
+  thermal_modal(c, k, mc, F);
+  modal_backward(F, f);
+}
+
+// element_apply_thermal from the faces' butterfly stages.
+template <typename T>
+__host__ __device__ __forceinline__ void element_apply_thermal_faces(const Face<T, 1>& b,
+                                                                     const Face<T, 1>& t, T k,
+                                                                     const ModalConsts<T>& mc,
+                                                                     T f[8][1]) {
+  ModalN<T, 1> c, F;
+  modal_from_faces(b, t, c);
+  thermal_modal(c, k, mc, F);
+  modal_backward(F, f);
+}
+
+template <typename T>
+inline ModalConsts<T> make_thermal_consts(double k0, double hx, double hy, double hz) {
+  const double gx2 = 4.0 / (hx * hx), gy2 = 4.0 / (hy * hy), gz2 = 4.0 / (hz * hz);
+  const double s = k0 * hx * hy * hz / 64.0;
+  ModalConsts<T> m{};
+  m.gx = T(2.0 / hx);
+  m.gy = T(2.0 / hy);
+  m.gz = T(2.0 / hz);
+  m.tw[0] = T(s * gx2);
+  m.tw[1] = T(s * gy2);
+  m.tw[2] = T(s * gz2);
+  m.tw[3] = T(s * (gx2 + gy2) / 3.0);
+  m.tw[4] = T(s * (gx2 + gz2) / 3.0);
+  m.tw[5] = T(s * (gy2 + gz2) / 3.0);
+  m.tw[6] = T(s * (gx2 + gy2 + gz2) / 9.0);
+  return m;
+}
+
 template <typename T>
 inline ModalConsts<T> make_modal_consts(double E0, double nu, double hx, double hy, double hz) {
   const double lam = E0 * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
   const double mu = E0 / (2.0 * (1.0 + nu));
   const double V = hx * hy * hz;
-  ModalConsts<T> m;
+  ModalConsts<T> m{};
   m.gx = T(2.0 / hx);
   m.gy = T(2.0 / hy);
   m.gz = T(2.0 / hz);

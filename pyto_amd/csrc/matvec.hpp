@@ -47,8 +47,8 @@ __device__ __forceinline__ double dpp_shr1(double v) {
 // 7 in mo when the node is outside the grid (the address is then clamped
 // into the grid).  The masking happens when the plane is stored to LDS, so
 // the loads stay in flight across a whole layer of element work.
-template <typename T>
-__device__ __forceinline__ void issue_node(T a[3], int& mo, int& ml, const T* __restrict__ u,
+template <typename T, int NC>
+__device__ __forceinline__ void issue_node(T a[NC], int& mo, int& ml, const T* __restrict__ u,
                                            const uint8_t* __restrict__ mask, int64_t nxy,
                                            bool inxy, int64_t plane, int kp, int sz,
                                            bool any_fixed) {
@@ -57,10 +57,9 @@ __device__ __forceinline__ void issue_node(T a[3], int& mo, int& ml, const T* __
   const bool inb = inxy && kp >= 0 && kp < sz;
   const int cz = min(max(kp, 0), sz - 1);
   const int64_t n = nxy + plane * cz;
-  const T* p = u + 3 * n;
-  a[0] = p[0];
-  a[1] = p[1];
-  a[2] = p[2];
+  const T* p = u + NC * n;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) a[c] = p[c];
   mo = inb ? 0 : 7;
   ml = 0;
   if (any_fixed) ml = mask[n];
@@ -76,11 +75,16 @@ __device__ __forceinline__ void issue_node(T a[3], int& mo, int& ml, const T* __
 // One workgroup's share of y = K u: logical block lb of the operator's
 // (tile, z-chunk) grid; its u.y partial goes to a.partials[pidx].  Shared by
 // the single-operator kernel and the batched kernel.
-template <typename T, bool CUBE>
+//
+// NC is the number of values per node: 3 (displacement, structural KE) or 1
+// (temperature, thermal KE_T); the tile structure is the same.  MASK_IN =
+// false keeps the fixed entries of u (only y's fixed entries are zeroed):
+// that is K_fc g for lifting non-zero Dirichlet values.
+template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true>
 __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int pidx) {
   if (a.done && *a.done) return;
-  __shared__ T pl[2][kPl][kPl][3];
-  __shared__ T xc[2][kTile][kTile][6];
+  __shared__ T pl[2][kPl][kPl][NC];
+  __shared__ T xc[2][kTile][kTile][2 * NC];
   __shared__ T es[2][kTile][kTile];   // E of the element layer below plane b
   __shared__ uint8_t pm[2][kPl][kPl];  // fixed bits of the staged nodes
   __shared__ double red[4];
@@ -113,7 +117,9 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
   const bool has1 = e1 < kPl * kPl;
   const int lx0 = e0 % kPl, ly0 = e0 / kPl;
   const int lx1 = has1 ? e1 % kPl : 0, ly1 = has1 ? e1 / kPl : 0;
-  T pv0[3], pv1[3] = {T(0), T(0), T(0)};
+  T pv0[NC], pv1[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) pv1[c] = T(0);
   int mo0 = 0, ml0 = 0, mo1 = 7, ml1 = 0;
   // E of element layer kp-1, loaded with node plane kp and staged with it
   const int64_t ecol = min(max(ex, 0), g.nx - 1) + (int64_t)g.nx * min(max(ey, 0), g.ny - 1);
@@ -130,19 +136,20 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
   const int64_t nxy1 = min(max(gx1, 0), g.sx - 1) + (int64_t)g.sx * min(max(gy1, 0), g.sy - 1);
   auto fetch = [&](int kp) {
     const bool fx = plane_fixed(kp);
-    issue_node<T>(pv0, mo0, ml0, a.u, a.mask, nxy0, inxy0, plane, kp, g.sz, fx);
-    if (has1) issue_node<T>(pv1, mo1, ml1, a.u, a.mask, nxy1, inxy1, plane, kp, g.sz, fx);
+    issue_node<T, NC>(pv0, mo0, ml0, a.u, a.mask, nxy0, inxy0, plane, kp, g.sz, fx);
+    if (has1) issue_node<T, NC>(pv1, mo1, ml1, a.u, a.mask, nxy1, inxy1, plane, kp, g.sz, fx);
     const int kl = min(max(kp - 1, 0), g.nz - 1);
     enext = a.Ee[ecol + (int64_t)g.nx * g.ny * kl];
   };
   auto store = [&](int b) {
     const int m0 = mo0 | ml0, m1 = mo1 | ml1;
+    const int i0 = MASK_IN ? m0 : mo0, i1 = MASK_IN ? m1 : mo1;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) pl[b][ly0][lx0][c] = ((m0 >> c) & 1) ? T(0) : pv0[c];
+    for (int c = 0; c < NC; ++c) pl[b][ly0][lx0][c] = ((i0 >> c) & 1) ? T(0) : pv0[c];
     pm[b][ly0][lx0] = (uint8_t)m0;
     if (has1) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) pl[b][ly1][lx1][c] = ((m1 >> c) & 1) ? T(0) : pv1[c];
+      for (int c = 0; c < NC; ++c) pl[b][ly1][lx1][c] = ((i1 >> c) & 1) ? T(0) : pv1[c];
       pm[b][ly1][lx1] = (uint8_t)m1;
     }
     es[b][ty][tx] = enext;
@@ -152,18 +159,19 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
   // the layer below (its top face), and u at this thread's node of that face.
   // FP64 keeps the corner values instead: carrying the face stages costs it
   // 8 VGPRs more and one wave per SIMD.
-  constexpr bool kFaces = sizeof(T) == 4;
-  Face<T> bface;
-  T bot[4][3];
-  T ucur[3];
+  // (one temperature per node: the faces are small in both precisions)
+  constexpr bool kFaces = sizeof(T) == 4 || NC == 1;
+  Face<T, NC> bface;
+  T bot[4][NC];
+  T ucur[NC];
   fetch(kz0 - 1);
   store(0);
   fetch(kz0);   // also E of layer kz0-1
   __syncthreads();
   {
-    T c0[3], c1[3], c2[3], c3[3];
+    T c0[NC], c1[NC], c2[NC], c3[NC];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < NC; ++c) {
       c0[c] = pl[0][ty][tx][c];
       c1[c] = pl[0][ty][tx + 1][c];
       c2[c] = pl[0][ty + 1][tx + 1][c];
@@ -174,14 +182,14 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
       bot[2][c] = c2[c];
       bot[3][c] = c3[c];
     }
-    if (kFaces) face_xy<T>(c0, c1, c2, c3, bface);
+    if (kFaces) face_xy<T, NC>(c0, c1, c2, c3, bface);
   }
   mq = pm[0][ty][tx];
-  T carry[3] = {T(0), T(0), T(0)};   // top-plane sum of the layer before last
-  T xb[3], xt[3], yb[3], yt[3];       // pending sums of the last layer
-  T ubot[3];                          // u at this thread's node of the pending plane
+  T carry[NC];                         // top-plane sum of the layer before last
+  T xb[NC], xt[NC], yb[NC], yt[NC];    // pending sums of the last layer
+  T ubot[NC];                          // u at this thread's node of the pending plane
 #pragma unroll
-  for (int c = 0; c < 3; ++c) xb[c] = xt[c] = yb[c] = yt[c] = ubot[c] = T(0);
+  for (int c = 0; c < NC; ++c) carry[c] = xb[c] = xt[c] = yb[c] = yt[c] = ubot[c] = T(0);
   double dot = 0.0;
   // this thread's own node in its plane (owners only write it)
   const int64_t own_xy = (int64_t)ex + (int64_t)g.sx * ey;
@@ -197,40 +205,40 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
     // and no load result is carried through a conditional path
     fetch(min(k + 2, kz1));
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < NC; ++c) {
       xc[xb_i][ty][tx][c] = yb[c];
-      xc[xb_i][ty][tx][3 + c] = yt[c];
+      xc[xb_i][ty][tx][NC + c] = yt[c];
     }
     __syncthreads();
     // finish node plane k-1 (bottom of layer k-1 + top of layer k-2)
     if (owner) {
       // all six exchange values are read unconditionally (branch-free)
-      T sb[3], st[3];
+      T sb[NC], st[NC];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
+      for (int c = 0; c < NC; ++c) {
         sb[c] = carry[c] + xb[c] + xc[xb_i][ty - 1][tx][c];
-        st[c] = xt[c] + xc[xb_i][ty - 1][tx][3 + c];
+        st[c] = xt[c] + xc[xb_i][ty - 1][tx][NC + c];
       }
       if (k - 1 >= kz0) {
         const int kp = k - 1;
         const int64_t nw = own_xy + plane * kp;
         const uint32_t m = (mq >> 8) & 0xffu;   // plane k-1 (byte 0: plane k)
-        T* yp = a.y + 3 * nw;
+        T* yp = a.y + NC * nw;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < NC; ++c) {
           const T v = ((m >> c) & 1u) ? T(0) : sb[c];
           yp[c] = v;
           dot += (double)v * (double)ubot[c];
         }
       }
 #pragma unroll
-      for (int c = 0; c < 3; ++c) carry[c] = st[c];
+      for (int c = 0; c < NC; ++c) carry[c] = st[c];
     }
     if (k >= kz1) return false;
-    Face<T> tface;
-    T top[4][3];
+    Face<T, NC> tface;
+    T top[4][NC];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < NC; ++c) {
       ubot[c] = ucur[c];
       top[0][c] = pl[b][ty][tx][c];
       top[1][c] = pl[b][ty][tx + 1][c];
@@ -238,12 +246,14 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
       top[3][c] = pl[b][ty + 1][tx][c];
       ucur[c] = top[0][c];
     }
-    if (kFaces) face_xy<T>(top[0], top[1], top[2], top[3], tface);
+    if (kFaces) face_xy<T, NC>(top[0], top[1], top[2], top[3], tface);
     mq = (mq << 8) | (uint32_t)pm[b][ty][tx];
-    T f[8][3];
+    T f[8][NC];
     if (col_valid && k >= 0 && k < g.nz) {
-      const T E = es[b][ty][tx];   // E of layer k, staged with plane k+1
-      if (kFaces) {
+      const T E = es[b][ty][tx];   // E (or k) of layer k, staged with plane k+1
+      if constexpr (NC == 1) {
+        element_apply_thermal_faces<T>(bface, tface, E, a.mc, f);
+      } else if constexpr (kFaces) {
         element_apply_faces<T, CUBE>(bface, tface, E, a.mc, f);
       } else {
         T ue[8][3];
@@ -263,11 +273,11 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
 #pragma unroll
       for (int j = 0; j < 8; ++j)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) f[j][c] = T(0);
+        for (int c = 0; c < NC; ++c) f[j][c] = T(0);
     }
     // x-neighbour sums (DPP) for the nodes (tx,ty) and (tx,ty+1) of both planes
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < NC; ++c) {
       xb[c] = f[0][c] + dpp_shr1(f[1][c]);
       yb[c] = f[3][c] + dpp_shr1(f[2][c]);
       xt[c] = f[4][c] + dpp_shr1(f[5][c]);
@@ -279,7 +289,7 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) bot[j][c] = top[j][c];
+        for (int c = 0; c < NC; ++c) bot[j][c] = top[j][c];
     }
     return true;
   };

@@ -62,9 +62,9 @@ __global__ void k_simp(const double* __restrict__ x, double* __restrict__ E64,
 
 // Jacobi: d(node,c) = KEdiag_c * sum of E_e over the node's elements;
 // |d| < 1e-12 -> 1 (src/linear_solvers.py:84-85); fixed DOFs get 0 so that
-// z = M r stays zero there.
+// z = M r stays zero there.  nc values per node (3, or 1 for temperature).
 __global__ void k_inv_diag(const double* __restrict__ E64, const uint8_t* __restrict__ mask,
-                           GridDims g, double kd0, double kd1, double kd2,
+                           GridDims g, int nc, double kd0, double kd1, double kd2,
                            double* __restrict__ invd64, float* __restrict__ invd32,
                            double* __restrict__ diag_out) {
   for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < g.nnodes;
@@ -84,24 +84,23 @@ __global__ void k_inv_diag(const double* __restrict__ E64, const uint8_t* __rest
         }
     const uint8_t m = mask[n];
     const double kd[3] = {kd0, kd1, kd2};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < nc; ++c) {
       const bool fixed = (m >> c) & 1;
       double d = kd[c] * s;
-      if (diag_out) diag_out[3 * n + c] = fixed ? 0.0 : d;
+      if (diag_out) diag_out[nc * n + c] = fixed ? 0.0 : d;
       if (invd64) {
         if (fabs(d) < 1e-12) d = 1.0;
         const double inv = fixed ? 0.0 : 1.0 / d;
-        invd64[3 * n + c] = inv;
-        invd32[3 * n + c] = (float)inv;
+        invd64[nc * n + c] = inv;
+        invd32[nc * n + c] = (float)inv;
       }
     }
   }
 }
 
-template <typename T, bool CUBE>
+template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true>
 __global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
-  matvec_tile<T, CUBE>(a, xcd_logical_block(blockIdx.x, gridDim.x), blockIdx.x);
+  matvec_tile<T, CUBE, NC, MASK_IN>(a, xcd_logical_block(blockIdx.x, gridDim.x), blockIdx.x);
 }
 
 // tileflags[tile * sz + kp] = 1 when node plane kp of the tile's 17x17
@@ -128,15 +127,16 @@ static hipError_t launch_tile_flags(Handle* h, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Resident workgroups of k_matvec<T, CUBE> on the whole device.
-template <typename T, bool CUBE>
+// Resident workgroups of k_matvec<T, CUBE, NC> on the whole device.
+template <typename T, bool CUBE, int NC>
 static int matvec_capacity() {
   static int cap = -1;
   if (cap < 0) {
     int nb = 0, dev = 0;
     hipGetDevice(&dev);
     hipDeviceProp_t prop;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_matvec<T, CUBE>, 256, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_matvec<T, CUBE, NC>, 256, 0) !=
+            hipSuccess ||
         hipGetDeviceProperties(&prop, dev) != hipSuccess || nb <= 0)
       return 1024;
     cap = nb * prop.multiProcessorCount;
@@ -154,7 +154,9 @@ static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& ti
   const int tiles_y = (g.sy + kOwn - 1) / kOwn;
   const int64_t tiles = (int64_t)tiles_x * tiles_y;
   const bool cube = g.hx == g.hy && g.hy == g.hz;
-  const int cap = cube ? matvec_capacity<T, true>() : matvec_capacity<T, false>();
+  const int cap = h->ncomp == 1 ? matvec_capacity<T, true, 1>()
+                  : cube        ? matvec_capacity<T, true, 3>()
+                                : matvec_capacity<T, false, 3>();
   int best = 16;
   double best_cost = 1e300;
   for (int zc = 4; zc <= 62; ++zc) {   // planes kz0-1 .. kz1 fit one 64-bit mask
@@ -201,10 +203,12 @@ int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalCon
   a.zero_vec = zero_vec;
   a.zero_n = zero_n;
   const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
-  if (cube)
-    hipLaunchKernelGGL((k_matvec<T, true>), grid, dim3(256), 0, s, a);
+  if (h->ncomp == 1)   // thermal: the modal weights already carry h
+    hipLaunchKernelGGL((k_matvec<T, true, 1>), grid, dim3(256), 0, s, a);
+  else if (cube)
+    hipLaunchKernelGGL((k_matvec<T, true, 3>), grid, dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((k_matvec<T, false>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_matvec<T, false, 3>), grid, dim3(256), 0, s, a);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -218,7 +222,7 @@ template int launch_matvec<float>(const Handle*, const float*, float*, const flo
 
 int ensure_solver_workspace(Handle* h) {
   if (h->r64) return PYTO_OK;
-  const size_t nd = (size_t)(3 * h->g.nnodes);
+  const size_t nd = (size_t)(h->ncomp * h->g.nnodes);
   PYTO_TRY(device_alloc(h, (void**)&h->r64, nd * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->p64, nd * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->q64, nd * 8));
@@ -245,6 +249,64 @@ static void host_ke(const Handle* h, double* ke) {
   }
 }
 
+// The same for the 8x8 thermal KE_T.
+static void host_ke_thermal(const ModalConsts<double>& mc, double* ke) {
+  for (int col = 0; col < 8; ++col) {
+    double u[8][1] = {};
+    u[col][0] = 1.0;
+    double f[8][1];
+    element_apply_thermal<double>(u, 1.0, mc, f);
+    for (int r = 0; r < 8; ++r) ke[r * 8 + col] = f[r][0];
+  }
+}
+
+// Handle with grid, fixed mask and per-density buffers for nc values per
+// node; the physics constants (mc64/mc32, kediag, SIMP law) are set by the
+// caller before this runs.
+static int create_common(pyto_handle* h, int nelx, int nely, int nelz, double hx, double hy,
+                         double hz, int nc, const uint8_t* fixed_mask, hipStream_t s) {
+  const int64_t nn = (int64_t)(nelx + 1) * (nely + 1) * (nelz + 1);
+  hipGetDevice(&h->device);
+  h->ncomp = nc;
+  h->g.nx = nelx; h->g.ny = nely; h->g.nz = nelz;
+  h->g.sx = nelx + 1; h->g.sy = nely + 1; h->g.sz = nelz + 1;
+  h->g.nnodes = nn;
+  h->g.nelems = (int64_t)nelx * nely * nelz;
+  h->g.hx = hx; h->g.hy = hy; h->g.hz = hz;
+  const int64_t ne = h->g.nelems;
+  int rc = PYTO_OK;
+  const int64_t ntiles = (int64_t)((h->g.sx + kOwn - 1) / kOwn) * ((h->g.sy + kOwn - 1) / kOwn);
+  if ((rc = device_alloc(h, (void**)&h->fixmask, nn)) ||
+      (rc = device_alloc(h, (void**)&h->tileflags, ntiles * h->g.sz)) ||
+      (rc = device_alloc(h, (void**)&h->Ee64, ne * 8)) ||
+      (rc = device_alloc(h, (void**)&h->Ee32, ne * 4)) ||
+      (rc = device_alloc(h, (void**)&h->x64, ne * 8)) ||
+      (rc = device_alloc(h, (void**)&h->invd64, nc * nn * 8)) ||
+      (rc = device_alloc(h, (void**)&h->invd32, nc * nn * 4)) ||
+      (rc = device_alloc(h, (void**)&h->partials, 2 * kMaxPartials * 8)) ||
+      (rc = device_alloc(h, (void**)&h->counters, kNumCounters * sizeof(unsigned))) ||
+      (rc = device_alloc(h, (void**)&h->scal, kNumScalars * 8)) ||
+      (rc = device_alloc(h, (void**)&h->flags, kNumFlags * sizeof(int))))
+    return rc;
+  if (hipHostMalloc((void**)&h->host_scal, kNumScalars * 8, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&h->host_flags, kNumFlags * sizeof(int), hipHostMallocDefault) !=
+          hipSuccess) {
+    set_error("hipHostMalloc failed");
+    return PYTO_ENOMEM;
+  }
+  if (hipMemcpyAsync(h->fixmask, fixed_mask, nn, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipMemsetAsync(h->tileflags, 0, ntiles * h->g.sz, s) != hipSuccess ||
+      launch_tile_flags(h, s) != hipSuccess ||
+      hipMemsetAsync(h->counters, 0, kNumCounters * sizeof(unsigned), s) != hipSuccess ||
+      hipMemsetAsync(h->scal, 0, kNumScalars * 8, s) != hipSuccess ||
+      hipMemsetAsync(h->flags, 0, kNumFlags * sizeof(int), s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("initialising the handle failed: %s", hipGetErrorString(hipGetLastError()));
+    return PYTO_EHIP;
+  }
+  return PYTO_OK;
+}
+
 }  // namespace pyto
 
 using namespace pyto;
@@ -264,6 +326,13 @@ int pyto_ke_hex8(double E, double nu, double hx, double hy, double hz, double* k
   return PYTO_OK;
 }
 
+int pyto_ke_hex8_thermal(double k0, double hx, double hy, double hz, double* ke_host) {
+  PYTO_REQUIRE(ke_host, PYTO_EINVAL, "ke_host is null");
+  PYTO_REQUIRE(hx > 0 && hy > 0 && hz > 0, PYTO_EINVAL, "element size must be positive");
+  host_ke_thermal(make_thermal_consts<double>(k0, hx, hy, hz), ke_host);
+  return PYTO_OK;
+}
+
 int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, double hy,
                 double hz, double E0, double nu, double penal, double eps_void,
                 const uint8_t* fixed_mask, void* stream) {
@@ -276,50 +345,43 @@ int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, doub
   const int64_t nn = (int64_t)(nelx + 1) * (nely + 1) * (nelz + 1);
   PYTO_REQUIRE(3 * nn < (int64_t)INT32_MAX * 4, PYTO_EINVAL, "grid too large");
   pyto_handle* h = new pyto_handle();
-  hipGetDevice(&h->device);
-  h->g.nx = nelx; h->g.ny = nely; h->g.nz = nelz;
-  h->g.sx = nelx + 1; h->g.sy = nely + 1; h->g.sz = nelz + 1;
-  h->g.nnodes = nn;
-  h->g.nelems = (int64_t)nelx * nely * nelz;
-  h->g.hx = hx; h->g.hy = hy; h->g.hz = hz;
   h->E0 = E0; h->nu = nu; h->penal = penal; h->eps_void = eps_void;
   h->mc64 = make_modal_consts<double>(E0, nu, hx, hy, hz);
   h->mc32 = make_modal_consts<float>(E0, nu, hx, hy, hz);
   double ke[576];
   host_ke(h, ke);
   for (int c = 0; c < 3; ++c) h->kediag[c] = ke[c * 24 + c];
-  const int64_t ne = h->g.nelems;
-  int rc = PYTO_OK;
-  auto fail = [&](int code) { pyto_destroy(h); return code; };
-  const int64_t ntiles = (int64_t)((h->g.sx + kOwn - 1) / kOwn) * ((h->g.sy + kOwn - 1) / kOwn);
-  if ((rc = device_alloc(h, (void**)&h->fixmask, nn)) ||
-      (rc = device_alloc(h, (void**)&h->tileflags, ntiles * h->g.sz)) ||
-      (rc = device_alloc(h, (void**)&h->Ee64, ne * 8)) ||
-      (rc = device_alloc(h, (void**)&h->Ee32, ne * 4)) ||
-      (rc = device_alloc(h, (void**)&h->x64, ne * 8)) ||
-      (rc = device_alloc(h, (void**)&h->invd64, 3 * nn * 8)) ||
-      (rc = device_alloc(h, (void**)&h->invd32, 3 * nn * 4)) ||
-      (rc = device_alloc(h, (void**)&h->partials, 2 * kMaxPartials * 8)) ||
-      (rc = device_alloc(h, (void**)&h->counters, kNumCounters * sizeof(unsigned))) ||
-      (rc = device_alloc(h, (void**)&h->scal, kNumScalars * 8)) ||
-      (rc = device_alloc(h, (void**)&h->flags, kNumFlags * sizeof(int))))
-    return fail(rc);
-  if (hipHostMalloc((void**)&h->host_scal, kNumScalars * 8, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&h->host_flags, kNumFlags * sizeof(int), hipHostMallocDefault) !=
-          hipSuccess) {
-    set_error("hipHostMalloc failed");
-    return fail(PYTO_ENOMEM);
+  const int rc = create_common(h, nelx, nely, nelz, hx, hy, hz, 3, fixed_mask, as_stream(stream));
+  if (rc != PYTO_OK) {
+    pyto_destroy(h);
+    return rc;
   }
-  hipStream_t s = as_stream(stream);
-  if (hipMemcpyAsync(h->fixmask, fixed_mask, nn, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-      hipMemsetAsync(h->tileflags, 0, ntiles * h->g.sz, s) != hipSuccess ||
-      launch_tile_flags(h, s) != hipSuccess ||
-      hipMemsetAsync(h->counters, 0, kNumCounters * sizeof(unsigned), s) != hipSuccess ||
-      hipMemsetAsync(h->scal, 0, kNumScalars * 8, s) != hipSuccess ||
-      hipMemsetAsync(h->flags, 0, kNumFlags * sizeof(int), s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess) {
-    set_error("initialising the handle failed: %s", hipGetErrorString(hipGetLastError()));
-    return fail(PYTO_EHIP);
+  *out = h;
+  return PYTO_OK;
+}
+
+int pyto_create_thermal(pyto_handle** out, int nelx, int nely, int nelz, double hx, double hy,
+                        double hz, double k0, double penal, double k_void,
+                        const uint8_t* fixed_mask, void* stream) {
+  PYTO_REQUIRE(out, PYTO_EINVAL, "out is null");
+  *out = nullptr;
+  PYTO_REQUIRE(nelx >= 1 && nely >= 1 && nelz >= 1, PYTO_EINVAL, "grid must be >= 1x1x1");
+  PYTO_REQUIRE(hx > 0 && hy > 0 && hz > 0, PYTO_EINVAL, "element size must be positive");
+  PYTO_REQUIRE(k0 > 0, PYTO_EINVAL, "conductivity must be positive");
+  PYTO_REQUIRE(fixed_mask, PYTO_EINVAL, "fixed_mask is null");
+  const int64_t nn = (int64_t)(nelx + 1) * (nely + 1) * (nelz + 1);
+  PYTO_REQUIRE(nn < (int64_t)INT32_MAX * 4, PYTO_EINVAL, "grid too large");
+  pyto_handle* h = new pyto_handle();
+  h->E0 = k0; h->nu = 0.0; h->penal = penal; h->eps_void = k_void;
+  h->mc64 = make_thermal_consts<double>(k0, hx, hy, hz);
+  h->mc32 = make_thermal_consts<float>(k0, hx, hy, hz);
+  double ke[64];
+  host_ke_thermal(h->mc64, ke);
+  h->kediag[0] = ke[0];
+  const int rc = create_common(h, nelx, nely, nelz, hx, hy, hz, 1, fixed_mask, as_stream(stream));
+  if (rc != PYTO_OK) {
+    pyto_destroy(h);
+    return rc;
   }
   *out = h;
   return PYTO_OK;
@@ -352,7 +414,7 @@ int pyto_info(pyto_handle* h, int64_t* nnodes, int64_t* nelems, int64_t* ndof,
   PYTO_REQUIRE(h, PYTO_EINVAL, "null handle");
   if (nnodes) *nnodes = h->g.nnodes;
   if (nelems) *nelems = h->g.nelems;
-  if (ndof) *ndof = 3 * h->g.nnodes;
+  if (ndof) *ndof = h->ncomp * h->g.nnodes;
   if (workspace_bytes) *workspace_bytes = h->workspace_bytes;
   return PYTO_OK;
 }
@@ -367,7 +429,7 @@ int pyto_set_density(pyto_handle* h, const double* x, void* stream) {
   PYTO_CHECK_HIP(hipGetLastError());
   blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 4096);
   hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(256), 0, s, h->Ee64, h->fixmask, h->g,
-                     h->kediag[0], h->kediag[1], h->kediag[2], h->invd64, h->invd32,
+                     h->ncomp, h->kediag[0], h->kediag[1], h->kediag[2], h->invd64, h->invd32,
                      (double*)nullptr);
   PYTO_CHECK_HIP(hipGetLastError());
   h->density_set = true;
@@ -380,7 +442,7 @@ int pyto_diagonal_f64(pyto_handle* h, double* d, void* stream) {
   PYTO_REQUIRE(h->density_set, PYTO_ESTATE, "pyto_set_density must be called first");
   int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 4096);
   hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(256), 0, as_stream(stream), h->Ee64,
-                     h->fixmask, h->g, h->kediag[0], h->kediag[1], h->kediag[2],
+                     h->fixmask, h->g, h->ncomp, h->kediag[0], h->kediag[1], h->kediag[2],
                      (double*)nullptr, (float*)nullptr, d);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
@@ -392,6 +454,37 @@ int pyto_matvec_f64(pyto_handle* h, const double* u, double* y, void* stream) {
   PYTO_REQUIRE(h->density_set, PYTO_ESTATE, "pyto_set_density must be called first");
   return launch_matvec<double>(h, u, y, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0,
                                as_stream(stream));
+}
+
+// y = K_fc g (g's fixed entries kept, y's fixed entries zeroed), FP64.
+int pyto_matvec_lift_f64(pyto_handle* h, const double* g, double* y, void* stream) {
+  PYTO_REQUIRE(h && g && y, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(g != y, PYTO_EINVAL, "g and y must not alias");
+  PYTO_REQUIRE(h->density_set, PYTO_ESTATE, "pyto_set_density must be called first");
+  MatvecArgs<double> a;
+  a.u = g;
+  a.y = y;
+  a.Ee = h->Ee64;
+  a.mask = h->fixmask;
+  a.tileflags = h->tileflags;
+  a.g = h->g;
+  a.mc = h->mc64;
+  dim3 grid;
+  matvec_launch_dims<double>(h, grid, a.zchunk, a.tiles_x, a.ntiles);
+  a.partials = nullptr;
+  a.done = nullptr;
+  a.zero_vec = nullptr;
+  a.zero_n = 0;
+  hipStream_t s = as_stream(stream);
+  const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
+  if (h->ncomp == 1)
+    hipLaunchKernelGGL((k_matvec<double, true, 1, false>), grid, dim3(256), 0, s, a);
+  else if (cube)
+    hipLaunchKernelGGL((k_matvec<double, true, 3, false>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_matvec<double, false, 3, false>), grid, dim3(256), 0, s, a);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
 }
 
 int pyto_matvec_f32(pyto_handle* h, const float* u, float* y, void* stream) {

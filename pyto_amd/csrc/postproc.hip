@@ -19,8 +19,9 @@
 
 namespace pyto {
 
+template <int NC = 3>
 __device__ __forceinline__ void gather_elem(const double* __restrict__ u, int64_t e,
-                                            const GridDims& g, double ue[8][3]) {
+                                            const GridDims& g, double ue[8][NC]) {
   int ix, iy, iz;
   elem_coords(e, g, ix, iy, iz);
 #pragma unroll
@@ -28,10 +29,23 @@ __device__ __forceinline__ void gather_elem(const double* __restrict__ u, int64_
     const int64_t n = (ix + corner_a(j)) + (int64_t)g.sx * ((iy + corner_b(j)) +
                                                             (int64_t)g.sy * (iz + corner_c(j)));
 #pragma unroll
-    for (int c = 0; c < 3; ++c) ue[j][c] = u[3 * n + c];
+    for (int c = 0; c < NC; ++c) ue[j][c] = u[NC * n + c];
   }
 }
 
+// f = KE u_e (NC = 3) or KE_T T_e (NC = 1), unit modulus
+template <int NC>
+__device__ __forceinline__ void apply_unit(const double ue[8][NC], const ModalConsts<double>& mc,
+                                           double f[8][NC]) {
+  if constexpr (NC == 3)
+    element_apply<double>(ue, 1.0, mc, f);
+  else
+    element_apply_thermal<double>(ue, 1.0, mc, f);
+}
+
+// NC = 1: the thermal compliance T^T K_T T with the conductivity law
+// (src/topopt_obj_cons_sensitivities.py:160-176, FEA_MODE.THERMAL).
+template <int NC>
 __global__ __launch_bounds__(256) void k_compliance(const double* __restrict__ u,
                                                     const double* __restrict__ x, GridDims g,
                                                     ModalConsts<double> mc, double penal,
@@ -44,14 +58,14 @@ __global__ __launch_bounds__(256) void k_compliance(const double* __restrict__ u
   double acc = 0.0;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < g.nelems;
        e += (int64_t)gridDim.x * blockDim.x) {
-    double ue[8][3], f[8][3];
-    gather_elem(u, e, g, ue);
-    element_apply<double>(ue, 1.0, mc, f);
+    double ue[8][NC], f[8][NC];
+    gather_elem<NC>(u, e, g, ue);
+    apply_unit<NC>(ue, mc, f);
     double ce = 0.0;
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) ce += ue[j][c] * f[j][c];
+      for (int c = 0; c < NC; ++c) ce += ue[j][c] * f[j][c];
     const double xe = x[e];
     const double Ee = eps + pow(xe, penal) * (1.0 - eps);
     acc += Ee * ce;
@@ -63,22 +77,40 @@ __global__ __launch_bounds__(256) void k_compliance(const double* __restrict__ u
                      [=](double tot) { *C = tot; });
 }
 
+template <int NC>
 __global__ __launch_bounds__(256) void k_bilinear(const double* __restrict__ a,
                                                   const double* __restrict__ b, GridDims g,
                                                   ModalConsts<double> mc,
                                                   double* __restrict__ out) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < g.nelems;
        e += (int64_t)gridDim.x * blockDim.x) {
-    double ae[8][3], be[8][3], f[8][3];
-    gather_elem(a, e, g, ae);
-    gather_elem(b, e, g, be);
-    element_apply<double>(be, 1.0, mc, f);
+    double ae[8][NC], be[8][NC], f[8][NC];
+    gather_elem<NC>(a, e, g, ae);
+    gather_elem<NC>(b, e, g, be);
+    apply_unit<NC>(be, mc, f);
     double s = 0.0;
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) s += ae[j][c] * f[j][c];
+      for (int c = 0; c < NC; ++c) s += ae[j][c] * f[j][c];
     out[e] = s;
+  }
+}
+
+// Centre gradient of the temperature in reference coordinates, (1/8) sum_j
+// s_k(j) T_j for k = x, y, z: the three mode-1 Hadamard sums over 8
+// (src/hex_thermal_fea.py:250-270, gradN @ sol[edof].T); out is 3 x N_el.
+__global__ __launch_bounds__(256) void k_thermal_grad(const double* __restrict__ T, GridDims g,
+                                                      double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < g.nelems;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double te[8][1];
+    gather_elem<1>(T, e, g, te);
+    ModalN<double, 1> c;
+    modal_forward(te, c);
+    out[e] = 0.125 * c.cx[0];
+    out[g.nelems + e] = 0.125 * c.cy[0];
+    out[2 * g.nelems + e] = 0.125 * c.cz[0];
   }
 }
 
@@ -310,9 +342,24 @@ int pyto_compliance(pyto_handle* h, const double* u, const double* x, double* ce
                     double* C, void* stream) {
   PYTO_REQUIRE(h && u && x && C, PYTO_EINVAL, "null argument");
   const int blocks = (int)std::min<int64_t>((h->g.nelems + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_compliance, dim3(blocks), dim3(256), 0, as_stream(stream), u, x, h->g,
-                     h->mc64, h->penal, h->eps_void, ce, grad, C, h->partials,
-                     h->counters + C_ELEM);
+  if (h->ncomp == 1)
+    hipLaunchKernelGGL(k_compliance<1>, dim3(blocks), dim3(256), 0, as_stream(stream), u, x,
+                       h->g, h->mc64, h->penal, h->eps_void, ce, grad, C, h->partials,
+                       h->counters + C_ELEM);
+  else
+    hipLaunchKernelGGL(k_compliance<3>, dim3(blocks), dim3(256), 0, as_stream(stream), u, x,
+                       h->g, h->mc64, h->penal, h->eps_void, ce, grad, C, h->partials,
+                       h->counters + C_ELEM);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+int pyto_thermal_gradient(pyto_handle* h, const double* T, double* grad, void* stream) {
+  PYTO_REQUIRE(h && T && grad, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(h->ncomp == 1, PYTO_EINVAL, "the temperature gradient needs a thermal handle");
+  const int blocks = (int)std::min<int64_t>((h->g.nelems + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_thermal_grad, dim3(blocks), dim3(256), 0, as_stream(stream), T, h->g,
+                     grad);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -321,8 +368,12 @@ int pyto_element_bilinear(pyto_handle* h, const double* a, const double* b, doub
                           void* stream) {
   PYTO_REQUIRE(h && a && b && be, PYTO_EINVAL, "null argument");
   const int blocks = (int)std::min<int64_t>((h->g.nelems + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_bilinear, dim3(blocks), dim3(256), 0, as_stream(stream), a, b, h->g,
-                     h->mc64, be);
+  if (h->ncomp == 1)
+    hipLaunchKernelGGL(k_bilinear<1>, dim3(blocks), dim3(256), 0, as_stream(stream), a, b, h->g,
+                       h->mc64, be);
+  else
+    hipLaunchKernelGGL(k_bilinear<3>, dim3(blocks), dim3(256), 0, as_stream(stream), a, b, h->g,
+                       h->mc64, be);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -337,6 +388,7 @@ int pyto_stress(pyto_handle* h, const double* u, const double* x, double q, doub
                 double* strain, double* stress, double* vm, double* energy, double* sum_vm_p,
                 void* stream) {
   PYTO_REQUIRE(h && u && x && vm && sum_vm_p, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(h->ncomp == 3, PYTO_EINVAL, "stress needs a structural handle");
   const int blocks = (int)std::min<int64_t>((h->g.nelems + 255) / 256, 2048);
   hipLaunchKernelGGL(k_stress, dim3(blocks), dim3(256), 0, as_stream(stream), u, x, h->g,
                      h->mc64, make_D(h), q, pexp, strain, stress, vm, energy, sum_vm_p, h->partials,
@@ -349,6 +401,7 @@ int pyto_pnorm_adjoint_rhs(pyto_handle* h, const double* u, const double* x, con
                            const double* sum_vm_p, double q, double pexp, double* g,
                            double* T1, void* stream) {
   PYTO_REQUIRE(h && u && x && sum_vm_p && g && T1, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(h->ncomp == 3, PYTO_EINVAL, "p-norm stress needs a structural handle");
   (void)vm;
   if (!h->elem_buf) PYTO_TRY(device_alloc(h, (void**)&h->elem_buf, (size_t)h->g.nelems * 6 * 8));
   hipStream_t s = as_stream(stream);

@@ -375,7 +375,7 @@ template <typename T>
 static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgResult& res,
                 hipStream_t s, bool resume = false) {
   Vecs<T> v = get_vecs<T>(h);
-  const int64_t n = 3 * h->g.nnodes;
+  const int64_t n = h->ncomp * h->g.nnodes;
   const bool defl = h->ngroups > 0;
   double* ct = h->ct;
   PYTO_TRY(ensure_streams(h));
@@ -498,7 +498,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
 // ||f - K u|| / ||f|| over free DOFs (invd64 != 0 marks them), FP64.
 static int true_residual(Handle* h, const double* f, const double* u, double* out,
                          hipStream_t s) {
-  const int64_t n = 3 * h->g.nnodes;
+  const int64_t n = h->ncomp * h->g.nnodes;
   PYTO_TRY(launch_matvec<double>(h, u, h->q64, h->Ee64, h->mc64, nullptr, nullptr, nullptr, 0, s));
   hipLaunchKernelGGL(k_norm2_diff, dim3(kVecBlocks), dim3(256), 0, s, f, h->q64, h->invd64, n,
                      h->partials);
@@ -575,7 +575,7 @@ int pyto_solve_dpcg_mixed(pyto_handle* h, const double* f, double* u, double rto
                PYTO_EINVAL, "bad tolerances or iteration limits");
   PYTO_TRY(check_ready(h));
   hipStream_t s = as_stream(stream);
-  const int64_t n = 3 * h->g.nnodes;
+  const int64_t n = h->ncomp * h->g.nnodes;
   const bool defl = h->ngroups > 0;
   // FP64 deflated start and outer residual
   if (defl) {

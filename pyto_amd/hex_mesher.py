@@ -92,6 +92,14 @@ class HexMesher:
             self._cache["edof"] = np.repeat(3 * e, 3, axis=1) + np.tile(np.arange(3), 8)
         return self._cache["edof"]
 
+    def createEdofMatThermal(self):
+        """src/hex_mesher.py:1120-1135: one DOF per node, the element's nodes."""
+        self.dofs_per_node = 1
+
+    @property
+    def edofMatThermal(self):
+        return self.elemArray.astype(np.int64)
+
     def getNodesOnBoundingBoxPlane(self, axis, minLimit):
         """src/hex_mesher.py:732-760."""
         if axis not in (0, 1, 2):

@@ -1,7 +1,8 @@
 """Linear solver dispatch (src/linear_solvers.py:126-202).
 
 ``solve(A0, b0, solver, bc, **kwargs)`` keeps the reference signature.  A0 is
-the ``GridOperator`` that ``HexStructuralFEA`` exposes as ``stiff_mtrx``.
+the ``GridOperator`` that ``HexStructuralFEA`` exposes as ``stiff_mtrx`` (or
+the ``ThermalOperator`` of ``HexThermalFEA``).
 DPCG runs the deflated PCG on the GPU (kwargs: dsolver, rtol, maxIters and
 ``precision`` = "fp64" (default, the reference algorithm) or "mixed"); PCG
 runs Jacobi PCG (no deflation).  The direct and external solvers of the
@@ -43,6 +44,12 @@ def solve(A0, b0, solver, bc, **kwargs):
     maxIters = kwargs.get('maxIters', 2000)
     precision = kwargs.get('precision', 'fp64')
     b = _to_device(A0, b0)
+    # non-zero Dirichlet values: b' = b - A[:, fixed] g, u[fixed] = g
+    # (src/bound_cond.py:57-66, src/linear_solvers.py:199-201)
+    g = bound_cond.dirichlet_vector(bc, A0.num_dofs)
+    if g is not None:
+        g = torch.from_numpy(g).to(A0.device)
+        b = b - A0.lift(g)
     if solver == Solvers.DPCG:
         dsolver = kwargs['dsolver']
         u = dsolver.deflatedPCG(A0, b, rtol=rtol, maxIters=maxIters, precision=precision)
@@ -55,6 +62,8 @@ def solve(A0, b0, solver, bc, **kwargs):
         raise NotImplementedError("%s is not part of the GPU path; use Solvers.DPCG" % solver.name)
     else:
         raise ValueError('Unknown solver type')
+    if g is not None:
+        u = u + g
     if isinstance(b0, torch.Tensor):
         return u
     return u.cpu().numpy()

@@ -3,8 +3,8 @@ of a voxel grid and everything that runs on it (solves, element kernels,
 cone filter).
 
 All vectors are torch CUDA tensors: DOF vectors are float64 of length
-3*num_nodes in the reference's AoS order, element vectors float64 of length
-num_elems.  The reference-facing modules (hex_structural_fea, linear_solvers,
+3*num_nodes in the reference's AoS order (num_nodes for ``ThermalOperator``),
+element vectors float64 of length num_elems.  The reference-facing modules (hex_structural_fea, linear_solvers,
 deflation, topopt_*) are thin layers over this class.
 """
 import ctypes
@@ -26,6 +26,16 @@ def fixed_mask_from_dofs(num_nodes, fixed_dofs):
         raise ValueError("fixed_dofs out of range")
     m = np.zeros(num_nodes, dtype=np.uint8)
     np.bitwise_or.at(m, fixed // 3, (1 << (fixed % 3)).astype(np.uint8))
+    return m
+
+
+def fixed_mask_from_nodes(num_nodes, fixed_nodes):
+    """uint8 per node, bit 0 set when the node's temperature is fixed."""
+    fixed = np.asarray(fixed_nodes, dtype=np.int64).ravel()
+    if fixed.size and (fixed.min() < 0 or fixed.max() >= num_nodes):
+        raise ValueError("fixed_dofs out of range")
+    m = np.zeros(num_nodes, dtype=np.uint8)
+    m[fixed] = 1
     return m
 
 
@@ -132,6 +142,16 @@ class GridOperator:
         return out
 
     __matmul__ = matvec
+
+    def lift(self, g, out=None):
+        """K_fc g: the operator with g's fixed entries kept and the output's
+        fixed entries zero (FP64); f - lift(g) eliminates the Dirichlet
+        values g (src/bound_cond.py:57-66)."""
+        g = self.as_dof(g)
+        out = torch.empty_like(g) if out is None else out
+        check(_lib.lib().pyto_matvec_lift_f64(self.handle, ptr(g, F64, self.num_dofs, "g"),
+                                              ptr(out, F64, self.num_dofs, "out"), stream_ptr()))
+        return out
 
     def diagonal(self):
         d = torch.empty(self.num_dofs, dtype=F64, device=self.device)
@@ -286,3 +306,51 @@ class GridOperator:
                                     off.ctypes.data_as(ctypes.c_void_p),
                                     w.ctypes.data_as(ctypes.c_void_p)))
         return off, w
+
+
+class ThermalOperator(GridOperator):
+    """K_T(x) of a structured grid: one temperature per node, the 8x8 KE_T of
+    src/hex_element_stiffness.py:339-385 scaled by the conductivity law
+    k_e = k0 (k_void + x^penal (1 - k_void)) (src/topopt_material_model.py:98-109).
+
+    Replaces the assembled ``stiff_mtrx`` of ``HexThermalFEA.solve``
+    (src/hex_thermal_fea.py:74-117).  matvec, diagonal, set_density, solve
+    (Jacobi PCG; no deflation groups) and compliance act on vectors of
+    length num_nodes.
+    """
+
+    def __init__(self, num_elems, elem_size, k0, fixed_nodes, penal=1.0, k_void=1e-8,
+                 device=None):
+        _lib.require_gpu()
+        self.device = torch.device(device or "cuda")
+        self.grid = tuple(int(v) for v in num_elems)
+        self.elem_size = tuple(float(v) for v in elem_size)
+        nelx, nely, nelz = self.grid
+        self.num_nodes = (nelx + 1) * (nely + 1) * (nelz + 1)
+        self.num_elems = nelx * nely * nelz
+        self.num_dofs = self.num_nodes
+        self.E0, self.nu, self.penal, self.eps_void = float(k0), 0.0, float(penal), float(k_void)
+        mask = fixed_mask_from_nodes(self.num_nodes, fixed_nodes)
+        self.fixed_mask = torch.from_numpy(mask).to(self.device)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().pyto_create_thermal(ctypes.byref(h), nelx, nely, nelz,
+                                                 *self.elem_size, self.E0, self.penal,
+                                                 self.eps_void, ptr(self.fixed_mask),
+                                                 stream_ptr()), "pyto_create_thermal")
+        self._h = h
+        self.ngroups = 0
+        self.x = None
+
+    def set_groups(self, node_group, centers):
+        if np.asarray(centers).reshape(-1, 3).shape[0] > 0:
+            raise NotImplementedError("deflation groups are not supported on the thermal "
+                                      "operator; its solves are Jacobi PCG")
+
+    def gradient(self, T):
+        """Centre temperature gradient in reference coordinates, 3 x num_elems
+        (src/hex_thermal_fea.py:250-270)."""
+        out = torch.empty((3, self.num_elems), dtype=F64, device=self.device)
+        check(_lib.lib().pyto_thermal_gradient(self.handle, ptr(self.as_dof(T), F64, self.num_dofs),
+                                               ptr(out), stream_ptr()), "pyto_thermal_gradient")
+        return out

@@ -15,6 +15,10 @@ _SIMP_STRUCTURAL_PENALTY = 3.0
 _SIMP_STRESS_RELAXATION = 0.5
 _EVOID_RELATIVE = 1e-8
 _PNORM_EXPONENT = 6
+_SIMP_THERMAL_PENALTY_MIN = 1
+_SIMP_THERMAL_PENALTY_MAX = 3
+_SIMP_THERMAL_PENALTY = 1.0
+_KVOID_RELATIVE = 1e-8
 
 
 class MaterialModel(enum.Enum):
@@ -100,3 +104,44 @@ def get_stress_relaxation_factor_sensitivity(x):
 
 def get_pNorm_exponent():
     return _PNORM_EXPONENT
+
+
+def initialize_SIMP_THERMAL_PENALTY(value=None):
+    """src/topopt_material_model.py:68-74."""
+    global _SIMP_THERMAL_PENALTY
+    _SIMP_THERMAL_PENALTY = 1.0 if value is None else float(value)
+
+
+def increment_SIMP_THERMAL_PENALTY(value):
+    """src/topopt_material_model.py:53-58."""
+    global _SIMP_THERMAL_PENALTY
+    _SIMP_THERMAL_PENALTY = max(min(_SIMP_THERMAL_PENALTY + value, _SIMP_THERMAL_PENALTY_MAX),
+                                _SIMP_THERMAL_PENALTY_MIN)
+    return _SIMP_THERMAL_PENALTY
+
+
+def get_SIMP_THERMAL_PENALTY():
+    return _SIMP_THERMAL_PENALTY
+
+
+def set_KVOID_RELATIVE(value):
+    global _KVOID_RELATIVE
+    _KVOID_RELATIVE = float(value)
+
+
+def get_KVOID_RELATIVE():
+    return _KVOID_RELATIVE
+
+
+def get_thermal_material_model_scaling(x, material_model):
+    """src/topopt_material_model.py:98-109 (SIMP)."""
+    if material_model is None:
+        return x
+    _require_simp(material_model)
+    return _KVOID_RELATIVE + (x ** _SIMP_THERMAL_PENALTY) * (1 - _KVOID_RELATIVE)
+
+
+def get_thermal_material_model_sensitivity(x, material_model):
+    """src/topopt_material_model.py:142-157 (SIMP)."""
+    _require_simp(material_model)
+    return _SIMP_THERMAL_PENALTY * (x ** (_SIMP_THERMAL_PENALTY - 1)) * (1 - _KVOID_RELATIVE)

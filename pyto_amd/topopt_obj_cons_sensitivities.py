@@ -36,9 +36,10 @@ def compute_compliance(sol, x, fe_solver, KE, material_model):
 
 def compute_compliance_and_gradient(feaMode, sol, x, fe_solver, KE, material_model):
     """src/topopt_obj_cons_sensitivities.py:148-179: C = sum E(x) u_e^T KE u_e,
-    dC/dx = -E'(x) u_e^T KE u_e (one fused element kernel)."""
-    if feaMode != FEA_MODE.STRUCTURAL:
-        raise ValueError("only the structural mode is part of the GPU path")
+    dC/dx = -E'(x) u_e^T KE u_e (one fused element kernel); in the thermal
+    mode C = sum k(x) T_e^T KE_T T_e with the conductivity law."""
+    if feaMode not in (FEA_MODE.STRUCTURAL, FEA_MODE.THERMAL):
+        raise ValueError("Invalid number of rows in element stiffness matrix.")
     tmm._require_simp(material_model)
     op = fe_solver.stiff_mtrx
     C, grad, _ = op.compliance(sol, x)

@@ -34,8 +34,8 @@ def topopt_optimality_criteria(feaMode, fe_solver, to_params, maxIterations=250,
         raise ValueError(f"Unsupported objective type: {to_params.Objective[0]}")
     if to_params.Constraints[0][0] != TO_QOI.VOLUME_FRACTION:
         raise ValueError(f"Unsupported constraint type: {to_params.Constraints[0][0]}")
-    if feaMode != FEA_MODE.STRUCTURAL:
-        raise ValueError("only the structural mode is part of the GPU path")
+    if feaMode not in (FEA_MODE.STRUCTURAL, FEA_MODE.THERMAL):
+        raise ValueError("only the structural and thermal modes are part of the GPU path")
     if plot_progress:
         raise NotImplementedError("plotting is not part of the GPU path")
     volFractionConstraint = to_params.Constraints[0][2]
@@ -52,11 +52,11 @@ def topopt_optimality_criteria(feaMode, fe_solver, to_params, maxIterations=250,
     if print_progress:
         log_message("Computing Filters ...")
     H, Hs = createFilters(fe_solver, to_params)
-    op = fe_solver.stiff_mtrx
-    dev = op.device
+    dev = fe_solver.stiff_mtrx.device
+    nDOFPerNode = 3 if feaMode == FEA_MODE.STRUCTURAL else 1
     elemsWithForces = torch.as_tensor(
-        find_elements_with_forces(fe_solver.mesh, fe_solver.bc.force, 3), dtype=torch.long,
-        device=dev)
+        find_elements_with_forces(fe_solver.mesh, fe_solver.bc.force, nDOFPerNode),
+        dtype=torch.long, device=dev)
     keep = None if to_params.ElemsToKeep is None else torch.as_tensor(
         np.asarray(to_params.ElemsToKeep), dtype=torch.long, device=dev)
 
@@ -101,7 +101,9 @@ def topopt_optimality_criteria(feaMode, fe_solver, to_params, maxIterations=250,
             grad_obj[keep] = grad_obj.min()
 
         xold = x
-        x = op.oc_update(x, grad_obj, volFractionConstraint, move=move, xmin=xmin, xmax=xmax)
+        # the handle may have been rebuilt by the solve (a new material law)
+        x = fe_solver.stiff_mtrx.oc_update(x, grad_obj, volFractionConstraint, move=move,
+                                           xmin=xmin, xmax=xmax)
         change = float((x - xold).abs().max())
         volfrac_now = float(x.mean())
         volConstraint = volfrac_now / volFractionConstraint - 1.0
@@ -128,6 +130,7 @@ def topopt_optimality_criteria(feaMode, fe_solver, to_params, maxIterations=250,
                 log_message("OC optimization converged.")
                 break
         if use_continuation and (iteration % 10 == 0):
+            tmm.increment_SIMP_THERMAL_PENALTY(0.5)
             tmm.increment_SIMP_STRUCTURAL_PENALTY(0.5)
     if iteration == maxIterations - 1:
         errorMsg = "Maximum iterations reached"

@@ -40,6 +40,16 @@ def test_host_ke_equals_oracle(h, E, nu):
     assert np.abs(A - B).max() <= 1e-14 * np.abs(B).max()
 
 
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 0.7, 1.3)])
+@pytest.mark.parametrize("k", [1.0, 205.0])
+def test_host_thermal_ke_equals_oracle(h, k):
+    from oracle import ke
+    from pyto_amd.hex_element_stiffness import hex8_stiffness_matrix_thermal
+    A = hex8_stiffness_matrix_thermal(k, h)
+    B = ke.hex8_stiffness_thermal_exact(k, h)
+    assert np.abs(A - B).max() <= 1e-14 * np.abs(B).max()
+
+
 def test_bad_arguments_raise():
     L = _lib.lib()
     ke = np.zeros(576)
