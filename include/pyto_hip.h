@@ -257,6 +257,30 @@ int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, do
                      double inner_rtol, int max_iters, int max_outer, pyto_solve_stats* stats,
                      void* stream);
 
+/* Thermo-elastic coupling.  H is a 24 x 8 row-major FP64 host array whose row
+ * 3 j + c belongs to element node j, displacement component c (how the
+ * reference indexes its getHMatrix, src/hex_thermal_fea.py:119-237); the
+ * law E(x) = eps + x^penal (1 - eps) is the structural one.  Any handle of
+ * the grid (structural or thermal) may be passed; T is one value per node,
+ * d and f three.
+ *
+ * pyto_thermoelastic_force: f = sum_e e_alpha E(x_e) H (T_e - t_ref) over all
+ * nodes (get_thermoelastic_force, src/hex_thermal_fea.py:198-237).
+ * pyto_thermoelastic_adjoint_rhs: r = sum_e coef E(x_e) H^T d_e (the
+ * right-hand side of solve_thermal_adjoint with coef = -2 E alpha,
+ * src/topopt_obj_cons_sensitivities.py:352-366).
+ * pyto_thermoelastic_bilinear: out_e = d_e^T H (T_e - t_ref) (the direct term
+ * of compute_thermoelastic_compliance_and_gradient,
+ * src/topopt_obj_cons_sensitivities.py:455-458). */
+int pyto_thermoelastic_force(pyto_handle* h, const double* T, const double* x, double penal,
+                             double eps, double e_alpha, double t_ref, const double* H_host,
+                             double* f, void* stream);
+int pyto_thermoelastic_adjoint_rhs(pyto_handle* h, const double* d, const double* x,
+                                   double penal, double eps, double coef, const double* H_host,
+                                   double* r, void* stream);
+int pyto_thermoelastic_bilinear(pyto_handle* h, const double* d, const double* T, double t_ref,
+                                const double* H_host, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,10 @@ Restates ``src/topopt_obj_cons_sensitivities.py``:
     (1,24) at ``:265`` (``get_stress_relaxation_correction`` reshapes to
     (-1,1), src/topopt_material_model.py:201) and ``g[edof] += g_e`` raises;
     here g_e is the flattened (24,) vector.  x is clipped to >= 1e-12 (``:252``)
-    for T1, g and T2 exactly as in the reference.
+    for T1, g and T2 exactly as in the reference;
+  * the thermo-elastic force, thermal adjoint right-hand side and coupled
+    compliance gradient, src/hex_thermal_fea.py:198-237 and
+    ``:322-470``.
 """
 import numpy as np
 import scipy.sparse.linalg as spla
@@ -123,3 +126,48 @@ def direct_solver(mesh, KE, x, fixed, gvals, penal=3.0, eps=1e-8):
             bm -= K[:, fixed] @ gvals
         return fea.scatter_solution(ndof, fixed, gvals, free, lu.solve(bm[free]))
     return solve, K
+
+
+# ---------------------------------------------------------------------------
+# Thermo-elastic coupling (src/hex_thermal_fea.py:198-237,
+# src/topopt_obj_cons_sensitivities.py:322-470).  H is indexed by row 3 j + c
+# for element node j and component c, as the reference indexes it.
+# ---------------------------------------------------------------------------
+def thermoelastic_force(mesh, T, x, H, e_alpha, t_ref, penal=3.0, eps=1e-8):
+    """f = sum_e E(x_e) e_alpha H (T_e - t_ref) assembled by nodes;
+    src/hex_thermal_fea.py:209-237 (single material)."""
+    s = fea.simp_scaling(x, penal, eps) * e_alpha
+    fe = s[:, None] * ((T[mesh.elemArray] - t_ref) @ H.T)          # (N_el, 24)
+    f = np.zeros(3 * mesh.num_nodes)
+    for j in range(8):
+        for c in range(3):
+            np.add.at(f, 3 * mesh.elemArray[:, j] + c, fe[:, 3 * j + c])
+    return f
+
+
+def thermal_adjoint_rhs(mesh, d, x, H, coef, penal=3.0, eps=1e-8):
+    """rhs = sum_e coef E(x_e) H^T d_e; src/topopt_obj_cons_sensitivities.py:352-366
+    (coef = -2 E alpha)."""
+    s = fea.simp_scaling(x, penal, eps) * coef
+    edof = mesh.edofMatStructural
+    re = s[:, None] * (d[edof] @ H)                                # (N_el, 8)
+    rhs = np.zeros(mesh.num_nodes)
+    for k in range(8):
+        np.add.at(rhs, mesh.elemArray[:, k], re[:, k])
+    return rhs
+
+
+def thermoelastic_gradient(mesh, x, T, d, lam, KE, KT, H, e_alpha, t_ref, penal_s=3.0,
+                           eps_s=1e-8, penal_t=1.0, kvoid=1e-8, exact=False):
+    """dJ/dx_e of J = d^T K d, src/topopt_obj_cons_sensitivities.py:428-468:
+    -E'(x) d_e^T KE d_e + 2 k'(x) E alpha d_e^T H (T_e - T_ref)
+    + k'(x) lambda_e^T KE_T T_e (exact: E'(x) in the middle term)."""
+    edof = mesh.edofMatStructural
+    de = d[edof]
+    Te = T[mesh.elemArray]
+    term1 = -fea.simp_sensitivity(x, penal_s, eps_s) * np.einsum("ei,ij,ej->e", de, KE, de)
+    kprime = fea.thermal_sensitivity(x, penal_t, kvoid)
+    dlaw = fea.simp_sensitivity(x, penal_s, eps_s) if exact else kprime
+    term2 = 2.0 * dlaw * e_alpha * np.einsum("ei,ik,ek->e", de, H, Te - t_ref)
+    term3 = kprime * np.einsum("ei,ij,ej->e", lam[mesh.elemArray], KT, Te)
+    return term1 + term2 + term3

@@ -50,6 +50,9 @@ _SIGS = {
     "pyto_ke_hex8_thermal": ([_d, _d, _d, _d, _vp], _i),
     "pyto_matvec_lift_f64": ([_vp, _vp, _vp, _vp], _i),
     "pyto_thermal_gradient": ([_vp, _vp, _vp, _vp], _i),
+    "pyto_thermoelastic_force": ([_vp, _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp], _i),
+    "pyto_thermoelastic_adjoint_rhs": ([_vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp], _i),
+    "pyto_thermoelastic_bilinear": ([_vp, _vp, _vp, _d, _vp, _vp, _vp], _i),
     "pyto_create_thermal": ([ctypes.POINTER(_vp), _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp, _vp],
                             _i),
     "pyto_destroy": ([_vp], _i),

@@ -25,13 +25,13 @@ class HexStructuralFEA:
                  thermo_elastic_force=None, **kwargs):
         if isinstance(mat_prop, list):
             raise NotImplementedError("multi-material KE stacks are not part of the GPU path")
-        if elem_body_force is not None or thermo_elastic_force is not None:
-            raise NotImplementedError("body and thermo-elastic forces are not part of the GPU path")
+        if elem_body_force is not None:
+            raise NotImplementedError("element body forces are not part of the GPU path")
         bound_cond.check_supported(bc)
         self.mesh, self.mat_prop, self.bc = mesh, mat_prop, bc
         self.solver, self.kwargs, self.dsolver = solver, kwargs, dsolver
         self.elem_body_force = None
-        self.thermo_elastic_force = None
+        self.thermo_elastic_force = thermo_elastic_force
         self.elem_stiff = np.expand_dims(hex_element_stiffness.hex8_stiffness_matrix_structural(
             mat_prop.youngs_modulus, mat_prop.poissons_ratio, mesh.elem_size), axis=0)
         self.stiff_mtrx = GridOperator(mesh.grid, mesh.elem_size, mat_prop.youngs_modulus,
@@ -77,6 +77,16 @@ class HexStructuralFEA:
         u = linear_solvers.solve(op, self._force, self.solver, self.bc, dsolver=self.dsolver,
                                  **self.kwargs)
         self.last_stats = getattr(self.dsolver, "last_stats", None) or getattr(op, "last_stats", None)
+        self.total_force = self.bc.force
+        u_elastic = None
+        if self.thermo_elastic_force is not None:
+            # elastic part first, then the solve with the thermal forces added
+            # (src/hex_structural_fea.py:134-147)
+            u_elastic = u
+            f_total = self._force + op.as_dof(self.thermo_elastic_force)
+            self.total_force = f_total
+            u = linear_solvers.solve(op, f_total, self.solver, self.bc, dsolver=self.dsolver,
+                                     **self.kwargs)
         self._u = u
         d = torch.sqrt(u[0::3] ** 2 + u[1::3] ** 2 + u[2::3] ** 2)
         if as_numpy:
@@ -88,6 +98,8 @@ class HexStructuralFEA:
             self.deformation = d
             self.max_deformation = float(d.max())
         self.solElastic = self.sol
+        if u_elastic is not None:
+            self.solElastic = u_elastic.cpu().numpy() if as_numpy else u_elastic
         return self.sol
 
     def postprocess(self):

@@ -14,12 +14,53 @@ thermal W as the rows ``W[free_dofs]`` of the structural rigid-body matrix,
 src/linear_solvers.py:184).  Arrays passed as numpy come back as numpy;
 CUDA tensors stay on the device.
 """
+import ctypes
+
 import numpy as np
 import torch
 
-from . import bound_cond, hex_element_stiffness, linear_solvers
+from . import _lib, bound_cond, hex_element_stiffness, linear_solvers
 from . import topopt_material_model as tmm
+from ._lib import check, ptr, stream_ptr
 from .operator import ThermalOperator
+
+_CORNERS = ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1))
+
+
+def thermoelastic_coupling_matrix(elem_size, nu, layout="reference"):
+    """24 x 8 H with f_e = E alpha E(x) H (T_e - T_ref), exact:
+    H[(c, j), k] = int dN_j/dx_c N_k dV / (1 - 2 nu), a product of 1D
+    integrals (+-1/2 along c; 1/3 or 1/6 along the other two axes).
+
+    layout "reference": row 8 c + j, as src/hex_thermal_fea.py:119-196
+    returns it (the reference then reads row 3 j + c for node j, component c,
+    :224-237, so its coupling is this matrix with permuted rows).
+    layout "physical": row 3 j + c, the node-major order of the DOFs.
+    """
+    h = np.asarray(elem_size, dtype=np.float64)
+    scale = float(np.prod(h)) / (1.0 - 2.0 * nu)
+    H = np.zeros((24, 8))
+    for j, cj in enumerate(_CORNERS):
+        for c in range(3):
+            row = 8 * c + j if layout == "reference" else 3 * j + c
+            for k, ck in enumerate(_CORNERS):
+                v = scale / h[c] * (0.5 if cj[c] else -0.5)
+                for a in range(3):
+                    if a != c:
+                        v *= (1.0 / 3.0) if cj[a] == ck[a] else (1.0 / 6.0)
+                H[row, k] = v
+    if layout not in ("reference", "physical"):
+        raise ValueError("layout must be 'reference' or 'physical'")
+    return H
+
+
+def structural_law(material_model):
+    """(penal, eps) of get_structural_material_model_scaling: x itself without
+    a model (src/topopt_material_model.py:84-85), else the SIMP law."""
+    if material_model is None:
+        return 1.0, 0.0
+    tmm._require_simp(material_model)
+    return tmm.get_SIMP_STRUCTURAL_PENALTY(), tmm.get_EVOID_RELATIVE()
 
 
 class HexThermalFEA:
@@ -36,6 +77,7 @@ class HexThermalFEA:
         self.thermoElasticReferenceTemperature = thermoElasticReferenceTemperature
         self.solver, self.kwargs, self.dsolver = solver, kwargs, dsolver
         self.elem_body_force = None
+        self.h_layout = "reference"
         self.sol = None
         self.x = None
         self.last_stats = None
@@ -103,3 +145,30 @@ class HexThermalFEA:
         grad = self.stiff_mtrx.gradient(self._T)
         self.strain = grad.cpu().numpy() if not isinstance(self.sol, torch.Tensor) else grad
         return
+
+    def getHMatrix(self, dx, dy, dz, nu):
+        """24 x 8 thermo-elastic coupling matrix (src/hex_thermal_fea.py:119-196);
+        ``self.h_layout`` = "reference" (default) or "physical"."""
+        return thermoelastic_coupling_matrix((dx, dy, dz), nu, layout=self.h_layout)
+
+    def get_thermoelastic_force(self, x=None, material_model=None):
+        """Structural nodal forces E alpha E(x) H (T_e - T_ref) summed over the
+        elements (src/hex_thermal_fea.py:198-237), single material."""
+        if self.sol is None:
+            raise ValueError("Solution not computed yet. Call solve() before get_thermal_force().")
+        op = self.stiff_mtrx
+        as_numpy = not isinstance(self.sol, torch.Tensor)
+        if x is None:
+            x = np.ones(self.mesh.num_elems)
+        self.x = x
+        xd = op.as_elem(x)
+        penal, eps = structural_law(material_model)
+        mp = self.mat_prop
+        H = np.ascontiguousarray(self.getHMatrix(*self.mesh.elem_size, mp.poissons_ratio))
+        f = torch.empty(3 * op.num_nodes, dtype=torch.float64, device=op.device)
+        check(_lib.lib().pyto_thermoelastic_force(
+            op.handle, ptr(op.as_dof(self._T), torch.float64, op.num_nodes), ptr(xd),
+            float(penal), float(eps), float(mp.youngs_modulus * mp.thermal_expansion_coefficient),
+            float(self.thermoElasticReferenceTemperature), H.ctypes.data_as(ctypes.c_void_p),
+            ptr(f), stream_ptr()), "pyto_thermoelastic_force")
+        return f.cpu().numpy() if as_numpy else f

@@ -82,3 +82,72 @@ def compute_objective_and_gradient(feaMode, to_params, sol, x, fe_solver, KE):
         pn, g, mx = compute_pnorm_stress_and_sensitivity(sol, x, fe_solver, KE, mm)
         return mx, g
     raise NotImplementedError(f"Objective {kind} is not implemented yet.")
+
+
+def solve_thermal_adjoint(x, d, fe_thermal_solver, fe_structural_solver, material_model):
+    """src/topopt_obj_cons_sensitivities.py:322-385: K_T lambda =
+    -sum_e 2 E alpha E(x_e) H^T d_e with zero temperatures on the thermal
+    fixed DOFs, K_T as last assembled by the thermal solve."""
+    import ctypes
+    from . import _lib, bound_cond
+    from .hex_thermal_fea import structural_law
+    ts, ss = fe_thermal_solver, fe_structural_solver
+    opT, opS = ts.stiff_mtrx, ss.stiff_mtrx
+    mp = ss.mat_prop
+    H = np.ascontiguousarray(ts.getHMatrix(*ss.mesh.elem_size, mp.poissons_ratio))
+    penal, eps = structural_law(material_model)
+    rhs = torch.empty(opT.num_nodes, dtype=torch.float64, device=opT.device)
+    _lib.check(_lib.lib().pyto_thermoelastic_adjoint_rhs(
+        opS.handle, _lib.ptr(opS.as_dof(d), torch.float64, opS.num_dofs),
+        _lib.ptr(opS.as_elem(x), torch.float64, opS.num_elems), float(penal), float(eps),
+        float(-2.0 * mp.youngs_modulus * mp.thermal_expansion_coefficient),
+        H.ctypes.data_as(ctypes.c_void_p), _lib.ptr(rhs), _lib.stream_ptr()),
+        "pyto_thermoelastic_adjoint_rhs")
+    force = np.zeros(opT.num_nodes)
+    bcAdjoint = bound_cond.BC(force=force, fixed_dofs=ts.bc.fixed_dofs,
+                              dirichlet_values=np.zeros(np.asarray(ts.bc.fixed_dofs).size))
+    solver = ts.solver
+    if solver == linear_solvers.Solvers.DPCG:
+        solver = linear_solvers.Solvers.PCG   # the thermal operator has no deflation space
+    kwargs = dict(ts.kwargs)
+    kwargs.setdefault("maxIters", 50000)
+    lam = linear_solvers.solve(opT, rhs, solver, bcAdjoint, **kwargs)
+    return _like(d, lam)
+
+
+def compute_thermoelastic_compliance_and_gradient(x, temperature, displacement, to_params,
+                                                  fe_thermal_solver, fe_structural_solver,
+                                                  exact=False):
+    """src/topopt_obj_cons_sensitivities.py:387-470: J = d^T K d and
+    dJ/dx_e = -E'(x) d_e^T KE d_e + 2 k'(x) E alpha d_e^T H (T_e - T_ref)
+              + k'(x) lambda_e^T KE_T T_e,
+    with lambda from ``solve_thermal_adjoint``.  The force carries the
+    structural law E(x), so its direct derivative is E'(x); the reference
+    writes k'(x) (:455-458).  exact=True uses E'(x) there: that is the
+    derivative of J (checked by finite differences); the default keeps the
+    reference's expression."""
+    import ctypes
+    from . import _lib
+    material_model = to_params.materialModel
+    tmm._require_simp(material_model)
+    ts, ss = fe_thermal_solver, fe_structural_solver
+    opT, opS = ts.stiff_mtrx, ss.stiff_mtrx
+    mp = ss.mat_prop
+    d = opS.as_dof(displacement)
+    T = opT.as_dof(temperature)
+    xd = opS.as_elem(x)
+    J = float(torch.dot(d, opS.matvec(d)))
+    lam = opT.as_dof(solve_thermal_adjoint(xd, d, ts, ss, material_model))
+    _, term1, _ = opS.compliance(d, xd)
+    H = np.ascontiguousarray(ts.getHMatrix(*ss.mesh.elem_size, mp.poissons_ratio))
+    bil = torch.empty(opS.num_elems, dtype=torch.float64, device=opS.device)
+    _lib.check(_lib.lib().pyto_thermoelastic_bilinear(
+        opS.handle, _lib.ptr(d, torch.float64, opS.num_dofs),
+        _lib.ptr(T, torch.float64, opT.num_nodes), float(ts.thermoElasticReferenceTemperature),
+        H.ctypes.data_as(ctypes.c_void_p), _lib.ptr(bil), _lib.stream_ptr()),
+        "pyto_thermoelastic_bilinear")
+    kprime = tmm.get_thermal_material_model_sensitivity(xd, material_model)
+    dlaw = tmm.get_structural_material_model_sensitivity(xd, material_model) if exact else kprime
+    term2 = 2.0 * dlaw * (mp.youngs_modulus * mp.thermal_expansion_coefficient) * bil
+    term3 = kprime * opT.bilinear(lam, T)
+    return J, _like(x, term1 + term2 + term3)
