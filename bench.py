@@ -122,6 +122,51 @@ def batch_line(torch, dev):
     return out
 
 
+def thermal_line(torch, dev, grid, x):
+    """The thermal operator on the same grid and densities (conductivity law
+    k = x, one temperature per node): FP32/FP64 launch times against the
+    algorithmic bytes, and one Jacobi PCG solve to rel res 1e-8 (sink T = 0
+    on x = 0, uniform heat flux into the x = max face)."""
+    import numpy as np
+    from pyto_amd.operator import ThermalOperator
+    nelx, nely, nelz = grid
+    sx, sy, sz = nelx + 1, nely + 1, nelz + 1
+    nn, ne = sx * sy * sz, nelx * nely * nelz
+    idx = np.arange(nn)
+    ix = idx % sx
+    op = ThermalOperator(grid, (1.0, 1.0, 1.0), 1.0, idx[ix == 0], penal=1.0, k_void=1e-3)
+    op.set_density(x)
+    q = torch.zeros(nn, dtype=torch.float64, device=dev)
+    q[torch.from_numpy(idx[ix == nelx]).to(dev)] = 1.0 / (sy * sz)
+    res = {}
+    for name, dt in (("f32", torch.float32), ("f64", torch.float64)):
+        u = torch.randn(nn, dtype=dt, device=dev)
+        y = torch.empty_like(u)
+        for _ in range(3):
+            op.matvec(u, y)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            op.matvec(u, y)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 20 * 1e-3
+        s = 4 if dt == torch.float32 else 8
+        nbytes = s * 2 * nn + s * ne
+        res["matvec_" + name + "_us"] = t * 1e6
+        res["matvec_" + name + "_GBs"] = nbytes / t / 1e9
+    op.solve(q, rtol=1e-8, max_iters=100000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    T, st = op.solve(q, rtol=1e-8, max_iters=100000)
+    torch.cuda.synchronize()
+    res.update({"grid": list(grid), "nodes": nn, "solve_ms": (time.perf_counter() - t0) * 1e3,
+                "iterations": st["iterations"], "converged": bool(st["converged"]),
+                "solver": "Jacobi PCG FP64 (no deflation space on the thermal operator)"})
+    op.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +179,8 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=0.1)
     ap.add_argument("--tuned-boxes", default="12,6,6",
                     help="second aggregate layout timed beside the named one ('' to skip)")
+    ap.add_argument("--no-thermal", action="store_true",
+                    help="skip the thermal-operator line")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 32-design line reported beside `value`")
     args = ap.parse_args()
@@ -269,6 +316,11 @@ def main():
         batch = batch_line(torch, dev)
         log("batch", batch)
 
+    thermal = None
+    if not args.no_thermal:
+        thermal = thermal_line(torch, dev, grid, x)
+        log("thermal", thermal)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(st64["iterations"], nd)
@@ -301,6 +353,7 @@ def main():
                      "fp64_kernel_us": t_mv64 * 1e6, "fp64_GBs": bytes64 / t_mv64 / 1e9},
         "tuned_aggregates": tuned,
         "batch_32_designs": batch,
+        "thermal": thermal,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
