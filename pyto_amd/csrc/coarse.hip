@@ -160,19 +160,20 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
 }
 
 // Per-chunk partials of W^T K v over the special elements (one wave per
-// chunk; k_coarse_gather sums them into t), where
-// v = invd o r + beta p (the new search direction before its coarse
-// correction; HAS_P, with beta = rz_new / rz formed from the update kernel's
-// block partials) or v = invd o r.  invd null: r already holds z = invd o r.  The element arithmetic is FP64
-// whatever the vector precision T.  Group centres are wave-uniform per slot.
+// chunk; k_coarse_gather sums them into t), where v = z + beta p (the new
+// search direction before its coarse correction; HAS_P, with beta = rz_new /
+// rz formed from the update kernel's block partials) or v = z, z = M^-1 r as
+// the solver stores it.  The element arithmetic is FP64 whatever the vector
+// precision T (the cube form for cube elements).  Group centres are
+// wave-uniform per slot.
 // A workgroup serves one SpecialBlock: it first stages v at the block's
 // sorted corner nodes in LDS (an element's corners would otherwise be 24
 // scattered loads, each wave instruction touching up to 64 cache lines on
 // interfaces normal to x), then each wave applies the element operator of
 // one chunk reading its corners from LDS.
-template <typename T, bool HAS_P>
+template <typename T, bool HAS_P, bool CUBE>
 __global__ __launch_bounds__(256) void k_restrict_special(
-    const T* __restrict__ r, const T* __restrict__ invd, const T* __restrict__ p,
+    const T* __restrict__ z, const T* __restrict__ p,
     const double* __restrict__ part_rz, int nrz, const double* __restrict__ rz_old,
     const SpecialElem* __restrict__ sinfo,
     const uint4* __restrict__ sidx, const SpecialChunk* __restrict__ chunks,
@@ -208,25 +209,25 @@ __global__ __launch_bounds__(256) void k_restrict_special(
   // DOFs wherever the nodes are consecutive.  Branch-free (indices clamped
   // into the block's list) so that all node-id loads, then all value loads,
   // are in flight together before the first LDS store.
+  // (32-bit DOF indices: the handle limits 3 * nodes below 2^32)
   const int nst = 3 * sb.nnodes;
-  int64_t dof[kStage];
+  uint32_t dof[kStage];
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
     const int i = min((int)threadIdx.x + 256 * t, nst - 1);
     const int q = i / 3;
-    dof[t] = 3 * (int64_t)sbnodes[sb.node0 + q] + (i - 3 * q);
+    dof[t] = 3u * (uint32_t)sbnodes[sb.node0 + q] + (uint32_t)(i - 3 * q);
   }
-  T ra[kStage], da[kStage], pa[kStage];
+  T za[kStage], pa[kStage];
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
-    ra[t] = r[dof[t]];
-    da[t] = invd ? invd[dof[t]] : T(1);
+    za[t] = z[dof[t]];
     pa[t] = HAS_P ? p[dof[t]] : T(0);
   }
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
     const int i = threadIdx.x + 256 * t;
-    if (i < nst) vs[i] = (double)ra[t] * (double)da[t] + (HAS_P ? beta * (double)pa[t] : 0.0);
+    if (i < nst) vs[i] = (double)za[t] + (HAS_P ? beta * (double)pa[t] : 0.0);
   }
   __syncthreads();
   if (w >= sb.nch) return;
@@ -245,7 +246,10 @@ __global__ __launch_bounds__(256) void k_restrict_special(
       for (int c = 0; c < 3; ++c)
         u[j][c] = ((info >> (24 + 3 * j + c)) & 1) ? 0.0 : vs[3 * l + c];
     }
-    element_apply<double>(u, Ev, mc, f);
+    if (CUBE)
+      element_apply_cube<double>(u, Ev, mc, f);
+    else
+      element_apply<double>(u, Ev, mc, f);
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
@@ -680,26 +684,25 @@ int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
   return PYTO_OK;
 }
 
-// t = W^T K v over the special elements (overwrites t; no zeroing needed).
-// Chunk partials of W^T K v over the special elements (no gather).  With p:
-// v = invd o r + beta p, beta from part_rz (nrz partials) and *rz_old.
+// Chunk partials of W^T K v over the special elements (no gather): v = z, or
+// with p, v = z + beta p with beta from part_rz (nrz partials) and *rz_old.
 template <typename T>
-int launch_restrict_special_parts(Handle* h, const T* r, const T* invd, const T* p,
-                                  const double* part_rz, int nrz, const double* rz_old,
-                                  const int* done, hipStream_t s) {
-  PYTO_REQUIRE(r && (p == nullptr || (part_rz != nullptr && rz_old != nullptr)),
-               PYTO_EINVAL, "special restriction needs r and (with p) the rz partials");
+int launch_restrict_special_parts(Handle* h, const T* z, const T* p, const double* part_rz,
+                                  int nrz, const double* rz_old, const int* done,
+                                  hipStream_t s) {
+  PYTO_REQUIRE(z && (p == nullptr || (part_rz != nullptr && rz_old != nullptr)),
+               PYTO_EINVAL, "special restriction needs z and (with p) the rz partials");
   if (h->nsblocks > 0) {
+    const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
+    auto go = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(h->nsblocks), dim3(256), 0, s, z, p, part_rz, nrz, rz_old,
+                         h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes, h->Ee64,
+                         h->centers, h->g, h->mc64, h->cpart, done);
+    };
     if (p)
-      hipLaunchKernelGGL((k_restrict_special<T, true>), dim3(h->nsblocks), dim3(256), 0, s, r,
-                         invd, p, part_rz, nrz, rz_old, h->sinfo, h->sidx, h->chunks,
-                         h->sblocks, h->sbnodes, h->Ee64, h->centers, h->g, h->mc64, h->cpart,
-                         done);
+      cube ? go(k_restrict_special<T, true, true>) : go(k_restrict_special<T, true, false>);
     else
-      hipLaunchKernelGGL((k_restrict_special<T, false>), dim3(h->nsblocks), dim3(256), 0, s, r,
-                         invd, p, part_rz, nrz, rz_old, h->sinfo, h->sidx, h->chunks,
-                         h->sblocks, h->sbnodes, h->Ee64, h->centers, h->g, h->mc64, h->cpart,
-                         done);
+      cube ? go(k_restrict_special<T, false, true>) : go(k_restrict_special<T, false, false>);
   }
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
@@ -712,12 +715,11 @@ int launch_coarse_gather(Handle* h, double* t, const int* done, hipStream_t s) {
   return PYTO_OK;
 }
 
-// t = W^T K (invd o r) over the special elements (overwrites t).
+// t = W^T K z over the special elements (overwrites t).
 template <typename T>
-int launch_restrict_special(Handle* h, const T* r, const T* invd, double* t, const int* done,
-                            hipStream_t s) {
-  PYTO_TRY(launch_restrict_special_parts<T>(h, r, invd, (const T*)nullptr, nullptr, 0, nullptr,
-                                            done, s));
+int launch_restrict_special(Handle* h, const T* z, double* t, const int* done, hipStream_t s) {
+  PYTO_TRY(launch_restrict_special_parts<T>(h, z, (const T*)nullptr, nullptr, 0, nullptr, done,
+                                            s));
   return launch_coarse_gather(h, t, done, s);
 }
 
@@ -734,16 +736,16 @@ int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double
 
 template int launch_restrict_full<double>(Handle*, const double*, double*, hipStream_t);
 template int launch_restrict_full<float>(Handle*, const float*, double*, hipStream_t);
-template int launch_restrict_special<double>(Handle*, const double*, const double*, double*,
-                                             const int*, hipStream_t);
-template int launch_restrict_special<float>(Handle*, const float*, const float*, double*,
-                                            const int*, hipStream_t);
+template int launch_restrict_special<double>(Handle*, const double*, double*, const int*,
+                                             hipStream_t);
+template int launch_restrict_special<float>(Handle*, const float*, double*, const int*,
+                                            hipStream_t);
 template int launch_restrict_special_parts<double>(Handle*, const double*, const double*,
-                                                   const double*, const double*, int,
-                                                   const double*, const int*, hipStream_t);
+                                                   const double*, int, const double*,
+                                                   const int*, hipStream_t);
 template int launch_restrict_special_parts<float>(Handle*, const float*, const float*,
-                                                  const float*, const double*, int,
-                                                  const double*, const int*, hipStream_t);
+                                                  const double*, int, const double*,
+                                                  const int*, hipStream_t);
 template int launch_prolong<double, false>(Handle*, double*, const double*, const double*,
                                            const double*, bool, const int*, hipStream_t,
                                            double*, int);

@@ -29,12 +29,11 @@ int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalCon
 template <typename T>
 int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s);
 template <typename T>
-int launch_restrict_special(Handle* h, const T* r, const T* invd, double* t, const int* done,
-                            hipStream_t s);
+int launch_restrict_special(Handle* h, const T* z, double* t, const int* done, hipStream_t s);
 template <typename T>
-int launch_restrict_special_parts(Handle* h, const T* r, const T* invd, const T* p,
-                                  const double* part_rz, int nrz, const double* rz_old,
-                                  const int* done, hipStream_t s);
+int launch_restrict_special_parts(Handle* h, const T* z, const T* p, const double* part_rz,
+                                  int nrz, const double* rz_old, const int* done,
+                                  hipStream_t s);
 int launch_coarse_gather(Handle* h, double* t, const int* done, hipStream_t s);
 int launch_gather_scalars(Handle* h, double* t, const double* part_pq, int npq,
                           const double* part_rz, int nrz, hipStream_t s);
@@ -400,13 +399,13 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     // beta = rz / rz_prev; no x update
     hipLaunchKernelGGL(k_resume_scalars, dim3(1), dim3(1), 0, s, h->scal);
     if (defl) {
-      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, (const T*)nullptr, v.p, part_rz,
-                                                kVecBlocks, h->scal + S_RZPREV, nullptr, s));
+      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, v.p, part_rz, kVecBlocks,
+                                                h->scal + S_RZPREV, nullptr, s));
       PYTO_TRY(launch_coarse_gather(h, ct, nullptr, s));
       PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
     }
   } else if (defl) {
-    PYTO_TRY(launch_restrict_special<T>(h, v.r, (const T*)nullptr, ct, nullptr, s));
+    PYTO_TRY(launch_restrict_special<T>(h, v.r, ct, nullptr, s));
     PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
   }
   // p0 = z0 - W mu, or (resume) z0 + beta p_prev - W mu
@@ -432,8 +431,8 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     if (defl) {
       // beta is formed inside the restriction from the update's partials, so
       // the scalar logic runs beside the per-group gather in one launch
-      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, (const T*)nullptr, v.p, part_rz,
-                                                kVecBlocks, h->scal + S_RZ, done, st));
+      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, v.p, part_rz, kVecBlocks,
+                                                h->scal + S_RZ, done, st));
       PYTO_TRY(launch_gather_scalars(h, ct, part_pq, npq, part_rz, kVecBlocks, st));
       PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, done, st, false));
     } else {
