@@ -6,7 +6,14 @@ with corners [n0, n0+1, n0+1+sx, n0+sx, ...+sx*sy] (:117-124), DOF 3*node+c
 (:1080-1095).  The large per-node / per-element arrays are built on first
 access (vectorised instead of the reference's Python loops); the GPU kernels
 never read them, they use the closed-form indices.
+
+``export_vtu_mesh`` writes the reference's VTU (src/hex_mesher.py:1446-1499)
+without pyvista: a VTK XML UnstructuredGrid of hexahedra with the element
+field averaged onto the points, base64-encoded binary arrays.
 """
+import base64
+import os
+
 import numpy as np
 
 
@@ -113,3 +120,61 @@ class HexMesher:
 
     def setPseudoDensity(self, rho):
         self.elemPseudoDensity = rho
+
+    def export_vtu_mesh(self, elem_field, mask_low_pseudodensity=False, density_field="density",
+                        file_name=None):
+        """src/hex_mesher.py:1446-1499: hexahedral cells (VTK type 12) over all
+        grid nodes, optionally only the elements with pseudo-density > 0.5;
+        the element field becomes point data, each point the mean over the
+        kept elements that share it (cells without point use get 0)."""
+        field = np.asarray(elem_field.cpu() if hasattr(elem_field, "cpu") else elem_field,
+                           dtype=np.float64).ravel()
+        elems = self.elemArray
+        if mask_low_pseudodensity:
+            rho = np.asarray(self.elemPseudoDensity, dtype=np.float64).ravel()
+            keep = rho > 0.5
+            elems, field = elems[keep], field[keep]
+        if len(elems) == 0:
+            print("No elements to plot after filtering")
+            return
+        file_name = "./temp.vtu" if file_name is None else file_name
+        write_vtu_hexes(file_name, self.node_xyz, elems, {density_field: field})
+
+
+def _vtu_array(name, arr, ncomp=1):
+    a = np.ascontiguousarray(arr)
+    vtk_type = {np.dtype(np.float64): "Float64", np.dtype(np.float32): "Float32",
+                np.dtype(np.int64): "Int64", np.dtype(np.int32): "Int32",
+                np.dtype(np.uint8): "UInt8"}[a.dtype]
+    raw = a.tobytes()
+    payload = base64.b64encode(np.uint64(len(raw)).tobytes() + raw).decode("ascii")
+    return ('<DataArray type="%s" Name="%s" NumberOfComponents="%d" format="binary">%s'
+            '</DataArray>' % (vtk_type, name, ncomp, payload))
+
+
+def write_vtu_hexes(file_name, points, cells, cell_fields_to_points):
+    """VTK XML UnstructuredGrid of 8-node hexahedra; each field (one value per
+    cell) is written as point data, the mean over the cells sharing a point."""
+    points = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    cells = np.asarray(cells, dtype=np.int64).reshape(-1, 8)
+    npts, ncells = points.shape[0], cells.shape[0]
+    count = np.bincount(cells.ravel(), minlength=npts).astype(np.float64)
+    parts = ['<?xml version="1.0"?>',
+             '<VTKFile type="UnstructuredGrid" version="1.0" byte_order="LittleEndian" '
+             'header_type="UInt64">',
+             '<UnstructuredGrid>',
+             '<Piece NumberOfPoints="%d" NumberOfCells="%d">' % (npts, ncells),
+             '<PointData>']
+    for name, f in cell_fields_to_points.items():
+        f = np.asarray(f, dtype=np.float64).ravel()
+        s = np.bincount(cells.ravel(), weights=np.repeat(f, 8), minlength=npts)
+        parts.append(_vtu_array(name, np.divide(s, count, out=np.zeros(npts), where=count > 0)))
+    parts += ['</PointData>', '<Points>', _vtu_array("Points", points, 3), '</Points>',
+              '<Cells>', _vtu_array("connectivity", cells.ravel()),
+              _vtu_array("offsets", 8 * np.arange(1, ncells + 1, dtype=np.int64)),
+              _vtu_array("types", np.full(ncells, 12, dtype=np.uint8)), '</Cells>',
+              '</Piece>', '</UnstructuredGrid>', '</VTKFile>']
+    d = os.path.dirname(os.path.abspath(file_name))
+    os.makedirs(d, exist_ok=True)
+    with open(file_name, "w") as fh:
+        fh.write("\n".join(parts) + "\n")

@@ -60,3 +60,22 @@ def createFilters(fe_solver, to_params):
     from .topopt_filters import createSmoothingFilter
     return createSmoothingFilter(fe_solver.mesh, rel_filter_radius=to_params.RelativeFilterRadius,
                                  operator=fe_solver.stiff_mtrx)
+
+
+def write_summary_row(result_csv_file, row):
+    """One result row into a per-method summary CSV, replacing the row of the
+    same 'name' or appending it (src/topopt_run_benchmarks.py:218-250)."""
+    import os
+    import pandas as pd
+    if os.path.exists(result_csv_file):
+        existing_df = pd.read_csv(result_csv_file)
+        idx = existing_df[existing_df["name"] == row["name"]].index
+        if len(idx) > 0:
+            existing_df.loc[idx[0]] = row
+        else:
+            existing_df = pd.concat([existing_df, pd.DataFrame([row])], ignore_index=True)
+        existing_df.to_csv(result_csv_file, index=False)
+    else:
+        d = os.path.dirname(os.path.abspath(result_csv_file))
+        os.makedirs(d, exist_ok=True)
+        pd.DataFrame([row]).to_csv(result_csv_file, index=False)
