@@ -163,26 +163,31 @@ __device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
 // chunk; k_coarse_gather sums them into t), where v = z + beta p (the new
 // search direction before its coarse correction; HAS_P, with beta = rz_new /
 // rz formed from the update kernel's block partials) or v = z, z = M^-1 r as
-// the solver stores it.  The element arithmetic is FP64 whatever the vector
-// precision T (the cube form for cube elements).  Group centres are
-// wave-uniform per slot.
+// the solver stores it.  The element arithmetic is in the solver's precision
+// T (the cube form for cube elements) and the per-group sums in FP64; in FP32
+// that halves the element registers, so more waves fit per SIMD.  Group
+// centres are wave-uniform per slot.
 // A workgroup serves one SpecialBlock: it first stages v at the block's
 // sorted corner nodes in LDS (an element's corners would otherwise be 24
 // scattered loads, each wave instruction touching up to 64 cache lines on
 // interfaces normal to x), then each wave applies the element operator of
 // one chunk reading its corners from LDS.
-template <typename T, bool HAS_P, bool CUBE>
-__global__ __launch_bounds__(256) void k_restrict_special(
+// WPE: minimum waves per SIMD the registers must allow (5 for FP32, which
+// fits 96 VGPRs without spilling: the 4-wave-per-SIMD grid of about a
+// thousand blocks would otherwise run a short second round).
+template <typename T, bool HAS_P, bool CUBE, int WPE = sizeof(T) == 4 ? 5 : 4>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void
+k_restrict_special(
     const T* __restrict__ z, const T* __restrict__ p,
     const double* __restrict__ part_rz, int nrz, const double* __restrict__ rz_old,
     const SpecialElem* __restrict__ sinfo,
     const uint4* __restrict__ sidx, const SpecialChunk* __restrict__ chunks,
     const SpecialBlock* __restrict__ sblocks, const int32_t* __restrict__ sbnodes,
-    const double* __restrict__ Ee, const double* __restrict__ centers, GridDims g,
-    ModalConsts<double> mc, double* __restrict__ cpart, const int* __restrict__ done) {
+    const T* __restrict__ Ee, const double* __restrict__ centers, GridDims g,
+    ModalConsts<T> mc, double* __restrict__ cpart, const int* __restrict__ done) {
   if (done && *done) return;
   constexpr int kStage = (3 * kSpecialBlockNodes + 255) / 256;   // staged DOFs per thread
-  __shared__ double vs[3 * kSpecialBlockNodes];
+  __shared__ T vs[3 * kSpecialBlockNodes];
   const SpecialBlock sb = sblocks[xcd_logical_block(blockIdx.x, gridDim.x)];
   // beta = rz_new / rz from the update kernel's partials, summed here in the
   // same order as by the scalar logic (no scalar kernel needed before this)
@@ -199,7 +204,7 @@ __global__ __launch_bounds__(256) void k_restrict_special(
   // this lane's element record, issued before the staging loads
   SpecialElem se{0, 0, 0};
   uint4 li = make_uint4(0, 0, 0, 0);
-  double Ev = 0.0;
+  T Ev = T(0);
   if (active) {
     se = sinfo[ch_start + lane];
     li = sidx[ch_start + lane];
@@ -227,55 +232,59 @@ __global__ __launch_bounds__(256) void k_restrict_special(
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
     const int i = threadIdx.x + 256 * t;
-    if (i < nst) vs[i] = (double)za[t] + (HAS_P ? beta * (double)pa[t] : 0.0);
+    if (i < nst) vs[i] = (T)((double)za[t] + (HAS_P ? beta * (double)pa[t] : 0.0));
   }
   __syncthreads();
   if (w >= sb.nch) return;
-  double f[8][3];
+  T f[8][3];
   uint64_t info = 0;
   int ix = 0, iy = 0, iz = 0;
   if (active) {
     const uint32_t lw[4] = {li.x, li.y, li.z, li.w};
     info = se.info;
     elem_coords(se.e, g, ix, iy, iz);
-    double u[8][3];
+    T u[8][3];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int l = (int)((lw[j >> 1] >> (16 * (j & 1))) & 0xffffu);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        u[j][c] = ((info >> (24 + 3 * j + c)) & 1) ? 0.0 : vs[3 * l + c];
+        u[j][c] = ((info >> (24 + 3 * j + c)) & 1) ? T(0) : vs[3 * l + c];
     }
     if (CUBE)
-      element_apply_cube<double>(u, Ev, mc, f);
+      element_apply_cube<T>(u, Ev, mc, f);
     else
-      element_apply<double>(u, Ev, mc, f);
+      element_apply<T>(u, Ev, mc, f);
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        if ((info >> (24 + 3 * j + c)) & 1) f[j][c] = 0.0;
+        if ((info >> (24 + 3 * j + c)) & 1) f[j][c] = T(0);
   } else {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) f[j][0] = f[j][1] = f[j][2] = 0.0;
+    for (int j = 0; j < 8; ++j) f[j][0] = f[j][1] = f[j][2] = T(0);
   }
+  // corner positions relative to the group centre and the moments r x f in
+  // the solver precision T, the sums in FP64
   for (int sl = 0; sl < ch_ng; ++sl) {
     const int gs = chunks[wave].groups[sl];
     const double cx = centers[3 * gs], cy = centers[3 * gs + 1], cz = centers[3 * gs + 2];
+    const T x0 = (T)(ix * g.hx - cx), y0 = (T)(iy * g.hy - cy), z0 = (T)(iz * g.hz - cz);
+    const T hx = (T)g.hx, hy = (T)g.hy, hz = (T)g.hz;
     double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       if (!active || (int)((info >> (3 * j)) & 7) != sl) continue;
-      const double rx = (ix + corner_a(j)) * g.hx - cx;
-      const double ry = (iy + corner_b(j)) * g.hy - cy;
-      const double rz = (iz + corner_c(j)) * g.hz - cz;
-      const double fx = f[j][0], fy = f[j][1], fz = f[j][2];
-      acc[0] += fx;
-      acc[1] += fy;
-      acc[2] += fz;
-      acc[3] += -rz * fy + ry * fz;
-      acc[4] += rz * fx - rx * fz;
-      acc[5] += -ry * fx + rx * fy;
+      const T rx = corner_a(j) ? x0 + hx : x0;
+      const T ry = corner_b(j) ? y0 + hy : y0;
+      const T rz = corner_c(j) ? z0 + hz : z0;
+      const T fx = f[j][0], fy = f[j][1], fz = f[j][2];
+      acc[0] += (double)fx;
+      acc[1] += (double)fy;
+      acc[2] += (double)fz;
+      acc[3] += (double)(ry * fz - rz * fy);
+      acc[4] += (double)(rz * fx - rx * fz);
+      acc[5] += (double)(rx * fy - ry * fx);
     }
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
@@ -684,6 +693,19 @@ int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
   return PYTO_OK;
 }
 
+template <typename T>
+static const T* ee_of(const Handle* h);
+template <>
+const double* ee_of<double>(const Handle* h) { return h->Ee64; }
+template <>
+const float* ee_of<float>(const Handle* h) { return h->Ee32; }
+template <typename T>
+static ModalConsts<T> mc_of(const Handle* h);
+template <>
+ModalConsts<double> mc_of<double>(const Handle* h) { return h->mc64; }
+template <>
+ModalConsts<float> mc_of<float>(const Handle* h) { return h->mc32; }
+
 // Chunk partials of W^T K v over the special elements (no gather): v = z, or
 // with p, v = z + beta p with beta from part_rz (nrz partials) and *rz_old.
 template <typename T>
@@ -696,8 +718,8 @@ int launch_restrict_special_parts(Handle* h, const T* z, const T* p, const doubl
     const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
     auto go = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, dim3(h->nsblocks), dim3(256), 0, s, z, p, part_rz, nrz, rz_old,
-                         h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes, h->Ee64,
-                         h->centers, h->g, h->mc64, h->cpart, done);
+                         h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes, ee_of<T>(h),
+                         h->centers, h->g, mc_of<T>(h), h->cpart, done);
     };
     if (p)
       cube ? go(k_restrict_special<T, true, true>) : go(k_restrict_special<T, true, false>);
