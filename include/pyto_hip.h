@@ -281,6 +281,45 @@ int pyto_thermoelastic_adjoint_rhs(pyto_handle* h, const double* d, const double
 int pyto_thermoelastic_bilinear(pyto_handle* h, const double* d, const double* T, double t_ref,
                                 const double* H_host, double* out, void* stream);
 
+/* ---- method of moving asymptotes ------------------------------------------ */
+/* One MMA step (mmasub) and its KKT residual (kktcheck) for n variables and
+ * 1 <= m <= 4 constraints, m < n.  These replace the mmapy calls of runMMA,
+ * src/mmaWrapper.py:90-91 and :108-109 (mmapy implements K. Svanberg's
+ * published MMA; same constants: asymptote factors 0.5/1.2/0.7, bounds 0.01
+ * and 10 times xmax - xmin, raa0 1e-5, albefa 0.1, epsimin 1e-7).
+ *
+ * pyto_mma_create: workspace for n variables and m constraints
+ * (8 (7 + 2m) n bytes).
+ * pyto_mma_sub: xval, xmin, xmax, xold1, xold2, df0dx (n) and dfdx (m x n,
+ * row-major) are device inputs; low and upp (n) are read (iteration > 2) and
+ * overwritten with the new asymptotes; xmma, xsi, eta (n) receive the
+ * subproblem solution.  fval_host, a_host, c_host, d_host hold m values.
+ * scal_host (may be null) receives y[m], z, lam[m], mu[m], zet, s[m]
+ * (4m + 2 values).  Synchronous: returns when the result is written.
+ * pyto_mma_kkt: the rows of the KKT residual over the variables,
+ * df0dx + dfdx^T lam - xsi + eta, xsi (x - xmin), eta (xmax - x); writes
+ * their sum of squares and max abs to out_host[0..1] (the m + 1 scalar rows
+ * are the caller's). */
+typedef struct pyto_mma pyto_mma;
+typedef struct pyto_mma_stats {
+  int newton_steps;      /* Newton steps over all barrier levels */
+  int evaluations;       /* residual evaluations (line-search trials + level starts) */
+  int levels;            /* barrier levels */
+  int rounds;            /* launch rounds queued (a multiple of 16) */
+  double residual_norm;  /* final residual norm of the subproblem */
+} pyto_mma_stats;
+int pyto_mma_create(pyto_mma** out, int64_t n, int m);
+int pyto_mma_destroy(pyto_mma* w);
+int pyto_mma_sub(pyto_mma* w, int iter, const double* xval, const double* xmin,
+                 const double* xmax, const double* xold1, const double* xold2,
+                 const double* df0dx, const double* dfdx, const double* fval_host, double a0,
+                 const double* a_host, const double* c_host, const double* d_host, double move,
+                 double* low, double* upp, double* xmma, double* xsi, double* eta,
+                 double* scal_host, pyto_mma_stats* stats, void* stream);
+int pyto_mma_kkt(pyto_mma* w, const double* x, const double* xsi, const double* eta,
+                 const double* xmin, const double* xmax, const double* df0dx, const double* dfdx,
+                 const double* lam_host, double* out_host, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,15 @@ class SolveStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MmaStats(ctypes.Structure):
+    _fields_ = [("newton_steps", ctypes.c_int), ("evaluations", ctypes.c_int),
+                ("levels", ctypes.c_int), ("rounds", ctypes.c_int),
+                ("residual_norm", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 _vp = ctypes.c_void_p
@@ -53,6 +62,11 @@ _SIGS = {
     "pyto_thermoelastic_force": ([_vp, _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp], _i),
     "pyto_thermoelastic_adjoint_rhs": ([_vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp], _i),
     "pyto_thermoelastic_bilinear": ([_vp, _vp, _vp, _d, _vp, _vp, _vp], _i),
+    "pyto_mma_create": ([ctypes.POINTER(_vp), ctypes.c_int64, _i], _i),
+    "pyto_mma_destroy": ([_vp], _i),
+    "pyto_mma_sub": ([_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, _d,
+                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "pyto_mma_kkt": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "pyto_create_thermal": ([ctypes.POINTER(_vp), _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp, _vp],
                             _i),
     "pyto_destroy": ([_vp], _i),

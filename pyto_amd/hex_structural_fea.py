@@ -32,15 +32,34 @@ class HexStructuralFEA:
         self.solver, self.kwargs, self.dsolver = solver, kwargs, dsolver
         self.elem_body_force = None
         self.thermo_elastic_force = thermo_elastic_force
-        self.elem_stiff = np.expand_dims(hex_element_stiffness.hex8_stiffness_matrix_structural(
-            mat_prop.youngs_modulus, mat_prop.poissons_ratio, mesh.elem_size), axis=0)
-        self.stiff_mtrx = GridOperator(mesh.grid, mesh.elem_size, mat_prop.youngs_modulus,
-                                       mat_prop.poissons_ratio, bc.fixed_dofs,
-                                       penal=tmm.get_SIMP_STRUCTURAL_PENALTY(),
-                                       eps_void=tmm.get_EVOID_RELATIVE())
+        self._material = None
+        self.stiff_mtrx = None
+        self.set_material(mat_prop)
         self._force = None
         self.x = None
         self.last_stats = None
+
+    def set_material(self, mat_prop):
+        """src/hex_structural_fea.py:56-65 (one material): KE of the material;
+        the operator is rebuilt when E or nu change."""
+        if isinstance(mat_prop, list):
+            raise NotImplementedError("multi-material KE stacks are not part of the GPU path")
+        self.elem_stiff = np.expand_dims(hex_element_stiffness.hex8_stiffness_matrix_structural(
+            mat_prop.youngs_modulus, mat_prop.poissons_ratio, self.mesh.elem_size), axis=0)
+        key = (mat_prop.youngs_modulus, mat_prop.poissons_ratio)
+        if key != self._material:
+            groups = getattr(self.stiff_mtrx, "_groups_from", None)
+            self.stiff_mtrx = GridOperator(self.mesh.grid, self.mesh.elem_size, key[0], key[1],
+                                           self.bc.fixed_dofs,
+                                           penal=tmm.get_SIMP_STRUCTURAL_PENALTY(),
+                                           eps_void=tmm.get_EVOID_RELATIVE())
+            if groups is not None:
+                groups.attach(self.stiff_mtrx)
+            self._material = key
+
+    def set_thermal_forces(self, thermo_elastic_force):
+        """src/hex_structural_fea.py:68-69."""
+        self.thermo_elastic_force = thermo_elastic_force
 
     def _sync_material_law(self):
         op = self.stiff_mtrx
@@ -49,9 +68,8 @@ class HexStructuralFEA:
             # the SIMP constants are baked into the handle: rebuild it
             groups = getattr(op, "_groups_from", None)
             self.stiff_mtrx = GridOperator(self.mesh.grid, self.mesh.elem_size,
-                                           self.mat_prop.youngs_modulus,
-                                           self.mat_prop.poissons_ratio, self.bc.fixed_dofs,
-                                           penal=p, eps_void=e)
+                                           self._material[0], self._material[1],
+                                           self.bc.fixed_dofs, penal=p, eps_void=e)
             if groups is not None:
                 groups.attach(self.stiff_mtrx)
 

@@ -11,14 +11,19 @@ DIRECT_SOLVER_DOF_CUTOFF = 100000  # src/topopt_common.py:18
 
 
 class TO_METHODS(enum.Enum):
-    OC = enum.auto()
-    MMA = enum.auto()
+    """src/topopt_common.py:20-24 (the two density methods are on the GPU
+    path: DENSITYOCM in topopt_ocm, DENSITYMMA in topopt_mma)."""
+    DENSITYMMA = enum.auto()
+    DENSITYOCM = enum.auto()
+    PARETO = enum.auto()
+    LEVELSET = enum.auto()
 
 
 class FEA_MODE(enum.Enum):
     """src/topopt_common.py:27-30."""
     STRUCTURAL = enum.auto()
     THERMAL = enum.auto()
+    THERMO_STRUCTURAL = enum.auto()
 
 
 class TO_QOI(enum.Enum):
@@ -43,6 +48,7 @@ class TOParams:
     RelativeFilterRadius: float = 1.5
     ElemsToKeep: list = None
     MaxIterations: int = 150
+    Eliminate_Hanging_Elements: bool = False
     materialModel: MaterialModel = MaterialModel.SIMP
 
 

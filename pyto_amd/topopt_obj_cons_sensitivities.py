@@ -75,6 +75,10 @@ def compute_objective_and_gradient(feaMode, to_params, sol, x, fe_solver, KE):
     if kind == TO_QOI.VOLUME_FRACTION:
         n = x.numel() if isinstance(x, torch.Tensor) else x.size
         return float(x.mean()), (torch.ones_like(x) if isinstance(x, torch.Tensor) else np.ones_like(x)) / n
+    if kind == TO_QOI.MASS:
+        w = _elem_mass(fe_solver)
+        ones = torch.ones_like(x) if isinstance(x, torch.Tensor) else np.ones_like(x)
+        return float(x.sum()) * w, ones * w
     if kind == TO_QOI.PNORM_STRESS:
         pn, g, _ = compute_pnorm_stress_and_sensitivity(sol, x, fe_solver, KE, mm)
         return pn, g
@@ -82,6 +86,52 @@ def compute_objective_and_gradient(feaMode, to_params, sol, x, fe_solver, KE):
         pn, g, mx = compute_pnorm_stress_and_sensitivity(sol, x, fe_solver, KE, mm)
         return mx, g
     raise NotImplementedError(f"Objective {kind} is not implemented yet.")
+
+
+def _elem_mass(fe_solver):
+    h = fe_solver.mesh.elem_size
+    return h[0] * h[1] * h[2] * fe_solver.mat_prop.mass_density
+
+
+def compute_constraint_and_gradient(feaMode, to_params, sol, x, fe_solver, KE):
+    """src/topopt_obj_cons_sensitivities.py:42-96 -> (c, dc): c is an (m, 1)
+    numpy array, dc (m, num_elems) in the kind of x (CUDA tensor or numpy).
+    The stress-scaled constraints keep the reference's running scale
+    ``compute_constraint_and_gradient.stress_scaling`` (:78-93)."""
+    tensor = isinstance(x, torch.Tensor)
+    n = x.numel() if tensor else x.size
+    nC = len(to_params.Constraints)
+    c = np.zeros((nC, 1))
+    dc = torch.zeros((nC, n), dtype=torch.float64, device=x.device) if tensor else np.zeros((nC, n))
+    mm = to_params.materialModel
+    for m in range(nC):
+        kind, limit = to_params.Constraints[m][0], to_params.Constraints[m][2]
+        if kind == TO_QOI.COMPLIANCE:
+            C, g = compute_compliance_and_gradient(feaMode, sol, x, fe_solver, KE, mm)
+            c[m, 0], dc[m, :] = C / limit - 1.0, g / limit
+        elif kind == TO_QOI.MASS:
+            w = _elem_mass(fe_solver)
+            c[m, 0] = float(x.sum()) * w / limit - 1.0
+            dc[m, :] = w / limit
+        elif kind == TO_QOI.VOLUME_FRACTION:
+            v, g = compute_volume_constraint_and_gradient(x, limit)
+            c[m, 0], dc[m, :] = v, g
+        elif kind == TO_QOI.PNORM_STRESS:
+            pn, g, _ = compute_pnorm_stress_and_sensitivity(sol, x, fe_solver, KE, mm)
+            c[m, 0], dc[m, :] = pn / limit - 1.0, g / limit
+        elif kind in (TO_QOI.MAX_VONMISES_STRESS, TO_QOI.STRESS_FAILURE_FACTOR):
+            pn, g, mx = compute_pnorm_stress_and_sensitivity(sol, x, fe_solver, KE, mm)
+            sc = compute_constraint_and_gradient.stress_scaling
+            div = limit if kind == TO_QOI.MAX_VONMISES_STRESS else \
+                fe_solver.mat_prop.yield_strength * limit
+            c[m, 0], dc[m, :] = sc * pn / div - 1.0, sc * g / div
+            compute_constraint_and_gradient.stress_scaling = 0.25 * mx / pn + 0.75 * sc
+        else:
+            raise NotImplementedError(f"Constraint {kind} is not implemented yet.")
+    return c, dc
+
+
+compute_constraint_and_gradient.stress_scaling = 1.0
 
 
 def solve_thermal_adjoint(x, d, fe_thermal_solver, fe_structural_solver, material_model):

@@ -58,3 +58,18 @@ def test_bad_arguments_raise():
     assert b"Poisson" in L.pyto_last_error()
     with pytest.raises(ValueError):
         _lib.check(rc)
+
+
+def test_mma_workspace_argument_checks():
+    """Rejected before any device call: more than four constraints, m >= n."""
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.pyto_mma_create(ctypes.byref(h), 100, 5) == _lib.PYTO_EINVAL
+    assert b"m <= 4" in L.pyto_last_error()
+    assert L.pyto_mma_create(ctypes.byref(h), 3, 3) == _lib.PYTO_EINVAL
+    assert b"m < n" in L.pyto_last_error()
+    from pyto_amd import mmaWrapper
+    z = np.zeros(10)
+    with pytest.raises(ValueError):
+        mmaWrapper.mmasub(5, 10, 1, z, z, z + 1, z, z, 0.0, z, np.zeros(5), np.zeros((5, 10)), z,
+                          z + 1, 1.0, np.zeros(5), np.ones(5), np.ones(5), 0.2)
