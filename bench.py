@@ -76,6 +76,65 @@ def cpu_baseline(gpu_iters_fp64, ndof_target, budget_s=20.0):
             "est_seconds_per_solve": t_solve}
 
 
+def mma_problem(n, seed=7):
+    """Synthetic data of one MMA step (iteration 3, one volume constraint
+    v_f = 0.3): densities in [1e-6, 1 - 1e-6], compliance-like negative
+    sensitivities, the two previous iterates a few per cent away."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    x = np.clip(rng.beta(0.6, 1.2, n), 1e-6, 1 - 1e-6)
+    df0dx = -3.0 * x * x * rng.lognormal(0.0, 1.0, n) - 1e-9
+    xo1 = np.clip(x + rng.normal(0, 0.02, n), 1e-6, 1 - 1e-6)
+    xo2 = np.clip(xo1 + rng.normal(0, 0.02, n), 1e-6, 1 - 1e-6)
+    low = x - rng.uniform(0.1, 0.5, n)
+    upp = x + rng.uniform(0.1, 0.5, n)
+    dfdx = np.full((1, n), 1.0 / (0.3 * n))
+    fval = np.array([x.mean() / 0.3 - 1.0])
+    return x, xo1, xo2, low, upp, df0dx, dfdx, fval
+
+
+def mma_cpu(n=262144):
+    """The oracle's mmasub on n variables of the same synthetic step."""
+    import numpy as np
+    from oracle import mma as OMMA
+    x, xo1, xo2, low, upp, df0dx, dfdx, fval = mma_problem(n)
+    t0 = time.perf_counter()
+    OMMA.mmasub(1, n, 3, x, np.full(n, 1e-6), np.full(n, 1 - 1e-6), xo1, xo2, 0.0, df0dx, fval,
+                dfdx, low, upp, 1.0, np.zeros(1), 100 * np.ones(1), np.ones(1), 0.05)
+    t = time.perf_counter() - t0
+    nw, ev, lv = OMMA.subsolv.last_counts
+    return {"variables": n, "ms": t * 1e3, "newton_steps": nw, "evaluations": ev,
+            "mvars_per_s": n / t / 1e6, "cores": 1, "kind": "port"}
+
+
+def mma_line(torch, dev, n):
+    """One device MMA step (mmasub) on n variables of the synthetic step:
+    time per call, Newton steps, and the algorithmic bandwidth of its element
+    passes, m = 1: setup 8 (20 + 3m) n bytes, each residual evaluation
+    8 (12 + 2m) n, each Newton direction 8 (18 + 2m) n, the final point 72 n."""
+    import numpy as np
+    from pyto_amd import mmaWrapper as W
+    x, xo1, xo2, low, upp, df0dx, dfdx, fval = mma_problem(n)
+    T = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+    args = [T(x), T(np.full(n, 1e-6)), T(np.full(n, 1 - 1e-6)), T(xo1), T(xo2), 0.0, T(df0dx),
+            fval, T(dfdx), T(low), T(upp), 1.0, np.zeros(1), 100 * np.ones(1), np.ones(1), 0.05]
+    W.mmasub(1, n, 3, *args)
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        W.mmasub(1, n, 3, *args)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    st = W.mmasub.last_stats
+    nbytes = 8 * n * (23 + 14 * st["evaluations"] + 20 * st["newton_steps"] + 9)
+    return {"variables": n, "constraints": 1, "ms": t * 1e3, "newton_steps": st["newton_steps"],
+            "evaluations": st["evaluations"], "levels": st["levels"],
+            "algorithmic_GBs": nbytes / t / 1e9,
+            "workload": "one MMA step (iteration 3, move 0.05, volume constraint) on the "
+                        "256x128x128 element count, synthetic compliance-like gradient"}
+
+
 def traffic_bytes():
     """HBM bytes per k_matvec<float> launch at this workload, from the two
     rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in the profile
@@ -181,6 +240,7 @@ def main():
                     help="second aggregate layout timed beside the named one ('' to skip)")
     ap.add_argument("--no-thermal", action="store_true",
                     help="skip the thermal-operator line")
+    ap.add_argument("--no-mma", action="store_true", help="skip the MMA step line")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 32-design line reported beside `value`")
     args = ap.parse_args()
@@ -316,6 +376,11 @@ def main():
         batch = batch_line(torch, dev)
         log("batch", batch)
 
+    mma = None
+    if not args.no_mma:
+        mma = mma_line(torch, dev, int(np.prod(grid)))
+        log("mma", mma)
+
     thermal = None
     if not args.no_thermal:
         thermal = thermal_line(torch, dev, grid, x)
@@ -324,6 +389,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(st64["iterations"], nd)
+        if mma is not None:
+            cpu["mma_step_oracle"] = mma_cpu()
         log("cpu baseline", cpu)
 
     s = stats[-1]
@@ -354,6 +421,7 @@ def main():
         "tuned_aggregates": tuned,
         "batch_32_designs": batch,
         "thermal": thermal,
+        "mma_step": mma,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
