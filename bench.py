@@ -253,19 +253,41 @@ def main():
     from pyto_amd import deflation, problems
     from pyto_amd.operator import GridOperator
 
+    from pyto_amd import builders
+
     cfg = problems.CONFIGS[args.config]
     grid = cfg["grid"]
+    boxes = tuple(int(v) for v in args.boxes.split(","))
+    dev = torch.device("cuda")
+    # problem set-up, host builders (vectorised numpy) against the device
+    # builders; both give the same bits, the device arrays are used
+    t0 = time.perf_counter()
     mesh, mat, bc = problems.cantilever(grid, load=cfg["load"], point=cfg.get("point"))
+    ds_host = deflation.DeflationSolver()
+    ds_host.create_box_groups(mesh, boxes)
+    t_setup_host = time.perf_counter() - t0
+    builders.fixed_mask((2, 2, 2), [(0, False, 7)], dev)     # first launch of the builders
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mask = builders.fixed_mask(grid, [(0, False, 7)], dev)
+    if cfg["load"] == "traction":
+        f = builders.face_load(grid, device=dev)
+    else:
+        f = torch.from_numpy(bc.force).to(dev)
+    ds = deflation.DeflationSolver()
+    ds.create_box_groups(mesh, boxes, device=dev)
+    torch.cuda.synchronize()
+    t_setup_dev = time.perf_counter() - t0
+    del ds_host
+    log("set-up: host builders %.1f ms, device builders %.1f ms"
+        % (t_setup_host * 1e3, t_setup_dev * 1e3))
     x_np = problems.config_density(args.config, grid)
     log("grid", grid, "dof", 3 * mesh.num_nodes)
-    dev = torch.device("cuda")
-    op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, bc.fixed_dofs, penal=3.0, eps_void=1e-9)
+    op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, None, penal=3.0, eps_void=1e-9,
+                      fixed_mask=mask)
     op.set_coarse_shift(0.0, 0.0)
-    ds = deflation.DeflationSolver()
-    ds.create_box_groups(mesh, tuple(int(v) for v in args.boxes.split(",")))
     ds.attach(op)
     x = torch.from_numpy(x_np).to(dev)
-    f = torch.from_numpy(bc.force).to(dev)
     u = torch.empty_like(f)
     ngroups = op.ngroups
     log("groups", ngroups, "workspace MB", op.workspace_bytes() / 1e6)
@@ -347,7 +369,7 @@ def main():
     tuned = None
     if args.tuned_boxes:
         ds2 = deflation.DeflationSolver()
-        ds2.create_box_groups(mesh, tuple(int(v) for v in args.tuned_boxes.split(",")))
+        ds2.create_box_groups(mesh, tuple(int(v) for v in args.tuned_boxes.split(",")), device=dev)
         ds2.attach(op)
         step("mixed")
         torch.cuda.synchronize()
@@ -418,6 +440,9 @@ def main():
                      "kernel": "k_matvec<float>", "kernel_us": t_mv32 * 1e6,
                      "algorithmic_bytes": bytes32,
                      "fp64_kernel_us": t_mv64 * 1e6, "fp64_GBs": bytes64 / t_mv64 / 1e9},
+        "setup_ms": {"host_builders": t_setup_host * 1e3, "device_builders": t_setup_dev * 1e3,
+                     "what": "fixed-DOF mask of the x=0 clamp, tip traction, %dx%dx%d box "
+                             "deflation groups (ids, counts, centres)" % boxes},
         "tuned_aggregates": tuned,
         "batch_32_designs": batch,
         "thermal": thermal,

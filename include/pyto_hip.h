@@ -320,6 +320,40 @@ int pyto_mma_kkt(pyto_mma* w, const double* x, const double* xsi, const double* 
                  const double* xmin, const double* xmax, const double* df0dx, const double* dfdx,
                  const double* lam_host, double* out_host, void* stream);
 
+/* ---- problem builders on the device --------------------------------------- */
+/* The per-node and per-element set-up of a voxel problem, bit-identical to the
+ * reference's arithmetic; all arrays are device memory.
+ * pyto_grid_arrays: node coordinates (3 per node, i * h), element corner
+ * nodes (8 per element, int32) and structural element DOFs (24 per element,
+ * int64); any output may be null.  Replaces grid_mesh's node and element
+ * loops and createEdofMatStructural (src/hex_mesher.py:104-152,1080-1095).
+ * pyto_plane_mask: ORs comp_bits (bit c = DOF c) into mask[n] for the nodes of
+ * the plane index[axis] == (at_max ? n_axis : 0); the fixed-DOF mask of
+ * pyto_create from getNodesOnBoundingBoxPlane (src/hex_mesher.py:732-760).
+ * pyto_face_load: f[dofs_per_node n + comp] = -w (total / sum w) on that plane,
+ * w = 4 inside, 2 on an edge, 1 at a corner (src/hex_structural_examples.py:
+ * 360-395); other entries are left as they are.
+ * pyto_node_groups: node_group[n] = box_to_group[tx[ix] + nbx (ty[iy] + nby
+ * tz[iz])] with per-axis box tables of nx+1, ny+1, nz+1 entries (-1 for a box
+ * index outside the table); the assignment step of create_deflation_groups
+ * (src/deflation.py:140-160,205-207).
+ * pyto_group_sums: per group g and coordinate c, the sum of the coordinates of
+ * the group's nodes in node order inside the half-open node box
+ * bbox[6g .. 6g+5] = (x0, x1, y0, y1, z0, z1), sequential like numpy's add.at
+ * (src/deflation.py:163-165,212-214); node_group null = every node of the box. */
+int pyto_grid_arrays(int nx, int ny, int nz, double hx, double hy, double hz, double* node_xyz,
+                     int32_t* elem_nodes, int64_t* edof, void* stream);
+int pyto_plane_mask(int nx, int ny, int nz, int axis, int at_max, int comp_bits, uint8_t* mask,
+                    void* stream);
+int pyto_face_load(int nx, int ny, int nz, int axis, int at_max, int comp, int dofs_per_node,
+                   double total, double* f, void* stream);
+int pyto_node_groups(int nx, int ny, int nz, const int32_t* tx, const int32_t* ty,
+                     const int32_t* tz, int nbx, int nby, int nbz, const int32_t* box_to_group,
+                     int32_t* node_group, void* stream);
+int pyto_group_sums(int nx, int ny, int nz, double hx, double hy, double hz,
+                    const int32_t* node_group, int ngroups, const int32_t* bbox, double* sums,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,11 @@ _SIGS = {
     "pyto_mma_sub": ([_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, _d,
                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "pyto_mma_kkt": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "pyto_grid_arrays": ([_i, _i, _i, _d, _d, _d, _vp, _vp, _vp, _vp], _i),
+    "pyto_plane_mask": ([_i, _i, _i, _i, _i, _i, _vp, _vp], _i),
+    "pyto_face_load": ([_i, _i, _i, _i, _i, _i, _i, _d, _vp, _vp], _i),
+    "pyto_node_groups": ([_i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp], _i),
+    "pyto_group_sums": ([_i, _i, _i, _d, _d, _d, _vp, _i, _vp, _vp, _vp], _i),
     "pyto_create_thermal": ([ctypes.POINTER(_vp), _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp, _vp],
                             _i),
     "pyto_destroy": ([_vp], _i),

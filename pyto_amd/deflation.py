@@ -31,8 +31,15 @@ class DeflationSolver:
         n = int(nDOF / self.dofPerGroup)
         return max(min(n, self.maxGroups), self.minGroups)
 
-    def create_deflation_groups(self, meshData, nGroupsDesired: int):
-        """src/deflation.py:86-229 (the blocking input() prompt becomes a ValueError)."""
+    def create_deflation_groups(self, meshData, nGroupsDesired: int, device=None):
+        """src/deflation.py:86-229 (the blocking input() prompt becomes a ValueError).
+        With ``device`` the groups are built on that GPU (pyto_amd.builders,
+        identical arrays; ws_nodeGroupNumber is then a CUDA tensor)."""
+        if device is not None:
+            from .builders import reference_boxes
+            return self._assign(meshData, reference_boxes(meshData.grid, meshData.elem_size,
+                                                          nGroupsDesired, self.minNodesPerGroup),
+                                device)
         nGroupsDesired = min(nGroupsDesired, int(meshData.num_nodes / (1 + self.minNodesPerGroup)))
         xyz = meshData.node_xyz
         mn = xyz.min(axis=0)
@@ -43,11 +50,20 @@ class DeflationSolver:
         nZ = max(round(alpha * L[2]), 1)
         return self._assign(meshData, (nX, nY, nZ))
 
-    def create_box_groups(self, meshData, boxes):
+    def create_box_groups(self, meshData, boxes, device=None):
         """Explicit nX x nY x nZ box partition (no reference counterpart)."""
-        return self._assign(meshData, tuple(int(b) for b in boxes))
+        return self._assign(meshData, tuple(int(b) for b in boxes), device)
 
-    def _assign(self, meshData, boxes):
+    def _assign(self, meshData, boxes, device=None):
+        if device is not None:
+            from .builders import box_groups
+            ngn, center, count = box_groups(meshData.grid, meshData.elem_size, boxes,
+                                            self.minNodesPerGroup, device)
+            self.ws_nGroups = int(center.shape[0])
+            self.ws_nodeGroupNumber = ngn
+            self.ws_groupCount = count
+            self.ws_groupCenter = center
+            return True
         nX, nY, nZ = boxes
         xyz = meshData.node_xyz
         mn = xyz.min(axis=0)

@@ -47,7 +47,9 @@ class GridOperator:
     """
 
     def __init__(self, num_elems, elem_size, E0, nu, fixed_dofs, penal=3.0, eps_void=1e-8,
-                 device=None):
+                 device=None, fixed_mask=None):
+        """``fixed_mask`` (a per-node uint8 CUDA tensor, bit c = DOF c fixed,
+        e.g. from pyto_amd.builders.fixed_mask) replaces ``fixed_dofs``."""
         _lib.require_gpu()
         self.device = torch.device(device or "cuda")
         self.grid = tuple(int(v) for v in num_elems)
@@ -57,8 +59,13 @@ class GridOperator:
         self.num_elems = nelx * nely * nelz
         self.num_dofs = 3 * self.num_nodes
         self.E0, self.nu, self.penal, self.eps_void = float(E0), float(nu), float(penal), float(eps_void)
-        mask = fixed_mask_from_dofs(self.num_nodes, fixed_dofs)
-        self.fixed_mask = torch.from_numpy(mask).to(self.device)
+        if fixed_mask is not None:
+            if fixed_mask.dtype != torch.uint8 or fixed_mask.numel() != self.num_nodes:
+                raise ValueError("fixed_mask must be a uint8 tensor of one entry per node")
+            self.fixed_mask = fixed_mask.to(self.device).contiguous()
+        else:
+            mask = fixed_mask_from_dofs(self.num_nodes, fixed_dofs)
+            self.fixed_mask = torch.from_numpy(mask).to(self.device)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             check(_lib.lib().pyto_create(ctypes.byref(h), nelx, nely, nelz, *self.elem_size,
@@ -94,7 +101,10 @@ class GridOperator:
     # -- state -------------------------------------------------------------
     def set_groups(self, node_group, centers):
         """Deflation groups (int32 per node) and centres (G x 3)."""
-        g = torch.as_tensor(np.ascontiguousarray(node_group, dtype=np.int32)).to(self.device)
+        if isinstance(node_group, torch.Tensor):
+            g = node_group.to(device=self.device, dtype=torch.int32).contiguous()
+        else:
+            g = torch.as_tensor(np.ascontiguousarray(node_group, dtype=np.int32)).to(self.device)
         c = torch.as_tensor(np.ascontiguousarray(centers, dtype=np.float64)).to(self.device)
         ng = int(c.shape[0])
         if ng == 0:
