@@ -32,13 +32,17 @@ struct MatvecArgs {
   int zero_n;
 };
 
-// Lane (tx-1)'s value within the 16-lane row (0 for tx = 0): DPP row_shr:1.
+// Lane (tx-1)'s value within the 16-lane row (0 for tx = 0): DPP row_shr:1
+// with bound_ctrl, which reads 0 where the row has no source lane, so no
+// "old" value has to be materialised and the add can take the DPP operand.
 __device__ __forceinline__ float dpp_shr1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+  const int b = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, 0x111, 0xF, 0xF, true));
 }
 __device__ __forceinline__ double dpp_shr1(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xF, 0xF, false);
+  const int l = __double2loint(v), h = __double2hiint(v);
+  const int lo = __builtin_amdgcn_update_dpp(l, l, 0x111, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(h, h, 0x111, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -249,8 +253,10 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
     if (kFaces) face_xy<T, NC>(top[0], top[1], top[2], top[3], tface);
     mq = (mq << 8) | (uint32_t)pm[b][ty][tx];
     T f[8][NC];
-    if (col_valid && k >= 0 && k < g.nz) {
-      const T E = es[b][ty][tx];   // E (or k) of layer k, staged with plane k+1
+    // wave-uniform test only: a column outside the grid applies E = 0 (its
+    // corner values are 0 as well), so no per-lane merge of the results
+    if (k >= 0 && k < g.nz) {
+      const T E = col_valid ? es[b][ty][tx] : T(0);   // E (or k) of layer k, staged with plane k+1
       if constexpr (NC == 1) {
         element_apply_thermal_faces<T>(bface, tface, E, a.mc, f);
       } else if constexpr (kFaces) {
