@@ -188,11 +188,8 @@ k_restrict_special(
   if (done && *done) return;
   constexpr int kStage = (3 * kSpecialBlockNodes + 255) / 256;   // staged DOFs per thread
   __shared__ T vs[3 * kSpecialBlockNodes];
+  __shared__ double csh[kSpecialBlockChunks][8 * 3];   // group centres per chunk and slot
   const SpecialBlock sb = sblocks[xcd_logical_block(blockIdx.x, gridDim.x)];
-  // beta = rz_new / rz from the update kernel's partials, summed here in the
-  // same order as by the scalar logic (no scalar kernel needed before this)
-  __shared__ double redb[5];
-  const double beta = HAS_P ? block_sum_partials(part_rz, nrz, redb) / *rz_old : 0.0;
   const int w = threadIdx.x >> 6;
   const int wave = sb.chunk0 + min(w, sb.nch - 1);
   const int lane = threadIdx.x & 63;
@@ -229,6 +226,21 @@ k_restrict_special(
     za[t] = z[dof[t]];
     pa[t] = HAS_P ? p[dof[t]] : T(0);
   }
+  // the centres of each chunk's groups, staged beside the values (threads
+  // 0..95 of the block: chunk, slot, coordinate)
+  if (threadIdx.x < kSpecialBlockChunks * 24) {
+    const int cw = threadIdx.x / 24, r = threadIdx.x - 24 * cw;
+    if (cw < sb.nch) {
+      const SpecialChunk* c = chunks + sb.chunk0 + cw;
+      const int sl = r / 3;
+      if (sl < c->ng) csh[cw][r] = centers[3 * c->groups[sl] + (r - 3 * sl)];
+    }
+  }
+  // beta = rz_new / rz from the update kernel's partials, summed here in the
+  // same order as by the scalar logic (no scalar kernel needed before this);
+  // formed after the staging loads are issued, so their latencies overlap
+  __shared__ double redb[5];
+  const double beta = HAS_P ? block_sum_partials(part_rz, nrz, redb) / *rz_old : 0.0;
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
     const int i = threadIdx.x + 256 * t;
@@ -266,9 +278,9 @@ k_restrict_special(
   }
   // corner positions relative to the group centre and the moments r x f in
   // the solver precision T, the sums in FP64
+  const double* cen = csh[w];
   for (int sl = 0; sl < ch_ng; ++sl) {
-    const int gs = chunks[wave].groups[sl];
-    const double cx = centers[3 * gs], cy = centers[3 * gs + 1], cz = centers[3 * gs + 2];
+    const double cx = cen[3 * sl], cy = cen[3 * sl + 1], cz = cen[3 * sl + 2];
     const T x0 = (T)(ix * g.hx - cx), y0 = (T)(iy * g.hy - cy), z0 = (T)(iz * g.hz - cz);
     const T hx = (T)g.hx, hy = (T)g.hy, hz = (T)g.hz;
     double acc[6] = {0, 0, 0, 0, 0, 0};
