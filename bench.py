@@ -76,6 +76,31 @@ def cpu_baseline(gpu_iters_fp64, ndof_target, budget_s=20.0):
             "est_seconds_per_solve": t_solve}
 
 
+def sensitivity_errors(op, x, u_mixed, u64, inner_rtol):
+    """Errors of the FP32+refine path against the FP64 path in the
+    sensitivities: the compliance gradient -E'(x) u_e^T KE u_e, and the p-norm
+    stress (p = 6, q = 0.5) with its gradient T1 + T2, whose adjoint is solved
+    in the same precision as the forward solve (one solve each)."""
+    import torch
+    Ep = 3.0 * x * x * (1.0 - 1e-9)   # SIMP p = 3, Emin = 1e-9
+    out = {}
+    _, gm, _ = op.compliance(u_mixed, x)
+    _, g6, _ = op.compliance(u64, x)
+    out["compliance_gradient_max_rel"] = float((gm - g6).abs().max() / g6.abs().max())
+    pn, grads = {}, {}
+    for name, u, prec in (("mixed", u_mixed, "mixed"), ("fp64", u64, "fp64")):
+        st = op.stress(u, x, q=0.5, pexp=6.0, want_strain=False, want_stress=False,
+                       want_energy=False)
+        pn[name] = float(st["sum_vm_p"][0]) ** (1.0 / 6.0)
+        g, T1 = op.pnorm_adjoint_rhs(u, x, st["vm"], st["sum_vm_p"], q=0.5, pexp=6.0)
+        lam, _ = op.solve(g, rtol=1e-8, max_iters=5000, precision=prec, inner_rtol=inner_rtol)
+        grads[name] = T1 - Ep * op.bilinear(lam, u)
+    out["pnorm_rel"] = abs(pn["mixed"] - pn["fp64"]) / pn["fp64"]
+    out["pnorm_gradient_max_rel"] = float((grads["mixed"] - grads["fp64"]).abs().max()
+                                          / grads["fp64"].abs().max())
+    return out
+
+
 def mma_problem(n, seed=7):
     """Synthetic data of one MMA step (iteration 3, one volume constraint
     v_f = 0.3): densities in [1e-6, 1 - 1e-6], compliance-like negative
@@ -327,6 +352,8 @@ def main():
     C_64 = float(torch.dot(f, u64))
     err_u = float((u_mixed - u64).abs().max() / u64.abs().max())
     err_C = abs(C_mixed - C_64) / abs(C_64)
+    err_sens = sensitivity_errors(op, x, u_mixed, u64, args.inner_rtol)
+    log("errors vs fp64: displacement %.2e, compliance %.2e," % (err_u, err_C), err_sens)
 
     # roofline of the dominant kernel: FP32 K*u, events on the launch stream
     nd, ne = op.num_dofs, op.num_elems
@@ -434,7 +461,8 @@ def main():
                        "rel": st64["rel"], "rel_true": st64["rel_true"],
                        "pcg_iterations_per_s": st64["iterations"] / t_fp64},
         "speedup_vs_fp64": t_fp64 / t_mixed,
-        "error_vs_fp64": {"displacement_max_rel": err_u, "compliance_rel": err_C},
+        "error_vs_fp64": dict({"displacement_max_rel": err_u, "compliance_rel": err_C},
+                              **err_sens),
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs32 / HBM_PEAK_GBS, "traffic": traffic_bytes(),
                      "kernel": "k_matvec<float>", "kernel_us": t_mv32 * 1e6,

@@ -534,11 +534,103 @@ __global__ __launch_bounds__(256) void k_dir_update(
     }
 }
 
+// The same direction update with one node (its three DOFs) per thread, for
+// the structural FP64 solver: W mu and the fixed bits are formed once per node
+// instead of for both nodes a 16-byte vector can span, and the per-DOF
+// arithmetic is the vector kernel's.  (In FP32 a node is 12 bytes, loaded as
+// separate dwords; the vector kernel is faster there.)
+template <typename T>
+__global__ __launch_bounds__(256) void k_dir_update_nodes(
+    T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
+    const double* __restrict__ mu, const int32_t* __restrict__ grp,
+    const double* __restrict__ centers, int ngroups, const uint8_t* __restrict__ mask,
+    GridDims g, const double* __restrict__ scal, bool first, const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ double aw[kLdsGroups * 6];
+  const bool lds = grp && ngroups <= kLdsGroups;
+  if (lds) {
+    for (int q = threadIdx.x; q < ngroups; q += blockDim.x) {
+      const double* m6 = mu + 6 * q;
+      const double wx = m6[3], wy = m6[4], wz = m6[5];
+      const double cx = centers[3 * q], cy = centers[3 * q + 1], cz = centers[3 * q + 2];
+      aw[6 * q + 0] = m6[0] - (wy * cz - wz * cy);
+      aw[6 * q + 1] = m6[1] - (wz * cx - wx * cz);
+      aw[6 * q + 2] = m6[2] - (wx * cy - wy * cx);
+      aw[6 * q + 3] = wx;
+      aw[6 * q + 4] = wy;
+      aw[6 * q + 5] = wz;
+    }
+    __syncthreads();
+  }
+  const double beta = first ? 0.0 : scal[S_BETA];
+  const double alpha = first ? 0.0 : scal[S_ALPHA_X];
+  for (int64_t nd = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; nd < g.nnodes;
+       nd += (int64_t)gridDim.x * blockDim.x) {
+    double w[3] = {0.0, 0.0, 0.0};
+    uint32_t fx = 0;
+    if (grp) {
+      int ix, iy, iz;
+      node_coords(nd, g, ix, iy, iz);
+      fx = mask[nd];
+      const int q = grp[nd];
+      const double X = ix * g.hx, Y = iy * g.hy, Z = iz * g.hz;
+      double e[6];
+      if (lds) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) e[k] = aw[6 * q + k];
+      } else {
+        const double* m6 = mu + 6 * q;
+        const double wx = m6[3], wy = m6[4], wz = m6[5];
+        const double cx = centers[3 * q], cy = centers[3 * q + 1], cz = centers[3 * q + 2];
+        e[0] = m6[0] - (wy * cz - wz * cy);
+        e[1] = m6[1] - (wz * cx - wx * cz);
+        e[2] = m6[2] - (wx * cy - wy * cx);
+        e[3] = wx;
+        e[4] = wy;
+        e[5] = wz;
+      }
+      w[0] = e[0] + e[4] * Z - e[5] * Y;
+      w[1] = e[1] + e[5] * X - e[3] * Z;
+      w[2] = e[2] + e[3] * Y - e[4] * X;
+    }
+    T* pn = p + 3 * nd;
+    T* xn = x + 3 * nd;
+    const T* zn = z + 3 * nd;
+    T pp[3], zz[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pp[c] = first ? T(0) : pn[c];
+      zz[c] = zn[c];
+    }
+    if (alpha != 0.0) {
+      T xx[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xx[c] = xn[c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xn[c] = (T)((double)xx[c] + alpha * (double)pp[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double wm = ((fx >> c) & 1u) ? 0.0 : w[c];
+      pn[c] = (T)((double)zz[c] + beta * (double)pp[c] - wm);
+    }
+  }
+}
+
 template <typename T>
 int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* done,
                       hipStream_t s) {
   // 2048 workgroups (8 per CU) striding over the vectors: each stages the
   // group maps once, so fewer, longer-lived blocks keep those L2 reads small
+  if (h->ncomp == 3 && sizeof(T) == 8) {
+    const int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_dir_update_nodes<T>, dim3(blocks), dim3(256), 0, s, x, p, z,
+                       (const double*)h->cmu,
+                       h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
+                       h->ngroups, h->fixmask, h->g, (const double*)h->scal, first, done);
+    PYTO_CHECK_HIP(hipGetLastError());
+    return PYTO_OK;
+  }
   const int64_t nvec = h->ncomp * h->g.nnodes / (16 / (int64_t)sizeof(T));
   const int blocks = (int)std::min<int64_t>((nvec + 255) / 256, 2048);
   hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), 0, s, x, p, z,
