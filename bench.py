@@ -174,7 +174,7 @@ def traffic_bytes():
     return None
 
 
-def batch_line(torch, dev):
+def batch_line(torch, dev, inner_rtol=None):
     from pyto_amd import problems
     from pyto_amd.batch import BatchSolver
     from pyto_amd.operator import GridOperator
@@ -187,10 +187,11 @@ def batch_line(torch, dev):
         ops.append(op)
     bs = BatchSolver(ops, [d["loads"] for d in designs])
     tune = bs.autotune()
-    bs.solve(rtol=1e-8)
+    kw = {} if inner_rtol is None else {"inner_rtol": inner_rtol}
+    bs.solve(rtol=1e-8, **kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, stats = bs.solve(rtol=1e-8)
+    _, stats = bs.solve(rtol=1e-8, **kw)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     it = [s["iterations"] for s in stats]
@@ -199,6 +200,7 @@ def batch_line(torch, dev):
            "seconds": t, "systems_per_s": bs.nsys / t,
            "all_converged": all(s["converged"] for s in stats),
            "inner_iterations_max": max(it), "inner_iterations_median": float(np.median(it)),
+           "outer_iterations_max": max(s["outer_iterations"] for s in stats),
            "zchunk": tune["zchunk"]}
     bs.close()
     for op in ops:

@@ -82,7 +82,7 @@ class BatchSolver:
         self.tuning = {"zchunk": best, "ms_per_matvec": times}
         return self.tuning
 
-    def solve(self, rtol=1e-8, inner_rtol=1e-4, max_iters=200000, max_outer=60, out=None):
+    def solve(self, rtol=1e-8, inner_rtol=1e-2, max_iters=200000, max_outer=60, out=None):
         """Returns (list of solutions per system in system order, list of
         stats dicts).  Systems are ordered design by design, load case by
         load case (``self.systems``)."""

@@ -13,9 +13,13 @@
 // Per system the algorithm is the plain Jacobi PCG of the reference's PCG
 // branch (src/linear_solvers.py:157-176 with _jacobi_preconditioner,
 // :71-88), with the FP64 refinement of the single-system mixed solver:
-// r = f - K u in FP64, inner FP32 PCG on K d = r / sqrt(r^T M r) to
-// min(0.1, max(inner_rtol, 0.3 rtol / rel)), u += sqrt(r^T M r) d, until the
-// FP64 sqrt(r^T M r / r0^T M r0) <= rtol.
+// r = f - K u in FP64, inner FP32 PCG on K d = r / sigma to
+// min(0.1, max(inner_rtol, 0.3 rtol / rel)), u += sigma d, until the FP64
+// sqrt(r^T M r / r0^T M r0) <= rtol.  sigma = sqrt(r0^T M r0) is fixed for the
+// solve, so an inner run can continue the previous run's search direction
+// (reliable update: beta = r^T M r / rz_prev) while that run converged and its
+// recurrence residual still tracks the new true one (within 1.25 in norm);
+// otherwise it restarts from p = M r.
 #include <algorithm>
 #include <vector>
 
@@ -43,8 +47,13 @@ struct BatchSys {
   int tiles_x, ntiles, zchunk;
 };
 
-// Outer-refinement flag: the system has met rtol (or has a zero load).
+// Outer-refinement flags: the system has met rtol (or has a zero load); the
+// next inner run continues the previous run's direction.
 constexpr int F_ODONE = 7;
+constexpr int F_RESUME = 8;
+// largest ratio of true to recurrence residual norm at which an inner run
+// continues the previous one's direction
+constexpr double kBatchResumeDrift = 1.25;
 
 __device__ __forceinline__ int find_sys(const int* __restrict__ pre, int nsys, int b) {
   int lo = 0, hi = nsys;   // pre[lo] <= b < pre[hi]
@@ -246,7 +255,9 @@ __global__ __launch_bounds__(256) void k_bscalars(const int* __restrict__ vpre,
   if (threadIdx.x == 0) pcg_scalar_step(sc, fl, pkp, rz_new, kFloor32);
 }
 
-// x += alpha p (unless FIRST), p = invd o r + beta p (FIRST: p = invd o r).
+// x += alpha p (unless FIRST), p = invd o r + beta p.  FIRST: the start of an
+// inner run, p = invd o r, or with F_RESUME p = invd o r + (rz / rz_prev) p
+// (x is 0 then).
 template <bool FIRST>
 __global__ __launch_bounds__(256) void k_bdir(const BatchSys* __restrict__ sys,
                                               const int* __restrict__ vpre, int nsys,
@@ -256,8 +267,9 @@ __global__ __launch_bounds__(256) void k_bdir(const BatchSys* __restrict__ sys,
   if (flags[v.s * kNumFlags + F_DONE]) return;
   const BatchSys& S = sys[v.s];
   const double* sc = scal + v.s * kNumScalars;
+  const bool resume = FIRST && flags[v.s * kNumFlags + F_RESUME];
   const float al = FIRST ? 0.f : (float)sc[S_ALPHA];
-  const float be = FIRST ? 0.f : (float)sc[S_BETA];
+  const float be = FIRST ? (resume ? (float)(sc[S_RZ] / sc[S_RZPREV]) : 0.f) : (float)sc[S_BETA];
   const int64_t nv = S.ndof / 4;
   float4* pv = reinterpret_cast<float4*>(S.p);
   float4* xv = reinterpret_cast<float4*>(S.x);
@@ -265,7 +277,7 @@ __global__ __launch_bounds__(256) void k_bdir(const BatchSys* __restrict__ sys,
   const float4* dv = reinterpret_cast<const float4*>(S.invd32);
   for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < nv; i += (int64_t)v.nb * 256) {
     const float4 rr = rv[i], dd = dv[i];
-    float4 pp = FIRST ? make_float4(0.f, 0.f, 0.f, 0.f) : pv[i];
+    float4 pp = (FIRST && !resume) ? make_float4(0.f, 0.f, 0.f, 0.f) : pv[i];
     if (!FIRST) {
       float4 xx = xv[i];
       xx.x += al * pp.x;
@@ -282,7 +294,7 @@ __global__ __launch_bounds__(256) void k_bdir(const BatchSys* __restrict__ sys,
   }
   if (v.lb == 0)
     for (int64_t i = nv * 4 + threadIdx.x; i < S.ndof; i += 256) {
-      const float pp = FIRST ? 0.f : S.p[i];
+      const float pp = (FIRST && !resume) ? 0.f : S.p[i];
       if (!FIRST) S.x[i] += al * pp;
       S.p[i] = S.invd32[i] * S.r[i] + be * pp;
     }
@@ -345,12 +357,23 @@ __global__ __launch_bounds__(256) void k_bouter(const int* __restrict__ vpre, in
   double* sc = scal + s * kNumScalars;
   const double rmr = block_sum_partials(part + vpre[s], vpre[s + 1] - vpre[s], red);
   if (threadIdx.x == 0) {
-    if (first) sc[S_TMP1] = rmr;
+    if (first) {
+      sc[S_TMP1] = rmr;
+      sc[S_SCALE] = sqrt(rmr);   // sigma: fixed for the solve
+      fl[F_RESUME] = 0;
+    }
     const double rel = sc[S_TMP1] > 0.0 ? sqrt(rmr / sc[S_TMP1]) : 0.0;
     sc[S_TMP2] = rel;
     sc[S_TMP0] = rmr;
-    if (rel <= rtol) fl[F_ODONE] = 1;
-    else sc[S_SCALE] = sqrt(rmr);
+    if (rel <= rtol) {
+      fl[F_ODONE] = 1;
+    } else if (!first) {
+      // the last inner run's recurrence r^T M r (S_RZ) against the new true
+      // one, both in the inner units (divided by sigma^2)
+      const double rz_true = rmr / (sc[S_SCALE] * sc[S_SCALE]);
+      const bool clean = fl[F_DONE] && !fl[F_BREAKDOWN] && fl[F_ITERS] > 0;
+      fl[F_RESUME] = clean && rz_true <= kBatchResumeDrift * kBatchResumeDrift * sc[S_RZ];
+    }
   }
 }
 

@@ -176,7 +176,7 @@ struct Handle {
 constexpr int kMaxPartials = 8192;
 constexpr int kNumCounters = 16;
 constexpr int kNumScalars = 32;
-constexpr int kNumFlags = 8;
+constexpr int kNumFlags = 10;
 
 // Device scalar slots.
 enum ScalarSlot {
