@@ -174,8 +174,10 @@ def traffic_bytes():
     return None
 
 
-def batch_line(torch, dev, inner_rtol=None):
-    from pyto_amd import problems
+def batch_line(torch, dev, inner_rtol=None, boxes=(8, 4, 2)):
+    """The 32-design batch; ``boxes``: the deflation box partition of every
+    design (None: Jacobi PCG only)."""
+    from pyto_amd import deflation, problems
     from pyto_amd.batch import BatchSolver
     from pyto_amd.operator import GridOperator
     designs = problems.batch_designs()
@@ -183,6 +185,11 @@ def batch_line(torch, dev, inner_rtol=None):
     for d in designs:
         op = GridOperator(d["grid"], (1, 1, 1), 1.0, 0.3, d["bc"].fixed_dofs, penal=3.0,
                           eps_void=1e-8)
+        if boxes is not None:
+            op.set_coarse_shift(0.0, 0.0)
+            ds = deflation.DeflationSolver()
+            ds.create_box_groups(d["mesh"], boxes)
+            ds.attach(op)
         op.set_density(torch.from_numpy(d["x"]).to(dev))
         ops.append(op)
     bs = BatchSolver(ops, [d["loads"] for d in designs])
@@ -201,6 +208,7 @@ def batch_line(torch, dev, inner_rtol=None):
            "all_converged": all(s["converged"] for s in stats),
            "inner_iterations_max": max(it), "inner_iterations_median": float(np.median(it)),
            "outer_iterations_max": max(s["outer_iterations"] for s in stats),
+           "deflation_boxes": None if boxes is None else list(boxes),
            "zchunk": tune["zchunk"]}
     bs.close()
     for op in ops:

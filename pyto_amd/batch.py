@@ -5,8 +5,10 @@
 vectors.  Every (design, load case) pair is one system; all systems iterate
 in lockstep through batched kernels (one launch per kernel per iteration for
 the whole batch, captured as a hipGraph), each with its own scalars and
-stopping test.  Per system: Jacobi PCG in FP32 inside FP64 iterative
-refinement.  ``autotune()`` picks the operator's tile depth (node planes per
+stopping test.  Per system: PCG in FP32 inside FP64 iterative refinement,
+deflated with the operator's coarse space when its handle has deflation
+groups (attach them, with ``set_coarse_shift(0, 0)``, before
+``set_density``), Jacobi PCG otherwise.  ``autotune()`` picks the operator's tile depth (node planes per
 workgroup) for this batch by timing the batched FP32 operator.
 """
 import ctypes

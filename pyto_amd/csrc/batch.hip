@@ -20,12 +20,29 @@
 // (reliable update: beta = r^T M r / rz_prev) while that run converged and its
 // recurrence residual still tracks the new true one (within 1.25 in norm);
 // otherwise it restarts from p = M r.
+//
+// Deflation: a system whose handle has deflation groups runs the deflated PCG
+// of the single-system solver inside the same launches.  Each inner run starts
+// from x0 = W E^-1 W^T b (per system, with the handle's full restriction and
+// coarse solve), and every direction is projected, p = M r + beta p -
+// W E^-1 W^T K (M r + beta p), with W^T K v restricted over the handle's
+// special elements; the restriction, the per-group gather and the coarse GEMV
+// are one launch each for all deflated systems of the batch.
 #include <algorithm>
 #include <vector>
 
 #include "matvec.hpp"
 
 namespace pyto {
+
+// coarse.hip
+template <typename T>
+int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s);
+int coarse_apply(Handle* h, const double* t1, const double* t2, const double* beta,
+                 double* mu, const int* done, hipStream_t s, bool refine);
+template <typename T, bool UPDATE>
+int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
+                   bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n);
 
 constexpr int kBatchGraphIters = 16;   // inner iterations per captured graph
 
@@ -45,6 +62,20 @@ struct BatchSys {
   double* u;
   int64_t ndof;
   int tiles_x, ntiles, zchunk;
+  // deflation (ngroups == 0: none); the handle's coarse space and the
+  // system's own coarse vectors
+  int ngroups, nc, ld;
+  const int32_t* node_group;
+  const double* centers;
+  const SpecialElem* sinfo;
+  const uint4* sidx;
+  const SpecialChunk* chunks;
+  const SpecialBlock* sblocks;
+  const int32_t* sbnodes;
+  const int32_t* gptr;
+  const int32_t* gidx;
+  const double* Einv;
+  double *cpart, *ct, *cmu;
 };
 
 // Outer-refinement flags: the system has met rtol (or has a zero load); the
@@ -87,6 +118,33 @@ __global__ __launch_bounds__(256) void k_bmatvec_inner(const BatchSys* __restric
   a.ntiles = S.ntiles;
   a.partials = partials;
   a.done = flags + s * kNumFlags + F_DONE;
+  a.zero_vec = nullptr;
+  a.zero_n = 0;
+  matvec_tile<float, CUBE>(a, gb - pre[s], gb);
+}
+
+// q = K x (FP32) for the systems still refining: the start of a deflated
+// inner run (x = x0).
+template <bool CUBE>
+__global__ __launch_bounds__(256) void k_bmatvec_start(const BatchSys* __restrict__ sys,
+                                                       const int* __restrict__ pre, int nsys,
+                                                       const int* __restrict__ flags) {
+  const int gb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int s = find_sys(pre, nsys, gb);
+  const BatchSys& S = sys[s];
+  MatvecArgs<float> a;
+  a.u = S.x;
+  a.y = S.q;
+  a.Ee = S.E32;
+  a.mask = S.mask;
+  a.tileflags = S.tileflags;
+  a.g = S.g;
+  a.mc = S.mc32;
+  a.zchunk = S.zchunk;
+  a.tiles_x = S.tiles_x;
+  a.ntiles = S.ntiles;
+  a.partials = nullptr;
+  a.done = flags + s * kNumFlags + F_ODONE;
   a.zero_vec = nullptr;
   a.zero_n = 0;
   matvec_tile<float, CUBE>(a, gb - pre[s], gb);
@@ -158,6 +216,52 @@ __global__ __launch_bounds__(256) void k_binit(const BatchSys* __restrict__ sys,
       S.r[i] = bb;
       S.x[i] = 0.f;
       acc += (double)bb * S.invd32[i] * bb;
+    }
+  const double bs = block_reduce_sum256(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// Start of a deflated inner run: x = 0 (a deflated system's x0 = W mu is
+// written after this by the prolongation).
+__global__ __launch_bounds__(256) void k_bzero_x(const BatchSys* __restrict__ sys,
+                                                 const int* __restrict__ vpre, int nsys,
+                                                 const int* __restrict__ flags) {
+  const VecShare v = vec_share(vpre, nsys);
+  if (flags[v.s * kNumFlags + F_ODONE]) return;
+  const BatchSys& S = sys[v.s];
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < S.ndof; i += (int64_t)v.nb * 256)
+    S.x[i] = 0.f;
+}
+
+// Start of a deflated inner run: r = b - q (q = K x0), partials of r^T M r,
+// in k_binit's order (a system without groups has q = 0 and gets k_binit's
+// bits).
+__global__ __launch_bounds__(256) void k_binit_q(const BatchSys* __restrict__ sys,
+                                                 const int* __restrict__ vpre, int nsys,
+                                                 const int* __restrict__ flags,
+                                                 double* __restrict__ part) {
+  const VecShare v = vec_share(vpre, nsys);
+  __shared__ double red[4];
+  if (flags[v.s * kNumFlags + F_ODONE]) return;
+  const BatchSys& S = sys[v.s];
+  const int64_t nv = S.ndof / 4;
+  const float4* bv = reinterpret_cast<const float4*>(S.b);
+  const float4* qv = reinterpret_cast<const float4*>(S.q);
+  const float4* dv = reinterpret_cast<const float4*>(S.invd32);
+  float4* rv = reinterpret_cast<float4*>(S.r);
+  double acc = 0.0;
+  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < nv; i += (int64_t)v.nb * 256) {
+    const float4 bb = bv[i], qq = qv[i], dd = dv[i];
+    const float4 rr = make_float4(bb.x - qq.x, bb.y - qq.y, bb.z - qq.z, bb.w - qq.w);
+    rv[i] = rr;
+    acc += (double)rr.x * dd.x * rr.x + (double)rr.y * dd.y * rr.y +
+           (double)rr.z * dd.z * rr.z + (double)rr.w * dd.w * rr.w;
+  }
+  if (v.lb == 0)
+    for (int64_t i = nv * 4 + threadIdx.x; i < S.ndof; i += 256) {
+      const float ri = S.b[i] - S.q[i];
+      S.r[i] = ri;
+      acc += (double)ri * S.invd32[i] * ri;
     }
   const double bs = block_reduce_sum256(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
@@ -257,17 +361,11 @@ __global__ __launch_bounds__(256) void k_bscalars(const int* __restrict__ vpre,
 
 // x += alpha p (unless FIRST), p = invd o r + beta p.  FIRST: the start of an
 // inner run, p = invd o r, or with F_RESUME p = invd o r + (rz / rz_prev) p
-// (x is 0 then).
+// (x is 0 then).  Also the direction update of the systems without groups in
+// a batch that has deflated ones, so that they take the same bits.
 template <bool FIRST>
-__global__ __launch_bounds__(256) void k_bdir(const BatchSys* __restrict__ sys,
-                                              const int* __restrict__ vpre, int nsys,
-                                              const double* __restrict__ scal,
-                                              const int* __restrict__ flags) {
-  const VecShare v = vec_share(vpre, nsys);
-  if (flags[v.s * kNumFlags + F_DONE]) return;
-  const BatchSys& S = sys[v.s];
-  const double* sc = scal + v.s * kNumScalars;
-  const bool resume = FIRST && flags[v.s * kNumFlags + F_RESUME];
+__device__ __forceinline__ void bdir_plain(const BatchSys& S, const VecShare& v,
+                                           const double* sc, bool resume) {
   const float al = FIRST ? 0.f : (float)sc[S_ALPHA];
   const float be = FIRST ? (resume ? (float)(sc[S_RZ] / sc[S_RZPREV]) : 0.f) : (float)sc[S_BETA];
   const int64_t nv = S.ndof / 4;
@@ -300,6 +398,91 @@ __global__ __launch_bounds__(256) void k_bdir(const BatchSys* __restrict__ sys,
     }
 }
 
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_bdir(const BatchSys* __restrict__ sys,
+                                              const int* __restrict__ vpre, int nsys,
+                                              const double* __restrict__ scal,
+                                              const int* __restrict__ flags) {
+  const VecShare v = vec_share(vpre, nsys);
+  if (flags[v.s * kNumFlags + F_DONE]) return;
+  bdir_plain<FIRST>(sys[v.s], v, scal + v.s * kNumScalars,
+                    FIRST && flags[v.s * kNumFlags + F_RESUME]);
+}
+
+// The direction update of deflated systems: x += alpha p (unless FIRST),
+// p = M r + beta p - W mu (FIRST: beta as in k_bdir), formed in FP64 as the
+// single-system solver does (z and W mu cancel in near-void groups).  A block
+// works on one system; W mu of group q at position X is a_q + w_q x X with
+// w = (mu3, mu4, mu5), a = (mu0, mu1, mu2) - w x c_q, staged in LDS.
+constexpr int kBLdsGroups = 512;
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_bdir_defl(const BatchSys* __restrict__ sys,
+                                                   const int* __restrict__ vpre, int nsys,
+                                                   const double* __restrict__ scal,
+                                                   const int* __restrict__ flags) {
+  const VecShare v = vec_share(vpre, nsys);
+  if (flags[v.s * kNumFlags + F_DONE]) return;
+  const BatchSys& S = sys[v.s];
+  const double* sc = scal + v.s * kNumScalars;
+  const bool resume = FIRST && flags[v.s * kNumFlags + F_RESUME];
+  if (S.ngroups == 0) {
+    bdir_plain<FIRST>(S, v, sc, resume);
+    return;
+  }
+  __shared__ double aw[kBLdsGroups * 6];
+  const bool lds = S.ngroups <= kBLdsGroups;
+  auto coeffs = [&](int q, double* e) {
+    const double* m6 = S.cmu + 6 * q;
+    const double wx = m6[3], wy = m6[4], wz = m6[5];
+    const double cx = S.centers[3 * q], cy = S.centers[3 * q + 1], cz = S.centers[3 * q + 2];
+    e[0] = m6[0] - (wy * cz - wz * cy);
+    e[1] = m6[1] - (wz * cx - wx * cz);
+    e[2] = m6[2] - (wx * cy - wy * cx);
+    e[3] = wx;
+    e[4] = wy;
+    e[5] = wz;
+  };
+  if (lds) {
+    for (int q = threadIdx.x; q < S.ngroups; q += blockDim.x) coeffs(q, aw + 6 * q);
+    __syncthreads();
+  }
+  auto wmu3 = [&](int64_t n, double* w) {
+    int ix, iy, iz;
+    node_coords(n, S.g, ix, iy, iz);
+    const int q = S.node_group[n];
+    double e[6];
+    if (lds) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) e[k] = aw[6 * q + k];
+    } else {
+      coeffs(q, e);
+    }
+    const double X = ix * S.g.hx, Y = iy * S.g.hy, Z = iz * S.g.hz;
+    w[0] = e[0] + e[4] * Z - e[5] * Y;
+    w[1] = e[1] + e[5] * X - e[3] * Z;
+    w[2] = e[2] + e[3] * Y - e[4] * X;
+  };
+  const double al = FIRST ? 0.0 : sc[S_ALPHA];
+  const double be = FIRST ? (resume ? sc[S_RZ] / sc[S_RZPREV] : 0.0) : sc[S_BETA];
+  const bool read_p = !FIRST || resume;
+  auto dof = [&](int64_t d, int64_t node, const double* w3, uint32_t m) {
+    const int c = (int)(d - 3 * node);
+    const float pd = read_p ? S.p[d] : 0.f;
+    if (!FIRST) S.x[d] = (float)((double)S.x[d] + al * (double)pd);
+    const double wm = ((m >> c) & 1u) ? 0.0 : w3[c];
+    S.p[d] = (float)((double)S.invd32[d] * (double)S.r[d] + be * (double)pd - wm);
+  };
+  // one node (three DOFs) per thread: W mu once per node
+  const int64_t nn = S.ndof / 3;
+  for (int64_t nd = (int64_t)v.lb * 256 + threadIdx.x; nd < nn; nd += (int64_t)v.nb * 256) {
+    double w3[3];
+    wmu3(nd, w3);
+    const uint32_t m = S.mask[nd];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dof(3 * nd + c, nd, w3, m);
+  }
+}
+
 // The x += alpha p that the stopping iteration leaves pending (converged
 // systems that iterated in this inner solve only).
 __global__ __launch_bounds__(256) void k_bfinal(const BatchSys* __restrict__ sys,
@@ -313,6 +496,173 @@ __global__ __launch_bounds__(256) void k_bfinal(const BatchSys* __restrict__ sys
   const float al = (float)scal[v.s * kNumScalars + S_ALPHA];
   for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < S.ndof; i += (int64_t)v.nb * 256)
     S.x[i] += al * S.p[i];
+}
+
+// ---- deflation ----------------------------------------------------------------
+// Chunk partials of W^T K v over a deflated system's special elements, v =
+// M r + beta p (beta: the iteration's, or at the start of an inner run the
+// resumed direction's rz / rz_prev, 0 when it restarts): one block per special
+// block of each deflated system (prefix sums rpre), the single-system
+// restriction's staging and element arithmetic.
+template <bool CUBE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_brestrict(
+    const BatchSys* __restrict__ sys, const int* __restrict__ rpre, int nsys,
+    const double* __restrict__ scal, const int* __restrict__ flags, bool start) {
+  constexpr int kStage = (3 * kSpecialBlockNodes + 255) / 256;
+  __shared__ float vs[3 * kSpecialBlockNodes];
+  __shared__ double csh[kSpecialBlockChunks][8 * 3];
+  const int gb = blockIdx.x;
+  const int s = find_sys(rpre, nsys, gb);
+  const int* fl = flags + s * kNumFlags;
+  if (fl[F_DONE]) return;
+  const BatchSys& S = sys[s];
+  const double* sc = scal + s * kNumScalars;
+  const double beta = start ? (fl[F_RESUME] ? sc[S_RZ] / sc[S_RZPREV] : 0.0) : sc[S_BETA];
+  const SpecialBlock sb = S.sblocks[gb - rpre[s]];
+  const int w = threadIdx.x >> 6;
+  const int wave = sb.chunk0 + min(w, sb.nch - 1);
+  const int lane = threadIdx.x & 63;
+  const int ch_start = S.chunks[wave].start, ch_count = S.chunks[wave].count;
+  const int ch_ng = S.chunks[wave].ng;
+  const bool active = w < sb.nch && lane < ch_count;
+  SpecialElem se{0, 0, 0};
+  uint4 li = make_uint4(0, 0, 0, 0);
+  float Ev = 0.f;
+  if (active) {
+    se = S.sinfo[ch_start + lane];
+    li = S.sidx[ch_start + lane];
+    Ev = S.E32[se.e];
+  }
+  const int nst = 3 * sb.nnodes;
+  uint32_t dof[kStage];
+#pragma unroll
+  for (int t = 0; t < kStage; ++t) {
+    const int i = min((int)threadIdx.x + 256 * t, nst - 1);
+    const int q = i / 3;
+    dof[t] = 3u * (uint32_t)S.sbnodes[sb.node0 + q] + (uint32_t)(i - 3 * q);
+  }
+  const bool has_p = beta != 0.0;
+  float ra[kStage], da[kStage], pa[kStage];
+#pragma unroll
+  for (int t = 0; t < kStage; ++t) {
+    ra[t] = S.r[dof[t]];
+    da[t] = S.invd32[dof[t]];
+    pa[t] = has_p ? S.p[dof[t]] : 0.f;
+  }
+  if (threadIdx.x < kSpecialBlockChunks * 24) {
+    const int cw = threadIdx.x / 24, r = threadIdx.x - 24 * cw;
+    if (cw < sb.nch) {
+      const SpecialChunk* c = S.chunks + sb.chunk0 + cw;
+      const int sl = r / 3;
+      if (sl < c->ng) csh[cw][r] = S.centers[3 * c->groups[sl] + (r - 3 * sl)];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kStage; ++t) {
+    const int i = threadIdx.x + 256 * t;
+    if (i < nst) vs[i] = (float)((double)(da[t] * ra[t]) + beta * (double)pa[t]);
+  }
+  __syncthreads();
+  if (w >= sb.nch) return;
+  float f[8][3];
+  uint64_t info = 0;
+  int ix = 0, iy = 0, iz = 0;
+  if (active) {
+    const uint32_t lw[4] = {li.x, li.y, li.z, li.w};
+    info = se.info;
+    elem_coords(se.e, S.g, ix, iy, iz);
+    float u[8][3];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int l = (int)((lw[j >> 1] >> (16 * (j & 1))) & 0xffffu);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        u[j][c] = ((info >> (24 + 3 * j + c)) & 1) ? 0.f : vs[3 * l + c];
+    }
+    if (CUBE)
+      element_apply_cube<float>(u, Ev, S.mc32, f);
+    else
+      element_apply<float>(u, Ev, S.mc32, f);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if ((info >> (24 + 3 * j + c)) & 1) f[j][c] = 0.f;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) f[j][0] = f[j][1] = f[j][2] = 0.f;
+  }
+  const double* cen = csh[w];
+  for (int sl = 0; sl < ch_ng; ++sl) {
+    const double cx = cen[3 * sl], cy = cen[3 * sl + 1], cz = cen[3 * sl + 2];
+    const float x0 = (float)(ix * S.g.hx - cx), y0 = (float)(iy * S.g.hy - cy),
+                z0 = (float)(iz * S.g.hz - cz);
+    const float hx = (float)S.g.hx, hy = (float)S.g.hy, hz = (float)S.g.hz;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (!active || (int)((info >> (3 * j)) & 7) != sl) continue;
+      const float rx = corner_a(j) ? x0 + hx : x0;
+      const float ry = corner_b(j) ? y0 + hy : y0;
+      const float rz = corner_c(j) ? z0 + hz : z0;
+      const float fx = f[j][0], fy = f[j][1], fz = f[j][2];
+      acc[0] += (double)fx;
+      acc[1] += (double)fy;
+      acc[2] += (double)fz;
+      acc[3] += (double)(ry * fz - rz * fy);
+      acc[4] += (double)(rz * fx - rx * fz);
+      acc[5] += (double)(rx * fy - ry * fx);
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double tot = wave_reduce_sum(acc[j]);
+      if (lane == 0) S.cpart[(int64_t)wave * 48 + 6 * sl + j] = tot;
+    }
+  }
+}
+
+// t = per-group sums of the chunk partials: one wave per group of each
+// deflated system (prefix sums gpre over groups), fixed order.
+__global__ __launch_bounds__(256) void k_bgather(const BatchSys* __restrict__ sys,
+                                                 const int* __restrict__ gpre, int nsys,
+                                                 const int* __restrict__ flags) {
+  const int G = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (G >= gpre[nsys]) return;
+  const int s = find_sys(gpre, nsys, G);
+  if (flags[s * kNumFlags + F_DONE]) return;
+  const BatchSys& S = sys[s];
+  const int g = G - gpre[s];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = S.gptr[g] + lane; k < S.gptr[g + 1]; k += 64) {
+    const double* c = S.cpart + (int64_t)S.gidx[k] * 6;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[j] += c[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const double t = wave_reduce_sum(acc[j]);
+    if (lane == 0) S.ct[6 * g + j] = t;
+  }
+}
+
+// mu = E^-1 t per deflated system: one wave per row (prefix sums cpre over
+// the systems' 6 G rows).
+__global__ __launch_bounds__(256) void k_bgemv(const BatchSys* __restrict__ sys,
+                                               const int* __restrict__ cpre, int nsys,
+                                               const int* __restrict__ flags) {
+  const int R = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (R >= cpre[nsys]) return;
+  const int s = find_sys(cpre, nsys, R);
+  if (flags[s * kNumFlags + F_DONE]) return;
+  const BatchSys& S = sys[s];
+  const int row = R - cpre[s];
+  const double* a = S.Einv + (int64_t)row * S.ld;
+  double acc = 0.0;
+  for (int j = lane; j < S.nc; j += 64) acc += a[j] * S.ct[j];
+  acc = wave_reduce_sum(acc);
+  if (lane == 0) S.cmu[row] = acc;
 }
 
 // ---- FP64 outer refinement --------------------------------------------------
@@ -432,6 +782,11 @@ struct Batch {
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   hipGraphExec_t graph = nullptr;
   int zchunk = 16;
+  // deflation: prefix sums over the deflated systems' special blocks, groups
+  // and coarse rows (host and device)
+  bool defl = false;
+  std::vector<int> h_rpre, h_gpre, h_cpre;
+  int *rpre = nullptr, *gpre = nullptr, *cpre = nullptr;
 };
 
 }  // namespace pyto
@@ -484,6 +839,26 @@ static void launch_bmatvec_inner(Batch* b, hipStream_t s) {
                        b->nsys, (const int*)b->flags, b->part_mv);
 }
 
+// mu = E^-1 W^T K (M r + beta p) for every deflated system still iterating:
+// special restriction, per-group gather, coarse GEMV.
+static void launch_bcoarse(Batch* b, hipStream_t s, bool start) {
+  const int nr = b->h_rpre[b->nsys], ng = b->h_gpre[b->nsys], nc = b->h_cpre[b->nsys];
+  if (nr > 0) {
+    if (b->cube)
+      hipLaunchKernelGGL(k_brestrict<true>, dim3(nr), dim3(256), 0, s, b->sys, b->rpre, b->nsys,
+                         (const double*)b->scal, (const int*)b->flags, start);
+    else
+      hipLaunchKernelGGL(k_brestrict<false>, dim3(nr), dim3(256), 0, s, b->sys, b->rpre,
+                         b->nsys, (const double*)b->scal, (const int*)b->flags, start);
+  }
+  if (ng > 0)
+    hipLaunchKernelGGL(k_bgather, dim3((ng * 64 + 255) / 256), dim3(256), 0, s, b->sys, b->gpre,
+                       b->nsys, (const int*)b->flags);
+  if (nc > 0)
+    hipLaunchKernelGGL(k_bgemv, dim3((nc * 64 + 255) / 256), dim3(256), 0, s, b->sys, b->cpre,
+                       b->nsys, (const int*)b->flags);
+}
+
 static int batch_iteration(Batch* b, hipStream_t s) {
   const int nv = b->h_vpre[b->nsys];
   launch_bmatvec_inner(b, s);
@@ -492,8 +867,14 @@ static int batch_iteration(Batch* b, hipStream_t s) {
                      b->part_v);
   hipLaunchKernelGGL(k_bscalars, dim3(b->nsys), dim3(256), 0, s, b->vpre, b->mpre, b->nsys,
                      b->scal, b->flags, (const double*)b->part_mv, (const double*)b->part_v);
-  hipLaunchKernelGGL(k_bdir<false>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, b->nsys,
-                     (const double*)b->scal, (const int*)b->flags);
+  if (b->defl) {
+    launch_bcoarse(b, s, false);
+    hipLaunchKernelGGL(k_bdir_defl<false>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, b->nsys,
+                       (const double*)b->scal, (const int*)b->flags);
+  } else {
+    hipLaunchKernelGGL(k_bdir<false>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, b->nsys,
+                       (const double*)b->scal, (const int*)b->flags);
+  }
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -586,8 +967,62 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
     }
     S.r64 = f64[0];
     S.q64 = f64[1];
+    if (h->ngroups > 0) {
+      if (!h->sblocks || !h->Einv || h->nsblocks <= 0) {
+        set_error("system %d: the handle's deflation groups have no coarse space", i);
+        return fail(PYTO_ESTATE);
+      }
+      S.ngroups = h->ngroups;
+      S.nc = h->nc;
+      S.ld = (h->nc + 63) / 64 * 64;
+      S.node_group = h->node_group;
+      S.centers = h->centers;
+      S.sinfo = h->sinfo;
+      S.sidx = reinterpret_cast<const uint4*>(h->sidx);
+      S.chunks = h->chunks;
+      S.sblocks = h->sblocks;
+      S.sbnodes = h->sbnodes;
+      S.gptr = h->gptr;
+      S.gidx = h->gidx;
+      S.Einv = h->Einv;
+      const size_t sizes[3] = {(size_t)h->nchunks * 48, (size_t)S.ld, (size_t)S.ld};
+      double* cb[3];
+      for (int k = 0; k < 3; ++k) {
+        void* p = nullptr;
+        if (hipMalloc(&p, sizes[k] * sizeof(double)) != hipSuccess) {
+          set_error("device allocation of %zu bytes failed", sizes[k] * sizeof(double));
+          return fail(PYTO_ENOMEM);
+        }
+        b->bufs.push_back(p);
+        hipMemsetAsync(p, 0, sizes[k] * sizeof(double), s);
+        cb[k] = (double*)p;
+      }
+      S.cpart = cb[0];
+      S.ct = cb[1];
+      S.cmu = cb[2];
+      b->defl = true;
+    }
     b->hs.push_back(S);
   }
+  // prefix sums of the deflated systems' special blocks, groups, coarse rows
+  b->h_rpre.assign(nsys + 1, 0);
+  b->h_gpre.assign(nsys + 1, 0);
+  b->h_cpre.assign(nsys + 1, 0);
+  for (int i = 0; i < nsys; ++i) {
+    const bool d = b->hs[i].ngroups > 0;
+    b->h_rpre[i + 1] = b->h_rpre[i] + (d ? handles[i]->nsblocks : 0);
+    b->h_gpre[i + 1] = b->h_gpre[i] + b->hs[i].ngroups;
+    b->h_cpre[i + 1] = b->h_cpre[i] + b->hs[i].nc;
+  }
+  if (hipMalloc((void**)&b->rpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&b->gpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&b->cpre, (nsys + 1) * sizeof(int)) != hipSuccess) {
+    set_error("batch allocation failed");
+    return fail(PYTO_ENOMEM);
+  }
+  hipMemcpyAsync(b->rpre, b->h_rpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(b->gpre, b->h_gpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(b->cpre, b->h_cpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
   if (hipMalloc((void**)&b->sys, nsys * sizeof(BatchSys)) != hipSuccess ||
       hipMalloc((void**)&b->mpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&b->vpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
@@ -622,7 +1057,8 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
 int pyto_batch_destroy(pyto_batch* b) {
   if (!b) return PYTO_OK;
   for (void* p : b->bufs) hipFree(p);
-  void* ptrs[] = {b->sys, b->mpre, b->vpre, b->scal, b->flags, b->part_mv, b->part_v};
+  void* ptrs[] = {b->sys, b->mpre, b->vpre, b->scal, b->flags, b->part_mv, b->part_v,
+                  b->rpre, b->gpre, b->cpre};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (b->graph) hipGraphExecDestroy(b->graph);
@@ -679,6 +1115,8 @@ int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, do
   for (int i = 0; i < b->nsys; ++i) {
     PYTO_REQUIRE(f[i] && u[i], PYTO_EINVAL, "null load or solution of system %d", i);
     PYTO_REQUIRE(b->handles[i]->density_set, PYTO_ESTATE, "system %d: density not set", i);
+    PYTO_REQUIRE(b->hs[i].ngroups == 0 || b->handles[i]->coarse_ready, PYTO_ESTATE,
+                 "system %d: the coarse matrix of its groups is not factorised", i);
     b->hs[i].f = f[i];
     b->hs[i].u = u[i];
   }
@@ -712,12 +1150,41 @@ int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, do
       if (!b->host_flags[i * kNumFlags + F_ODONE]) outer_iters[i]++;
     hipLaunchKernelGGL(k_bscale, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
                        (const double*)b->scal, (const int*)b->flags);
-    hipLaunchKernelGGL(k_binit, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
-                       (const int*)b->flags, b->part_v);
-    hipLaunchKernelGGL(k_bsetup, dim3(nsys), dim3(256), 0, s, b->vpre, nsys, b->scal, b->flags,
-                       (const double*)b->part_v, rtol, inner_rtol);
-    hipLaunchKernelGGL(k_bdir<true>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
-                       (const double*)b->scal, (const int*)b->flags);
+    if (b->defl) {
+      // deflated start: x0 = W E^-1 W^T b per deflated system (x0 = 0 for the
+      // others), r = b - K x0, then the projected first direction
+      hipLaunchKernelGGL(k_bzero_x, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                         (const int*)b->flags);
+      for (int i = 0; i < nsys; ++i) {
+        const BatchSys& S = b->hs[i];
+        if (S.ngroups == 0 || b->host_flags[i * kNumFlags + F_ODONE]) continue;
+        Handle* h = b->handles[i];
+        PYTO_TRY(launch_restrict_full<float>(h, S.b, S.ct, s));
+        PYTO_TRY(coarse_apply(h, S.ct, nullptr, nullptr, S.cmu, nullptr, s, false));
+        PYTO_TRY(launch_prolong<float, false>(h, S.x, nullptr, S.cmu, nullptr, true, nullptr, s,
+                                              nullptr, 0));
+      }
+      if (b->cube)
+        hipLaunchKernelGGL(k_bmatvec_start<true>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
+                           nsys, (const int*)b->flags);
+      else
+        hipLaunchKernelGGL(k_bmatvec_start<false>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
+                           nsys, (const int*)b->flags);
+      hipLaunchKernelGGL(k_binit_q, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                         (const int*)b->flags, b->part_v);
+      hipLaunchKernelGGL(k_bsetup, dim3(nsys), dim3(256), 0, s, b->vpre, nsys, b->scal,
+                         b->flags, (const double*)b->part_v, rtol, inner_rtol);
+      launch_bcoarse(b, s, true);
+      hipLaunchKernelGGL(k_bdir_defl<true>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                         (const double*)b->scal, (const int*)b->flags);
+    } else {
+      hipLaunchKernelGGL(k_binit, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                         (const int*)b->flags, b->part_v);
+      hipLaunchKernelGGL(k_bsetup, dim3(nsys), dim3(256), 0, s, b->vpre, nsys, b->scal,
+                         b->flags, (const double*)b->part_v, rtol, inner_rtol);
+      hipLaunchKernelGGL(k_bdir<true>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+                         (const double*)b->scal, (const int*)b->flags);
+    }
     PYTO_CHECK_HIP(hipGetLastError());
     // inner iterations: replays of the captured batch of kGraphBatch
     if (!b->graph) {
