@@ -174,9 +174,10 @@ def traffic_bytes():
     return None
 
 
-def batch_line(torch, dev, inner_rtol=None, boxes=(8, 4, 2)):
+def batch_line(torch, dev, inner_rtol=None, boxes=12):
     """The 32-design batch; ``boxes``: the deflation box partition of every
-    design (None: Jacobi PCG only)."""
+    design, or a number (a box edge in elements: each design gets
+    max(1, round(n / edge)) boxes per axis), or None (Jacobi PCG only)."""
     from pyto_amd import deflation, problems
     from pyto_amd.batch import BatchSolver
     from pyto_amd.operator import GridOperator
@@ -188,7 +189,9 @@ def batch_line(torch, dev, inner_rtol=None, boxes=(8, 4, 2)):
         if boxes is not None:
             op.set_coarse_shift(0.0, 0.0)
             ds = deflation.DeflationSolver()
-            ds.create_box_groups(d["mesh"], boxes)
+            nb = boxes if isinstance(boxes, (tuple, list)) else \
+                tuple(max(1, int(round(n / boxes))) for n in d["grid"])
+            ds.create_box_groups(d["mesh"], nb)
             ds.attach(op)
         op.set_density(torch.from_numpy(d["x"]).to(dev))
         ops.append(op)
@@ -201,14 +204,24 @@ def batch_line(torch, dev, inner_rtol=None, boxes=(8, 4, 2)):
     _, stats = bs.solve(rtol=1e-8, **kw)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
+    # the per-design-update set-up: E_e, the Jacobi diagonal and (deflated)
+    # the Galerkin coarse matrix and its factor, for every design
+    xs = [op.x.clone() for op in ops]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for op, x in zip(ops, xs):
+        op.set_density(x)
+    torch.cuda.synchronize()
+    t_set = time.perf_counter() - t0
     it = [s["iterations"] for s in stats]
     ndof = sum(3 * d["mesh"].num_nodes * len(d["loads"]) for d in designs)
     out = {"designs": len(designs), "systems": bs.nsys, "dof_total": ndof,
-           "seconds": t, "systems_per_s": bs.nsys / t,
+           "seconds": t, "systems_per_s": bs.nsys / t, "density_update_seconds": t_set,
            "all_converged": all(s["converged"] for s in stats),
            "inner_iterations_max": max(it), "inner_iterations_median": float(np.median(it)),
            "outer_iterations_max": max(s["outer_iterations"] for s in stats),
-           "deflation_boxes": None if boxes is None else list(boxes),
+           "deflation_boxes": (None if boxes is None else list(boxes)
+                               if isinstance(boxes, (tuple, list)) else "edge %g" % boxes),
            "zchunk": tune["zchunk"]}
     bs.close()
     for op in ops:

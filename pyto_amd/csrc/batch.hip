@@ -23,26 +23,16 @@
 //
 // Deflation: a system whose handle has deflation groups runs the deflated PCG
 // of the single-system solver inside the same launches.  Each inner run starts
-// from x0 = W E^-1 W^T b (per system, with the handle's full restriction and
-// coarse solve), and every direction is projected, p = M r + beta p -
+// from x0 = W E^-1 W^T b, and every direction is projected, p = M r + beta p -
 // W E^-1 W^T K (M r + beta p), with W^T K v restricted over the handle's
-// special elements; the restriction, the per-group gather and the coarse GEMV
-// are one launch each for all deflated systems of the batch.
+// special elements.  The restrictions, the per-group gathers, the coarse GEMV
+// and the prolongation are one launch each for all deflated systems.
 #include <algorithm>
 #include <vector>
 
 #include "matvec.hpp"
 
 namespace pyto {
-
-// coarse.hip
-template <typename T>
-int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s);
-int coarse_apply(Handle* h, const double* t1, const double* t2, const double* beta,
-                 double* mu, const int* done, hipStream_t s, bool refine);
-template <typename T, bool UPDATE>
-int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
-                   bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n);
 
 constexpr int kBatchGraphIters = 16;   // inner iterations per captured graph
 
@@ -76,6 +66,14 @@ struct BatchSys {
   const int32_t* gidx;
   const double* Einv;
   double *cpart, *ct, *cmu;
+  // the full restriction W^T v of the start of an inner run: node blocks of
+  // one group each, their partials, per-group lists of them
+  const int32_t* gnodes;
+  const int2* nchunk;
+  const int32_t* nchunk_grp;
+  const int32_t* nptr;
+  const int32_t* nidx;
+  double* npart;
 };
 
 // Outer-refinement flags: the system has met rtol (or has a zero load); the
@@ -221,16 +219,35 @@ __global__ __launch_bounds__(256) void k_binit(const BatchSys* __restrict__ sys,
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
-// Start of a deflated inner run: x = 0 (a deflated system's x0 = W mu is
-// written after this by the prolongation).
-__global__ __launch_bounds__(256) void k_bzero_x(const BatchSys* __restrict__ sys,
-                                                 const int* __restrict__ vpre, int nsys,
-                                                 const int* __restrict__ flags) {
+// Start of an inner run in a batch with deflated systems: x0 = W mu (fixed
+// DOFs 0) for a deflated system, with mu = E^-1 W^T b from k_bgemv<true>;
+// x0 = 0 for the others.
+__global__ __launch_bounds__(256) void k_bstart_x(const BatchSys* __restrict__ sys,
+                                                  const int* __restrict__ vpre, int nsys,
+                                                  const int* __restrict__ flags) {
   const VecShare v = vec_share(vpre, nsys);
   if (flags[v.s * kNumFlags + F_ODONE]) return;
   const BatchSys& S = sys[v.s];
-  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < S.ndof; i += (int64_t)v.nb * 256)
-    S.x[i] = 0.f;
+  if (S.ngroups == 0) {
+    for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < S.ndof; i += (int64_t)v.nb * 256)
+      S.x[i] = 0.f;
+    return;
+  }
+  const int64_t nn = S.ndof / 3;
+  for (int64_t n = (int64_t)v.lb * 256 + threadIdx.x; n < nn; n += (int64_t)v.nb * 256) {
+    int ix, iy, iz;
+    node_coords(n, S.g, ix, iy, iz);
+    const int gid = S.node_group[n];
+    const double rx = ix * S.g.hx - S.centers[3 * gid];
+    const double ry = iy * S.g.hy - S.centers[3 * gid + 1];
+    const double rz = iz * S.g.hz - S.centers[3 * gid + 2];
+    const double* m6 = S.cmu + 6 * gid;
+    const double wm[3] = {m6[0] + rz * m6[4] - ry * m6[5], m6[1] - rz * m6[3] + rx * m6[5],
+                          m6[2] + ry * m6[3] - rx * m6[4]};
+    const uint8_t m = S.mask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) S.x[3 * n + c] = ((m >> c) & 1) ? 0.f : (float)wm[c];
+  }
 }
 
 // Start of a deflated inner run: r = b - q (q = K x0), partials of r^T M r,
@@ -621,8 +638,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
   }
 }
 
-// t = per-group sums of the chunk partials: one wave per group of each
-// deflated system (prefix sums gpre over groups), fixed order.
+// Block partials of t = W^T b over all nodes (fixed DOFs excluded) at the
+// start of an inner run: one block per node block of one group of each
+// deflated system (prefix sums npre), the single-system full restriction's
+// arithmetic and order.
+__global__ __launch_bounds__(256) void k_brestrict_full(const BatchSys* __restrict__ sys,
+                                                        const int* __restrict__ npre, int nsys,
+                                                        const int* __restrict__ flags) {
+  __shared__ double red[4][6];
+  const int s = find_sys(npre, nsys, blockIdx.x);
+  if (flags[s * kNumFlags + F_ODONE]) return;
+  const BatchSys& S = sys[s];
+  const int blk = blockIdx.x - npre[s];
+  const int2 ch = S.nchunk[blk];
+  const int gid = S.nchunk_grp[blk];
+  double c[6] = {0, 0, 0, 0, 0, 0};
+  if ((int)threadIdx.x < ch.y) {
+    const int64_t n = S.gnodes[ch.x + threadIdx.x];
+    int ix, iy, iz;
+    node_coords(n, S.g, ix, iy, iz);
+    const double rx = ix * S.g.hx - S.centers[3 * gid];
+    const double ry = iy * S.g.hy - S.centers[3 * gid + 1];
+    const double rz = iz * S.g.hz - S.centers[3 * gid + 2];
+    const uint8_t m = S.mask[n];
+    const double vx = (m & 1) ? 0.0 : (double)S.b[3 * n];
+    const double vy = (m & 2) ? 0.0 : (double)S.b[3 * n + 1];
+    const double vz = (m & 4) ? 0.0 : (double)S.b[3 * n + 2];
+    c[0] = vx; c[1] = vy; c[2] = vz;
+    c[3] = -rz * vy + ry * vz;
+    c[4] = rz * vx - rx * vz;
+    c[5] = -ry * vx + rx * vy;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const double sj = wave_reduce_sum(c[j]);
+    if (lane == 0) red[w][j] = sj;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int j = threadIdx.x;
+    S.npart[(int64_t)blk * 48 + j] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+  }
+}
+
+// t = per-group sums of the partials: one wave per group of each deflated
+// system (prefix sums gpre over groups), fixed order.  START: the full
+// restriction's node-block partials of a new inner run (systems still
+// refining); otherwise the special restriction's chunk partials (systems
+// still iterating).
+template <bool START>
 __global__ __launch_bounds__(256) void k_bgather(const BatchSys* __restrict__ sys,
                                                  const int* __restrict__ gpre, int nsys,
                                                  const int* __restrict__ flags) {
@@ -630,12 +695,15 @@ __global__ __launch_bounds__(256) void k_bgather(const BatchSys* __restrict__ sy
   const int lane = threadIdx.x & 63;
   if (G >= gpre[nsys]) return;
   const int s = find_sys(gpre, nsys, G);
-  if (flags[s * kNumFlags + F_DONE]) return;
+  if (flags[s * kNumFlags + (START ? F_ODONE : F_DONE)]) return;
   const BatchSys& S = sys[s];
   const int g = G - gpre[s];
+  const int32_t* ptr = START ? S.nptr : S.gptr;
+  const int32_t* idx = START ? S.nidx : S.gidx;
+  const double* part = START ? S.npart : S.cpart;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = S.gptr[g] + lane; k < S.gptr[g + 1]; k += 64) {
-    const double* c = S.cpart + (int64_t)S.gidx[k] * 6;
+  for (int k = ptr[g] + lane; k < ptr[g + 1]; k += 64) {
+    const double* c = part + (int64_t)idx[k] * 6;
 #pragma unroll
     for (int j = 0; j < 6; ++j) acc[j] += c[j];
   }
@@ -647,7 +715,8 @@ __global__ __launch_bounds__(256) void k_bgather(const BatchSys* __restrict__ sy
 }
 
 // mu = E^-1 t per deflated system: one wave per row (prefix sums cpre over
-// the systems' 6 G rows).
+// the systems' 6 G rows).  START as in k_bgather.
+template <bool START>
 __global__ __launch_bounds__(256) void k_bgemv(const BatchSys* __restrict__ sys,
                                                const int* __restrict__ cpre, int nsys,
                                                const int* __restrict__ flags) {
@@ -655,7 +724,7 @@ __global__ __launch_bounds__(256) void k_bgemv(const BatchSys* __restrict__ sys,
   const int lane = threadIdx.x & 63;
   if (R >= cpre[nsys]) return;
   const int s = find_sys(cpre, nsys, R);
-  if (flags[s * kNumFlags + F_DONE]) return;
+  if (flags[s * kNumFlags + (START ? F_ODONE : F_DONE)]) return;
   const BatchSys& S = sys[s];
   const int row = R - cpre[s];
   const double* a = S.Einv + (int64_t)row * S.ld;
@@ -785,8 +854,8 @@ struct Batch {
   // deflation: prefix sums over the deflated systems' special blocks, groups
   // and coarse rows (host and device)
   bool defl = false;
-  std::vector<int> h_rpre, h_gpre, h_cpre;
-  int *rpre = nullptr, *gpre = nullptr, *cpre = nullptr;
+  std::vector<int> h_rpre, h_gpre, h_cpre, h_npre;
+  int *rpre = nullptr, *gpre = nullptr, *cpre = nullptr, *npre = nullptr;
 };
 
 }  // namespace pyto
@@ -852,11 +921,28 @@ static void launch_bcoarse(Batch* b, hipStream_t s, bool start) {
                          b->nsys, (const double*)b->scal, (const int*)b->flags, start);
   }
   if (ng > 0)
-    hipLaunchKernelGGL(k_bgather, dim3((ng * 64 + 255) / 256), dim3(256), 0, s, b->sys, b->gpre,
-                       b->nsys, (const int*)b->flags);
+    hipLaunchKernelGGL(k_bgather<false>, dim3((ng * 64 + 255) / 256), dim3(256), 0, s, b->sys,
+                       b->gpre, b->nsys, (const int*)b->flags);
   if (nc > 0)
-    hipLaunchKernelGGL(k_bgemv, dim3((nc * 64 + 255) / 256), dim3(256), 0, s, b->sys, b->cpre,
-                       b->nsys, (const int*)b->flags);
+    hipLaunchKernelGGL(k_bgemv<false>, dim3((nc * 64 + 255) / 256), dim3(256), 0, s, b->sys,
+                       b->cpre, b->nsys, (const int*)b->flags);
+}
+
+// x0 = W E^-1 W^T b for every deflated system still refining, x0 = 0 for the
+// others: full restriction, per-group gather, coarse GEMV, prolongation.
+static void launch_bstart_x(Batch* b, hipStream_t s) {
+  const int nn = b->h_npre[b->nsys], ng = b->h_gpre[b->nsys], nc = b->h_cpre[b->nsys];
+  if (nn > 0)
+    hipLaunchKernelGGL(k_brestrict_full, dim3(nn), dim3(256), 0, s, b->sys, b->npre, b->nsys,
+                       (const int*)b->flags);
+  if (ng > 0)
+    hipLaunchKernelGGL(k_bgather<true>, dim3((ng * 64 + 255) / 256), dim3(256), 0, s, b->sys,
+                       b->gpre, b->nsys, (const int*)b->flags);
+  if (nc > 0)
+    hipLaunchKernelGGL(k_bgemv<true>, dim3((nc * 64 + 255) / 256), dim3(256), 0, s, b->sys,
+                       b->cpre, b->nsys, (const int*)b->flags);
+  hipLaunchKernelGGL(k_bstart_x, dim3(b->h_vpre[b->nsys]), dim3(256), 0, s, b->sys, b->vpre,
+                     b->nsys, (const int*)b->flags);
 }
 
 static int batch_iteration(Batch* b, hipStream_t s) {
@@ -985,9 +1071,19 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
       S.gptr = h->gptr;
       S.gidx = h->gidx;
       S.Einv = h->Einv;
-      const size_t sizes[3] = {(size_t)h->nchunks * 48, (size_t)S.ld, (size_t)S.ld};
-      double* cb[3];
-      for (int k = 0; k < 3; ++k) {
+      if (!h->gnodes || h->nnchunks <= 0) {
+        set_error("system %d: the handle's deflation groups have no node blocks", i);
+        return fail(PYTO_ESTATE);
+      }
+      S.gnodes = h->gnodes;
+      S.nchunk = h->nchunk;
+      S.nchunk_grp = h->nchunk_grp;
+      S.nptr = h->nptr;
+      S.nidx = h->nidx;
+      const size_t sizes[4] = {(size_t)h->nchunks * 48, (size_t)S.ld, (size_t)S.ld,
+                               (size_t)h->nnchunks * 48};
+      double* cb[4];
+      for (int k = 0; k < 4; ++k) {
         void* p = nullptr;
         if (hipMalloc(&p, sizes[k] * sizeof(double)) != hipSuccess) {
           set_error("device allocation of %zu bytes failed", sizes[k] * sizeof(double));
@@ -1000,6 +1096,7 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
       S.cpart = cb[0];
       S.ct = cb[1];
       S.cmu = cb[2];
+      S.npart = cb[3];
       b->defl = true;
     }
     b->hs.push_back(S);
@@ -1008,21 +1105,25 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
   b->h_rpre.assign(nsys + 1, 0);
   b->h_gpre.assign(nsys + 1, 0);
   b->h_cpre.assign(nsys + 1, 0);
+  b->h_npre.assign(nsys + 1, 0);
   for (int i = 0; i < nsys; ++i) {
     const bool d = b->hs[i].ngroups > 0;
     b->h_rpre[i + 1] = b->h_rpre[i] + (d ? handles[i]->nsblocks : 0);
     b->h_gpre[i + 1] = b->h_gpre[i] + b->hs[i].ngroups;
     b->h_cpre[i + 1] = b->h_cpre[i] + b->hs[i].nc;
+    b->h_npre[i + 1] = b->h_npre[i] + (d ? handles[i]->nnchunks : 0);
   }
   if (hipMalloc((void**)&b->rpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&b->gpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&b->cpre, (nsys + 1) * sizeof(int)) != hipSuccess) {
+      hipMalloc((void**)&b->cpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&b->npre, (nsys + 1) * sizeof(int)) != hipSuccess) {
     set_error("batch allocation failed");
     return fail(PYTO_ENOMEM);
   }
   hipMemcpyAsync(b->rpre, b->h_rpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
   hipMemcpyAsync(b->gpre, b->h_gpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
   hipMemcpyAsync(b->cpre, b->h_cpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(b->npre, b->h_npre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
   if (hipMalloc((void**)&b->sys, nsys * sizeof(BatchSys)) != hipSuccess ||
       hipMalloc((void**)&b->mpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&b->vpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
@@ -1058,7 +1159,7 @@ int pyto_batch_destroy(pyto_batch* b) {
   if (!b) return PYTO_OK;
   for (void* p : b->bufs) hipFree(p);
   void* ptrs[] = {b->sys, b->mpre, b->vpre, b->scal, b->flags, b->part_mv, b->part_v,
-                  b->rpre, b->gpre, b->cpre};
+                  b->rpre, b->gpre, b->cpre, b->npre};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (b->graph) hipGraphExecDestroy(b->graph);
@@ -1115,7 +1216,12 @@ int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, do
   for (int i = 0; i < b->nsys; ++i) {
     PYTO_REQUIRE(f[i] && u[i], PYTO_EINVAL, "null load or solution of system %d", i);
     PYTO_REQUIRE(b->handles[i]->density_set, PYTO_ESTATE, "system %d: density not set", i);
-    PYTO_REQUIRE(b->hs[i].ngroups == 0 || b->handles[i]->coarse_ready, PYTO_ESTATE,
+    const Handle* h = b->handles[i];
+    PYTO_REQUIRE(h->ngroups == b->hs[i].ngroups && h->Einv == b->hs[i].Einv &&
+                     h->sblocks == b->hs[i].sblocks, PYTO_ESTATE,
+                 "system %d: its handle's deflation groups changed after the batch was "
+                 "created; create the batch again", i);
+    PYTO_REQUIRE(b->hs[i].ngroups == 0 || h->coarse_ready, PYTO_ESTATE,
                  "system %d: the coarse matrix of its groups is not factorised", i);
     b->hs[i].f = f[i];
     b->hs[i].u = u[i];
@@ -1153,17 +1259,7 @@ int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, do
     if (b->defl) {
       // deflated start: x0 = W E^-1 W^T b per deflated system (x0 = 0 for the
       // others), r = b - K x0, then the projected first direction
-      hipLaunchKernelGGL(k_bzero_x, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
-                         (const int*)b->flags);
-      for (int i = 0; i < nsys; ++i) {
-        const BatchSys& S = b->hs[i];
-        if (S.ngroups == 0 || b->host_flags[i * kNumFlags + F_ODONE]) continue;
-        Handle* h = b->handles[i];
-        PYTO_TRY(launch_restrict_full<float>(h, S.b, S.ct, s));
-        PYTO_TRY(coarse_apply(h, S.ct, nullptr, nullptr, S.cmu, nullptr, s, false));
-        PYTO_TRY(launch_prolong<float, false>(h, S.x, nullptr, S.cmu, nullptr, true, nullptr, s,
-                                              nullptr, 0));
-      }
+      launch_bstart_x(b, s);
       if (b->cube)
         hipLaunchKernelGGL(k_bmatvec_start<true>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
                            nsys, (const int*)b->flags);

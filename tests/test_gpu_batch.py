@@ -83,6 +83,21 @@ def test_deflated_batch_matches_single_solves(gpu):
             assert st["iterations"] < 0.6 * sp["iterations"]
         else:
             assert st["iterations"] == sp["iterations"]
+    # new densities keep the batch valid; new groups need a new batch
+    x_new = ops[0].x * 0.9
+    ops[0].set_density(x_new)
+    plain_ops[0].set_density(x_new)
+    us2, stats2 = bs.solve(rtol=1e-8)
+    assert all(st["converged"] for st in stats2)
+    res, eu, eC = _check_against_single(plain_ops[0], loads[0][0], us2[0])
+    assert res <= 1e-6 and eu <= 1e-5 and eC <= 1e-7
+    from pyto_amd import deflation, problems
+    mesh0, _, _ = problems.cantilever(specs[0][0], load="point")
+    ds = deflation.DeflationSolver()
+    ds.create_box_groups(mesh0, (2, 2, 2))
+    ds.attach(ops[0])
+    with pytest.raises(RuntimeError, match="groups changed"):
+        bs.solve(rtol=1e-8)
     bs.close()
     bp.close()
 
