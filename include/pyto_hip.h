@@ -234,11 +234,15 @@ int pyto_oc_update(pyto_handle* h, const double* x, const double* dc, int64_t n,
  * designs (handles, each with its own grid, densities and BCs) and several
  * load cases per design (the same handle listed once per load case).  Every
  * kernel of the iteration is one launch for the whole batch; each system has
- * its own scalars and stopping test.  Per system the algorithm is Jacobi PCG
- * (the reference's PCG branch, src/linear_solvers.py:157-176, no deflation)
- * in FP32 inside FP64 iterative refinement, as pyto_solve_dpcg_mixed.  No
- * reference counterpart: the reference solves one system per call.
- * All handles must use cube elements, or all bricks. */
+ * its own scalars and stopping test.  Per system the algorithm is PCG in FP32
+ * inside FP64 iterative refinement, as pyto_solve_dpcg_mixed: the deflated
+ * PCG of src/deflation.py:504-584 when the handle has deflation groups
+ * (pyto_set_groups before the densities), otherwise Jacobi PCG (the
+ * reference's PCG branch, src/linear_solvers.py:157-176).  No reference
+ * counterpart: the reference solves one system per call.
+ * All handles must use cube elements, or all bricks.  New densities keep a
+ * batch valid; new deflation groups on a handle need a new batch
+ * (pyto_batch_solve returns PYTO_ESTATE). */
 typedef struct pyto_batch pyto_batch;
 int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, void* stream);
 int pyto_batch_destroy(pyto_batch* b);
