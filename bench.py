@@ -277,11 +277,11 @@ def thermal_line(torch, dev, grid, x):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3")
     ap.add_argument("--boxes", default="4,4,4")
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inner-rtol", type=float, default=0.1)
     ap.add_argument("--tuned-boxes", default="12,6,6",
@@ -354,10 +354,14 @@ def main():
     torch.cuda.synchronize()
     log("set_density (SIMP, Jacobi, Galerkin + Cholesky + inverse): %.2f ms"
         % ((time.perf_counter() - t0) * 1e3))
+    # each solve returns after reading its scalars back, so the per-step
+    # clock brackets a finished solve
     t0 = time.perf_counter()
-    stats = []
+    stats, per_mixed = [], []
     for _ in range(args.steps):
+        t1 = time.perf_counter()
         stats.append(step("mixed"))
+        per_mixed.append(time.perf_counter() - t1)
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
     u_mixed = u.clone()
@@ -365,8 +369,11 @@ def main():
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    per_fp64 = []
     for _ in range(args.fp64_steps):
+        t1 = time.perf_counter()
         st64 = step("fp64")
+        per_fp64.append(time.perf_counter() - t1)
     torch.cuda.synchronize()
     t_fp64 = (time.perf_counter() - t0) / args.fp64_steps
     u64 = u.clone()
@@ -479,10 +486,14 @@ def main():
         "fp32_refined": {"solves_per_s": 1.0 / t_mixed, "inner_iterations": s["iterations"],
                          "outer_iterations": s["outer_iterations"], "rel": s["rel"],
                          "rel_true": s["rel_true"],
-                         "pcg_iterations_per_s": s["iterations"] / t_mixed},
+                         "pcg_iterations_per_s": s["iterations"] / t_mixed,
+                         "ms_median": float(np.median(per_mixed)) * 1e3,
+                         "ms_min": min(per_mixed) * 1e3},
         "fp64_plain": {"solves_per_s": 1.0 / t_fp64, "iterations": st64["iterations"],
                        "rel": st64["rel"], "rel_true": st64["rel_true"],
-                       "pcg_iterations_per_s": st64["iterations"] / t_fp64},
+                       "pcg_iterations_per_s": st64["iterations"] / t_fp64,
+                       "ms_median": float(np.median(per_fp64)) * 1e3,
+                       "ms_min": min(per_fp64) * 1e3},
         "speedup_vs_fp64": t_fp64 / t_mixed,
         "error_vs_fp64": dict({"displacement_max_rel": err_u, "compliance_rel": err_C},
                               **err_sens),

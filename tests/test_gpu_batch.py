@@ -8,10 +8,10 @@ from tests.helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _design(grid, seed, nloads, gpu, boxes=None):
+def _design(grid, seed, nloads, gpu, boxes=None, h=(1.0, 1.0, 1.0)):
     from pyto_amd import deflation, problems
     from pyto_amd.operator import GridOperator
-    mesh, mat, bc = problems.cantilever(grid, load="point")
+    mesh, mat, bc = problems.cantilever(grid, elem_size=h, load="point")
     rng = np.random.default_rng(seed)
     x = rng.uniform(0.01, 1.0, mesh.num_elems)
     sx, sy = grid[0] + 1, grid[1] + 1
@@ -20,7 +20,7 @@ def _design(grid, seed, nloads, gpu, boxes=None):
         f = np.zeros(3 * mesh.num_nodes)
         f[3 * (grid[0] + sx * (fn % sy + sy * (fn // sy))) + 2] = -1.0
         loads.append(f)
-    op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, bc.fixed_dofs, penal=3.0, eps_void=1e-8)
+    op = GridOperator(grid, h, 1.0, 0.3, bc.fixed_dofs, penal=3.0, eps_void=1e-8)
     if boxes is not None:
         op.set_coarse_shift(0.0, 0.0)
         ds = deflation.DeflationSolver()
@@ -100,6 +100,26 @@ def test_deflated_batch_matches_single_solves(gpu):
         bs.solve(rtol=1e-8)
     bs.close()
     bp.close()
+
+
+def test_deflated_batch_brick_elements(gpu):
+    """Brick elements (the non-cube element form in the batched restriction
+    and operator): deflated and plain designs against single solves."""
+    from pyto_amd.batch import BatchSolver
+    print("deflated batch, bricks", flush=True)
+    h = (0.5, 1.0, 1.5)
+    specs = [((24, 12, 8), 21, 2, (3, 2, 1)), ((16, 8, 8), 22, 1, None),
+             ((32, 16, 8), 23, 2, (4, 2, 2))]
+    ops, loads = zip(*[_design(g, s, n, gpu, bx, h) for g, s, n, bx in specs])
+    bs = BatchSolver(ops, loads)
+    us, stats = bs.solve(rtol=1e-8)
+    for (d, k), u, st in zip(bs.systems, us, stats):
+        assert st["converged"] and st["rel"] <= 1e-8
+        res, eu, eC = _check_against_single(ops[d], loads[d][k], u)
+        print(d, k, st["iterations"], st["outer_iterations"],
+              "res %.1e err u %.1e C %.1e" % (res, eu, eC), flush=True)
+        assert res <= 1e-6 and eu <= 1e-5 and eC <= 1e-7
+    bs.close()
 
 
 def test_batch_autotune_and_resolve(gpu):
