@@ -834,6 +834,7 @@ struct Batch {
   int nsys = 0;
   bool cube = true;
   std::vector<pyto_handle*> handles;
+  std::vector<int> groups_gen;     // each handle's groups generation at creation
   std::vector<BatchSys> hs;        // host copy of the table
   BatchSys* sys = nullptr;
   std::vector<int> h_mpre, h_vpre;
@@ -1100,6 +1101,7 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
       b->defl = true;
     }
     b->hs.push_back(S);
+    b->groups_gen.push_back(h->groups_gen);
   }
   // prefix sums of the deflated systems' special blocks, groups, coarse rows
   b->h_rpre.assign(nsys + 1, 0);
@@ -1217,8 +1219,7 @@ int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, do
     PYTO_REQUIRE(f[i] && u[i], PYTO_EINVAL, "null load or solution of system %d", i);
     PYTO_REQUIRE(b->handles[i]->density_set, PYTO_ESTATE, "system %d: density not set", i);
     const Handle* h = b->handles[i];
-    PYTO_REQUIRE(h->ngroups == b->hs[i].ngroups && h->Einv == b->hs[i].Einv &&
-                     h->sblocks == b->hs[i].sblocks, PYTO_ESTATE,
+    PYTO_REQUIRE(h->groups_gen == b->groups_gen[i], PYTO_ESTATE,
                  "system %d: its handle's deflation groups changed after the batch was "
                  "created; create the batch again", i);
     PYTO_REQUIRE(b->hs[i].ngroups == 0 || h->coarse_ready, PYTO_ESTATE,

@@ -130,35 +130,6 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ out, const T* _
   }
 }
 
-// Gather of an element's 24 nodal values (fixed DOFs zeroed), node ids and
-// the node coordinates relative to their group centre.
-struct ElemNodes {
-  int64_t node[8];
-  int gid[8];
-  uint8_t m[8];
-  double r[8][3];
-};
-
-__device__ __forceinline__ void elem_nodes(int64_t e, const GridDims& g,
-                                           const int32_t* __restrict__ grp,
-                                           const double* __restrict__ centers,
-                                           const uint8_t* __restrict__ mask, ElemNodes& en) {
-  int ix, iy, iz;
-  elem_coords(e, g, ix, iy, iz);
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int a = corner_a(j), b = corner_b(j), c = corner_c(j);
-    const int64_t n = (ix + a) + (int64_t)g.sx * ((iy + b) + (int64_t)g.sy * (iz + c));
-    en.node[j] = n;
-    const int gid = grp[n];
-    en.gid[j] = gid;
-    en.m[j] = mask[n];
-    en.r[j][0] = (ix + a) * g.hx - centers[3 * gid];
-    en.r[j][1] = (iy + b) * g.hy - centers[3 * gid + 1];
-    en.r[j][2] = (iz + c) * g.hz - centers[3 * gid + 2];
-  }
-}
-
 // Per-chunk partials of W^T K v over the special elements (one wave per
 // chunk; k_coarse_gather sums them into t), where v = z + beta p (the new
 // search direction before its coarse correction; HAS_P, with beta = rz_new /
@@ -646,67 +617,99 @@ template int launch_dir_update<double>(Handle*, double*, double*, const double*,
 template int launch_dir_update<float>(Handle*, float*, float*, const float*, bool, const int*,
                                       hipStream_t);
 
-// E += W_b^T K W_a over the special elements; E is nc_pad x nc_pad row-major.
-__global__ __launch_bounds__(256) void k_galerkin(
-    const int32_t* __restrict__ special, const SpecialChunk* __restrict__ chunks, int nchunks,
-    const double* __restrict__ Ee, const int32_t* __restrict__ grp,
-    const double* __restrict__ centers, const uint8_t* __restrict__ mask, GridDims g,
-    ModalConsts<double> mc, double* __restrict__ E, int ld) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+// Galerkin matrix E = W^T K W over the special elements, without atomics.
+// k_galerkin_parts: one wave per chunk and coarse column ja (blockIdx.y) of
+// each of the chunk's groups: the element operator applied to column ja of
+// group ga (W restricted to the corners in ga, fixed DOFs 0), then for every
+// group gb of the chunk the 6 products with gb's columns, summed over the
+// chunk's elements.  The 6x6 block of the pair (ga, gb) lands at
+// part[off[chunk] + (sa ng + sb) 36 + 6 ja + jb].  The node positions
+// relative to the group centres and the slot and fixed bits come from the
+// packed special-element records.
+template <bool CUBE>
+__global__ __launch_bounds__(256) void k_galerkin_parts(
+    const SpecialElem* __restrict__ sinfo, const SpecialChunk* __restrict__ chunks, int nchunks,
+    const int32_t* __restrict__ off, const double* __restrict__ Ee,
+    const double* __restrict__ centers, GridDims g, ModalConsts<double> mc,
+    double* __restrict__ part) {
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  const int ja = blockIdx.y;
   if (wave >= nchunks) return;
-  const SpecialChunk ch = chunks[wave];
-  const bool active = lane < ch.count;
-  ElemNodes en;
+  const SpecialChunk* ch = chunks + wave;
+  const int ng = ch->ng;
+  const bool active = lane < ch->count;
+  uint64_t info = 0;
   double Ev = 0.0;
+  int ix = 0, iy = 0, iz = 0;
   if (active) {
-    const int64_t e = special[ch.start + lane];
-    elem_nodes(e, g, grp, centers, mask, en);
-    Ev = Ee[e];
-  } else {
+    const SpecialElem se = sinfo[ch->start + lane];
+    info = se.info;
+    Ev = Ee[se.e];
+    elem_coords(se.e, g, ix, iy, iz);
+  }
+  double* out = part + off[wave];
+  for (int sa = 0; sa < ng; ++sa) {
+    const int ga = ch->groups[sa];
+    const double ax = centers[3 * ga], ay = centers[3 * ga + 1], az = centers[3 * ga + 2];
+    double u[8][3], f[8][3];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      en.node[j] = 0; en.gid[j] = -1; en.m[j] = 7;
-      en.r[j][0] = en.r[j][1] = en.r[j][2] = 0.0;
+      const bool in = active && (int)((info >> (3 * j)) & 7) == sa;
+      double w[6];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        w_row(c, (ix + corner_a(j)) * g.hx - ax, (iy + corner_b(j)) * g.hy - ay,
+              (iz + corner_c(j)) * g.hz - az, w);
+        u[j][c] = (in && !((info >> (24 + 3 * j + c)) & 1)) ? w[ja] : 0.0;
+      }
     }
-  }
-  for (int sa = 0; sa < ch.ng; ++sa) {
-    const int ga = ch.groups[sa];
-    for (int ja = 0; ja < 6; ++ja) {
-      double u[8][3], f[8][3];
+    if (CUBE)
+      element_apply_cube<double>(u, Ev, mc, f);
+    else
+      element_apply<double>(u, Ev, mc, f);
+    for (int sb = 0; sb < ng; ++sb) {
+      const int gb = ch->groups[sb];
+      const double bx = centers[3 * gb], by = centers[3 * gb + 1], bz = centers[3 * gb + 2];
+      double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        double w[6];
+        if (!active || (int)((info >> (3 * j)) & 7) != sb) continue;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          w_row(c, en.r[j][0], en.r[j][1], en.r[j][2], w);
-          u[j][c] = (en.gid[j] == ga && !((en.m[j] >> c) & 1)) ? w[ja] : 0.0;
+          if ((info >> (24 + 3 * j + c)) & 1) continue;
+          double w[6];
+          w_row(c, (ix + corner_a(j)) * g.hx - bx, (iy + corner_b(j)) * g.hy - by,
+                (iz + corner_c(j)) * g.hz - bz, w);
+#pragma unroll
+          for (int jb = 0; jb < 6; ++jb) acc[jb] += w[jb] * f[j][c];
         }
       }
-      element_apply<double>(u, Ev, mc, f);
-      for (int sb = 0; sb < ch.ng; ++sb) {
-        const int gb = ch.groups[sb];
-        double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          if (en.gid[j] != gb) continue;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            if ((en.m[j] >> c) & 1) continue;
-            double w[6];
-            w_row(c, en.r[j][0], en.r[j][1], en.r[j][2], w);
-#pragma unroll
-            for (int jb = 0; jb < 6; ++jb) acc[jb] += w[jb] * f[j][c];
-          }
-        }
-#pragma unroll
-        for (int jb = 0; jb < 6; ++jb) {
-          const double tot = wave_reduce_sum(acc[jb]);
-          if (lane == 0) atomicAdd(&E[(int64_t)(6 * gb + jb) * ld + (6 * ga + ja)], tot);
-        }
+      for (int jb = 0; jb < 6; ++jb) {
+        const double tot = wave_reduce_sum(acc[jb]);
+        if (lane == 0) out[(sa * ng + sb) * 36 + 6 * ja + jb] = tot;
       }
     }
   }
+}
+
+// E's 6x6 block of each group pair (ga, gb) that shares a chunk: one wave per
+// pair, lane 6 ja + jb sums the pair's chunk blocks in list order (fixed
+// order, so E is the same bits on every run) into E[6 gb + jb][6 ga + ja].
+__global__ __launch_bounds__(256) void k_galerkin_reduce(const int2* __restrict__ pairs,
+                                                         const int32_t* __restrict__ pptr,
+                                                         const int32_t* __restrict__ pidx,
+                                                         int npairs, const double* __restrict__ part,
+                                                         double* __restrict__ E, int ld) {
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p >= npairs || lane >= 36) return;
+  double acc = 0.0;
+  for (int k = pptr[p]; k < pptr[p + 1]; ++k) acc += part[pidx[k] + lane];
+  const int2 gp = pairs[p];
+  const int ja = lane / 6, jb = lane - 6 * ja;
+  E[(int64_t)(6 * gp.y + jb) * ld + (6 * gp.x + ja)] = acc;
 }
 
 // E <- (E + E^T)/2 + (sa + sr |E_ii|) I on the nc x nc block, identity on the
@@ -933,10 +936,19 @@ int coarse_setup(Handle* h, hipStream_t s) {
   const int ld = round_up64(h->nc);
   PYTO_CHECK_HIP(hipMemsetAsync(h->Ec, 0, sizeof(double) * ld * ld, s));
   if (h->nchunks > 0) {
-    const int blocks = (h->nchunks * 64 + 255) / 256;
-    hipLaunchKernelGGL(k_galerkin, dim3(blocks), dim3(256), 0, s, h->special, h->chunks,
-                       h->nchunks, h->Ee64, h->node_group, h->centers, h->fixmask, h->g,
-                       h->mc64, h->Ec, ld);
+    const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
+    const dim3 grid((h->nchunks + 3) / 4, 6);
+    if (cube)
+      hipLaunchKernelGGL(k_galerkin_parts<true>, grid, dim3(256), 0, s, h->sinfo, h->chunks,
+                         h->nchunks, h->gal_off, h->Ee64, h->centers, h->g, h->mc64,
+                         h->gal_part);
+    else
+      hipLaunchKernelGGL(k_galerkin_parts<false>, grid, dim3(256), 0, s, h->sinfo, h->chunks,
+                         h->nchunks, h->gal_off, h->Ee64, h->centers, h->g, h->mc64,
+                         h->gal_part);
+    hipLaunchKernelGGL(k_galerkin_reduce, dim3((h->npairs * 64 + 255) / 256), dim3(256), 0, s,
+                       h->gal_pairs, h->gal_pptr, h->gal_pidx, h->npairs,
+                       (const double*)h->gal_part, h->Ec, ld);
     PYTO_CHECK_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_symmetrize, dim3((ld + 255) / 256, ld), dim3(256), 0, s, h->Ec, h->nc, ld,
@@ -974,7 +986,8 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   void* olds[] = {h->node_group, h->centers, h->special, h->sinfo, h->chunks, h->cpart,
                   h->sblocks, h->sbnodes, h->sidx, h->gptr, h->gidx,
                   h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
-                  h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres};
+                  h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres,
+                  h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx};
   for (void* p : olds)
     if (p) hipFree(p);
   h->node_group = nullptr; h->centers = nullptr; h->special = nullptr; h->chunks = nullptr;
@@ -984,7 +997,10 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   h->gnodes = nullptr; h->nchunk = nullptr; h->nchunk_grp = nullptr; h->npart = nullptr;
   h->nptr = nullptr; h->nidx = nullptr; h->nnchunks = 0;
   h->Ec = h->Einv = h->cwork = h->ct = h->cmu = h->cres = nullptr;
+  h->gal_off = nullptr; h->gal_part = nullptr; h->gal_pairs = nullptr; h->gal_pptr = nullptr;
+  h->gal_pidx = nullptr; h->npairs = 0;
   h->ngroups = 0; h->nc = 0; h->nspecial = 0; h->nchunks = 0; h->coarse_ready = false;
+  h->groups_gen++;
   if (ngroups == 0) return PYTO_OK;
   PYTO_REQUIRE(node_group && centers, PYTO_EINVAL, "null group arrays");
   const int64_t nn = h->g.nnodes;
@@ -1112,6 +1128,37 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
       gidx.insert(gidx.end(), per[gi].begin(), per[gi].end());
     }
   }
+  // Galerkin partial blocks: per chunk ng^2 blocks of 36 at gal_off[chunk];
+  // per group pair (ga, gb) sharing a chunk, the list of its blocks in chunk
+  // order
+  std::vector<int32_t> gal_off(chunks.size());
+  std::vector<int2> gal_pairs;
+  std::vector<int32_t> gal_pptr(1, 0), gal_pidx;
+  int64_t galsz = 0;
+  {
+    std::vector<std::pair<int64_t, int32_t>> ent;   // (pair key, block offset)
+    for (size_t c = 0; c < chunks.size(); ++c) {
+      PYTO_REQUIRE(galsz < (int64_t)1 << 31, PYTO_EINVAL, "Galerkin partials exceed 2^31 values");
+      gal_off[c] = (int32_t)galsz;
+      const int ng = chunks[c].ng;
+      for (int sa = 0; sa < ng; ++sa)
+        for (int sb = 0; sb < ng; ++sb)
+          ent.push_back({(int64_t)chunks[c].groups[sa] * ngroups + chunks[c].groups[sb],
+                         (int32_t)(galsz + (sa * ng + sb) * 36)});
+      galsz += (int64_t)36 * ng * ng;
+    }
+    std::stable_sort(ent.begin(), ent.end(),
+                     [](const std::pair<int64_t, int32_t>& x,
+                        const std::pair<int64_t, int32_t>& y) { return x.first < y.first; });
+    for (size_t k = 0; k < ent.size(); ++k) {
+      if (k == 0 || ent[k].first != ent[k - 1].first) {
+        if (k > 0) gal_pptr.push_back((int32_t)k);
+        gal_pairs.push_back(make_int2((int)(ent[k].first / ngroups), (int)(ent[k].first % ngroups)));
+      }
+      gal_pidx.push_back(ent[k].second);
+    }
+    if (!ent.empty()) gal_pptr.push_back((int32_t)ent.size());
+  }
   // nodes sorted by group, in runs of <= 256 of one group
   std::vector<int32_t> gnodes(nn);
   {
@@ -1165,6 +1212,12 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_TRY(device_alloc(h, (void**)&h->nidx, nidx.size() * sizeof(int32_t)));
   PYTO_TRY(device_alloc(h, (void**)&h->gptr, gptr.size() * sizeof(int32_t)));
   PYTO_TRY(device_alloc(h, (void**)&h->gidx, std::max<size_t>(gidx.size(), 1) * sizeof(int32_t)));
+  h->npairs = (int)gal_pairs.size();
+  PYTO_TRY(device_alloc(h, (void**)&h->gal_off, std::max<size_t>(gal_off.size(), 1) * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->gal_part, std::max<int64_t>(galsz, 1) * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->gal_pairs, std::max<size_t>(gal_pairs.size(), 1) * sizeof(int2)));
+  PYTO_TRY(device_alloc(h, (void**)&h->gal_pptr, gal_pptr.size() * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->gal_pidx, std::max<size_t>(gal_pidx.size(), 1) * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->Ec, (size_t)ld * ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->Einv, (size_t)ld * ld * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->cwork, (size_t)ld * 8));
@@ -1185,6 +1238,14 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     PYTO_CHECK_HIP(hipMemcpyAsync(h->sbnodes, sbnodes.data(), sbnodes.size() * sizeof(int32_t),
                                   hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->sidx, sidx.data(), sidx.size() * sizeof(uint16_t),
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_off, gal_off.data(), gal_off.size() * 4,
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pairs, gal_pairs.data(), gal_pairs.size() * sizeof(int2),
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pptr, gal_pptr.data(), gal_pptr.size() * 4,
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pidx, gal_pidx.data(), gal_pidx.size() * 4,
                                   hipMemcpyHostToDevice, s));
   }
   PYTO_CHECK_HIP(hipMemcpyAsync(h->gnodes, gnodes.data(), nn * sizeof(int32_t),

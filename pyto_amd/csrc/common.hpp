@@ -144,6 +144,14 @@ struct Handle {
   double* npart = nullptr;      // per node block partials (nnchunks x 8 x 6, slot 0 used)
   int32_t* nptr = nullptr;      // per group: range in nidx
   int32_t* nidx = nullptr;
+  // Galerkin assembly: per chunk ng^2 6x6 partial blocks (at gal_off), per
+  // group pair the list of its blocks
+  int npairs = 0;
+  int32_t* gal_off = nullptr;
+  double* gal_part = nullptr;
+  int2* gal_pairs = nullptr;    // (ga, gb)
+  int32_t* gal_pptr = nullptr;
+  int32_t* gal_pidx = nullptr;
   double* Ec = nullptr;         // coarse matrix (nc x nc), then Cholesky factor
   double* Einv = nullptr;       // coarse inverse (nc x nc)
   double* cwork = nullptr;      // ld workspace (coarse refinement correction)
@@ -152,6 +160,7 @@ struct Handle {
   double* cmu = nullptr;        // coarse solution (nc)
   double* cres = nullptr;       // coarse residual (nc)
   bool coarse_ready = false;
+  int groups_gen = 0;           // counts pyto_set_groups calls (batches check it)
   double shift_abs = 1e-10;     // coarse diagonal shift (reference default)
   double shift_rel = 0.0;
 

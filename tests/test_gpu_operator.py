@@ -72,10 +72,11 @@ def test_symmetry_at_full_size(gpu):
     assert float(Ku[fixed].abs().max()) == 0.0
 
 
-@pytest.mark.parametrize("dims", [(12, 6, 4), (60, 20, 10)])
-def test_coarse_space(gpu, dims):
-    print("coarse", dims, flush=True)
-    c = Case(dims, seed=5)
+@pytest.mark.parametrize("dims,h", [((12, 6, 4), (1.0, 1.0, 1.0)), ((60, 20, 10), (1.0, 1.0, 1.0)),
+                                    ((14, 7, 5), (0.5, 1.0, 1.5))])
+def test_coarse_space(gpu, dims, h):
+    print("coarse", dims, h, flush=True)
+    c = Case(dims, seed=5, h=h)
     op = c.operator(gpu, groups=True)
     W = c.W_free()
     rng = np.random.default_rng(8)
@@ -95,6 +96,18 @@ def test_coarse_space(gpu, dims):
     ref = np.linalg.solve(E_ref, rhs)
     assert np.linalg.norm(E_ref @ sol - rhs) <= 1e-10 * np.linalg.norm(rhs)
     assert rel_err(sol, ref) <= 1e-8
+
+
+def test_coarse_matrix_is_reproducible(gpu):
+    """The Galerkin matrix is summed in a fixed order: the same densities give
+    the same bits on every assembly."""
+    print("coarse reproducible", flush=True)
+    c = Case((40, 20, 12), seed=6)
+    op = c.operator(gpu, groups=True)
+    E1 = op.coarse_matrix().clone()
+    for _ in range(3):
+        op.set_density(c.x)
+        assert torch.equal(op.coarse_matrix(), E1)
 
 
 def test_bad_calls_raise(gpu):
