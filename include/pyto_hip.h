@@ -53,6 +53,8 @@ typedef struct pyto_solve_stats {
   double setup_ms;       /* density set-up time inside the call, if any */
 } pyto_solve_stats;
 
+/* No reference counterpart: the calling thread's last error message (the
+ * reference raises Python exceptions instead) and the ABI version. */
 const char* pyto_last_error(void);
 int pyto_version(void);
 
