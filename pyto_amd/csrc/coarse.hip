@@ -138,11 +138,15 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ out, const T* _
 // T (the cube form for cube elements) and the per-group sums in FP64; in FP32
 // that halves the element registers, so more waves fit per SIMD.  Group
 // centres are wave-uniform per slot.
-// A workgroup serves one SpecialBlock: it first stages v at the block's
-// sorted corner nodes in LDS (an element's corners would otherwise be 24
-// scattered loads, each wave instruction touching up to 64 cache lines on
-// interfaces normal to x), then each wave applies the element operator of
-// one chunk reading its corners from LDS.
+// A workgroup serves one block of up to kSpecialBlockChunks chunks: it first
+// stages v at the block's sorted corner nodes in LDS (an element's corners
+// would otherwise be 24 scattered loads, each wave instruction touching up to
+// 64 cache lines on interfaces normal to x), then each wave applies the
+// element operator of one chunk reading its corners from LDS.
+// The block tables are padded (pwave, pslot, pnodes, pcen, pE), so every
+// load but the staged values depends on the block index alone and they are
+// all in flight at once; the stop flag is tested after they are issued, and
+// before the value loads.
 // WPE: minimum waves per SIMD the registers must allow (5 for FP32, which
 // fits 96 VGPRs without spilling: the 4-wave-per-SIMD grid of about a
 // thousand blocks would otherwise run a short second round).
@@ -151,61 +155,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 k_restrict_special(
     const T* __restrict__ z, const T* __restrict__ p,
     const double* __restrict__ part_rz, int nrz, const double* __restrict__ rz_old,
-    const SpecialElem* __restrict__ sinfo,
-    const uint4* __restrict__ sidx, const SpecialChunk* __restrict__ chunks,
-    const SpecialBlock* __restrict__ sblocks, const int32_t* __restrict__ sbnodes,
-    const T* __restrict__ Ee, const double* __restrict__ centers, GridDims g,
-    ModalConsts<T> mc, double* __restrict__ cpart, const int* __restrict__ done) {
-  if (done && *done) return;
+    const int2* __restrict__ pwave, const SpecialSlot* __restrict__ pslot,
+    const int32_t* __restrict__ pnodes, const double* __restrict__ pcen,
+    const T* __restrict__ pE, GridDims g, ModalConsts<T> mc, double* __restrict__ cpart,
+    const int* __restrict__ done) {
   constexpr int kStage = (3 * kSpecialBlockNodes + 255) / 256;   // staged DOFs per thread
   __shared__ T vs[3 * kSpecialBlockNodes];
   __shared__ double csh[kSpecialBlockChunks][8 * 3];   // group centres per chunk and slot
-  const SpecialBlock sb = sblocks[xcd_logical_block(blockIdx.x, gridDim.x)];
+  const int b = xcd_logical_block(blockIdx.x, gridDim.x);
   const int w = threadIdx.x >> 6;
-  const int wave = sb.chunk0 + min(w, sb.nch - 1);
   const int lane = threadIdx.x & 63;
-  // count and ng by value; the group list is read in place (a local copy of
-  // it indexed by the loop variable would live in scratch memory)
-  const int ch_start = chunks[wave].start, ch_count = chunks[wave].count;
-  const int ch_ng = chunks[wave].ng;
-  const bool active = w < sb.nch && lane < ch_count;
-  // this lane's element record, issued before the staging loads
-  SpecialElem se{0, 0, 0};
-  uint4 li = make_uint4(0, 0, 0, 0);
-  T Ev = T(0);
-  if (active) {
-    se = sinfo[ch_start + lane];
-    li = sidx[ch_start + lane];
-    Ev = Ee[se.e];
-  }
-  // stage v at the block's corner nodes: consecutive lanes read consecutive
-  // DOFs wherever the nodes are consecutive.  Branch-free (indices clamped
-  // into the block's list) so that all node-id loads, then all value loads,
-  // are in flight together before the first LDS store.
-  // (32-bit DOF indices: the handle limits 3 * nodes below 2^32)
-  const int nst = 3 * sb.nnodes;
+  const int stopped = done ? *done : 0;
+  const int2 pw = pwave[b * kSpecialBlockChunks + w];   // (chunk or -1, ng)
+  const int64_t slot = ((int64_t)b * kSpecialBlockChunks + w) * 64 + lane;
+  const SpecialSlot se = pslot[slot];
+  const T Ev = pE[slot];
+  // the block's corner nodes (padded to kSpecialBlockNodes by repeating the
+  // last one): consecutive lanes read consecutive DOFs wherever the nodes are
+  // consecutive.  (32-bit DOF indices: the handle limits 3 * nodes below 2^32)
   uint32_t dof[kStage];
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
-    const int i = min((int)threadIdx.x + 256 * t, nst - 1);
+    const int i = (int)threadIdx.x + 256 * t;
     const int q = i / 3;
-    dof[t] = 3u * (uint32_t)sbnodes[sb.node0 + q] + (uint32_t)(i - 3 * q);
+    dof[t] = 3u * (uint32_t)pnodes[(int64_t)b * kSpecialBlockNodes + q] + (uint32_t)(i - 3 * q);
   }
+  if (threadIdx.x < kSpecialBlockChunks * 24)
+    (&csh[0][0])[threadIdx.x] = pcen[(int64_t)b * kSpecialBlockChunks * 24 + threadIdx.x];
+  if (stopped) return;   // the same flag for the whole block
   T za[kStage], pa[kStage];
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
     za[t] = z[dof[t]];
     pa[t] = HAS_P ? p[dof[t]] : T(0);
-  }
-  // the centres of each chunk's groups, staged beside the values (threads
-  // 0..95 of the block: chunk, slot, coordinate)
-  if (threadIdx.x < kSpecialBlockChunks * 24) {
-    const int cw = threadIdx.x / 24, r = threadIdx.x - 24 * cw;
-    if (cw < sb.nch) {
-      const SpecialChunk* c = chunks + sb.chunk0 + cw;
-      const int sl = r / 3;
-      if (sl < c->ng) csh[cw][r] = centers[3 * c->groups[sl] + (r - 3 * sl)];
-    }
   }
   // beta = rz_new / rz from the update kernel's partials, summed here in the
   // same order as by the scalar logic (no scalar kernel needed before this);
@@ -213,17 +195,16 @@ k_restrict_special(
   __shared__ double redb[5];
   const double beta = HAS_P ? block_sum_partials(part_rz, nrz, redb) / *rz_old : 0.0;
 #pragma unroll
-  for (int t = 0; t < kStage; ++t) {
-    const int i = threadIdx.x + 256 * t;
-    if (i < nst) vs[i] = (T)((double)za[t] + (HAS_P ? beta * (double)pa[t] : 0.0));
-  }
+  for (int t = 0; t < kStage; ++t)
+    vs[threadIdx.x + 256 * t] = (T)((double)za[t] + (HAS_P ? beta * (double)pa[t] : 0.0));
   __syncthreads();
-  if (w >= sb.nch) return;
+  if (pw.x < 0) return;
+  const bool active = se.e >= 0;
   T f[8][3];
   uint64_t info = 0;
   int ix = 0, iy = 0, iz = 0;
   if (active) {
-    const uint32_t lw[4] = {li.x, li.y, li.z, li.w};
+    const uint32_t lw[4] = {se.li.x, se.li.y, se.li.z, se.li.w};
     info = se.info;
     elem_coords(se.e, g, ix, iy, iz);
     T u[8][3];
@@ -250,7 +231,7 @@ k_restrict_special(
   // corner positions relative to the group centre and the moments r x f in
   // the solver precision T, the sums in FP64
   const double* cen = csh[w];
-  for (int sl = 0; sl < ch_ng; ++sl) {
+  for (int sl = 0; sl < pw.y; ++sl) {
     const double cx = cen[3 * sl], cy = cen[3 * sl + 1], cz = cen[3 * sl + 2];
     const T x0 = (T)(ix * g.hx - cx), y0 = (T)(iy * g.hy - cy), z0 = (T)(iz * g.hz - cz);
     const T hx = (T)g.hx, hy = (T)g.hy, hz = (T)g.hz;
@@ -272,9 +253,21 @@ k_restrict_special(
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
       const double tot = wave_reduce_sum(acc[j]);
-      if (lane == 0) cpart[(int64_t)wave * 48 + 6 * sl + j] = tot;
+      if (lane == 0) cpart[(int64_t)pw.x * 48 + 6 * sl + j] = tot;
     }
   }
+}
+
+// E_e of every lane of the padded restriction blocks (0 without an element),
+// both precisions; refreshed with every density.
+__global__ void k_special_E(const SpecialSlot* __restrict__ pslot, int64_t n,
+                            const double* __restrict__ E64, const float* __restrict__ E32,
+                            double* __restrict__ pE64, float* __restrict__ pE32) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int e = pslot[i].e;
+  pE64[i] = e >= 0 ? E64[e] : 0.0;
+  pE32[i] = e >= 0 ? E32[e] : 0.f;
 }
 
 // t[6g..6g+5] = sum of the chunk partials of group g: one wave per group,
@@ -807,6 +800,12 @@ const double* ee_of<double>(const Handle* h) { return h->Ee64; }
 template <>
 const float* ee_of<float>(const Handle* h) { return h->Ee32; }
 template <typename T>
+static const T* pe_of(const Handle* h);
+template <>
+const double* pe_of<double>(const Handle* h) { return h->pE64; }
+template <>
+const float* pe_of<float>(const Handle* h) { return h->pE32; }
+template <typename T>
 static ModalConsts<T> mc_of(const Handle* h);
 template <>
 ModalConsts<double> mc_of<double>(const Handle* h) { return h->mc64; }
@@ -825,8 +824,8 @@ int launch_restrict_special_parts(Handle* h, const T* z, const T* p, const doubl
     const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
     auto go = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, dim3(h->nsblocks), dim3(256), 0, s, z, p, part_rz, nrz, rz_old,
-                         h->sinfo, h->sidx, h->chunks, h->sblocks, h->sbnodes, ee_of<T>(h),
-                         h->centers, h->g, mc_of<T>(h), h->cpart, done);
+                         h->pwave, h->pslot, h->pnodes, h->pcen, pe_of<T>(h), h->g, mc_of<T>(h),
+                         h->cpart, done);
     };
     if (p)
       cube ? go(k_restrict_special<T, true, true>) : go(k_restrict_special<T, true, false>);
@@ -934,6 +933,11 @@ void release_blas(Handle* h) {
 
 int coarse_setup(Handle* h, hipStream_t s) {
   const int ld = round_up64(h->nc);
+  if (h->nsblocks > 0) {
+    const int64_t n = (int64_t)h->nsblocks * kSpecialBlockChunks * 64;
+    hipLaunchKernelGGL(k_special_E, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, h->pslot,
+                       n, (const double*)h->Ee64, (const float*)h->Ee32, h->pE64, h->pE32);
+  }
   PYTO_CHECK_HIP(hipMemsetAsync(h->Ec, 0, sizeof(double) * ld * ld, s));
   if (h->nchunks > 0) {
     const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
@@ -987,7 +991,8 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
                   h->sblocks, h->sbnodes, h->sidx, h->gptr, h->gidx,
                   h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
                   h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres,
-                  h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx};
+                  h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
+                  h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32};
   for (void* p : olds)
     if (p) hipFree(p);
   h->node_group = nullptr; h->centers = nullptr; h->special = nullptr; h->chunks = nullptr;
@@ -999,6 +1004,8 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   h->Ec = h->Einv = h->cwork = h->ct = h->cmu = h->cres = nullptr;
   h->gal_off = nullptr; h->gal_part = nullptr; h->gal_pairs = nullptr; h->gal_pptr = nullptr;
   h->gal_pidx = nullptr; h->npairs = 0;
+  h->pwave = nullptr; h->pslot = nullptr; h->pnodes = nullptr; h->pcen = nullptr;
+  h->pE64 = nullptr; h->pE32 = nullptr;
   h->ngroups = 0; h->nc = 0; h->nspecial = 0; h->nchunks = 0; h->coarse_ready = false;
   h->groups_gen++;
   if (ngroups == 0) return PYTO_OK;
@@ -1117,6 +1124,47 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
       sblocks.push_back(sb);
     }
   }
+  // the blocks padded: per (block, wave) its chunk and group count, per lane
+  // its element record, per block kSpecialBlockNodes node ids (the last one
+  // repeated), per (block, wave) the centres of the chunk's groups
+  const size_t nsb = sblocks.size();
+  std::vector<int2> pwave(nsb * kSpecialBlockChunks, make_int2(-1, 0));
+  std::vector<SpecialSlot> pslot(nsb * kSpecialBlockChunks * 64);
+  std::vector<int32_t> pnodes(nsb * kSpecialBlockNodes);
+  std::vector<double> pcen(nsb * kSpecialBlockChunks * 24, 0.0);
+  {
+    std::vector<double> cen_h((size_t)ngroups * 3);
+    PYTO_CHECK_HIP(hipMemcpyAsync(cen_h.data(), centers, cen_h.size() * 8, hipMemcpyDeviceToHost, s));
+    PYTO_CHECK_HIP(hipStreamSynchronize(s));
+    for (SpecialSlot& sl : pslot) {
+      sl.e = -1;
+      sl.pad = 0;
+      sl.info = 0;
+      sl.li = make_uint4(0, 0, 0, 0);
+    }
+    for (size_t bi = 0; bi < nsb; ++bi) {
+      const SpecialBlock& sb = sblocks[bi];
+      for (int q = 0; q < kSpecialBlockNodes; ++q)
+        pnodes[bi * kSpecialBlockNodes + q] = sbnodes[sb.node0 + std::min(q, sb.nnodes - 1)];
+      for (int w = 0; w < sb.nch; ++w) {
+        const int c = sb.chunk0 + w;
+        const SpecialChunk& ch = chunks[c];
+        const size_t bw = bi * kSpecialBlockChunks + w;
+        pwave[bw] = make_int2(c, ch.ng);
+        for (int sl = 0; sl < ch.ng; ++sl)
+          for (int k = 0; k < 3; ++k) pcen[bw * 24 + 3 * sl + k] = cen_h[3 * ch.groups[sl] + k];
+        for (int l = 0; l < ch.count; ++l) {
+          const size_t i = (size_t)ch.start + l;
+          SpecialSlot& o = pslot[bw * 64 + l];
+          o.e = sinfo[i].e;
+          o.info = sinfo[i].info;
+          const uint16_t* x = &sidx[i * 8];
+          o.li = make_uint4(x[0] | ((uint32_t)x[1] << 16), x[2] | ((uint32_t)x[3] << 16),
+                            x[4] | ((uint32_t)x[5] << 16), x[6] | ((uint32_t)x[7] << 16));
+        }
+      }
+    }
+  }
   // per group: the (chunk, slot) partials that belong to it
   std::vector<int32_t> gptr(ngroups + 1, 0), gidx;
   {
@@ -1213,6 +1261,12 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_TRY(device_alloc(h, (void**)&h->gptr, gptr.size() * sizeof(int32_t)));
   PYTO_TRY(device_alloc(h, (void**)&h->gidx, std::max<size_t>(gidx.size(), 1) * sizeof(int32_t)));
   h->npairs = (int)gal_pairs.size();
+  PYTO_TRY(device_alloc(h, (void**)&h->pwave, std::max<size_t>(pwave.size(), 1) * sizeof(int2)));
+  PYTO_TRY(device_alloc(h, (void**)&h->pslot, std::max<size_t>(pslot.size(), 1) * sizeof(SpecialSlot)));
+  PYTO_TRY(device_alloc(h, (void**)&h->pnodes, std::max<size_t>(pnodes.size(), 1) * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->pcen, std::max<size_t>(pcen.size(), 1) * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->pE64, std::max<size_t>(pslot.size(), 1) * 8));
+  PYTO_TRY(device_alloc(h, (void**)&h->pE32, std::max<size_t>(pslot.size(), 1) * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->gal_off, std::max<size_t>(gal_off.size(), 1) * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->gal_part, std::max<int64_t>(galsz, 1) * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->gal_pairs, std::max<size_t>(gal_pairs.size(), 1) * sizeof(int2)));
@@ -1241,6 +1295,13 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
                                   hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_off, gal_off.data(), gal_off.size() * 4,
                                   hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->pwave, pwave.data(), pwave.size() * sizeof(int2),
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->pslot, pslot.data(), pslot.size() * sizeof(SpecialSlot),
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->pnodes, pnodes.data(), pnodes.size() * 4,
+                                  hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->pcen, pcen.data(), pcen.size() * 8, hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pairs, gal_pairs.data(), gal_pairs.size() * sizeof(int2),
                                   hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pptr, gal_pptr.data(), gal_pptr.size() * 4,

@@ -96,6 +96,17 @@ struct SpecialBlock {
 constexpr int kSpecialBlockChunks = 4;
 constexpr int kSpecialBlockNodes = 1024;
 
+// One lane of the single-system restriction in block-padded order
+// (block b, wave w, lane l at (b kSpecialBlockChunks + w) 64 + l): the
+// element (-1: no element), its packed slot and fixed bits, and its 8
+// block-local corner indices (u16).
+struct SpecialSlot {
+  int32_t e;
+  int32_t pad;
+  uint64_t info;
+  uint4 li;
+};
+
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   const void* x = nullptr;
@@ -134,6 +145,14 @@ struct Handle {
   SpecialBlock* sblocks = nullptr;
   int32_t* sbnodes = nullptr;   // per block: its sorted unique corner nodes
   uint4* sidx = nullptr;        // per special element: 8 block-local corner indices (u16)
+  // the same blocks padded, so that every load of a restriction block except
+  // the staged vector values depends on the block index alone
+  int2* pwave = nullptr;        // per (block, wave): (chunk index or -1, ng)
+  SpecialSlot* pslot = nullptr; // per (block, wave, lane)
+  int32_t* pnodes = nullptr;    // per block: kSpecialBlockNodes node ids (padded)
+  double* pcen = nullptr;       // per (block, wave): 8 slots x 3 group-centre coordinates
+  double* pE64 = nullptr;       // per (block, wave, lane): E_e (0 without an element)
+  float* pE32 = nullptr;
   int32_t* gptr = nullptr;      // per group: range of (chunk*8+slot) in gidx
   int32_t* gidx = nullptr;
   // nodes sorted by group, cut into blocks of <= 256 nodes of one group
