@@ -714,24 +714,50 @@ __global__ __launch_bounds__(256) void k_bgather(const BatchSys* __restrict__ sy
   }
 }
 
-// mu = E^-1 t per deflated system: one wave per row (prefix sums cpre over
-// the systems' 6 G rows).  START as in k_bgather.
+// mu = E^-1 t for the deflated systems: one wave per coarse row of each run
+// of consecutive systems on one handle (the load cases of a design, at most
+// kBatchRhs; prefix sums dpre over the runs' rows), so that each row of the
+// shared E^-1 is read once for all of them.  START as in k_bgather.  Per
+// system the order of the sums is that of a single GEMV.
+constexpr int kBatchRhs = 8;
 template <bool START>
 __global__ __launch_bounds__(256) void k_bgemv(const BatchSys* __restrict__ sys,
-                                               const int* __restrict__ cpre, int nsys,
+                                               const int2* __restrict__ drun,
+                                               const int* __restrict__ dpre, int nruns,
                                                const int* __restrict__ flags) {
   const int R = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (R >= cpre[nsys]) return;
-  const int s = find_sys(cpre, nsys, R);
-  if (flags[s * kNumFlags + (START ? F_ODONE : F_DONE)]) return;
-  const BatchSys& S = sys[s];
-  const int row = R - cpre[s];
+  if (R >= dpre[nruns]) return;
+  const int d = find_sys(dpre, nruns, R);
+  const int2 run = drun[d];   // (first system, count)
+  const int row = R - dpre[d];
+  bool act[kBatchRhs];
+  const double* t[kBatchRhs];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < kBatchRhs; ++k) {
+    act[k] = k < run.y && !flags[(run.x + k) * kNumFlags + (START ? F_ODONE : F_DONE)];
+    t[k] = act[k] ? sys[run.x + k].ct : nullptr;
+    any = any || act[k];
+  }
+  if (!any) return;
+  const BatchSys& S = sys[run.x];
   const double* a = S.Einv + (int64_t)row * S.ld;
-  double acc = 0.0;
-  for (int j = lane; j < S.nc; j += 64) acc += a[j] * S.ct[j];
-  acc = wave_reduce_sum(acc);
-  if (lane == 0) S.cmu[row] = acc;
+  double acc[kBatchRhs];
+#pragma unroll
+  for (int k = 0; k < kBatchRhs; ++k) acc[k] = 0.0;
+  for (int j = lane; j < S.nc; j += 64) {
+    const double aj = a[j];
+#pragma unroll
+    for (int k = 0; k < kBatchRhs; ++k)
+      if (act[k]) acc[k] += aj * t[k][j];
+  }
+#pragma unroll
+  for (int k = 0; k < kBatchRhs; ++k) {
+    if (!act[k]) continue;
+    const double v = wave_reduce_sum(acc[k]);
+    if (lane == 0) sys[run.x + k].cmu[row] = v;
+  }
 }
 
 // ---- FP64 outer refinement --------------------------------------------------
@@ -855,8 +881,10 @@ struct Batch {
   // deflation: prefix sums over the deflated systems' special blocks, groups
   // and coarse rows (host and device)
   bool defl = false;
-  std::vector<int> h_rpre, h_gpre, h_cpre, h_npre;
-  int *rpre = nullptr, *gpre = nullptr, *cpre = nullptr, *npre = nullptr;
+  std::vector<int> h_rpre, h_gpre, h_npre, h_dpre;
+  std::vector<int2> h_drun;        // runs of systems sharing one deflated handle
+  int *rpre = nullptr, *gpre = nullptr, *npre = nullptr, *dpre = nullptr;
+  int2* drun = nullptr;
 };
 
 }  // namespace pyto
@@ -912,7 +940,8 @@ static void launch_bmatvec_inner(Batch* b, hipStream_t s) {
 // mu = E^-1 W^T K (M r + beta p) for every deflated system still iterating:
 // special restriction, per-group gather, coarse GEMV.
 static void launch_bcoarse(Batch* b, hipStream_t s, bool start) {
-  const int nr = b->h_rpre[b->nsys], ng = b->h_gpre[b->nsys], nc = b->h_cpre[b->nsys];
+  const int nr = b->h_rpre[b->nsys], ng = b->h_gpre[b->nsys];
+  const int nruns = (int)b->h_drun.size(), nc = nruns ? b->h_dpre[nruns] : 0;
   if (nr > 0) {
     if (b->cube)
       hipLaunchKernelGGL(k_brestrict<true>, dim3(nr), dim3(256), 0, s, b->sys, b->rpre, b->nsys,
@@ -926,13 +955,14 @@ static void launch_bcoarse(Batch* b, hipStream_t s, bool start) {
                        b->gpre, b->nsys, (const int*)b->flags);
   if (nc > 0)
     hipLaunchKernelGGL(k_bgemv<false>, dim3((nc * 64 + 255) / 256), dim3(256), 0, s, b->sys,
-                       b->cpre, b->nsys, (const int*)b->flags);
+                       b->drun, b->dpre, nruns, (const int*)b->flags);
 }
 
 // x0 = W E^-1 W^T b for every deflated system still refining, x0 = 0 for the
 // others: full restriction, per-group gather, coarse GEMV, prolongation.
 static void launch_bstart_x(Batch* b, hipStream_t s) {
-  const int nn = b->h_npre[b->nsys], ng = b->h_gpre[b->nsys], nc = b->h_cpre[b->nsys];
+  const int nn = b->h_npre[b->nsys], ng = b->h_gpre[b->nsys];
+  const int nruns = (int)b->h_drun.size(), nc = nruns ? b->h_dpre[nruns] : 0;
   if (nn > 0)
     hipLaunchKernelGGL(k_brestrict_full, dim3(nn), dim3(256), 0, s, b->sys, b->npre, b->nsys,
                        (const int*)b->flags);
@@ -941,7 +971,7 @@ static void launch_bstart_x(Batch* b, hipStream_t s) {
                        b->gpre, b->nsys, (const int*)b->flags);
   if (nc > 0)
     hipLaunchKernelGGL(k_bgemv<true>, dim3((nc * 64 + 255) / 256), dim3(256), 0, s, b->sys,
-                       b->cpre, b->nsys, (const int*)b->flags);
+                       b->drun, b->dpre, nruns, (const int*)b->flags);
   hipLaunchKernelGGL(k_bstart_x, dim3(b->h_vpre[b->nsys]), dim3(256), 0, s, b->sys, b->vpre,
                      b->nsys, (const int*)b->flags);
 }
@@ -1106,25 +1136,40 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
   // prefix sums of the deflated systems' special blocks, groups, coarse rows
   b->h_rpre.assign(nsys + 1, 0);
   b->h_gpre.assign(nsys + 1, 0);
-  b->h_cpre.assign(nsys + 1, 0);
   b->h_npre.assign(nsys + 1, 0);
   for (int i = 0; i < nsys; ++i) {
     const bool d = b->hs[i].ngroups > 0;
     b->h_rpre[i + 1] = b->h_rpre[i] + (d ? handles[i]->nsblocks : 0);
     b->h_gpre[i + 1] = b->h_gpre[i] + b->hs[i].ngroups;
-    b->h_cpre[i + 1] = b->h_cpre[i] + b->hs[i].nc;
     b->h_npre[i + 1] = b->h_npre[i] + (d ? handles[i]->nnchunks : 0);
+    if (!d) continue;
+    // runs of consecutive deflated systems on one handle share E^-1
+    if (!b->h_drun.empty() && handles[b->h_drun.back().x] == handles[i] &&
+        b->h_drun.back().x + b->h_drun.back().y == i && b->h_drun.back().y < kBatchRhs)
+      b->h_drun.back().y++;
+    else
+      b->h_drun.push_back(make_int2(i, 1));
   }
+  b->h_dpre.assign(b->h_drun.size() + 1, 0);
+  for (size_t r = 0; r < b->h_drun.size(); ++r)
+    b->h_dpre[r + 1] = b->h_dpre[r] + b->hs[b->h_drun[r].x].nc;
+  const size_t nruns = std::max<size_t>(b->h_drun.size(), 1);
   if (hipMalloc((void**)&b->rpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&b->gpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&b->cpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&b->npre, (nsys + 1) * sizeof(int)) != hipSuccess) {
+      hipMalloc((void**)&b->npre, (nsys + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&b->dpre, (nruns + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&b->drun, nruns * sizeof(int2)) != hipSuccess) {
     set_error("batch allocation failed");
     return fail(PYTO_ENOMEM);
   }
   hipMemcpyAsync(b->rpre, b->h_rpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
   hipMemcpyAsync(b->gpre, b->h_gpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(b->cpre, b->h_cpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
+  if (!b->h_drun.empty()) {
+    hipMemcpyAsync(b->dpre, b->h_dpre.data(), b->h_dpre.size() * sizeof(int),
+                   hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(b->drun, b->h_drun.data(), b->h_drun.size() * sizeof(int2),
+                   hipMemcpyHostToDevice, s);
+  }
   hipMemcpyAsync(b->npre, b->h_npre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
   if (hipMalloc((void**)&b->sys, nsys * sizeof(BatchSys)) != hipSuccess ||
       hipMalloc((void**)&b->mpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
@@ -1161,7 +1206,7 @@ int pyto_batch_destroy(pyto_batch* b) {
   if (!b) return PYTO_OK;
   for (void* p : b->bufs) hipFree(p);
   void* ptrs[] = {b->sys, b->mpre, b->vpre, b->scal, b->flags, b->part_mv, b->part_v,
-                  b->rpre, b->gpre, b->cpre, b->npre};
+                  b->rpre, b->gpre, b->npre, b->dpre, b->drun};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (b->graph) hipGraphExecDestroy(b->graph);
