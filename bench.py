@@ -291,6 +291,9 @@ def main():
     ap.add_argument("--no-mma", action="store_true", help="skip the MMA step line")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 32-design line reported beside `value`")
+    ap.add_argument("--operator-only", action="store_true",
+                    help="only 20 launches of the FP32 operator on the workload (for "
+                         "counter collection); prints their HIP-event average")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -330,6 +333,24 @@ def main():
     log("set-up: host builders %.1f ms, device builders %.1f ms"
         % (t_setup_host * 1e3, t_setup_dev * 1e3))
     x_np = problems.config_density(args.config, grid)
+    if args.operator_only:
+        op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, None, penal=3.0, eps_void=1e-9,
+                          fixed_mask=mask)
+        op.set_density(torch.from_numpy(x_np).to(dev))
+        p32 = torch.randn(op.num_dofs, dtype=torch.float32, device=dev)
+        q32 = torch.empty_like(p32)
+        for _ in range(3):
+            op.matvec(p32, q32)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            op.matvec(p32, q32)
+        e1.record()
+        torch.cuda.synchronize()
+        print(json.dumps({"operator_only": True, "kernel": "k_matvec<float>",
+                          "us_per_launch": e0.elapsed_time(e1) / 20 * 1e3}))
+        return
     log("grid", grid, "dof", 3 * mesh.num_nodes)
     op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, None, penal=3.0, eps_void=1e-9,
                       fixed_mask=mask)

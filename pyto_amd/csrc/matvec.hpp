@@ -88,7 +88,7 @@ template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true>
 __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int pidx) {
   if (a.done && *a.done) return;
   __shared__ T pl[2][kPl][kPl][NC];
-  __shared__ T xc[2][kTile][kTile][2 * NC];
+  __shared__ T xc[2][kTile][kTile][2 * NC + 1];   // odd stride: no bank conflicts between rows
   __shared__ T es[2][kTile][kTile];   // E of the element layer below plane b
   __shared__ uint8_t pm[2][kPl][kPl];  // fixed bits of the staged nodes
   __shared__ double red[4];
