@@ -84,14 +84,11 @@ constexpr int F_RESUME = 8;
 // continues the previous one's direction
 constexpr double kBatchResumeDrift = 1.25;
 
-__device__ __forceinline__ int find_sys(const int* __restrict__ pre, int nsys, int b) {
-  int lo = 0, hi = nsys;   // pre[lo] <= b < pre[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (pre[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  return lo;
+// Every prefix-sum array pre[0..n] (blocks, groups or rows per system) is
+// followed on the device by its map: pre[n + 1 + b] is the system of item b
+// (pre[s] <= b < pre[s + 1]); one load instead of a search.
+__device__ __forceinline__ int find_sys(const int* __restrict__ pre, int n, int b) {
+  return pre[n + 1 + b];
 }
 
 // ---- operator --------------------------------------------------------------
@@ -893,6 +890,23 @@ struct pyto_batch : public pyto::Batch {};
 
 namespace pyto {
 
+// Upload prefix sums pre[0..n] with the item -> system map appended (the
+// layout find_sys reads), replacing *dev.  Synchronises `s`.
+static int upload_prefix(const std::vector<int>& pre, int** dev, hipStream_t s) {
+  const int n = (int)pre.size() - 1;
+  std::vector<int> buf(pre);
+  buf.reserve(pre.size() + (size_t)pre[n]);
+  for (int i = 0; i < n; ++i)
+    for (int k = pre[i]; k < pre[i + 1]; ++k) buf.push_back(i);
+  PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  if (*dev) PYTO_CHECK_HIP(hipFree(*dev));
+  *dev = nullptr;
+  PYTO_CHECK_HIP(hipMalloc((void**)dev, buf.size() * sizeof(int)));
+  PYTO_CHECK_HIP(hipMemcpyAsync(*dev, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  return PYTO_OK;
+}
+
 static int batch_geometry(Batch* b, int zc, hipStream_t s) {
   b->h_mpre.assign(b->nsys + 1, 0);
   b->h_vpre.assign(b->nsys + 1, 0);
@@ -915,10 +929,8 @@ static int batch_geometry(Batch* b, int zc, hipStream_t s) {
   }
   PYTO_CHECK_HIP(hipMemcpyAsync(b->sys, b->hs.data(), b->hs.size() * sizeof(BatchSys),
                                 hipMemcpyHostToDevice, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(b->mpre, b->h_mpre.data(), (b->nsys + 1) * sizeof(int),
-                                hipMemcpyHostToDevice, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(b->vpre, b->h_vpre.data(), (b->nsys + 1) * sizeof(int),
-                                hipMemcpyHostToDevice, s));
+  PYTO_TRY(upload_prefix(b->h_mpre, &b->mpre, s));
+  PYTO_TRY(upload_prefix(b->h_vpre, &b->vpre, s));
   PYTO_CHECK_HIP(hipStreamSynchronize(s));
   if (b->graph) {
     hipGraphExecDestroy(b->graph);
@@ -1154,26 +1166,21 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
   for (size_t r = 0; r < b->h_drun.size(); ++r)
     b->h_dpre[r + 1] = b->h_dpre[r] + b->hs[b->h_drun[r].x].nc;
   const size_t nruns = std::max<size_t>(b->h_drun.size(), 1);
-  if (hipMalloc((void**)&b->rpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&b->gpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&b->npre, (nsys + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&b->dpre, (nruns + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&b->drun, nruns * sizeof(int2)) != hipSuccess) {
+  if (hipMalloc((void**)&b->drun, nruns * sizeof(int2)) != hipSuccess) {
     set_error("batch allocation failed");
     return fail(PYTO_ENOMEM);
   }
-  hipMemcpyAsync(b->rpre, b->h_rpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(b->gpre, b->h_gpre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
-  if (!b->h_drun.empty()) {
-    hipMemcpyAsync(b->dpre, b->h_dpre.data(), b->h_dpre.size() * sizeof(int),
-                   hipMemcpyHostToDevice, s);
+  if (!b->h_drun.empty())
     hipMemcpyAsync(b->drun, b->h_drun.data(), b->h_drun.size() * sizeof(int2),
                    hipMemcpyHostToDevice, s);
+  {
+    int rc = upload_prefix(b->h_rpre, &b->rpre, s);
+    if (!rc) rc = upload_prefix(b->h_gpre, &b->gpre, s);
+    if (!rc) rc = upload_prefix(b->h_npre, &b->npre, s);
+    if (!rc) rc = upload_prefix(b->h_dpre, &b->dpre, s);
+    if (rc) return fail(rc);
   }
-  hipMemcpyAsync(b->npre, b->h_npre.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, s);
   if (hipMalloc((void**)&b->sys, nsys * sizeof(BatchSys)) != hipSuccess ||
-      hipMalloc((void**)&b->mpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&b->vpre, (nsys + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&b->scal, (size_t)nsys * kNumScalars * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&b->flags, (size_t)nsys * kNumFlags * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void**)&b->host_scal, (size_t)nsys * kNumScalars * sizeof(double),
