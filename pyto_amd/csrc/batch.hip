@@ -57,11 +57,12 @@ struct BatchSys {
   int ngroups, nc, ld;
   const int32_t* node_group;
   const double* centers;
-  const SpecialElem* sinfo;
-  const uint4* sidx;
-  const SpecialChunk* chunks;
-  const SpecialBlock* sblocks;
-  const int32_t* sbnodes;
+  // the handle's block-padded restriction tables
+  const int2* pwave;
+  const SpecialSlot* pslot;
+  const int32_t* pnodes;
+  const double* pcen;
+  const float* pE32;
   const int32_t* gptr;
   const int32_t* gidx;
   const double* Einv;
@@ -531,30 +532,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
   if (fl[F_DONE]) return;
   const BatchSys& S = sys[s];
   const double* sc = scal + s * kNumScalars;
-  const double beta = start ? (fl[F_RESUME] ? sc[S_RZ] / sc[S_RZPREV] : 0.0) : sc[S_BETA];
-  const SpecialBlock sb = S.sblocks[gb - rpre[s]];
+  const int b = gb - rpre[s];
   const int w = threadIdx.x >> 6;
-  const int wave = sb.chunk0 + min(w, sb.nch - 1);
   const int lane = threadIdx.x & 63;
-  const int ch_start = S.chunks[wave].start, ch_count = S.chunks[wave].count;
-  const int ch_ng = S.chunks[wave].ng;
-  const bool active = w < sb.nch && lane < ch_count;
-  SpecialElem se{0, 0, 0};
-  uint4 li = make_uint4(0, 0, 0, 0);
-  float Ev = 0.f;
-  if (active) {
-    se = S.sinfo[ch_start + lane];
-    li = S.sidx[ch_start + lane];
-    Ev = S.E32[se.e];
-  }
-  const int nst = 3 * sb.nnodes;
+  // the handle's padded tables: every load but the staged values depends on
+  // the block alone (as in the single-system restriction)
+  const int2 pw = S.pwave[b * kSpecialBlockChunks + w];   // (chunk or -1, ng)
+  const int64_t slot = ((int64_t)b * kSpecialBlockChunks + w) * 64 + lane;
+  const SpecialSlot se = S.pslot[slot];
+  const float Ev = S.pE32[slot];
   uint32_t dof[kStage];
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
-    const int i = min((int)threadIdx.x + 256 * t, nst - 1);
+    const int i = (int)threadIdx.x + 256 * t;
     const int q = i / 3;
-    dof[t] = 3u * (uint32_t)S.sbnodes[sb.node0 + q] + (uint32_t)(i - 3 * q);
+    dof[t] = 3u * (uint32_t)S.pnodes[(int64_t)b * kSpecialBlockNodes + q] + (uint32_t)(i - 3 * q);
   }
+  if (threadIdx.x < kSpecialBlockChunks * 24)
+    (&csh[0][0])[threadIdx.x] = S.pcen[(int64_t)b * kSpecialBlockChunks * 24 + threadIdx.x];
+  const double beta = start ? (fl[F_RESUME] ? sc[S_RZ] / sc[S_RZPREV] : 0.0) : sc[S_BETA];
   const bool has_p = beta != 0.0;
   float ra[kStage], da[kStage], pa[kStage];
 #pragma unroll
@@ -563,26 +559,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     da[t] = S.invd32[dof[t]];
     pa[t] = has_p ? S.p[dof[t]] : 0.f;
   }
-  if (threadIdx.x < kSpecialBlockChunks * 24) {
-    const int cw = threadIdx.x / 24, r = threadIdx.x - 24 * cw;
-    if (cw < sb.nch) {
-      const SpecialChunk* c = S.chunks + sb.chunk0 + cw;
-      const int sl = r / 3;
-      if (sl < c->ng) csh[cw][r] = S.centers[3 * c->groups[sl] + (r - 3 * sl)];
-    }
-  }
 #pragma unroll
-  for (int t = 0; t < kStage; ++t) {
-    const int i = threadIdx.x + 256 * t;
-    if (i < nst) vs[i] = (float)((double)(da[t] * ra[t]) + beta * (double)pa[t]);
-  }
+  for (int t = 0; t < kStage; ++t)
+    vs[threadIdx.x + 256 * t] = (float)((double)(da[t] * ra[t]) + beta * (double)pa[t]);
   __syncthreads();
-  if (w >= sb.nch) return;
+  if (pw.x < 0) return;
+  const bool active = se.e >= 0;
   float f[8][3];
   uint64_t info = 0;
   int ix = 0, iy = 0, iz = 0;
   if (active) {
-    const uint32_t lw[4] = {li.x, li.y, li.z, li.w};
+    const uint32_t lw[4] = {se.li.x, se.li.y, se.li.z, se.li.w};
     info = se.info;
     elem_coords(se.e, S.g, ix, iy, iz);
     float u[8][3];
@@ -607,7 +594,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     for (int j = 0; j < 8; ++j) f[j][0] = f[j][1] = f[j][2] = 0.f;
   }
   const double* cen = csh[w];
-  for (int sl = 0; sl < ch_ng; ++sl) {
+  for (int sl = 0; sl < pw.y; ++sl) {
     const double cx = cen[3 * sl], cy = cen[3 * sl + 1], cz = cen[3 * sl + 2];
     const float x0 = (float)(ix * S.g.hx - cx), y0 = (float)(iy * S.g.hy - cy),
                 z0 = (float)(iz * S.g.hz - cz);
@@ -630,7 +617,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
       const double tot = wave_reduce_sum(acc[j]);
-      if (lane == 0) S.cpart[(int64_t)wave * 48 + 6 * sl + j] = tot;
+      if (lane == 0) S.cpart[(int64_t)pw.x * 48 + 6 * sl + j] = tot;
     }
   }
 }
@@ -1097,7 +1084,7 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
     S.r64 = f64[0];
     S.q64 = f64[1];
     if (h->ngroups > 0) {
-      if (!h->sblocks || !h->Einv || h->nsblocks <= 0) {
+      if (!h->pslot || !h->Einv || h->nsblocks <= 0) {
         set_error("system %d: the handle's deflation groups have no coarse space", i);
         return fail(PYTO_ESTATE);
       }
@@ -1106,11 +1093,11 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
       S.ld = (h->nc + 63) / 64 * 64;
       S.node_group = h->node_group;
       S.centers = h->centers;
-      S.sinfo = h->sinfo;
-      S.sidx = reinterpret_cast<const uint4*>(h->sidx);
-      S.chunks = h->chunks;
-      S.sblocks = h->sblocks;
-      S.sbnodes = h->sbnodes;
+      S.pwave = h->pwave;
+      S.pslot = h->pslot;
+      S.pnodes = h->pnodes;
+      S.pcen = h->pcen;
+      S.pE32 = h->pE32;
       S.gptr = h->gptr;
       S.gidx = h->gidx;
       S.Einv = h->Einv;
