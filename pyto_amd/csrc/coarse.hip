@@ -987,18 +987,18 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
       gc.exec = nullptr;
     }
   // release a previous coarse space
-  void* olds[] = {h->node_group, h->centers, h->special, h->sinfo, h->chunks, h->cpart,
-                  h->sblocks, h->sbnodes, h->sidx, h->gptr, h->gidx,
+  void* olds[] = {h->node_group, h->centers, h->sinfo, h->chunks, h->cpart,
+                  h->gptr, h->gidx,
                   h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
                   h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres,
                   h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
                   h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32};
   for (void* p : olds)
     if (p) hipFree(p);
-  h->node_group = nullptr; h->centers = nullptr; h->special = nullptr; h->chunks = nullptr;
+  h->node_group = nullptr; h->centers = nullptr; h->chunks = nullptr;
   h->sinfo = nullptr;
   h->cpart = nullptr; h->gptr = nullptr; h->gidx = nullptr;
-  h->sblocks = nullptr; h->sbnodes = nullptr; h->sidx = nullptr; h->nsblocks = 0;
+  h->nsblocks = 0;
   h->gnodes = nullptr; h->nchunk = nullptr; h->nchunk_grp = nullptr; h->npart = nullptr;
   h->nptr = nullptr; h->nidx = nullptr; h->nnchunks = 0;
   h->Ec = h->Einv = h->cwork = h->ct = h->cmu = h->cres = nullptr;
@@ -1245,13 +1245,9 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   const int ld = round_up64(h->nc);
   PYTO_TRY(device_alloc(h, (void**)&h->node_group, nn * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->centers, (size_t)ngroups * 3 * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->special, special.size() * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->sinfo, sinfo.size() * sizeof(SpecialElem)));
   PYTO_TRY(device_alloc(h, (void**)&h->chunks, chunks.size() * sizeof(SpecialChunk)));
   PYTO_TRY(device_alloc(h, (void**)&h->cpart, chunks.size() * 48 * sizeof(double)));
-  PYTO_TRY(device_alloc(h, (void**)&h->sblocks, std::max<size_t>(sblocks.size(), 1) * sizeof(SpecialBlock)));
-  PYTO_TRY(device_alloc(h, (void**)&h->sbnodes, std::max<size_t>(sbnodes.size(), 1) * sizeof(int32_t)));
-  PYTO_TRY(device_alloc(h, (void**)&h->sidx, std::max<size_t>(sidx.size(), 8) * sizeof(uint16_t)));
   PYTO_TRY(device_alloc(h, (void**)&h->gnodes, nn * sizeof(int32_t)));
   PYTO_TRY(device_alloc(h, (void**)&h->nchunk, nchunk.size() * sizeof(int2)));
   PYTO_TRY(device_alloc(h, (void**)&h->nchunk_grp, nchunk.size() * sizeof(int32_t)));
@@ -1281,17 +1277,9 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_CHECK_HIP(hipMemcpyAsync(h->node_group, node_group, nn * 4, hipMemcpyDeviceToDevice, s));
   PYTO_CHECK_HIP(hipMemcpyAsync(h->centers, centers, (size_t)ngroups * 24, hipMemcpyDeviceToDevice, s));
   if (!special.empty()) {
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->special, special.data(), special.size() * 4,
-                                  hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->sinfo, sinfo.data(), sinfo.size() * sizeof(SpecialElem),
                                   hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->chunks, chunks.data(), chunks.size() * sizeof(SpecialChunk),
-                                  hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->sblocks, sblocks.data(), sblocks.size() * sizeof(SpecialBlock),
-                                  hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->sbnodes, sbnodes.data(), sbnodes.size() * sizeof(int32_t),
-                                  hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->sidx, sidx.data(), sidx.size() * sizeof(uint16_t),
                                   hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_off, gal_off.data(), gal_off.size() * 4,
                                   hipMemcpyHostToDevice, s));

@@ -137,14 +137,10 @@ struct Handle {
   int32_t* node_group = nullptr;
   double* centers = nullptr;
   int nspecial = 0, nchunks = 0;
-  int32_t* special = nullptr;   // element ids sorted by group set
   SpecialElem* sinfo = nullptr; // same order, with packed slots and fixed bits
   SpecialChunk* chunks = nullptr;
   double* cpart = nullptr;      // per-chunk partial W^T K v (nchunks x 8 x 6)
   int nsblocks = 0;
-  SpecialBlock* sblocks = nullptr;
-  int32_t* sbnodes = nullptr;   // per block: its sorted unique corner nodes
-  uint4* sidx = nullptr;        // per special element: 8 block-local corner indices (u16)
   // the same blocks padded, so that every load of a restriction block except
   // the staged vector values depends on the block index alone
   int2* pwave = nullptr;        // per (block, wave): (chunk index or -1, ng)
