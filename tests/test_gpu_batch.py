@@ -102,6 +102,29 @@ def test_deflated_batch_matches_single_solves(gpu):
     bp.close()
 
 
+def test_deflated_batch_many_load_cases(gpu):
+    """A design with 10 load cases (the coarse GEMV serves at most 8 systems
+    of one design per run) and the same design listed again after another
+    one: every system against single solves."""
+    from pyto_amd.batch import BatchSolver
+    print("deflated batch, 10 load cases", flush=True)
+    op_a, loads_a = _design((32, 16, 16), 31, 10, gpu, (4, 2, 2))
+    op_b, loads_b = _design((24, 12, 8), 32, 2, gpu, (3, 1, 1))
+    bs = BatchSolver([op_a, op_b, op_a], [loads_a, loads_b, loads_a[:3]])
+    us, stats = bs.solve(rtol=1e-8)
+    assert bs.nsys == 15
+    ops, loads = [op_a, op_b, op_a], [loads_a, loads_b, loads_a[:3]]
+    for (d, k), u, st in zip(bs.systems, us, stats):
+        assert st["converged"] and st["rel"] <= 1e-8
+        res, eu, eC = _check_against_single(ops[d], loads[d][k], u)
+        print(d, k, st["iterations"], "res %.1e err u %.1e C %.1e" % (res, eu, eC), flush=True)
+        assert res <= 1e-6 and eu <= 1e-5 and eC <= 1e-7
+    # the third design repeats the first one's first three load cases
+    for k in range(3):
+        assert torch.equal(us[12 + k], us[k])
+    bs.close()
+
+
 def test_deflated_batch_brick_elements(gpu):
     """Brick elements (the non-cube element form in the batched restriction
     and operator): deflated and plain designs against single solves."""
