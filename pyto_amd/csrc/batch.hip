@@ -614,11 +614,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
       acc[4] += (double)(rz * fx - rx * fz);
       acc[5] += (double)(rx * fy - ry * fx);
     }
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const double tot = wave_reduce_sum(acc[j]);
-      if (lane == 0) S.cpart[(int64_t)pw.x * 48 + 6 * sl + j] = tot;
-    }
+    const double tot = wave_reduce6(acc);
+    if ((lane & 7) == 0 && lane < 48) S.cpart[(int64_t)pw.x * 48 + 6 * sl + (lane >> 3)] = tot;
   }
 }
 
@@ -654,10 +651,9 @@ __global__ __launch_bounds__(256) void k_brestrict_full(const BatchSys* __restri
     c[5] = -ry * vx + rx * vy;
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const double sj = wave_reduce_sum(c[j]);
-    if (lane == 0) red[w][j] = sj;
+  {
+    const double sj = wave_reduce6(c);
+    if ((lane & 7) == 0 && lane < 48) red[w][lane >> 3] = sj;
   }
   __syncthreads();
   if (threadIdx.x < 6) {
@@ -691,11 +687,8 @@ __global__ __launch_bounds__(256) void k_bgather(const BatchSys* __restrict__ sy
 #pragma unroll
     for (int j = 0; j < 6; ++j) acc[j] += c[j];
   }
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const double t = wave_reduce_sum(acc[j]);
-    if (lane == 0) S.ct[6 * g + j] = t;
-  }
+  const double t = wave_reduce6(acc);
+  if ((lane & 7) == 0 && lane < 48) S.ct[6 * g + (lane >> 3)] = t;
 }
 
 // mu = E^-1 t for the deflated systems: one wave per coarse row of each run

@@ -74,10 +74,9 @@ __global__ __launch_bounds__(256) void k_restrict_full(const T* __restrict__ v,
     c[5] = -ry * vx + rx * vy;
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const double sj = wave_reduce_sum(c[j]);
-    if (lane == 0) red[w][j] = sj;
+  {
+    const double sj = wave_reduce6(c);
+    if ((lane & 7) == 0 && lane < 48) red[w][lane >> 3] = sj;
   }
   __syncthreads();
   if (threadIdx.x < 6) {
@@ -250,11 +249,8 @@ k_restrict_special(
       acc[4] += (double)(rz * fx - rx * fz);
       acc[5] += (double)(rx * fy - ry * fx);
     }
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const double tot = wave_reduce_sum(acc[j]);
-      if (lane == 0) cpart[(int64_t)pw.x * 48 + 6 * sl + j] = tot;
-    }
+    const double tot = wave_reduce6(acc);
+    if ((lane & 7) == 0 && lane < 48) cpart[(int64_t)pw.x * 48 + 6 * sl + (lane >> 3)] = tot;
   }
 }
 
@@ -289,11 +285,8 @@ __global__ __launch_bounds__(256) void k_coarse_gather(const double* __restrict_
 #pragma unroll
     for (int j = 0; j < 6; ++j) acc[j] += c[j];
   }
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const double s = wave_reduce_sum(acc[j]);
-    if (lane == 0) t[6 * g + j] = s;
-  }
+  const double s = wave_reduce6(acc);
+  if ((lane & 7) == 0 && lane < 48) t[6 * g + (lane >> 3)] = s;
 }
 
 static int round_up64(int n) { return (n + 63) / 64 * 64; }
@@ -330,11 +323,8 @@ __global__ __launch_bounds__(256) void k_gather_scalars(
 #pragma unroll
     for (int j = 0; j < 6; ++j) acc[j] += c[j];
   }
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const double sj = wave_reduce_sum(acc[j]);
-    if (lane == 0) t[6 * g + j] = sj;
-  }
+  const double sj = wave_reduce6(acc);
+  if ((lane & 7) == 0 && lane < 48) t[6 * g + (lane >> 3)] = sj;
 }
 
 int launch_gather_scalars(Handle* h, double* t, const double* part_pq, int npq,
@@ -678,11 +668,8 @@ __global__ __launch_bounds__(256) void k_galerkin_parts(
           for (int jb = 0; jb < 6; ++jb) acc[jb] += w[jb] * f[j][c];
         }
       }
-#pragma unroll
-      for (int jb = 0; jb < 6; ++jb) {
-        const double tot = wave_reduce_sum(acc[jb]);
-        if (lane == 0) out[(sa * ng + sb) * 36 + 6 * ja + jb] = tot;
-      }
+      const double tot = wave_reduce6(acc);
+      if ((lane & 7) == 0 && lane < 48) out[(sa * ng + sb) * 36 + 6 * ja + (lane >> 3)] = tot;
     }
   }
 }

@@ -260,6 +260,34 @@ __device__ __forceinline__ double wave_reduce_sum(double v) {
   return v;
 }
 
+// Wave sums of six per-lane values at once: every lane l ends with the total
+// over the wave of value l >> 3 (lanes 48..63 with a sum of zeros).  Three
+// exchange steps (lane offsets 32, 16, 8) halve the values a lane carries,
+// three more sum the last one within groups of 8 lanes: 12 cross-lane steps
+// instead of 36 for six separate reductions, in a fixed order.
+__device__ __forceinline__ double wave_reduce6(const double a[6]) {
+  const int lane = threadIdx.x & 63;
+  const double v[8] = {a[0], a[1], a[2], a[3], a[4], a[5], 0.0, 0.0};
+  const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0, h3 = (lane & 8) != 0;
+  double k4[4], k2[2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double give = h5 ? v[i] : v[4 + i];
+    k4[i] = (h5 ? v[4 + i] : v[i]) + __shfl_xor(give, 32, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double give = h4 ? k4[i] : k4[2 + i];
+    k2[i] = (h4 ? k4[2 + i] : k4[i]) + __shfl_xor(give, 16, 64);
+  }
+  const double give = h3 ? k2[0] : k2[1];
+  double k1 = (h3 ? k2[1] : k2[0]) + __shfl_xor(give, 8, 64);
+  k1 += __shfl_xor(k1, 4, 64);
+  k1 += __shfl_xor(k1, 2, 64);
+  k1 += __shfl_xor(k1, 1, 64);
+  return k1;
+}
+
 // Block-wide FP64 sum for blockDim.x == 256; result valid in thread 0.
 __device__ __forceinline__ double block_reduce_sum256(double v, double* lds4) {
   v = wave_reduce_sum(v);
