@@ -444,7 +444,9 @@ __global__ __launch_bounds__(256) void k_bdir_defl(const BatchSys* __restrict__ 
     bdir_plain<FIRST>(S, v, sc, resume);
     return;
   }
-  __shared__ double aw[kBLdsGroups * 6];
+  // 6 values per group, sized by the launch for the batch's largest staged
+  // group count (at most kBLdsGroups)
+  extern __shared__ double aw[];
   const bool lds = S.ngroups <= kBLdsGroups;
   auto coeffs = [&](int q, double* e) {
     const double* m6 = S.cmu + 6 * q;
@@ -858,6 +860,7 @@ struct Batch {
   // deflation: prefix sums over the deflated systems' special blocks, groups
   // and coarse rows (host and device)
   bool defl = false;
+  size_t dir_lds = 0;              // dynamic LDS of k_bdir_defl
   std::vector<int> h_rpre, h_gpre, h_npre, h_dpre;
   std::vector<int2> h_drun;        // runs of systems sharing one deflated handle
   int *rpre = nullptr, *gpre = nullptr, *npre = nullptr, *dpre = nullptr;
@@ -978,7 +981,7 @@ static int batch_iteration(Batch* b, hipStream_t s) {
                      b->scal, b->flags, (const double*)b->part_mv, (const double*)b->part_v);
   if (b->defl) {
     launch_bcoarse(b, s, false);
-    hipLaunchKernelGGL(k_bdir_defl<false>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, b->nsys,
+    hipLaunchKernelGGL(k_bdir_defl<false>, dim3(nv), dim3(256), b->dir_lds, s, b->sys, b->vpre, b->nsys,
                        (const double*)b->scal, (const int*)b->flags);
   } else {
     hipLaunchKernelGGL(k_bdir<false>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, b->nsys,
@@ -1142,6 +1145,9 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
     else
       b->h_drun.push_back(make_int2(i, 1));
   }
+  for (int i = 0; i < nsys; ++i)
+    if (b->hs[i].ngroups > 0 && b->hs[i].ngroups <= kBLdsGroups)
+      b->dir_lds = std::max(b->dir_lds, (size_t)b->hs[i].ngroups * 48);
   b->h_dpre.assign(b->h_drun.size() + 1, 0);
   for (size_t r = 0; r < b->h_drun.size(); ++r)
     b->h_dpre[r + 1] = b->h_dpre[r] + b->hs[b->h_drun[r].x].nc;
@@ -1304,7 +1310,7 @@ int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, do
       hipLaunchKernelGGL(k_bsetup, dim3(nsys), dim3(256), 0, s, b->vpre, nsys, b->scal,
                          b->flags, (const double*)b->part_v, rtol, inner_rtol);
       launch_bcoarse(b, s, true);
-      hipLaunchKernelGGL(k_bdir_defl<true>, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
+      hipLaunchKernelGGL(k_bdir_defl<true>, dim3(nv), dim3(256), b->dir_lds, s, b->sys, b->vpre, nsys,
                          (const double*)b->scal, (const int*)b->flags);
     } else {
       hipLaunchKernelGGL(k_binit, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,

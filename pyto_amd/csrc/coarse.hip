@@ -379,7 +379,8 @@ __global__ __launch_bounds__(256) void k_dir_update(
     GridDims g, int nc, const double* __restrict__ scal, bool first,
     const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double aw[kLdsGroups * 6];
+  // 6 values per group, sized by the launch (ngroups * 48 bytes when staged)
+  extern __shared__ double aw[];
   const bool lds = grp && ngroups <= kLdsGroups;
   if (lds) {
     for (int q = threadIdx.x; q < ngroups; q += blockDim.x) {
@@ -500,7 +501,8 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
     const double* __restrict__ centers, int ngroups, const uint8_t* __restrict__ mask,
     GridDims g, const double* __restrict__ scal, bool first, const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double aw[kLdsGroups * 6];
+  // 6 values per group, sized by the launch (ngroups * 48 bytes when staged)
+  extern __shared__ double aw[];
   const bool lds = grp && ngroups <= kLdsGroups;
   if (lds) {
     for (int q = threadIdx.x; q < ngroups; q += blockDim.x) {
@@ -576,9 +578,12 @@ int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* 
                       hipStream_t s) {
   // 2048 workgroups (8 per CU) striding over the vectors: each stages the
   // group maps once, so fewer, longer-lived blocks keep those L2 reads small
+  // the group table in LDS takes only what the groups need, so that all
+  // workgroups fit on the chip at once
+  const size_t lds = (h->ngroups > 0 && h->ngroups <= kLdsGroups) ? (size_t)h->ngroups * 48 : 0;
   if (h->ncomp == 3 && sizeof(T) == 8) {
     const int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_dir_update_nodes<T>, dim3(blocks), dim3(256), 0, s, x, p, z,
+    hipLaunchKernelGGL(k_dir_update_nodes<T>, dim3(blocks), dim3(256), lds, s, x, p, z,
                        (const double*)h->cmu,
                        h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
                        h->ngroups, h->fixmask, h->g, (const double*)h->scal, first, done);
@@ -587,7 +592,7 @@ int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* 
   }
   const int64_t nvec = h->ncomp * h->g.nnodes / (16 / (int64_t)sizeof(T));
   const int blocks = (int)std::min<int64_t>((nvec + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), 0, s, x, p, z,
+  hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), lds, s, x, p, z,
                      (const double*)h->cmu,
                      h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
                      h->ngroups, h->fixmask, h->g, h->ncomp, (const double*)h->scal, first,
