@@ -924,6 +924,7 @@ void release_blas(Handle* h) {
 }
 
 int coarse_setup(Handle* h, hipStream_t s) {
+  h->coarse_ready = false;   // until the new factor is complete
   const int ld = round_up64(h->nc);
   if (h->nsblocks > 0) {
     const int64_t n = (int64_t)h->nsblocks * kSpecialBlockChunks * 64;
@@ -956,6 +957,31 @@ int coarse_setup(Handle* h, hipStream_t s) {
   return PYTO_OK;
 }
 
+// Free a handle's coarse space and return it to the no-groups state.
+static void release_coarse(Handle* h) {
+  void* olds[] = {h->node_group, h->centers, h->sinfo, h->chunks, h->cpart,
+                  h->gptr, h->gidx,
+                  h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
+                  h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres,
+                  h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
+                  h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32};
+  for (void* p : olds)
+    if (p) hipFree(p);
+  h->node_group = nullptr; h->centers = nullptr; h->chunks = nullptr;
+  h->sinfo = nullptr;
+  h->cpart = nullptr; h->gptr = nullptr; h->gidx = nullptr;
+  h->nsblocks = 0;
+  h->gnodes = nullptr; h->nchunk = nullptr; h->nchunk_grp = nullptr; h->npart = nullptr;
+  h->nptr = nullptr; h->nidx = nullptr; h->nnchunks = 0;
+  h->Ec = h->Einv = h->cwork = h->ct = h->cmu = h->cres = nullptr;
+  h->gal_off = nullptr; h->gal_part = nullptr; h->gal_pairs = nullptr; h->gal_pptr = nullptr;
+  h->gal_pidx = nullptr; h->npairs = 0;
+  h->pwave = nullptr; h->pslot = nullptr; h->pnodes = nullptr; h->pcen = nullptr;
+  h->pE64 = nullptr; h->pE32 = nullptr;
+  h->ngroups = 0; h->nc = 0; h->nspecial = 0; h->nchunks = 0; h->coarse_ready = false;
+  h->groups_gen++;
+}
+
 }  // namespace pyto
 
 using namespace pyto;
@@ -978,28 +1004,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
       hipGraphExecDestroy(gc.exec);
       gc.exec = nullptr;
     }
-  // release a previous coarse space
-  void* olds[] = {h->node_group, h->centers, h->sinfo, h->chunks, h->cpart,
-                  h->gptr, h->gidx,
-                  h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
-                  h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres,
-                  h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
-                  h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32};
-  for (void* p : olds)
-    if (p) hipFree(p);
-  h->node_group = nullptr; h->centers = nullptr; h->chunks = nullptr;
-  h->sinfo = nullptr;
-  h->cpart = nullptr; h->gptr = nullptr; h->gidx = nullptr;
-  h->nsblocks = 0;
-  h->gnodes = nullptr; h->nchunk = nullptr; h->nchunk_grp = nullptr; h->npart = nullptr;
-  h->nptr = nullptr; h->nidx = nullptr; h->nnchunks = 0;
-  h->Ec = h->Einv = h->cwork = h->ct = h->cmu = h->cres = nullptr;
-  h->gal_off = nullptr; h->gal_part = nullptr; h->gal_pairs = nullptr; h->gal_pptr = nullptr;
-  h->gal_pidx = nullptr; h->npairs = 0;
-  h->pwave = nullptr; h->pslot = nullptr; h->pnodes = nullptr; h->pcen = nullptr;
-  h->pE64 = nullptr; h->pE32 = nullptr;
-  h->ngroups = 0; h->nc = 0; h->nspecial = 0; h->nchunks = 0; h->coarse_ready = false;
-  h->groups_gen++;
+  release_coarse(h);
   if (ngroups == 0) return PYTO_OK;
   PYTO_REQUIRE(node_group && centers, PYTO_EINVAL, "null group arrays");
   const int64_t nn = h->g.nnodes;
@@ -1228,85 +1233,94 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     std::vector<int32_t> pos(nptr.begin(), nptr.end() - 1);
     for (size_t c = 0; c < nchunk.size(); ++c) nidx[pos[nchunk_grp[c]]++] = (int32_t)(c * 8);
   }
-  h->ngroups = ngroups;
-  h->nc = 6 * ngroups;
-  h->nnchunks = (int)nchunk.size();
-  h->nspecial = (int)special.size();
-  h->nchunks = (int)chunks.size();
-  h->nsblocks = (int)sblocks.size();
-  const int ld = round_up64(h->nc);
-  PYTO_TRY(device_alloc(h, (void**)&h->node_group, nn * 4));
-  PYTO_TRY(device_alloc(h, (void**)&h->centers, (size_t)ngroups * 3 * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->sinfo, sinfo.size() * sizeof(SpecialElem)));
-  PYTO_TRY(device_alloc(h, (void**)&h->chunks, chunks.size() * sizeof(SpecialChunk)));
-  PYTO_TRY(device_alloc(h, (void**)&h->cpart, chunks.size() * 48 * sizeof(double)));
-  PYTO_TRY(device_alloc(h, (void**)&h->gnodes, nn * sizeof(int32_t)));
-  PYTO_TRY(device_alloc(h, (void**)&h->nchunk, nchunk.size() * sizeof(int2)));
-  PYTO_TRY(device_alloc(h, (void**)&h->nchunk_grp, nchunk.size() * sizeof(int32_t)));
-  PYTO_TRY(device_alloc(h, (void**)&h->npart, nchunk.size() * 48 * sizeof(double)));
-  PYTO_TRY(device_alloc(h, (void**)&h->nptr, nptr.size() * sizeof(int32_t)));
-  PYTO_TRY(device_alloc(h, (void**)&h->nidx, nidx.size() * sizeof(int32_t)));
-  PYTO_TRY(device_alloc(h, (void**)&h->gptr, gptr.size() * sizeof(int32_t)));
-  PYTO_TRY(device_alloc(h, (void**)&h->gidx, std::max<size_t>(gidx.size(), 1) * sizeof(int32_t)));
-  h->npairs = (int)gal_pairs.size();
-  PYTO_TRY(device_alloc(h, (void**)&h->pwave, std::max<size_t>(pwave.size(), 1) * sizeof(int2)));
-  PYTO_TRY(device_alloc(h, (void**)&h->pslot, std::max<size_t>(pslot.size(), 1) * sizeof(SpecialSlot)));
-  PYTO_TRY(device_alloc(h, (void**)&h->pnodes, std::max<size_t>(pnodes.size(), 1) * 4));
-  PYTO_TRY(device_alloc(h, (void**)&h->pcen, std::max<size_t>(pcen.size(), 1) * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->pE64, std::max<size_t>(pslot.size(), 1) * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->pE32, std::max<size_t>(pslot.size(), 1) * 4));
-  PYTO_TRY(device_alloc(h, (void**)&h->gal_off, std::max<size_t>(gal_off.size(), 1) * 4));
-  PYTO_TRY(device_alloc(h, (void**)&h->gal_part, std::max<int64_t>(galsz, 1) * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->gal_pairs, std::max<size_t>(gal_pairs.size(), 1) * sizeof(int2)));
-  PYTO_TRY(device_alloc(h, (void**)&h->gal_pptr, gal_pptr.size() * 4));
-  PYTO_TRY(device_alloc(h, (void**)&h->gal_pidx, std::max<size_t>(gal_pidx.size(), 1) * 4));
-  PYTO_TRY(device_alloc(h, (void**)&h->Ec, (size_t)ld * ld * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->Einv, (size_t)ld * ld * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->cwork, (size_t)ld * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->ct, (size_t)2 * ld * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->cmu, (size_t)ld * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->cres, (size_t)ld * 8));
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->node_group, node_group, nn * 4, hipMemcpyDeviceToDevice, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->centers, centers, (size_t)ngroups * 24, hipMemcpyDeviceToDevice, s));
-  if (!special.empty()) {
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->sinfo, sinfo.data(), sinfo.size() * sizeof(SpecialElem),
+  // install the new space; a failure part-way leaves the handle without groups
+  auto install = [&]() -> int {
+    h->ngroups = ngroups;
+    h->nc = 6 * ngroups;
+    h->nnchunks = (int)nchunk.size();
+    h->nspecial = (int)special.size();
+    h->nchunks = (int)chunks.size();
+    h->nsblocks = (int)sblocks.size();
+    const int ld = round_up64(h->nc);
+    PYTO_TRY(device_alloc(h, (void**)&h->node_group, nn * 4));
+    PYTO_TRY(device_alloc(h, (void**)&h->centers, (size_t)ngroups * 3 * 8));
+    PYTO_TRY(device_alloc(h, (void**)&h->sinfo, sinfo.size() * sizeof(SpecialElem)));
+    PYTO_TRY(device_alloc(h, (void**)&h->chunks, chunks.size() * sizeof(SpecialChunk)));
+    PYTO_TRY(device_alloc(h, (void**)&h->cpart, chunks.size() * 48 * sizeof(double)));
+    PYTO_TRY(device_alloc(h, (void**)&h->gnodes, nn * sizeof(int32_t)));
+    PYTO_TRY(device_alloc(h, (void**)&h->nchunk, nchunk.size() * sizeof(int2)));
+    PYTO_TRY(device_alloc(h, (void**)&h->nchunk_grp, nchunk.size() * sizeof(int32_t)));
+    PYTO_TRY(device_alloc(h, (void**)&h->npart, nchunk.size() * 48 * sizeof(double)));
+    PYTO_TRY(device_alloc(h, (void**)&h->nptr, nptr.size() * sizeof(int32_t)));
+    PYTO_TRY(device_alloc(h, (void**)&h->nidx, nidx.size() * sizeof(int32_t)));
+    PYTO_TRY(device_alloc(h, (void**)&h->gptr, gptr.size() * sizeof(int32_t)));
+    PYTO_TRY(device_alloc(h, (void**)&h->gidx, std::max<size_t>(gidx.size(), 1) * sizeof(int32_t)));
+    h->npairs = (int)gal_pairs.size();
+    PYTO_TRY(device_alloc(h, (void**)&h->pwave, std::max<size_t>(pwave.size(), 1) * sizeof(int2)));
+    PYTO_TRY(device_alloc(h, (void**)&h->pslot, std::max<size_t>(pslot.size(), 1) * sizeof(SpecialSlot)));
+    PYTO_TRY(device_alloc(h, (void**)&h->pnodes, std::max<size_t>(pnodes.size(), 1) * 4));
+    PYTO_TRY(device_alloc(h, (void**)&h->pcen, std::max<size_t>(pcen.size(), 1) * 8));
+    PYTO_TRY(device_alloc(h, (void**)&h->pE64, std::max<size_t>(pslot.size(), 1) * 8));
+    PYTO_TRY(device_alloc(h, (void**)&h->pE32, std::max<size_t>(pslot.size(), 1) * 4));
+    PYTO_TRY(device_alloc(h, (void**)&h->gal_off, std::max<size_t>(gal_off.size(), 1) * 4));
+    PYTO_TRY(device_alloc(h, (void**)&h->gal_part, std::max<int64_t>(galsz, 1) * 8));
+    PYTO_TRY(device_alloc(h, (void**)&h->gal_pairs, std::max<size_t>(gal_pairs.size(), 1) * sizeof(int2)));
+    PYTO_TRY(device_alloc(h, (void**)&h->gal_pptr, gal_pptr.size() * 4));
+    PYTO_TRY(device_alloc(h, (void**)&h->gal_pidx, std::max<size_t>(gal_pidx.size(), 1) * 4));
+    PYTO_TRY(device_alloc(h, (void**)&h->Ec, (size_t)ld * ld * 8));
+    PYTO_TRY(device_alloc(h, (void**)&h->Einv, (size_t)ld * ld * 8));
+    PYTO_TRY(device_alloc(h, (void**)&h->cwork, (size_t)ld * 8));
+    PYTO_TRY(device_alloc(h, (void**)&h->ct, (size_t)2 * ld * 8));
+    PYTO_TRY(device_alloc(h, (void**)&h->cmu, (size_t)ld * 8));
+    PYTO_TRY(device_alloc(h, (void**)&h->cres, (size_t)ld * 8));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->node_group, node_group, nn * 4, hipMemcpyDeviceToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->centers, centers, (size_t)ngroups * 24, hipMemcpyDeviceToDevice, s));
+    if (!special.empty()) {
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->sinfo, sinfo.data(), sinfo.size() * sizeof(SpecialElem),
+                                    hipMemcpyHostToDevice, s));
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->chunks, chunks.data(), chunks.size() * sizeof(SpecialChunk),
+                                    hipMemcpyHostToDevice, s));
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_off, gal_off.data(), gal_off.size() * 4,
+                                    hipMemcpyHostToDevice, s));
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->pwave, pwave.data(), pwave.size() * sizeof(int2),
+                                    hipMemcpyHostToDevice, s));
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->pslot, pslot.data(), pslot.size() * sizeof(SpecialSlot),
+                                    hipMemcpyHostToDevice, s));
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->pnodes, pnodes.data(), pnodes.size() * 4,
+                                    hipMemcpyHostToDevice, s));
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->pcen, pcen.data(), pcen.size() * 8, hipMemcpyHostToDevice, s));
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pairs, gal_pairs.data(), gal_pairs.size() * sizeof(int2),
+                                    hipMemcpyHostToDevice, s));
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pptr, gal_pptr.data(), gal_pptr.size() * 4,
+                                    hipMemcpyHostToDevice, s));
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pidx, gal_pidx.data(), gal_pidx.size() * 4,
+                                    hipMemcpyHostToDevice, s));
+    }
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->gnodes, gnodes.data(), nn * sizeof(int32_t),
                                   hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->chunks, chunks.data(), chunks.size() * sizeof(SpecialChunk),
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->nchunk, nchunk.data(), nchunk.size() * sizeof(int2),
                                   hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_off, gal_off.data(), gal_off.size() * 4,
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->nchunk_grp, nchunk_grp.data(),
+                                  nchunk_grp.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->nptr, nptr.data(), nptr.size() * sizeof(int32_t),
                                   hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->pwave, pwave.data(), pwave.size() * sizeof(int2),
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->nidx, nidx.data(), nidx.size() * sizeof(int32_t),
                                   hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->pslot, pslot.data(), pslot.size() * sizeof(SpecialSlot),
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->gptr, gptr.data(), gptr.size() * sizeof(int32_t),
                                   hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->pnodes, pnodes.data(), pnodes.size() * 4,
-                                  hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->pcen, pcen.data(), pcen.size() * 8, hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pairs, gal_pairs.data(), gal_pairs.size() * sizeof(int2),
-                                  hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pptr, gal_pptr.data(), gal_pptr.size() * 4,
-                                  hipMemcpyHostToDevice, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->gal_pidx, gal_pidx.data(), gal_pidx.size() * 4,
-                                  hipMemcpyHostToDevice, s));
+    if (!gidx.empty())
+      PYTO_CHECK_HIP(hipMemcpyAsync(h->gidx, gidx.data(), gidx.size() * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipMemsetAsync(h->ct, 0, (size_t)2 * ld * 8, s));
+    PYTO_CHECK_HIP(hipMemsetAsync(h->cmu, 0, (size_t)ld * 8, s));
+    PYTO_CHECK_HIP(hipStreamSynchronize(s));
+    return PYTO_OK;
+  };
+  const int rc = install();
+  if (rc != PYTO_OK) {
+    release_coarse(h);
+    return rc;
   }
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->gnodes, gnodes.data(), nn * sizeof(int32_t),
-                                hipMemcpyHostToDevice, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->nchunk, nchunk.data(), nchunk.size() * sizeof(int2),
-                                hipMemcpyHostToDevice, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->nchunk_grp, nchunk_grp.data(),
-                                nchunk_grp.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->nptr, nptr.data(), nptr.size() * sizeof(int32_t),
-                                hipMemcpyHostToDevice, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->nidx, nidx.data(), nidx.size() * sizeof(int32_t),
-                                hipMemcpyHostToDevice, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->gptr, gptr.data(), gptr.size() * sizeof(int32_t),
-                                hipMemcpyHostToDevice, s));
-  if (!gidx.empty())
-    PYTO_CHECK_HIP(hipMemcpyAsync(h->gidx, gidx.data(), gidx.size() * sizeof(int32_t),
-                                  hipMemcpyHostToDevice, s));
-  PYTO_CHECK_HIP(hipMemsetAsync(h->ct, 0, (size_t)2 * ld * 8, s));
-  PYTO_CHECK_HIP(hipMemsetAsync(h->cmu, 0, (size_t)ld * 8, s));
-  PYTO_CHECK_HIP(hipStreamSynchronize(s));
   if (h->density_set) PYTO_TRY(coarse_setup(h, s));
   return PYTO_OK;
 }
