@@ -270,6 +270,17 @@ def thermal_line(torch, dev, grid, x):
     res.update({"grid": list(grid), "nodes": nn, "solve_ms": (time.perf_counter() - t0) * 1e3,
                 "iterations": st["iterations"], "converged": bool(st["converged"]),
                 "solver": "Jacobi PCG FP64 (no deflation space on the thermal operator)"})
+    # the same solve in FP32 inside FP64 refinement
+    op.solve(q, rtol=1e-8, max_iters=100000, precision="mixed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Tm, sm = op.solve(q, rtol=1e-8, max_iters=100000, precision="mixed")
+    torch.cuda.synchronize()
+    C64, Cm = float(torch.dot(q, T)), float(torch.dot(q, Tm))
+    res.update({"mixed_solve_ms": (time.perf_counter() - t0) * 1e3,
+                "mixed_iterations": sm["iterations"], "mixed_refinements": sm["outer_iterations"],
+                "mixed_converged": bool(sm["converged"]),
+                "mixed_compliance_rel_vs_fp64": abs(Cm - C64) / abs(C64)})
     op.close()
     return res
 

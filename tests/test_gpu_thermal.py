@@ -224,3 +224,29 @@ def test_thermal_oc_loop_matches_oracle(gpu):
     assert n == 5
     assert np.allclose(hist["objective"][:5], hist_ref, rtol=1e-8, atol=0)
     assert np.abs(fe.stiff_mtrx.x.cpu().numpy() - x_ref).max() <= 1e-6
+
+
+@pytest.mark.parametrize("dims", [(24, 12, 8), (64, 32, 32)])
+def test_thermal_mixed_matches_fp64(gpu, dims):
+    """The FP32-inside-FP64-refinement solve on the thermal operator (one
+    temperature per node, Jacobi PCG) against the FP64 solve: thermal
+    compliance within 1e-6, temperatures within 1e-5, both to rel res 1e-8."""
+    from pyto_amd.operator import ThermalOperator
+    print("thermal mixed", dims, flush=True)
+    nelx, nely, nelz = dims
+    sx, sy, sz = nelx + 1, nely + 1, nelz + 1
+    idx = np.arange(sx * sy * sz)
+    ix = idx % sx
+    x = np.random.default_rng(4).uniform(0.01, 1.0, nelx * nely * nelz)
+    op = ThermalOperator(dims, (1.0, 1.0, 1.0), 1.0, idx[ix == 0], penal=3.0, k_void=1e-6)
+    op.set_density(x)
+    q = torch.zeros(idx.size, dtype=torch.float64, device=gpu)
+    q[torch.from_numpy(idx[ix == nelx]).to(gpu)] = 1.0 / (sy * sz)
+    T64, s64 = op.solve(q, rtol=1e-8, max_iters=100000)
+    Tm, sm = op.solve(q, rtol=1e-8, max_iters=100000, precision="mixed")
+    print("fp64", s64["iterations"], "mixed", sm["iterations"], sm["outer_iterations"], flush=True)
+    assert s64["converged"] and sm["converged"] and sm["rel"] <= 1e-8
+    C64, Cm = float(torch.dot(q, T64)), float(torch.dot(q, Tm))
+    assert abs(C64 - Cm) <= 1e-6 * abs(C64)
+    assert rel_err(Tm.cpu().numpy(), T64.cpu().numpy()) <= 1e-5
+    op.close()
