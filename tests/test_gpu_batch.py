@@ -125,6 +125,25 @@ def test_deflated_batch_many_load_cases(gpu):
     bs.close()
 
 
+def test_deflated_batch_more_groups_than_the_lds_table(gpu):
+    """A design with 640 groups (beyond the 512 staged in LDS by the batched
+    direction update) beside a small one: both against single solves."""
+    from pyto_amd.batch import BatchSolver
+    print("deflated batch, 640 groups", flush=True)
+    op_a, loads_a = _design((48, 24, 24), 41, 2, gpu, (10, 8, 8))
+    op_b, loads_b = _design((24, 12, 8), 42, 1, gpu, (3, 1, 1))
+    assert op_a.ngroups > 512
+    bs = BatchSolver([op_a, op_b], [loads_a, loads_b])
+    us, stats = bs.solve(rtol=1e-8)
+    ops, loads = [op_a, op_b], [loads_a, loads_b]
+    for (d, k), u, st in zip(bs.systems, us, stats):
+        assert st["converged"] and st["rel"] <= 1e-8
+        res, eu, eC = _check_against_single(ops[d], loads[d][k], u)
+        print(d, k, st["iterations"], "res %.1e err u %.1e C %.1e" % (res, eu, eC), flush=True)
+        assert res <= 1e-6 and eu <= 1e-5 and eC <= 1e-7
+    bs.close()
+
+
 def test_deflated_batch_brick_elements(gpu):
     """Brick elements (the non-cube element form in the batched restriction
     and operator): deflated and plain designs against single solves."""

@@ -160,3 +160,33 @@ def test_bench_recipe_high_contrast(gpu, grid):
     assert s64["rel_true"] < 1e-3 and sm["rel_true"] < 1e-3
     C64, Cm = float(torch.dot(f, u64)), float(torch.dot(f, um))
     assert abs(C64 - Cm) <= 1e-6 * abs(C64)
+
+
+def test_more_groups_than_the_lds_table(gpu):
+    """640 box groups (more than the 512 whose W mu coefficients the direction
+    updates stage in LDS): the FP64 and mixed deflated solves take the path
+    that forms them per node, and agree with a Jacobi PCG solve."""
+    from pyto_amd import deflation, problems
+    from pyto_amd.operator import GridOperator
+    print("640 groups", flush=True)
+    grid = (48, 24, 24)
+    mesh, _, bc = problems.cantilever(grid)
+    x = problems.density_uniform(mesh.num_elems, 0.05, 1.0, 9)
+    op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, bc.fixed_dofs, eps_void=1e-9)
+    op.set_coarse_shift(0.0, 0.0)
+    ds = deflation.DeflationSolver()
+    ds.create_box_groups(mesh, (10, 8, 8))
+    ds.attach(op)
+    assert op.ngroups > 512
+    op.set_density(x)
+    plain = GridOperator(grid, (1, 1, 1), 1.0, 0.3, bc.fixed_dofs, eps_void=1e-9)
+    plain.set_density(x)
+    f = torch.from_numpy(bc.force).to(gpu)
+    u_ref, s_ref = plain.solve(f, rtol=1e-10, max_iters=20000)
+    C_ref = float(torch.dot(f, u_ref))
+    for prec in ("fp64", "mixed"):
+        u, st = op.solve(f, rtol=1e-10, max_iters=5000, precision=prec)
+        print(prec, st["iterations"], s_ref["iterations"], flush=True)
+        assert st["converged"] and st["iterations"] < s_ref["iterations"]
+        assert abs(float(torch.dot(f, u)) - C_ref) <= 1e-8 * abs(C_ref)
+        assert rel_err(u.cpu().numpy(), u_ref.cpu().numpy()) <= 1e-6
