@@ -786,12 +786,6 @@ int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
 }
 
 template <typename T>
-static const T* ee_of(const Handle* h);
-template <>
-const double* ee_of<double>(const Handle* h) { return h->Ee64; }
-template <>
-const float* ee_of<float>(const Handle* h) { return h->Ee32; }
-template <typename T>
 static const T* pe_of(const Handle* h);
 template <>
 const double* pe_of<double>(const Handle* h) { return h->pE64; }
