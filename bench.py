@@ -5,8 +5,11 @@ moduli, Jacobi diagonal, Galerkin coarse matrix + FP64 Cholesky and inverse)
 and solve K u = f to sqrt(r^T M r / r0^T M r0) <= 1e-8 with FP32 deflated PCG
 inside FP64 iterative refinement.  The same solve in plain FP64 is timed
 beside it, and the dominant kernel (the FP32 matrix-free K*u) is timed with
-events on its stream for the roofline.  The CPU baseline is the oracle
-(assembled CSR, two-SpMV DPCG, SuperLU coarse solve) on a bounded sample.
+events on its stream for the roofline.  The CPU baseline is the oracle in the
+reference's form (assembled CSR, two-SpMV DPCG, SuperLU refactorisation per
+coarse solve) on a bounded sample of config 1 (60x20x10), timed beside the
+complete config-1 OC loop on the GPU; the bench workload itself does not fit
+the reference's form on a host (its COO assembly needs ~100 GB).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3]
 """
@@ -29,51 +32,113 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(gpu_iters_fp64, ndof_target, budget_s=20.0):
-    """Oracle DPCG (reference form) on a 64x32x32 sample of the same workload,
-    extrapolated to the target size: setup and per-iteration costs scale
-    with DOF, iterations = the GPU FP64 run's count."""
-    from oracle import deflation as ODF
+def config1_lines(torch, dev, budget_s=20.0, run_cpu=True):
+    """Config 1, the reference's own CPU-runnable case (60x20x10, 42,273 DOF,
+    x0 = 0.5, unit -z point load at node (60,10,5), Emin = 1e-9, sensitivity
+    filter r = 1.5, volume fraction 0.5, DPCG rtol 1e-8 with the reference's
+    group heuristic, 96 groups) on both sides of the same run.
+
+    GPU: the complete device OC loop, 50 iterations plus the tail (binary
+    threshold, largest connected component, final solve), wall clock.
+    CPU: the oracle in the reference's form (COO assembly and CSR per solve,
+    Dirichlet slicing, Jacobi, two-SpMV DPCG with scipy spsolve
+    refactorising the coarse matrix at every coarse solve) for as many OC
+    iterations as fit in about budget_s, one BLAS thread (scipy's sparse
+    kernels are single-threaded)."""
+    from pyto_amd import deflation, hex_structural_fea, linear_solvers, problems
+    from pyto_amd import topopt_common, topopt_material_model as tmm, topopt_ocm
+    mesh, mat, bc = problems.cantilever((60, 20, 10), load="point", point=(60, 10, 5))
+    ng = min(2000, max(10, round(3 * mesh.num_nodes / 500)))
+    tmm.set_EVOID_RELATIVE(1e-9)
+    try:
+        ds = deflation.DeflationSolver()
+        ds.create_deflation_groups(mesh, ng)
+        fe = hex_structural_fea.HexStructuralFEA(mesh, mat, bc, linear_solvers.Solvers.DPCG,
+                                                 dsolver=ds, rtol=1e-8, maxIters=2000)
+        params = topopt_common.TOParams()
+        topopt_ocm.topopt_optimality_criteria(topopt_common.FEA_MODE.STRUCTURAL, fe, params,
+                                              maxIterations=2, print_progress=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hist, _, _, nfea = topopt_ocm.topopt_optimality_criteria(
+            topopt_common.FEA_MODE.STRUCTURAL, fe, params, maxIterations=50,
+            print_progress=False)
+        torch.cuda.synchronize()
+        t_gpu = time.perf_counter() - t0
+    finally:
+        tmm.set_EVOID_RELATIVE(1e-8)
+    gpu = {"oc_iterations": nfea, "solves": nfea + 1, "seconds": t_gpu,
+           "solves_per_s": (nfea + 1) / t_gpu, "final_objective": hist["objective"][-1],
+           "groups": ds.ws_nGroups}
+    if not run_cpu:
+        return gpu, None
+    from threadpoolctl import threadpool_limits
     from oracle import fea as OF
     from oracle import ke as OK
     from oracle import mesh as OM
-    from pyto_amd import problems
-    dims = (64, 32, 32)
-    t0 = time.perf_counter()
-    g = OM.GridMesh(dims, (1, 1, 1))
-    g.createEdofMatStructural()
+    from oracle import topopt as OT
+    g = OM.GridMesh((60, 20, 10), (1.0, 1.0, 1.0))
     KE = OK.hex8_stiffness(1.0, 0.3, (1, 1, 1))
-    x = problems.density_bumps(dims, seed=2)
-    K = OF.assemble_K(g, KE, OF.simp_scaling(x, 3.0, 1e-9))
-    fixed, gv = OF.clamp_x0(g)
-    f = OF.face_traction_load(g)
-    A, b, free = OF.impose_dirichlet(K, f, fixed, gv)
-    Minv = OF.jacobi_inverse_diag(A)
-    ngn, cnt, cen = ODF.create_groups(g.node_xyz, ODF.recommended_groups_ocm(g.num_nodes))
-    W = ODF.deflation_matrix(g.node_xyz, ngn, cen)[free, :]
-    t_setup = time.perf_counter() - t0
-    # time a bounded number of iterations (the coarse factorisation is in the first call)
-    t0 = time.perf_counter()
-    _, info = ODF.deflated_pcg(A, b, W, Minv, rtol=1e-30, maxIters=2)
-    t_first = time.perf_counter() - t0
-    n_it = 2
-    while True:
+    fixed, _ = OF.clamp_x0(g)
+    f = OF.point_load(g, 60, 10, 5)
+    with threadpool_limits(limits=1):
+        ms = OT.reference_dpcg_solver(g, KE, fixed, eps=1e-9, rtol=1e-8)
         t0 = time.perf_counter()
-        _, info = ODF.deflated_pcg(A, b, W, Minv, rtol=1e-30, maxIters=n_it + 10)
-        t_more = time.perf_counter() - t0
-        if t_more > budget_s / 2 or n_it >= 40:
-            break
-        n_it += 10
-    per_iter = (t_more - t_first) / (n_it + 10 - 2)
-    scale = ndof_target / (3 * g.num_nodes)
-    t_solve = (t_setup + (t_first - 2 * per_iter)) * scale + per_iter * scale * gpu_iters_fp64
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "oracle (assembled CSR K, Jacobi, reference group heuristic %d groups, two-SpMV "
-                      "DPCG with SuperLU coarse solve) on the 64x32x32 cantilever with the same "
-                      "density recipe: setup %.2fs, %.3fs/iteration at %d DOF, extrapolated x%.1f "
-                      "in DOF and to %d iterations (GPU FP64 count)"
-                      % (len(cnt), t_setup, per_iter, 3 * g.num_nodes, scale, gpu_iters_fp64),
-            "est_seconds_per_solve": t_solve}
+        OT.oc_loop(g, KE, f, fixed, 0.5, ms, max_iterations=1, eps=1e-9)
+        t_two = time.perf_counter() - t0           # one iteration + the tail's solve
+        k = max(2, int(budget_s / (t_two / 2)) - 1)
+        ms = OT.reference_dpcg_solver(g, KE, fixed, eps=1e-9, rtol=1e-8)
+        t0 = time.perf_counter()
+        r = OT.oc_loop(g, KE, f, fixed, 0.5, ms, max_iterations=k, eps=1e-9)
+        t_cpu = time.perf_counter() - t0
+    solves = r["nFEAs"] + 1
+    env = {k_: os.environ.get(k_) for k_ in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS",
+                                             "MKL_NUM_THREADS")}
+    cpu = {"value": solves / t_cpu, "unit": "solves/s (config 1: 60x20x10, 42,273 DOF)",
+           "cores": 1, "kind": "port",
+           "sample": "config 1 OC loop, first %d iterations plus the tail's final solve (%d "
+                     "solves, %.1f s) of the oracle in the reference's form: COO assembly and "
+                     "CSR per solve, Dirichlet slicing, Jacobi, %d reference-heuristic groups, "
+                     "two-SpMV DPCG to rtol 1e-8 with spsolve refactorising the coarse matrix "
+                     "at every coarse solve (DPCG iterations per solve %d..%d), sensitivity "
+                     "filter and OC update; one thread"
+                     % (r["nFEAs"], solves, t_cpu, ng, min(ms.stats["iterations"]),
+                        max(ms.stats["iterations"])),
+           "host_cores_available": len(os.sched_getaffinity(0)),
+           "blas_threads_used": 1, "thread_env": env,
+           "gpu_same_workload": gpu,
+           "gpu_over_cpu_same_workload": gpu["solves_per_s"] / (solves / t_cpu),
+           "bench_workload": "N/A on the CPU in the reference's form: its COO assembly of the "
+                             "256x128x128 grid needs about 100 GB of host memory"}
+    return gpu, cpu
+
+
+def reference_groups_line(op, mesh, f, u, step, dev):
+    """The bench workload with the reference's own coarse space:
+    create_deflation_groups(mesh, min(2000, max(10, round(3 N / 500)))), 20 x
+    10 x 10 = 2000 groups (12,000 coarse unknowns, a 1.15 GB dense FP64
+    inverse applied every iteration).  Reported beside `value`."""
+    from pyto_amd import deflation
+    ds = deflation.DeflationSolver()
+    ng = min(2000, max(10, round(3 * mesh.num_nodes / 500)))
+    t0 = time.perf_counter()
+    ds.create_deflation_groups(mesh, ng, device=dev)
+    ds.attach(op)
+    step("mixed")
+    import torch
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    out = {"groups": op.ngroups, "set_groups_and_first_solve_s": t_setup}
+    for prec in ("mixed", "fp64"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = step(prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        out[prec] = {"ms_per_solve": t * 1e3, "iterations": st["iterations"],
+                     "ms_per_iteration": t * 1e3 / max(st["iterations"], 1),
+                     "converged": st["converged"], "rel": st["rel"]}
+    return out
 
 
 def sensitivity_errors(op, x, u_mixed, u64, inner_rtol):
@@ -300,6 +365,8 @@ def main():
     ap.add_argument("--no-thermal", action="store_true",
                     help="skip the thermal-operator line")
     ap.add_argument("--no-mma", action="store_true", help="skip the MMA step line")
+    ap.add_argument("--no-reference-groups", action="store_true",
+                    help="skip the solves with the reference's 2000-group coarse space")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 32-design line reported beside `value`")
     ap.add_argument("--operator-only", action="store_true",
@@ -478,6 +545,12 @@ def main():
         log("tuned aggregates %s: mixed %.3f ms, fp64 %.3f ms" % (args.tuned_boxes, t2m * 1e3,
                                                                    t2d * 1e3))
 
+    # the reference's own coarse space (2000 groups) on the bench workload
+    ref_groups = None
+    if not args.no_reference_groups:
+        ref_groups = reference_groups_line(op, mesh, f, u, step, dev)
+        log("reference groups", ref_groups)
+
     # the batching workload, reported beside `value`: 32 designs (seed 5)
     # with 1-6 point-load cases each, all systems to rtol 1e-8 in one batch
     batch = None
@@ -497,9 +570,9 @@ def main():
         thermal = thermal_line(torch, dev, grid, x)
         log("thermal", thermal)
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(st64["iterations"], nd)
+    c1_gpu, cpu = config1_lines(torch, dev, run_cpu=not args.no_cpu)
+    log("config 1 GPU OC loop", c1_gpu)
+    if cpu is not None:
         if mma is not None:
             cpu["mma_step_oracle"] = mma_cpu()
         log("cpu baseline", cpu)
@@ -538,6 +611,8 @@ def main():
                      "what": "fixed-DOF mask of the x=0 clamp, tip traction, %dx%dx%d box "
                              "deflation groups (ids, counts, centres)" % boxes},
         "tuned_aggregates": tuned,
+        "reference_groups": ref_groups,
+        "config1_oc_loop_gpu": c1_gpu,
         "batch_32_designs": batch,
         "thermal": thermal,
         "mma_step": mma,

@@ -20,7 +20,14 @@
  *   - `stream` is a hipStream_t passed as void* (0 = null stream).  Calls are
  *     asynchronous except the drivers that return scalars to the host.
  *   - A handle owns only its constants and scratch workspace, which
- *     pyto_destroy frees.  One thread at a time may use a handle.
+ *     pyto_destroy frees.  One thread at a time may use a handle, and the
+ *     calls on one handle must be ordered by the caller (one stream, or
+ *     streams the caller synchronises).  Use of separate handles from
+ *     several host threads at once is not verified: density updates of
+ *     separate handles from concurrent threads once gave a non-positive
+ *     coarse pivot that sequential updates of the same handles do not, with
+ *     the cause unexplained.  Drive the handles of one process from one
+ *     thread.
  */
 #ifndef PYTO_HIP_H
 #define PYTO_HIP_H
@@ -242,7 +249,8 @@ int pyto_oc_update(pyto_handle* h, const double* x, const double* dc, int64_t n,
  * (pyto_set_groups before the densities), otherwise Jacobi PCG (the
  * reference's PCG branch, src/linear_solvers.py:157-176).  No reference
  * counterpart: the reference solves one system per call.
- * All handles must use cube elements, or all bricks.  New densities keep a
+ * All handles must be structural (pyto_create; a thermal handle is
+ * refused with PYTO_EINVAL) and use cube elements, or all bricks.  New densities keep a
  * batch valid; new deflation groups on a handle need a new batch
  * (pyto_batch_solve returns PYTO_ESTATE). */
 typedef struct pyto_batch pyto_batch;
@@ -359,6 +367,18 @@ int pyto_node_groups(int nx, int ny, int nz, const int32_t* tx, const int32_t* t
 int pyto_group_sums(int nx, int ny, int nz, double hx, double hy, double hz,
                     const int32_t* node_group, int ngroups, const int32_t* bbox, double* sums,
                     void* stream);
+
+/* ---- design clean-up ----------------------------------------------------- */
+/* Connected components of the elements with x > threshold under the
+ * reference's 27-entry element neighbour array (face, edge and corner
+ * neighbours, src/hex_mesher.py:126-151): labels (int32 per element, device)
+ * receives -1 for an absent element and otherwise the smallest element index
+ * of its component, so the components come ordered as
+ * HexMesher.find_connected_components (src/hex_mesher.py:1285-1339) lists
+ * them.  Synchronises `stream`; *rounds_host (may be null) = propagation
+ * rounds run. */
+int pyto_element_components(int nx, int ny, int nz, const double* x, double threshold,
+                            int32_t* labels, int* rounds_host, void* stream);
 
 #ifdef __cplusplus
 }

@@ -11,9 +11,11 @@ Restates ``src/deflation.py``:
     start, two products with K per iteration, stop when
     sqrt(rz_new/rz0) <= rtol, maxIters 2000.
 
-The coarse matrix is factorised once with SuperLU (``splu``) instead of being
-refactorised by ``spsolve`` on every call (src/deflation.py:533,547,575): the
-factors, and so the solves, are the same up to rounding.
+By default the coarse matrix is factorised once with SuperLU (``splu``)
+instead of being refactorised by ``spsolve`` on every call
+(src/deflation.py:533,547,575): the factors, and so the solves, are the same
+up to rounding.  ``coarse="spsolve"`` keeps the reference's per-call
+refactorisation (the CPU baseline times that form).
 """
 import numpy as np
 import scipy.sparse as sp
@@ -95,7 +97,7 @@ def deflation_matrix(node_xyz, ngn, center):
     return sp.coo_matrix((vals, (rows, cols)), shape=(3 * nn, 6 * G)).tocsr()
 
 
-def deflated_pcg(K, f, W, Minv, rtol=1e-8, maxIters=2000, shift=1e-10):
+def deflated_pcg(K, f, W, Minv, rtol=1e-8, maxIters=2000, shift=1e-10, coarse="splu"):
     """src/deflation.py:504-584.  Minv is the Jacobi diagonal (vector).
 
     Returns (x, info) where info has the iteration count, the final
@@ -103,12 +105,26 @@ def deflated_pcg(K, f, W, Minv, rtol=1e-8, maxIters=2000, shift=1e-10):
     the reference's 1e-10 diagonal shift of the coarse matrix (:521); it
     makes the deflation inexact, and below rtol ~1e-9 the iteration stalls
     and diverges (tests/test_oracle_fea.py pins both behaviours).
+    ``coarse="splu"`` factorises the coarse matrix once; ``"spsolve"`` calls
+    scipy's spsolve on the CSR coarse matrix at every coarse solve, which
+    refactorises it each time as the reference does (:533,547,575).
     """
     WT = W.T.tocsr()
     KW = K @ W
     WKW = (WT @ KW).toarray()
     WKW = (WKW + WKW.T) / 2 + shift * np.eye(WKW.shape[0])
-    lu = spla.splu(sp.csc_matrix(WKW))
+    if coarse == "splu":
+        lu = spla.splu(sp.csc_matrix(WKW))
+    elif coarse == "spsolve":
+        WKW_csr = sp.csr_matrix(WKW)
+
+        class _Refactor:
+            @staticmethod
+            def solve(b):
+                return spla.spsolve(WKW_csr, b)
+        lu = _Refactor
+    else:
+        raise ValueError("coarse must be 'splu' or 'spsolve'")
     mu = lu.solve(WT @ f)
     x = W @ mu
     r = f - K @ x

@@ -72,6 +72,7 @@ _SIGS = {
     "pyto_face_load": ([_i, _i, _i, _i, _i, _i, _i, _d, _vp, _vp], _i),
     "pyto_node_groups": ([_i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp], _i),
     "pyto_group_sums": ([_i, _i, _i, _d, _d, _d, _vp, _i, _vp, _vp, _vp], _i),
+    "pyto_element_components": ([_i, _i, _i, _vp, _d, _vp, ctypes.POINTER(_i), _vp], _i),
     "pyto_create_thermal": ([ctypes.POINTER(_vp), _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp, _vp],
                             _i),
     "pyto_destroy": ([_vp], _i),

@@ -6,7 +6,9 @@ code:
 
 * ``responses(x, fe, pnorm=True) -> (C, PN)``: compliance and the von Mises
   p-norm from ONE forward solve.  Backward: compliance is self-adjoint
-  (lambda = u, no extra solve, src/topopt_obj_cons_sensitivities.py:148-179);
+  (lambda = u, no extra solve, src/topopt_obj_cons_sensitivities.py:148-179)
+  when every Dirichlet value is zero; with prescribed displacements it takes
+  one adjoint solve K_ff lambda_f = f_f;
   the p-norm runs one adjoint deflated-PCG solve K lambda = g with the same
   operator and boundary conditions (:181-287, intended math with g_e as the
   24-vector).  Both terms share one adjoint when both are requested:
@@ -16,12 +18,13 @@ code:
 
 Nothing element-wise is stored between forward and backward apart from u,
 x and the von Mises field: the backward kernels recompute the element
-strains from u.  ``fe`` is a ``HexStructuralFEA``; x is a float64 CUDA
+strains from u.  Every adjoint solve uses the forward problem's fixed DOFs
+with zero values.  ``fe`` is a ``HexStructuralFEA``; x is a float64 CUDA
 tensor of element densities.
 """
 import torch
 
-from . import linear_solvers
+from . import bound_cond, linear_solvers
 from . import topopt_material_model as tmm
 
 
@@ -33,8 +36,16 @@ def _ensure_density(op, x):
 
 
 def _adjoint_solve(fe, rhs):
+    """K_ff lambda_f = rhs_f with lambda = 0 on every fixed DOF (homogeneous
+    Dirichlet values whatever the forward problem prescribes)."""
     op = fe.stiff_mtrx
-    return linear_solvers.solve(op, rhs, fe.solver, fe.bc, dsolver=fe.dsolver, **fe.kwargs)
+    return linear_solvers.solve(op, rhs, fe.solver, bound_cond.homogeneous(fe.bc),
+                                dsolver=fe.dsolver, **fe.kwargs)
+
+
+def _prescribed(fe):
+    """True when the forward problem prescribes non-zero displacements."""
+    return bound_cond.dirichlet_vector(fe.bc, fe.stiff_mtrx.num_dofs) is not None
 
 
 class _Responses(torch.autograd.Function):
@@ -68,6 +79,14 @@ class _Responses(torch.autograd.Function):
         grad = torch.zeros_like(xd)
         if gC is not None and bool(gC != 0):
             _, dC, _ = op.compliance(u, xd)           # -E'(x) u_e^T KE u_e
+            if _prescribed(fe):
+                # C = u^T K u with u = g on the fixed DOFs: the implicit term
+                # is 2 (K u)_f^T du_f/dx = -2 lam^T K' u with K_ff lam_f = f_f
+                # (zero on the fixed DOFs), so dC/dx_e = E'(x_e) (ce_e -
+                # 2 lam_e^T KE u_e); with g = 0, lam = u and it is -E' ce.
+                lam = _adjoint_solve(fe, op.as_dof(fe.total_force))
+                dE = tmm.get_structural_material_model_sensitivity(xd, tmm.MaterialModel.SIMP)
+                dC = -dC - 2.0 * dE * op.bilinear(lam, u)
             grad += gC * dC
         if ctx.want_pnorm and gP is not None and bool(gP != 0):
             g, T1 = op.pnorm_adjoint_rhs(u, xd, vm, svp, q=ctx.q, pexp=ctx.p)
@@ -107,7 +126,9 @@ def responses(x, fe, pnorm=True):
 
 
 def compliance(x, fe):
-    """C(x) = f^T u(x); differentiable in x (self-adjoint backward)."""
+    """C(x) = u^T K(x) u (= f^T u when every Dirichlet value is zero);
+    differentiable in x (self-adjoint backward, or one adjoint solve with
+    prescribed displacements)."""
     return responses(x, fe, pnorm=False)[0]
 
 

@@ -51,6 +51,17 @@ def dirichlet_vector(bc, num_dofs):
     return g
 
 
+def homogeneous(bc):
+    """The same fixed DOFs with zero Dirichlet values: the boundary conditions
+    of every adjoint solve (the adjoint variable vanishes where the state is
+    prescribed, whatever the prescribed value).  Returns bc itself when its
+    values are already zero."""
+    if dirichlet_vector(bc, bc.num_dofs) is None:
+        return bc
+    fixed = np.asarray(bc.fixed_dofs)
+    return BC(force=bc.force, fixed_dofs=fixed, dirichlet_values=np.zeros(fixed.size))
+
+
 def impose_dirichlet_bc(A, b, bc):
     """src/bound_cond.py:40-71 for the matrix-free operator: returns the
     operator (which applies the fixed-DOF mask itself) and

@@ -198,3 +198,38 @@ def box_groups(grid, elem_size, boxes, min_nodes=15, device=None):
         raise ValueError("deflation group with fewer than %d nodes (smallest %d)"
                          % (min_nodes, int(count.min())))
     return node_group, center, count
+
+
+def element_components(grid, x, threshold=0.01, device=None):
+    """Connected components of the elements with x > threshold on the
+    device (find_connected_components, src/hex_mesher.py:1285-1339): an
+    int32 label per element, -1 for an absent element, otherwise the
+    smallest element index of its component."""
+    nx, ny, nz = _grid(grid)
+    dev = _dev(device)
+    xt = torch.as_tensor(x, dtype=torch.float64).to(dev).contiguous().reshape(-1)
+    if xt.numel() != nx * ny * nz:
+        raise ValueError("x has %d entries, the grid %d elements" % (xt.numel(), nx * ny * nz))
+    labels = torch.empty(nx * ny * nz, dtype=torch.int32, device=dev)
+    rounds = ctypes.c_int()
+    with torch.cuda.device(dev):
+        _lib.check(_lib.lib().pyto_element_components(nx, ny, nz, _p(xt), float(threshold),
+                                                      _p(labels), ctypes.byref(rounds),
+                                                      _stream(dev)),
+                   "pyto_element_components")
+    return labels
+
+
+def largest_component(grid, x, threshold=0.01, device=None):
+    """(number of components, mask of the largest one) of element_components;
+    among equally large components the one with the smallest element index
+    wins, as max(components, key=len) picks the first of the reference's
+    list (src/topopt_ocm.py:318-325)."""
+    labels = element_components(grid, x, threshold, device)
+    present = labels >= 0
+    if not bool(present.any()):
+        return 0, present
+    counts = torch.bincount(labels[present].long(), minlength=labels.numel())
+    ncomp = int((counts > 0).sum())
+    best = int(torch.argmax(counts))      # first maximal count: smallest label
+    return ncomp, labels == best

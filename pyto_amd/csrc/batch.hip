@@ -1030,6 +1030,12 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
       set_error("handle %d is null", i);
       return fail(PYTO_EINVAL);
     }
+    if (h->ncomp != 3) {
+      // the batched operator and vector kernels assume 3 DOFs per node
+      set_error("a batch holds structural handles only (system %d has %d value(s) per node)",
+                i, h->ncomp);
+      return fail(PYTO_EINVAL);
+    }
     const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
     if (i == 0) b->cube = cube;
     if (cube != b->cube) {

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 REPO = os.path.dirname(PKG)
 SOURCES = ["operator.hip", "coarse.hip", "solver.hip", "postproc.hip", "optimizer.hip", "batch.hip",
-           "thermo.hip", "mma.hip", "builders.hip"]
+           "thermo.hip", "mma.hip", "builders.hip", "topology.hip"]
 LIB = os.path.join(PKG, "libpytohip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -370,6 +370,11 @@ struct DirVec<double> {
 };
 
 constexpr int kLdsGroups = 512;
+// Largest dense coarse space: 6 x 4096 = 24576 unknowns, whose FP64 matrix
+// and inverse take 4.8 GB each.  The reference's own heuristic asks for at
+// most 2000 groups and its box rounding can exceed that (26 x 9 x 9 = 2106
+// on a 192 x 64 x 64 grid).
+constexpr int kMaxCoarseGroups = 4096;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_dir_update(
@@ -988,9 +993,10 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   PYTO_REQUIRE(ngroups >= 0, PYTO_EINVAL, "ngroups must be >= 0");
   PYTO_REQUIRE(ngroups == 0 || h->ncomp == 3, PYTO_EINVAL,
                "rigid-body deflation groups need a structural handle");
-  PYTO_REQUIRE(6 * (int64_t)ngroups <= 12288, PYTO_EINVAL,
-               "coarse space of %d groups (%d unknowns) exceeds the dense GPU limit of 2048 "
-               "groups; use fewer, larger deflation groups", ngroups, 6 * ngroups);
+  PYTO_REQUIRE(ngroups <= kMaxCoarseGroups, PYTO_EINVAL,
+               "coarse space of %d groups (%d unknowns) exceeds the dense GPU limit of %d "
+               "groups; use fewer, larger deflation groups", ngroups, 6 * ngroups,
+               kMaxCoarseGroups);
   hipStream_t s = as_stream(stream);
   // captured iteration graphs hold the old coarse-space pointers
   for (auto& gc : h->graphs)

@@ -396,7 +396,7 @@ int pyto_destroy(pyto_handle* h) {
                   h->cmu, h->cres, h->r64, h->p64, h->q64, h->r32, h->p32, h->q32, h->x32,
                   h->b32, h->partials, h->counters, h->scal, h->flags, h->elem_buf,
                   h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
-                  h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32};
+                  h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32, h->stencil};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& gc : h->graphs)

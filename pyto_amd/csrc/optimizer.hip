@@ -23,6 +23,7 @@
 namespace pyto {
 
 constexpr int kOcMaxBlocks = kMaxPartials / 2;
+constexpr int kOcMaxSteps = 4096;
 
 __device__ __forceinline__ double oc_xnew(double x, double dc, double dv, double lmid,
                                           double move, double xmin, double xmax) {
@@ -37,7 +38,8 @@ __global__ __launch_bounds__(256) void k_oc_step(const double* __restrict__ x,
                                                  double xmin, double xmax, int step,
                                                  double* __restrict__ scal,
                                                  int* __restrict__ flags,
-                                                 double* __restrict__ partials, int nblocks) {
+                                                 double* __restrict__ partials, int nblocks,
+                                                 int pstride) {
   __shared__ double lds[5];
   // state written by block 0 of the previous launch (never by this launch)
   const double* st = scal + S_OC_STATE + 4 * (step & 1);
@@ -49,7 +51,7 @@ __global__ __launch_bounds__(256) void k_oc_step(const double* __restrict__ x,
   }
   double l1 = st[0], l2 = st[1], lmid = st[2];
   if (step > 0) {
-    const double sum = block_sum_partials(partials + kOcMaxBlocks * ((step - 1) & 1), nblocks, lds);
+    const double sum = block_sum_partials(partials + pstride * ((step - 1) & 1), nblocks, lds);
     const double vol = (sum / (double)n) / volfrac - 1.0;
     if (vol > 0) l1 = lmid;
     else l2 = lmid;
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(256) void k_oc_step(const double* __restrict__ x,
   for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
     s += oc_xnew(x[i], dc[i], dv, lmid, move, xmin, xmax);
   const double bs = block_reduce_sum256(s, lds);
-  if (threadIdx.x == 0) partials[kOcMaxBlocks * (step & 1) + blockIdx.x] = bs;
+  if (threadIdx.x == 0) partials[pstride * (step & 1) + blockIdx.x] = bs;
   if (first) {
     double* nx = scal + S_OC_STATE + 4 * ((step + 1) & 1);
     nx[0] = l1;
@@ -103,6 +105,44 @@ __global__ __launch_bounds__(256) void k_oc_apply(const double* __restrict__ x,
     xnew[i] = oc_xnew(x[i], dc[i], dv, lmid, move, xmin, xmax);
 }
 
+// The launch sequence of one OC update: init, max_steps bisection steps,
+// the final apply; scal/flags/partials are the caller's scratch (the
+// handle's, or the allocation-free variant's static array).
+static int oc_launch(const double* x, const double* dc, int64_t n, double dv, double volfrac,
+                     double move, double xmin, double xmax, int max_steps, double* xnew,
+                     double* scal, int* flags, double* partials, int pstride,
+                     double* lmid_host, int* steps_host, hipStream_t s) {
+  const int nblocks = (int)std::min<int64_t>((n + 255) / 256, kOcMaxBlocks);
+  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, scal, flags);
+  for (int k = 0; k < max_steps; ++k)
+    hipLaunchKernelGGL(k_oc_step, dim3(nblocks), dim3(256), 0, s, x, dc, n, dv, volfrac, move,
+                       xmin, xmax, k, scal, flags, partials, nblocks, pstride);
+  hipLaunchKernelGGL(k_oc_apply, dim3(nblocks), dim3(256), 0, s, x, dc, n, dv, move, xmin, xmax,
+                     (const double*)scal, xnew);
+  PYTO_CHECK_HIP(hipGetLastError());
+  if (lmid_host || steps_host) {
+    int fl[kNumFlags];
+    double sc[kNumScalars];
+    PYTO_CHECK_HIP(hipMemcpyAsync(fl, flags, sizeof(fl), hipMemcpyDeviceToHost, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(sc, scal, sizeof(sc), hipMemcpyDeviceToHost, s));
+    PYTO_CHECK_HIP(hipStreamSynchronize(s));
+    if (lmid_host) *lmid_host = sc[S_OC_LAST];
+    if (steps_host) *steps_host = fl[F_OC_DONE] ? fl[F_OC_STEPS] : -fl[F_OC_STEPS];
+  }
+  return PYTO_OK;
+}
+
+static int oc_check(const double* x, const double* dc, int64_t n, double dv, double volfrac,
+                    double move, double xmin, double xmax, int max_steps, double* xnew) {
+  PYTO_REQUIRE(x && dc && xnew, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(n > 0, PYTO_EINVAL, "n must be positive");
+  PYTO_REQUIRE(dv > 0 && volfrac > 0 && move >= 0 && xmin <= xmax, PYTO_EINVAL,
+               "need dv > 0, volfrac > 0, move >= 0 and xmin <= xmax");
+  PYTO_REQUIRE(max_steps > 0 && max_steps <= kOcMaxSteps, PYTO_EINVAL,
+               "max_steps %d outside [1, %d]", max_steps, kOcMaxSteps);
+  return PYTO_OK;
+}
+
 }  // namespace pyto
 
 using namespace pyto;
@@ -112,30 +152,10 @@ extern "C" {
 int pyto_oc_update(pyto_handle* h, const double* x, const double* dc, int64_t n, double dv,
                    double volfrac, double move, double xmin, double xmax, int max_steps,
                    double* xnew, double* lmid_host, int* steps_host, void* stream) {
-  PYTO_REQUIRE(h && x && dc && xnew, PYTO_EINVAL, "null argument");
-  PYTO_REQUIRE(n > 0, PYTO_EINVAL, "n must be positive");
-  PYTO_REQUIRE(dv > 0 && volfrac > 0 && move >= 0 && xmin <= xmax, PYTO_EINVAL,
-               "need dv > 0, volfrac > 0, move >= 0 and xmin <= xmax");
-  PYTO_REQUIRE(max_steps > 0 && max_steps <= 4096, PYTO_EINVAL, "max_steps in [1, 4096]");
-  hipStream_t s = as_stream(stream);
-  const int nblocks = (int)std::min<int64_t>((n + 255) / 256, kOcMaxBlocks);
-  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, h->scal, h->flags);
-  for (int k = 0; k < max_steps; ++k)
-    hipLaunchKernelGGL(k_oc_step, dim3(nblocks), dim3(256), 0, s, x, dc, n, dv, volfrac, move,
-                       xmin, xmax, k, h->scal, h->flags, h->partials, nblocks);
-  hipLaunchKernelGGL(k_oc_apply, dim3(nblocks), dim3(256), 0, s, x, dc, n, dv, move, xmin, xmax,
-                     (const double*)h->scal, xnew);
-  PYTO_CHECK_HIP(hipGetLastError());
-  if (lmid_host || steps_host) {
-    int fl[kNumFlags];
-    double sc[kNumScalars];
-    PYTO_CHECK_HIP(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, s));
-    PYTO_CHECK_HIP(hipMemcpyAsync(sc, h->scal, sizeof(sc), hipMemcpyDeviceToHost, s));
-    PYTO_CHECK_HIP(hipStreamSynchronize(s));
-    if (lmid_host) *lmid_host = sc[S_OC_LAST];
-    if (steps_host) *steps_host = fl[F_OC_DONE] ? fl[F_OC_STEPS] : -fl[F_OC_STEPS];
-  }
-  return PYTO_OK;
+  PYTO_REQUIRE(h, PYTO_EINVAL, "null handle");
+  PYTO_TRY(oc_check(x, dc, n, dv, volfrac, move, xmin, xmax, max_steps, xnew));
+  return oc_launch(x, dc, n, dv, volfrac, move, xmin, xmax, max_steps, xnew, h->scal, h->flags,
+                   h->partials, kOcMaxBlocks, lmid_host, steps_host, as_stream(stream));
 }
 
 }  // extern "C"

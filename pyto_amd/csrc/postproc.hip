@@ -290,24 +290,24 @@ static int build_stencil(const GridDims& g, double rel, Stencil& st) {
   return PYTO_OK;
 }
 
-__constant__ Stencil c_stencil;
-
+// The stencil lives in a per-handle device buffer (st), read with
+// wave-uniform indices.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_filter(const double* __restrict__ v,
                                                 const double* __restrict__ aux, GridDims g,
+                                                const Stencil* __restrict__ st,
                                                 double* __restrict__ out) {
-  const int ns = c_stencil.n;
+  const int ns = st->n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < g.nelems;
        e += (int64_t)gridDim.x * blockDim.x) {
     int ix, iy, iz;
     elem_coords(e, g, ix, iy, iz);
     double hv = 0.0, hs = 0.0;
     for (int k = 0; k < ns; ++k) {
-      const int jx = ix + c_stencil.off[k][0], jy = iy + c_stencil.off[k][1],
-                jz = iz + c_stencil.off[k][2];
+      const int jx = ix + st->off[k][0], jy = iy + st->off[k][1], jz = iz + st->off[k][2];
       if (jx < 0 || jx >= g.nx || jy < 0 || jy >= g.ny || jz < 0 || jz >= g.nz) continue;
       const int64_t j = jx + (int64_t)g.nx * (jy + (int64_t)g.ny * jz);
-      const double w = c_stencil.w[k];
+      const double w = st->w[k];
       hs += w;
       if (MODE == 0 || MODE == 1) hv += w * v[j];
       if (MODE == 2) hv += w * aux[j] * v[j];
@@ -315,10 +315,9 @@ __global__ __launch_bounds__(256) void k_filter(const double* __restrict__ v,
         // H (v / Hs): needs Hs of the neighbour; recompute it
         double hsj = 0.0;
         for (int l = 0; l < ns; ++l) {
-          const int kx = jx + c_stencil.off[l][0], ky = jy + c_stencil.off[l][1],
-                    kz = jz + c_stencil.off[l][2];
+          const int kx = jx + st->off[l][0], ky = jy + st->off[l][1], kz = jz + st->off[l][2];
           if (kx < 0 || kx >= g.nx || ky < 0 || ky >= g.ny || kz < 0 || kz >= g.nz) continue;
-          hsj += c_stencil.w[l];
+          hsj += st->w[l];
         }
         hv += w * v[j] / hsj;
       }
@@ -438,22 +437,29 @@ int pyto_filter_apply(pyto_handle* h, double rel_radius, int mode, const double*
   PYTO_REQUIRE(mode == 4 || v, PYTO_EINVAL, "null input");
   PYTO_REQUIRE(mode != 2 || aux, PYTO_EINVAL, "mode 2 needs the density (aux)");
   PYTO_REQUIRE(rel_radius > 0, PYTO_EINVAL, "filter radius must be positive");
-  Stencil st;
-  PYTO_TRY(build_stencil(h->g, rel_radius, st));
   hipStream_t s = as_stream(stream);
-  PYTO_CHECK_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_stencil), &st, sizeof(Stencil), 0,
-                                        hipMemcpyHostToDevice, s));
+  // The handle keeps the stencil of the last radius on the device; a new
+  // radius is uploaded in stream order (the call then waits for the copy,
+  // so the host staging struct may go out of scope).
+  if (!h->stencil) PYTO_TRY(device_alloc(h, &h->stencil, sizeof(Stencil)));
+  if (h->stencil_rel != rel_radius) {
+    Stencil st;
+    PYTO_TRY(build_stencil(h->g, rel_radius, st));
+    h->stencil_rel = 0.0;
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->stencil, &st, sizeof(Stencil), hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipStreamSynchronize(s));
+    h->stencil_rel = rel_radius;
+  }
+  const Stencil* st = (const Stencil*)h->stencil;
   const int blocks = (int)std::min<int64_t>((h->g.nelems + 255) / 256, 4096);
   switch (mode) {
-    case 0: hipLaunchKernelGGL(k_filter<0>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, out); break;
-    case 1: hipLaunchKernelGGL(k_filter<1>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, out); break;
-    case 2: hipLaunchKernelGGL(k_filter<2>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, out); break;
-    case 3: hipLaunchKernelGGL(k_filter<3>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, out); break;
-    default: hipLaunchKernelGGL(k_filter<4>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, out); break;
+    case 0: hipLaunchKernelGGL(k_filter<0>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, st, out); break;
+    case 1: hipLaunchKernelGGL(k_filter<1>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, st, out); break;
+    case 2: hipLaunchKernelGGL(k_filter<2>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, st, out); break;
+    case 3: hipLaunchKernelGGL(k_filter<3>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, st, out); break;
+    default: hipLaunchKernelGGL(k_filter<4>, dim3(blocks), dim3(256), 0, s, v, aux, h->g, st, out); break;
   }
   PYTO_CHECK_HIP(hipGetLastError());
-  // the constant-memory stencil must not be overwritten while in use
-  PYTO_CHECK_HIP(hipStreamSynchronize(s));
   return PYTO_OK;
 }
 

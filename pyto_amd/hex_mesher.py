@@ -121,6 +121,22 @@ class HexMesher:
     def setPseudoDensity(self, rho):
         self.elemPseudoDensity = rho
 
+    def find_connected_components(self, threshold=0.01):
+        """src/hex_mesher.py:1285-1339: element-index arrays of the connected
+        sets of elements with elemPseudoDensity > threshold (face, edge and
+        corner neighbours), ordered by their smallest element index as the
+        reference lists them.  Labelled on the GPU; each array is sorted
+        (the reference keeps its search's visiting order)."""
+        from .builders import element_components
+        labels = element_components(self.grid, self.elemPseudoDensity, threshold).cpu().numpy()
+        present = np.nonzero(labels >= 0)[0]
+        if present.size == 0:
+            return []
+        order = np.argsort(labels[present], kind="stable")
+        lab_sorted = labels[present][order]
+        cuts = np.nonzero(np.diff(lab_sorted))[0] + 1
+        return [np.sort(c) for c in np.split(present[order], cuts)]
+
     def export_vtu_mesh(self, elem_field, mask_low_pseudodensity=False, density_field="density",
                         file_name=None):
         """src/hex_mesher.py:1446-1499: hexahedral cells (VTK type 12) over all

@@ -10,7 +10,7 @@ the 24-vector), with one adjoint DPCG solve on the same operator and BCs.
 import numpy as np
 import torch
 
-from . import linear_solvers
+from . import bound_cond, linear_solvers
 from . import topopt_material_model as tmm
 from .topopt_common import FEA_MODE, TO_QOI
 
@@ -57,8 +57,9 @@ def compute_pnorm_stress_and_sensitivity(sol, x, fe_solver, KE, material_model):
     st = op.stress(u, xd, q=q, pexp=p, want_strain=False, want_stress=False, want_energy=False)
     pn = float(st["sum_vm_p"][0]) ** (1.0 / p)
     g, T1 = op.pnorm_adjoint_rhs(u, xd, st["vm"], st["sum_vm_p"], q=q, pexp=p)
-    lam = linear_solvers.solve(op, g, fe_solver.solver, fe_solver.bc, dsolver=fe_solver.dsolver,
-                               **fe_solver.kwargs)
+    # adjoint: same fixed DOFs, zero values (lambda vanishes where u is prescribed)
+    lam = linear_solvers.solve(op, g, fe_solver.solver, bound_cond.homogeneous(fe_solver.bc),
+                               dsolver=fe_solver.dsolver, **fe_solver.kwargs)
     xc = torch.clamp(xd, min=1e-12)
     ce = op.bilinear(lam, u)
     T2 = -tmm.get_structural_material_model_sensitivity(xc, material_model) * ce

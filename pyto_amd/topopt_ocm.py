@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from . import topopt_material_model as tmm
+from .builders import largest_component
 from .topopt_common import FEA_MODE, TO_QOI, createFilters, find_elements_with_forces
 from .topopt_obj_cons_sensitivities import compute_objective_and_gradient
 
@@ -140,13 +141,21 @@ def topopt_optimality_criteria(feaMode, fe_solver, to_params, maxIterations=250,
 
     if binarize_topology:
         # extract a binary topology that keeps the volume fraction
-        # (src/topopt_ocm.py:312-316); the reference's connected-component
-        # clean-up that follows (:318-327) is a host-side mesh utility and is
-        # not part of this path
+        # (src/topopt_ocm.py:312-316)
         x_sorted = torch.sort(x).values
         threshold = x_sorted[int((1 - float(x.mean())) * num_elems)]
         x = torch.where(x < threshold, torch.zeros_like(x), torch.ones_like(x))
     volfrac = float(x.mean())
+    # keep only the largest connected component (src/topopt_ocm.py:317-327),
+    # labelled on the device
+    fe_solver.mesh.setPseudoDensity(x)
+    ncomp, largest = largest_component(fe_solver.mesh.grid, x, 0.01, dev)
+    if ncomp > 1:
+        if print_progress:
+            log_message("Disconnected topology detected. Removing hanging elements.")
+        x = largest.to(torch.float64)
+        fe_solver.mesh.setPseudoDensity(x)
+        volfrac = float(x.mean())
     sol = fe_solver.solve(x, material_model)
     if objective is None:
         obj_t, _ = compute_objective_and_gradient(feaMode, to_params, sol, x, fe_solver,

@@ -9,11 +9,12 @@ c1.npz  : the 60x20x10 cantilever of config 1 (x = 0.5, unit -z point load
           iterate at rtol 1e-8 (u_dpcg, C_dpcg): the reference stops there,
           1.5e-6 short of the direct-solve compliance.
 tiny.npz: 4x3x2 grid, densities U[0.1,1] seed 3: K (dense), KE, f, u.
+c1_oc.npz: the 50-iteration OC loop of config 1 with its tail (see c1_oc).
 
 Provenance: the reference cannot be imported here (numba, pyvista, pyamg,
 numpy-stl absent), so these vectors come from the restatement in oracle/,
 which tests/test_oracle_*.py pin against closed forms, brute force and FD.
-Run: python tests/golden/make_golden.py
+Run: python tests/golden/make_golden.py [c1] [tiny] [c1_oc]
 """
 import os
 import sys
@@ -59,6 +60,38 @@ def c1():
                         H_indptr=H.indptr, H_indices=H.indices, H_data=H.data, Hs=Hs)
 
 
+def c1_oc():
+    """The 50-iteration OC loop of config 1 (60x20x10, x0 = 0.5, unit -z
+    point load at node (60,10,5), x = 0 clamped, E0 = 1, nu = 0.3, p = 3,
+    Emin = 1e-9, sensitivity filter r = 1.5, volume fraction 0.5, move 0.2)
+    with its tail (binarise, largest connected component, final solve):
+    once with the exact Galerkin DPCG to rtol 1e-11 ("exact"), once in the
+    reference's own form (coarse shift 1e-10, spsolve refactorisation,
+    rtol 1e-8, "reference")."""
+    from oracle import topopt as OT
+    g = OM.GridMesh((60, 20, 10), (1.0, 1.0, 1.0))
+    g.createEdofMatStructural()
+    KE = OK.hex8_stiffness(1.0, 0.3, (1, 1, 1))
+    fixed, _ = OF.clamp_x0(g)
+    f = OF.point_load(g, 60, 10, 5)
+    out = {}
+    for name, kw in (("exact", dict(rtol=1e-11, shift=0.0, coarse="splu")),
+                     ("reference", dict(rtol=1e-8, shift=1e-10, coarse="spsolve"))):
+        ms = OT.reference_dpcg_solver(g, KE, fixed, eps=1e-9, **kw)
+        r = OT.oc_loop(g, KE, f, fixed, 0.5, ms, max_iterations=50, eps=1e-9)
+        h = r["history"]
+        out[name + "_objective"] = np.array(h["objective"])
+        out[name + "_volfrac"] = np.array(h["volfrac"])
+        out[name + "_change"] = np.array(h["change"])
+        out[name + "_x_loop"] = r["x_loop"]
+        out[name + "_x_final"] = r["x_final"].astype(np.uint8)
+        out[name + "_components"] = r["components"]
+        out[name + "_nFEAs"] = r["nFEAs"]
+        out[name + "_dpcg_iterations"] = np.array(ms.stats["iterations"])
+        print(name, "final objective", h["objective"][-1], "components", r["components"])
+    np.savez_compressed(os.path.join(HERE, "c1_oc.npz"), **out)
+
+
 def tiny():
     g = OM.GridMesh((4, 3, 2), (1.0, 1.0, 1.0))
     g.createEdofMatStructural()
@@ -74,6 +107,7 @@ def tiny():
 
 
 if __name__ == "__main__":
-    c1()
-    tiny()
+    which = sys.argv[1:] or ["c1", "tiny", "c1_oc"]
+    for name in which:
+        {"c1": c1, "tiny": tiny, "c1_oc": c1_oc}[name]()
     print("wrote", os.listdir(HERE))

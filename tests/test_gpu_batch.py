@@ -206,3 +206,17 @@ def test_batch_32_designs(gpu):
         d, k = bs.systems[i]
         res, eu, eC = _check_against_single(ops[d], designs[d]["loads"][k], us[i])
         assert res <= 1e-6 and eu <= 1e-5 and eC <= 1e-7, (i, res, eu, eC)
+
+
+def test_batch_refuses_thermal_handle(gpu):
+    """A thermal handle (one value per node) cannot join a batch: creation
+    fails before any kernel runs."""
+    from pyto_amd.batch import BatchSolver
+    from pyto_amd.operator import ThermalOperator
+    print("batch refuses thermal handle", flush=True)
+    op = ThermalOperator((8, 4, 4), (1.0, 1.0, 1.0), 1.0, [0, 1, 2], penal=1.0, k_void=1e-3)
+    op.set_density(np.full(op.num_elems, 0.5))
+    f = np.zeros(op.num_dofs)
+    f[-1] = 1.0
+    with pytest.raises(ValueError, match="structural handles only"):
+        BatchSolver([op], [[f]])

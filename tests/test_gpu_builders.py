@@ -84,6 +84,12 @@ def _host_groups(grid, h, boxes=None, ngroups=None):
     ((10, 3, 3), (0.5, 1.0, 2.0), (9, 2, 1), None),  # small boxes merged, brick elements
     ((256, 128, 128), (1, 1, 1), (4, 4, 4), None),
     ((256, 128, 128), (1, 1, 1), None, 2000),       # 20 x 10 x 10
+    # the hand-built merge cases whose groups tests/test_oracle_groups.py
+    # writes out: nearest valid box, distance tie to the first valid box,
+    # empty box skipped
+    ((6, 2, 3), (1, 1, 1), None, 5),
+    ((4, 2, 3), (1.0, 0.5, 0.5), None, 3),
+    ((2, 5, 5), (1.0, 0.125, 0.125), None, 6),
 ])
 def test_box_groups_match_host_and_oracle(gpu, grid, h, boxes, ngroups):
     from pyto_amd import deflation

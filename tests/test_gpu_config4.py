@@ -1,6 +1,6 @@
 """The differentiation workload at full size: 128x64x64 hexes (1.64M DOF),
 densities U[0.2, 1] seed 1, p-norm p = 8, q = 0.5, plus compliance, through
-the autograd wrapper: central finite differences on 20 seeded elements and
+the autograd wrapper: directional central finite differences at h = 1e-6 and
 30 OC iterations end to end."""
 import numpy as np
 import pytest
@@ -41,8 +41,19 @@ def pnorm8():
 
 
 def test_config4_gradient_fd(gpu, pnorm8):
-    """The FP64 DPCG reaches rtol 1e-12 reliably at this size (its true
-    residual settles near 4e-11; 1e-13 sits at the attainable limit)."""
+    """Directional central differences of J = C/C0 + PN/P0 at h = 1e-6.
+
+    A solve to relative residual rtol leaves a first-order error ~rtol |J| in
+    J, so a central difference carries noise up to ~4 rtol |J| / h (8e-6 at
+    rtol 1e-12, h = 1e-6, J ~ 2).  A single element's sensitivity (1e-7 to
+    1e-5 in a 524k-element sum) is below that floor; along a direction that
+    moves every element the signal is the sum of the sensitivities.  Two
+    directions: d1 = sign(g) over all elements (g.d1 = ||g||_1) and d2 =
+    sign(g) on a seeded half of the elements, zero elsewhere.  Both signals
+    are asserted to exceed the noise floor by more than 1e5 / 5, so the
+    1e-5 relative comparison is resolved.  The FP64 DPCG reaches rtol 1e-12
+    reliably at this size (its true residual settles near 4e-11; 1e-13 sits
+    at the attainable limit)."""
     from pyto_amd import autograd as AG
     print("config4 FD", flush=True)
     rtol = 1e-12
@@ -62,29 +73,20 @@ def test_config4_gradient_fd(gpu, pnorm8):
     xr = xt.clone().requires_grad_(True)
     J0 = J(xr)
     (g,) = torch.autograd.grad(J0, xr)
-    g = g.cpu().numpy()
-    assert np.all(np.isfinite(g)) and np.abs(g).max() > 0
-    idx = np.random.default_rng(1).choice(n, 20, replace=False)
-    # A solve to relative residual rtol leaves a first-order error ~rtol |J|
-    # in PN, so a central difference carries noise ~rtol |J| / h (allowed
-    # 4x).  At h = 1e-6 that floor dominates the small gradients of a
-    # 524k-element sum; h = 1e-4 (truncation ~h^2 / x^2 < 1e-7 relative for
-    # x >= 0.2) checks every element at 1e-5 relative.
-    bad = []
+    assert bool(torch.isfinite(g).all()) and float(g.abs().max()) > 0
+    h = 1e-6
+    floor = 4 * rtol * abs(float(J0)) / h
+    half = torch.from_numpy(np.random.default_rng(1).random(n) < 0.5).to(gpu)
+    d1 = torch.sign(g)
+    d2 = torch.where(half, torch.sign(g), torch.zeros_like(g))
     with torch.no_grad():
-        for h in (1e-6, 1e-4):
-            floor = 4 * rtol * abs(float(J0)) / h
-            for e in idx:
-                vals = []
-                for s in (+1, -1):
-                    xx = xt.clone()
-                    xx[e] += s * h
-                    vals.append(float(J(xx)))
-                fd = (vals[0] - vals[1]) / (2 * h)
-                print("h %.0e elem %d adjoint %.8e fd %.8e" % (h, e, g[e], fd), flush=True)
-                if not (abs(g[e] - fd) <= 1e-5 * abs(g[e]) + floor):
-                    bad.append((h, int(e), g[e], fd))
-    assert not bad, bad
+        for name, d in (("sign(g)", d1), ("sign(g) on a seeded half", d2)):
+            slope = float(torch.dot(g, d))
+            assert floor <= 0.2e-5 * abs(slope), (name, floor, slope)
+            fd = (float(J(xt + h * d)) - float(J(xt - h * d))) / (2 * h)
+            print("direction %s: adjoint %.10e fd %.10e rel %.2e floor %.1e"
+                  % (name, slope, fd, abs(slope - fd) / abs(slope), floor), flush=True)
+            assert abs(slope - fd) <= 1e-5 * abs(slope), (name, slope, fd)
 
 
 def test_config4_oc_30_iterations(gpu, pnorm8):

@@ -38,3 +38,30 @@ def test_filter_structure_and_values(gpu, dims, h, r):
     # device tensors stay on the device
     out = H * torch.from_numpy(v).to(gpu)
     assert out.is_cuda
+
+
+def test_handles_keep_their_own_stencils(gpu):
+    """Two operators with different radii and element sizes, called
+    alternately: each handle filters with its own stencil."""
+    from pyto_amd.operator import GridOperator
+    print("filter: alternating handles", flush=True)
+    cases = [((12, 9, 7), (1.0, 1.0, 1.0), 1.5), ((10, 8, 6), (1.0, 0.8, 1.3), 2.2)]
+    rng = np.random.default_rng(3)
+    ops, refs, vs = [], [], []
+    for dims, h, r in cases:
+        ops.append(GridOperator(dims, h, 1.0, 0.3, [], penal=3.0, eps_void=1e-8))
+        Href, Hs_ref = OFl.cone_filter(OM.GridMesh(dims, h), r)
+        refs.append((Href, Hs_ref, r))
+        vs.append(rng.uniform(0.1, 1.0, int(np.prod(dims))))
+    for rep in range(3):
+        outs = []
+        for op, v, (Href, Hs_ref, r) in zip(ops, vs, refs):
+            outs.append((op.filter(v, r, mode=1), Href, Hs_ref, v))
+        for out, Href, Hs_ref, v in outs:
+            assert rel_err(out.cpu().numpy(), OFl.density_filter(Href, Hs_ref, v)) <= 1e-14
+    # a radius change on one handle leaves the other's stencil alone
+    out0 = ops[0].filter(vs[0], 2.2, mode=0)
+    H22, _ = OFl.cone_filter(OM.GridMesh(cases[0][0], cases[0][1]), 2.2)
+    out1 = ops[1].filter(vs[1], 2.2, mode=0)
+    assert rel_err(out0.cpu().numpy(), H22 @ vs[0]) <= 1e-14
+    assert rel_err(out1.cpu().numpy(), refs[1][0] @ vs[1]) <= 1e-14

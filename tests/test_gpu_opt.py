@@ -189,3 +189,63 @@ def test_oc_loop_matches_oracle(gpu):
         assert np.abs(x_dev - x_ref).max() <= 1e-6
     finally:
         tmm.set_EVOID_RELATIVE(1e-8)
+
+
+def test_adjoints_with_prescribed_displacement_fd(gpu):
+    """A cantilever whose far face is pulled down by a prescribed
+    displacement (u_z = -0.01 on the x = max nodes, no load): the adjoints
+    vanish on every fixed DOF, so the backward of C = u^T K u, of the p-norm
+    and of sum(u^2) through the displacement adjoint all match central
+    differences on the 8 largest sensitivities (h = 1e-5, solves to rtol
+    1e-13: the FD noise floor 4 rtol |J| / h is asserted to stay below
+    2e-6 |g_e|, under the 1e-5 relative target)."""
+    from pyto_amd import autograd as AG
+    from pyto_amd import bound_cond, deflation, hex_mesher, hex_structural_fea, linear_solvers
+    from pyto_amd import mat_lib
+    print("adjoints with prescribed displacement", flush=True)
+    dims = (12, 6, 4)
+    mesh = hex_mesher.HexMesher().grid_mesh(dims, (1, 1, 1))
+    clamp = mesh.getNodesOnBoundingBoxPlane(0, True)
+    tip = mesh.getNodesOnBoundingBoxPlane(0, False)
+    fixed = np.concatenate([3 * clamp, 3 * clamp + 1, 3 * clamp + 2, 3 * tip + 2])
+    order = np.argsort(fixed)
+    vals = np.concatenate([np.zeros(3 * clamp.size), np.full(tip.size, -0.01)])[order]
+    force = np.zeros(3 * mesh.num_nodes)
+    force[3 * (6 + 13 * (3 + 7 * 4)) + 0] = 2e-3     # a load as well, so f_f != 0
+    bc = bound_cond.BC(force=force, fixed_dofs=fixed[order], dirichlet_values=vals)
+    ds = deflation.DeflationSolver()
+    ds.create_deflation_groups(mesh, 8)
+    fe = hex_structural_fea.HexStructuralFEA(mesh, mat_lib.Material("Unit", 1.0, 0.3), bc,
+                                             linear_solvers.Solvers.DPCG, dsolver=ds, rtol=1e-13,
+                                             maxIters=5000)
+    fe.stiff_mtrx.set_coarse_shift(0.0, 0.0)
+    x = np.random.default_rng(7).uniform(0.3, 1.0, mesh.num_elems)
+    xt = torch.from_numpy(x).to(gpu)
+    u0 = fe.solve(xt)
+    assert abs(float(u0[3 * tip[0] + 2]) + 0.01) <= 1e-15     # the prescribed value holds
+
+    objectives = {
+        "compliance": lambda v: AG.responses(v, fe, pnorm=False)[0],
+        "pnorm": lambda v: AG.responses(v, fe, pnorm=True)[1],
+        "sum_u2": lambda v: (AG.displacement(v, fe) ** 2).sum(),
+    }
+    h, rtol = 1e-5, 1e-13
+    for name, J in objectives.items():
+        xg = xt.clone().requires_grad_(True)
+        J0 = J(xg)
+        (g,) = torch.autograd.grad(J0, xg)
+        g = g.cpu().numpy()
+        # the 8 largest sensitivities: there the FD noise floor is far below
+        # the 1e-5 relative target
+        idx = np.argsort(-np.abs(g))[:8]
+        floor = 4 * rtol * abs(float(J0)) / h
+        assert floor <= 0.2e-5 * np.abs(g[idx]).min(), (name, floor, np.abs(g[idx]).min())
+        with torch.no_grad():
+            for e in idx:
+                v = []
+                for s in (+1, -1):
+                    xx = x.copy()
+                    xx[e] += s * h
+                    v.append(float(J(torch.from_numpy(xx).to(gpu))))
+                fd = (v[0] - v[1]) / (2 * h)
+                assert abs(g[e] - fd) <= 1e-5 * abs(g[e]) + floor, (name, e, g[e], fd)

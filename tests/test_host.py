@@ -25,10 +25,13 @@ def test_mesher_equals_oracle(dims, h):
             assert np.array_equal(m.getNodesOnBoundingBoxPlane(ax, lo), o.getNodesOnBoundingBoxPlane(ax, lo))
 
 
-@pytest.mark.parametrize("dims,ng", [((60, 20, 10), None), ((12, 6, 4), None), ((30, 7, 5), 40),
-                                     ((8, 8, 8), 300)])
-def test_deflation_groups_equal_oracle(dims, ng):
-    m = hex_mesher.HexMesher().grid_mesh(dims, (1, 1, 1))
+@pytest.mark.parametrize("dims,ng,h", [((60, 20, 10), None, (1, 1, 1)), ((12, 6, 4), None, (1, 1, 1)),
+                                       ((30, 7, 5), 40, (1, 1, 1)), ((8, 8, 8), 300, (1, 1, 1)),
+                                       # the hand-built merge cases of test_oracle_groups
+                                       ((6, 2, 3), 5, (1, 1, 1)), ((4, 2, 3), 3, (1.0, 0.5, 0.5)),
+                                       ((2, 5, 5), 6, (1.0, 0.125, 0.125))])
+def test_deflation_groups_equal_oracle(dims, ng, h):
+    m = hex_mesher.HexMesher().grid_mesh(dims, h)
     if ng is None:
         ng = ODF.recommended_groups_ocm(m.num_nodes)
     d = deflation.DeflationSolver()
