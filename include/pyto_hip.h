@@ -237,6 +237,20 @@ int pyto_filter_stencil(pyto_handle* h, double rel_radius, int capacity, int* co
 int pyto_oc_update(pyto_handle* h, const double* x, const double* dc, int64_t n, double dv,
                    double volfrac, double move, double xmin, double xmax, int max_steps,
                    double* xnew, double* lmid_host, int* steps_host, void* stream);
+/* No reference counterpart: pyto_oc_update for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight optimisation or
+ * serving loops, harnesses that forbid hipMalloc).  It needs no handle and
+ * calls no hipMalloc/hipFree or allocating library routine: its scratch (the
+ * bisection's block partials, state and flags) is one static device array of
+ * 65.9 KB per GPU, held until the process ends, whose size does not depend
+ * on n.  Same kernels, same launch shape and so the same results as
+ * pyto_oc_update, bit for bit.  Every call on a GPU shares that GPU's array
+ * (rewritten at the start of each call): calls that overlap on one GPU, on
+ * different streams or threads, race.  lmid_host/steps_host as in
+ * pyto_oc_update (they synchronise the stream; pass null under capture). */
+int pyto_oc_update_noalloc(const double* x, const double* dc, int64_t n, double dv,
+                           double volfrac, double move, double xmin, double xmax, int max_steps,
+                           double* xnew, double* lmid_host, int* steps_host, void* stream);
 
 /* ---- batches of independent systems ------------------------------------- */
 /* A batch solves many independent systems K_i u_i = f_i at once: several

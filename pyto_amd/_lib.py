@@ -103,6 +103,8 @@ _SIGS = {
     "pyto_batch_solve": ([_vp, _vp, _vp, _d, _d, _i, _i, _vp, _vp], _i),
     "pyto_oc_update": ([_vp, _vp, _vp, ctypes.c_int64, _d, _d, _d, _d, _d, _i, _vp,
                         ctypes.POINTER(_d), ctypes.POINTER(_i), _vp], _i),
+    "pyto_oc_update_noalloc": ([_vp, _vp, ctypes.c_int64, _d, _d, _d, _d, _d, _i, _vp,
+                                ctypes.POINTER(_d), ctypes.POINTER(_i), _vp], _i),
 }
 
 EXPORTED = tuple(_SIGS)

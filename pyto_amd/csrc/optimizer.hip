@@ -18,6 +18,8 @@
 // condition fails, block 0 marks the next state done and later launches
 // return at once, so the host enqueues a bounded number of steps without reading
 // anything back.  k_oc_apply then writes xnew at the last evaluated lmid.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace pyto {
@@ -143,11 +145,60 @@ static int oc_check(const double* x, const double* dc, int64_t n, double dv, dou
   return PYTO_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the same kernels on a file-scope device array.
+// Layout (doubles): two parities of kOcMaxBlocks block partials, then
+// kNumScalars scalar slots, then kNumFlags ints.  65.9 KB per GPU, whatever
+// n is (the kernels stride over the elements with at most kOcMaxBlocks
+// workgroups).  k_oc_init rewrites the bisection state and the flags at the
+// start of every call, and every partial is written before it is read, so
+// nothing needs clearing between calls; calls that overlap on one GPU race
+// on this array.
+// ---------------------------------------------------------------------------
+constexpr int kScratchDoubles = 2 * kOcMaxBlocks + kNumScalars + (kNumFlags + 1) / 2;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// Device address of `scratch` per device, looked up once per device with
+// that device current (the array's name is a host-side stand-in, not a
+// device address).
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(double** out) {
+  int dev = 0;
+  PYTO_CHECK_HIP(hipGetDevice(&dev));
+  PYTO_REQUIRE(dev >= 0 && dev < kMaxDevices, PYTO_EINVAL, "device %d beyond %d", dev,
+               kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PYTO_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PYTO_REQUIRE(p != nullptr, PYTO_EHIP, "hipGetSymbolAddress returned a null address");
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PYTO_OK;
+}
+
 }  // namespace pyto
 
 using namespace pyto;
 
 extern "C" {
+
+int pyto_oc_update_noalloc(const double* x, const double* dc, int64_t n, double dv,
+                           double volfrac, double move, double xmin, double xmax, int max_steps,
+                           double* xnew, double* lmid_host, int* steps_host, void* stream) {
+  PYTO_TRY(oc_check(x, dc, n, dv, volfrac, move, xmin, xmax, max_steps, xnew));
+  double* base = nullptr;
+  PYTO_TRY(scratch_ptr(&base));
+  double* partials = base;
+  double* scal = base + 2 * kOcMaxBlocks;
+  int* flags = reinterpret_cast<int*>(scal + kNumScalars);
+  return oc_launch(x, dc, n, dv, volfrac, move, xmin, xmax, max_steps, xnew, scal, flags,
+                   partials, kOcMaxBlocks, lmid_host, steps_host, as_stream(stream));
+}
 
 int pyto_oc_update(pyto_handle* h, const double* x, const double* dc, int64_t n, double dv,
                    double volfrac, double move, double xmin, double xmax, int max_steps,
