@@ -57,6 +57,7 @@ struct ModalConsts {
   T gx, gy, gz;   // 2/h per axis
   T lam, mu;      // Lame constants of E0 times V/64 (Hadamard scaling folded)
   T tw[7];        // thermal: k0 V/64 times the modal weights (element.hpp)
+  T cl, cm, cl3, cm3, ck9;   // cube elements: lam g^2, mu g^2, their thirds, (cl + 4 cm)/9
 };
 
 struct GridDims {

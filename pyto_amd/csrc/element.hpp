@@ -126,27 +126,40 @@ __host__ __device__ __forceinline__ void modal_stress_cube(const Modal<T>& c, T 
     F.cy[0] = s01; F.cy[1] = ltr + M2 * c.cy[1]; F.cy[2] = s12;
     F.cz[0] = s02; F.cz[1] = s12; F.cz[2] = ltr + M2 * c.cz[2];
   }
-  const T M32 = M3 + M3;
   // b = cxy (as d/dy in mode xi), cxz (d/dz in mode xi); cxy (d/dx in mode
-  // eta), cyz (d/dz in mode eta); cxz (d/dx in mode zeta), cyz (d/dy in zeta)
+  // eta), cyz (d/dz in mode eta); cxz (d/dx in mode zeta), cyz (d/dy in zeta).
+  // A diagonal term collects the shear M/3 and the normal 2M/3 of its two
+  // modes: (M/3 + 2M/3) b = M b.
   const T lxi = L3 * (c.cxy[1] + c.cxz[2]);
   const T leta = L3 * (c.cxy[0] + c.cyz[2]);
   const T lze = L3 * (c.cxz[0] + c.cyz[1]);
   const T sxi21 = M3 * (c.cxy[2] + c.cxz[1]);
   const T seta20 = M3 * (c.cxy[2] + c.cyz[0]);
   const T sze10 = M3 * (c.cxz[1] + c.cyz[0]);
-  F.cxy[0] = M3 * c.cxy[0] + (leta + M32 * c.cxy[0]);
-  F.cxy[1] = (lxi + M32 * c.cxy[1]) + M3 * c.cxy[1];
+  F.cxy[0] = M * c.cxy[0] + leta;
+  F.cxy[1] = M * c.cxy[1] + lxi;
   F.cxy[2] = sxi21 + seta20;
-  F.cxz[0] = M3 * c.cxz[0] + (lze + M32 * c.cxz[0]);
+  F.cxz[0] = M * c.cxz[0] + lze;
   F.cxz[1] = sxi21 + sze10;
-  F.cxz[2] = (lxi + M32 * c.cxz[2]) + M3 * c.cxz[2];
+  F.cxz[2] = M * c.cxz[2] + lxi;
   F.cyz[0] = seta20 + sze10;
-  F.cyz[1] = M3 * c.cyz[1] + (lze + M32 * c.cyz[1]);
-  F.cyz[2] = (leta + M32 * c.cyz[2]) + M3 * c.cyz[2];
+  F.cyz[1] = M * c.cyz[1] + lze;
+  F.cyz[2] = M * c.cyz[2] + leta;
   F.cxyz[0] = K9 * c.cxyz[0];
   F.cxyz[1] = K9 * c.cxyz[1];
   F.cxyz[2] = K9 * c.cxyz[2];
+}
+
+// The cube constants of modal_stress_cube for relative modulus E: the unit
+// values (ModalConsts cl, cm, cl3, cm3, ck9, host-computed in FP64) times E.
+template <typename T>
+__host__ __device__ __forceinline__ void cube_consts(T E, const ModalConsts<T>& mc, T& L, T& M,
+                                                     T& L3, T& M3, T& K9) {
+  L = E * mc.cl;
+  M = E * mc.cm;
+  L3 = E * mc.cl3;
+  M3 = E * mc.cm3;
+  K9 = E * mc.ck9;
 }
 
 // Transposed butterfly: f[n][i] = sum_m s_m(n) F_m,i (no constant mode).
@@ -193,10 +206,8 @@ __host__ __device__ __forceinline__ void element_apply_cube(const T u[8][3], T E
                                                             T f[8][3]) {
   Modal<T> c, F;
   modal_forward(u, c);
-  const T g2 = mc.gx * mc.gx;
-  const T L = E * (mc.lam * g2), M = E * (mc.mu * g2);
-  const T L3 = L * T(1.0 / 3.0), M3 = M * T(1.0 / 3.0);
-  const T K9 = (L + T(4) * M) * T(1.0 / 9.0);
+  T L, M, L3, M3, K9;
+  cube_consts(E, mc, L, M, L3, M3, K9);
   modal_stress_cube(c, L, M, L3, M3, K9, F);
   modal_backward(F, f);
 }
@@ -249,15 +260,75 @@ __host__ __device__ __forceinline__ void element_apply_faces(const Face<T>& b, c
   Modal<T> c, F;
   modal_from_faces(b, t, c);
   if (CUBE) {
-    const T g2 = mc.gx * mc.gx;
-    const T L = E * (mc.lam * g2), M = E * (mc.mu * g2);
-    const T L3 = L * T(1.0 / 3.0), M3 = M * T(1.0 / 3.0);
-    const T K9 = (L + T(4) * M) * T(1.0 / 9.0);
+    T L, M, L3, M3, K9;
+    cube_consts(E, mc, L, M, L3, M3, K9);
     modal_stress_cube(c, L, M, L3, M3, K9, F);
   } else {
     modal_stress(c, mc.gx, mc.gy, mc.gz, E * mc.lam, E * mc.mu, F);
   }
   modal_backward(F, f);
+}
+
+// Modal forces F of one element from its faces' butterfly stages (the
+// element_apply_faces work before the transposed butterfly).
+template <typename T, bool CUBE>
+__host__ __device__ __forceinline__ void element_modal_faces(const Face<T>& b, const Face<T>& t,
+                                                             T E, const ModalConsts<T>& mc,
+                                                             Modal<T>& F) {
+  Modal<T> c;
+  modal_from_faces(b, t, c);
+  if (CUBE) {
+    T L, M, L3, M3, K9;
+    cube_consts(E, mc, L, M, L3, M3, K9);
+    modal_stress_cube(c, L, M, L3, M3, K9, F);
+  } else {
+    modal_stress(c, mc.gx, mc.gy, mc.gz, E * mc.lam, E * mc.mu, F);
+  }
+}
+
+// Modal forces F of one element from its 8 corner values.
+template <typename T, bool CUBE>
+__host__ __device__ __forceinline__ void element_modal(const T u[8][3], T E,
+                                                       const ModalConsts<T>& mc, Modal<T>& F) {
+  Modal<T> c;
+  modal_forward(u, c);
+  if (CUBE) {
+    T L, M, L3, M3, K9;
+    cube_consts(E, mc, L, M, L3, M3, K9);
+    modal_stress_cube(c, L, M, L3, M3, K9, F);
+  } else {
+    modal_stress(c, mc.gx, mc.gy, mc.gz, E * mc.lam, E * mc.mu, F);
+  }
+}
+
+// Transposed butterfly of one node plane shared by two element layers.
+// modal_backward's z stage splits an element's modal forces into its bottom
+// face (s = -F_z, sy = F_y - F_yz, dx = F_x - F_xz, dxy = F_xy - F_xyz) and
+// its top face (the same with + signs); the y and x stages are linear, so
+// the plane between element layers k-1 and k is finished once from the sum
+// of layer k-1's top face (carried in `top`) and layer k's bottom face.  On
+// return c holds the plane's corner forces (corners (0,0), (1,0), (1,1),
+// (0,1) of the element's face) and `top` this element's top face.
+template <typename T, int NC>
+__host__ __device__ __forceinline__ void face_backward(const ModalN<T, NC>& F, Face<T, NC>& top,
+                                                       T c[4][NC]) {
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    const T s = top.s[i] - F.cz[i];
+    const T sy = top.sy[i] + (F.cy[i] - F.cyz[i]);
+    const T dx = top.dx[i] + (F.cx[i] - F.cxz[i]);
+    const T dxy = top.dxy[i] + (F.cxy[i] - F.cxyz[i]);
+    top.s[i] = F.cz[i];
+    top.sy[i] = F.cy[i] + F.cyz[i];
+    top.dx[i] = F.cx[i] + F.cxz[i];
+    top.dxy[i] = F.cxy[i] + F.cxyz[i];
+    const T sx0 = s - sy, sx1 = s + sy;
+    const T dx0 = dx - dxy, dx1 = dx + dxy;
+    c[0][i] = sx0 - dx0;
+    c[1][i] = sx0 + dx0;
+    c[2][i] = sx1 + dx1;
+    c[3][i] = sx1 - dx1;
+  }
 }
 
 // Thermal brick (one temperature per node): the conduction energy of each
@@ -288,6 +359,17 @@ __host__ __device__ __forceinline__ void element_apply_thermal(const T u[8][1], 
   modal_forward(u, c);
   thermal_modal(c, k, mc, F);
   modal_backward(F, f);
+}
+
+// Modal fluxes of one thermal element from its faces' butterfly stages.
+template <typename T>
+__host__ __device__ __forceinline__ void thermal_modal_faces(const Face<T, 1>& b,
+                                                             const Face<T, 1>& t, T k,
+                                                             const ModalConsts<T>& mc,
+                                                             ModalN<T, 1>& F) {
+  ModalN<T, 1> c;
+  modal_from_faces(b, t, c);
+  thermal_modal(c, k, mc, F);
 }
 
 // element_apply_thermal from the faces' butterfly stages.
@@ -331,6 +413,15 @@ inline ModalConsts<T> make_modal_consts(double E0, double nu, double hx, double 
   m.gz = T(2.0 / hz);
   m.lam = T(lam * V / 64.0);
   m.mu = T(mu * V / 64.0);
+  // cube elements (hx = hy = hz): g^2 folded into the Lame constants, and the
+  // 1/3 and 1/9 mode weights into per-mode copies (modal_stress_cube)
+  const double g2 = (2.0 / hx) * (2.0 / hx);
+  const double cl = lam * V / 64.0 * g2, cm = mu * V / 64.0 * g2;
+  m.cl = T(cl);
+  m.cm = T(cm);
+  m.cl3 = T(cl / 3.0);
+  m.cm3 = T(cm / 3.0);
+  m.ck9 = T((cl + 4.0 * cm) / 9.0);
   return m;
 }
 

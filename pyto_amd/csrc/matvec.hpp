@@ -53,27 +53,38 @@ __device__ __forceinline__ double dpp_shr1(double v) {
 // the loads stay in flight across a whole layer of element work.
 template <typename T, int NC>
 __device__ __forceinline__ void issue_node(T a[NC], int& mo, int& ml, const T* __restrict__ u,
-                                           const uint8_t* __restrict__ mask, int64_t nxy,
-                                           bool inxy, int64_t plane, int kp, int sz,
-                                           bool any_fixed) {
-  // nxy: the node's (clamped) index within its plane; inxy: x, y inside the
-  // grid.  Only the plane offset changes as the workgroup marches in z.
+                                           const uint8_t* __restrict__ mask, uint32_t nxy,
+                                           uint32_t nxyb, bool inxy, uint32_t plane, int kp,
+                                           int sz, bool any_fixed) {
+  // nxy: the node's (clamped) index within its plane, nxyb the same in bytes
+  // of a node; inxy: x, y inside the grid.  Only the plane offset changes as
+  // the workgroup marches in z.
+  // Indices and byte offsets are 32-bit (the handle limits the grid so that
+  // every byte offset of a DOF vector fits), so the loads take the vector
+  // offset form against the kernel argument's base address; the per-thread
+  // part of the byte offset is a constant and the plane's part uniform.
+  constexpr uint32_t kNode = NC * sizeof(T);
   const bool inb = inxy && kp >= 0 && kp < sz;
-  const int cz = min(max(kp, 0), sz - 1);
-  const int64_t n = nxy + plane * cz;
-  const T* p = u + NC * n;
+  const uint32_t cz = (uint32_t)min(max(kp, 0), sz - 1);
+  uint32_t off = nxyb + (plane * kNode) * cz;
+  // an opaque 32-bit offset: keeps the compiler from folding the constant
+  // part into a 64-bit per-thread pointer (one 32-bit add per load instead
+  // of a 64-bit one)
+  asm volatile("" : "+v"(off));
+  const T* p = reinterpret_cast<const T*>(reinterpret_cast<const char*>(u) + off);
 #pragma unroll
   for (int c = 0; c < NC; ++c) a[c] = p[c];
   mo = inb ? 0 : 7;
   ml = 0;
-  if (any_fixed) ml = mask[n];
+  if (any_fixed) ml = mask[nxy + plane * cz];
 }
 
 // Layer loop, per element layer k (nodes k and k+1):
-//   store plane k+1 (prefetched into registers one layer earlier) to LDS,
-//   issue the global loads of plane k+2, write this thread's y-exchange of
-//   layer k-1, one barrier, finish node plane k-1 (owners), then read the
-//   corners of plane k+1 and apply element (ex, ey, k).
+//   store plane k+1 (prefetched into registers PF layers earlier) to LDS,
+//   issue the global loads of plane k+1+PF, one barrier, finish node plane
+//   k-1 (owners: its x sum and the y exchange of the row below), then read
+//   the corners of plane k+1, apply element (ex, ey, k), finish the x sum of
+//   plane k and write its y exchange.
 // x-neighbour sums use DPP within the 16-lane rows (tx is the lane's row
 // position); only the y-neighbour sums go through LDS.
 // One workgroup's share of y = K u: logical block lb of the operator's
@@ -84,11 +95,22 @@ __device__ __forceinline__ void issue_node(T a[NC], int& mo, int& ml, const T* _
 // (temperature, thermal KE_T); the tile structure is the same.  MASK_IN =
 // false keeps the fixed entries of u (only y's fixed entries are zeroed):
 // that is K_fc g for lifting non-zero Dirichlet values.
-template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true>
+// One node plane's loads in flight: this thread's two staging nodes (raw
+// values, outside-grid and fixed bits) and the E of the element layer below.
+template <typename T, int NC>
+struct PlaneStage {
+  T pv0[NC], pv1[NC];
+  int mo0, ml0, mo1, ml1;
+  T e;
+};
+
+// PF: node planes in flight ahead of the one being stored.  One suffices: a
+// second plane in flight (11 more VGPRs in FP32) measured no faster.
+template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true, int PF = 1>
 __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int pidx) {
   if (a.done && *a.done) return;
   __shared__ T pl[2][kPl][kPl][NC];
-  __shared__ T xc[2][kTile][kTile][2 * NC + 1];   // odd stride: no bank conflicts between rows
+  __shared__ T xc[2][kTile][kTile][NC];   // odd stride (NC = 3 or 1): no bank conflicts between rows
   __shared__ T es[2][kTile][kTile];   // E of the element layer below plane b
   __shared__ uint8_t pm[2][kPl][kPl];  // fixed bits of the staged nodes
   __shared__ double red[4];
@@ -121,42 +143,68 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
   const bool has1 = e1 < kPl * kPl;
   const int lx0 = e0 % kPl, ly0 = e0 / kPl;
   const int lx1 = has1 ? e1 % kPl : 0, ly1 = has1 ? e1 / kPl : 0;
-  T pv0[NC], pv1[NC];
+  PlaneStage<T, NC> st[PF];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) pv1[c] = T(0);
-  int mo0 = 0, ml0 = 0, mo1 = 7, ml1 = 0;
+  for (int q = 0; q < PF; ++q) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) st[q].pv1[c] = T(0);
+    st[q].mo0 = st[q].ml0 = st[q].ml1 = 0;
+    st[q].mo1 = 7;
+    st[q].e = T(0);
+  }
   // E of element layer kp-1, loaded with node plane kp and staged with it
-  const int64_t ecol = min(max(ex, 0), g.nx - 1) + (int64_t)g.nx * min(max(ey, 0), g.ny - 1);
-  T enext = T(0);
+  uint32_t ecolb = (uint32_t)(min(max(ex, 0), g.nx - 1) + g.nx * min(max(ey, 0), g.ny - 1)) *
+                   (uint32_t)sizeof(T);
+  asm volatile("" : "+v"(ecolb));
+  const uint32_t elayer = (uint32_t)(g.nx * g.ny);
   // fixed bits of this thread's own node, read from the staged plane when
   // it becomes the top plane: mq byte 0 = newest plane
   uint32_t mq = 0;
   // in-plane parts of the two staged nodes' indices (constant over z)
-  const int64_t plane = (int64_t)g.sx * g.sy;
+  const uint32_t plane = (uint32_t)(g.sx * g.sy);
   const int gx0 = x0 + lx0, gy0 = y0 + ly0, gx1 = x0 + lx1, gy1 = y0 + ly1;
   const bool inxy0 = gx0 >= 0 && gx0 < g.sx && gy0 >= 0 && gy0 < g.sy;
   const bool inxy1 = gx1 >= 0 && gx1 < g.sx && gy1 >= 0 && gy1 < g.sy;
-  const int64_t nxy0 = min(max(gx0, 0), g.sx - 1) + (int64_t)g.sx * min(max(gy0, 0), g.sy - 1);
-  const int64_t nxy1 = min(max(gx1, 0), g.sx - 1) + (int64_t)g.sx * min(max(gy1, 0), g.sy - 1);
-  auto fetch = [&](int kp) {
+  const uint32_t nxy0 = (uint32_t)(min(max(gx0, 0), g.sx - 1) + g.sx * min(max(gy0, 0), g.sy - 1));
+  const uint32_t nxy1 = (uint32_t)(min(max(gx1, 0), g.sx - 1) + g.sx * min(max(gy1, 0), g.sy - 1));
+  // the same in bytes, opaque to the compiler so that it adds the plane's
+  // uniform byte offset instead of multiplying the node index per load
+  uint32_t nxyb0 = nxy0 * (uint32_t)(NC * sizeof(T)), nxyb1 = nxy1 * (uint32_t)(NC * sizeof(T));
+  asm volatile("" : "+v"(nxyb0), "+v"(nxyb1));
+  auto fetch = [&](int kp, PlaneStage<T, NC>& s) {
     const bool fx = plane_fixed(kp);
-    issue_node<T, NC>(pv0, mo0, ml0, a.u, a.mask, nxy0, inxy0, plane, kp, g.sz, fx);
-    if (has1) issue_node<T, NC>(pv1, mo1, ml1, a.u, a.mask, nxy1, inxy1, plane, kp, g.sz, fx);
+    issue_node<T, NC>(s.pv0, s.mo0, s.ml0, a.u, a.mask, nxy0, nxyb0, inxy0, plane, kp, g.sz, fx);
+    if (has1)
+      issue_node<T, NC>(s.pv1, s.mo1, s.ml1, a.u, a.mask, nxy1, nxyb1, inxy1, plane, kp, g.sz,
+                        fx);
     const int kl = min(max(kp - 1, 0), g.nz - 1);
-    enext = a.Ee[ecol + (int64_t)g.nx * g.ny * kl];
+    uint32_t eoff = ecolb + (elayer * (uint32_t)sizeof(T)) * (uint32_t)kl;
+    asm volatile("" : "+v"(eoff));
+    s.e = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a.Ee) + eoff);
   };
-  auto store = [&](int b) {
-    const int m0 = mo0 | ml0, m1 = mo1 | ml1;
-    const int i0 = MASK_IN ? m0 : mo0, i1 = MASK_IN ? m1 : mo1;
+  // value v with bit c of the fixed bits i set -> 0 (FP32: the sign-extended
+  // bit clears the word, two operations instead of a compare and a select)
+  auto masked = [](T v, int i, int c) -> T {
+    if constexpr (sizeof(T) == 4) {
+      int clear = __builtin_amdgcn_sbfe(i, c, 1);   // -1 when the bit is set
+      asm volatile("" : "+v"(clear));   // keeps the AND from becoming a compare and select
+      return __int_as_float(__float_as_int(v) & ~clear);
+    } else {
+      return ((i >> c) & 1) ? T(0) : v;
+    }
+  };
+  auto store = [&](int b, const PlaneStage<T, NC>& s) {
+    const int m0 = s.mo0 | s.ml0, m1 = s.mo1 | s.ml1;
+    const int i0 = MASK_IN ? m0 : s.mo0, i1 = MASK_IN ? m1 : s.mo1;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) pl[b][ly0][lx0][c] = ((i0 >> c) & 1) ? T(0) : pv0[c];
+    for (int c = 0; c < NC; ++c) pl[b][ly0][lx0][c] = masked(s.pv0[c], i0, c);
     pm[b][ly0][lx0] = (uint8_t)m0;
     if (has1) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) pl[b][ly1][lx1][c] = ((i1 >> c) & 1) ? T(0) : pv1[c];
+      for (int c = 0; c < NC; ++c) pl[b][ly1][lx1][c] = masked(s.pv1[c], i1, c);
       pm[b][ly1][lx1] = (uint8_t)m1;
     }
-    es[b][ty][tx] = enext;
+    es[b][ty][tx] = s.e;
   };
 
   // FP32: x/y butterfly stages of the element's bottom face, carried from
@@ -168,9 +216,11 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
   Face<T, NC> bface;
   T bot[4][NC];
   T ucur[NC];
-  fetch(kz0 - 1);
-  store(0);
-  fetch(kz0);   // also E of layer kz0-1
+  fetch(kz0 - 1, st[0]);
+  store(0, st[0]);
+  // planes kz0 .. kz0 + PF - 1 in flight (with them E of layers kz0-1 ..)
+#pragma unroll
+  for (int q = 0; q < PF; ++q) fetch(min(kz0 + q, kz1), st[q]);
   __syncthreads();
   {
     T c0[NC], c1[NC], c2[NC], c3[NC];
@@ -189,54 +239,56 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
     if (kFaces) face_xy<T, NC>(c0, c1, c2, c3, bface);
   }
   mq = pm[0][ty][tx];
-  T carry[NC];                         // top-plane sum of the layer before last
-  T xb[NC], xt[NC], yb[NC], yt[NC];    // pending sums of the last layer
-  T ubot[NC];                          // u at this thread's node of the pending plane
+  // top face of the previous element layer (z stage of its transposed
+  // butterfly), added to this layer's bottom face: each node plane's forces
+  // go through the y and x stages once
+  Face<T, NC> btop;
+  T xs[NC], ys[NC];    // pending sums of the last finished plane (x sum, y exchange)
+  T ubot[NC];          // u at this thread's node of that plane
 #pragma unroll
-  for (int c = 0; c < NC; ++c) carry[c] = xb[c] = xt[c] = yb[c] = yt[c] = ubot[c] = T(0);
-  double dot = 0.0;
+  for (int c = 0; c < NC; ++c) {
+    btop.s[c] = btop.sy[c] = btop.dx[c] = btop.dxy[c] = T(0);
+    xs[c] = ys[c] = ubot[c] = T(0);
+  }
+  // this thread's share of u.y over its owned nodes, in T: a thread sums at
+  // most 3 * zchunk products; the block sum and everything after is FP64
+  T dot = T(0);
   // this thread's own node in its plane (owners only write it)
-  const int64_t own_xy = (int64_t)ex + (int64_t)g.sx * ey;
+  uint32_t own_xyb = (uint32_t)(ex + g.sx * ey) * (uint32_t)(NC * sizeof(T));
+  asm volatile("" : "+v"(own_xyb));
   // one element layer k (see the comment above the kernel); false once the
   // last node plane of the chunk is finished.  Called twice per loop trip so
   // that the values carried from layer to layer alternate between register
   // sets instead of being copied.
-  auto layer = [&](int k) -> bool {
+  auto layer = [&](int k, PlaneStage<T, NC>& s) -> bool {
     const int b = (k - kz0 + 2) & 1;   // LDS plane buffer of node plane k+1
-    const int xb_i = k & 1;            // y-exchange buffer of layer k-1
-    if (k < kz1) store(b);
+    // y-exchange buffer of plane k-1 (written in layer k-1).  Parities
+    // relative to kz0 are fixed for each of the two calls of a loop trip,
+    // so the LDS offsets are constants.
+    const int xb_i = (k - kz0) & 1;
+    if (k < kz1) store(b, s);
     // unconditional (clamped to plane kz1): a plane past kz1 is never stored,
     // and no load result is carried through a conditional path
-    fetch(min(k + 2, kz1));
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      xc[xb_i][ty][tx][c] = yb[c];
-      xc[xb_i][ty][tx][NC + c] = yt[c];
-    }
+    fetch(min(k + 1 + PF, kz1), s);
     __syncthreads();
-    // finish node plane k-1 (bottom of layer k-1 + top of layer k-2)
+    // finish node plane k-1: its x sum plus the y exchange of the row below
     if (owner) {
-      // all six exchange values are read unconditionally (branch-free)
-      T sb[NC], st[NC];
+      T sb[NC];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        sb[c] = carry[c] + xb[c] + xc[xb_i][ty - 1][tx][c];
-        st[c] = xt[c] + xc[xb_i][ty - 1][tx][NC + c];
-      }
+      for (int c = 0; c < NC; ++c) sb[c] = xs[c] + xc[xb_i][ty - 1][tx][c];
       if (k - 1 >= kz0) {
         const int kp = k - 1;
-        const int64_t nw = own_xy + plane * kp;
         const uint32_t m = (mq >> 8) & 0xffu;   // plane k-1 (byte 0: plane k)
-        T* yp = a.y + NC * nw;
+        uint32_t yoff = own_xyb + (plane * (uint32_t)(NC * sizeof(T))) * (uint32_t)kp;
+        asm volatile("" : "+v"(yoff));
+        T* yp = reinterpret_cast<T*>(reinterpret_cast<char*>(a.y) + yoff);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           const T v = ((m >> c) & 1u) ? T(0) : sb[c];
           yp[c] = v;
-          dot += (double)v * (double)ubot[c];
+          dot += v * ubot[c];
         }
       }
-#pragma unroll
-      for (int c = 0; c < NC; ++c) carry[c] = st[c];
     }
     if (k >= kz1) return false;
     Face<T, NC> tface;
@@ -252,15 +304,15 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
     }
     if (kFaces) face_xy<T, NC>(top[0], top[1], top[2], top[3], tface);
     mq = (mq << 8) | (uint32_t)pm[b][ty][tx];
-    T f[8][NC];
+    ModalN<T, NC> F;
     // wave-uniform test only: a column outside the grid applies E = 0 (its
     // corner values are 0 as well), so no per-lane merge of the results
     if (k >= 0 && k < g.nz) {
       const T E = col_valid ? es[b][ty][tx] : T(0);   // E (or k) of layer k, staged with plane k+1
       if constexpr (NC == 1) {
-        element_apply_thermal_faces<T>(bface, tface, E, a.mc, f);
+        thermal_modal_faces<T>(bface, tface, E, a.mc, F);
       } else if constexpr (kFaces) {
-        element_apply_faces<T, CUBE>(bface, tface, E, a.mc, f);
+        element_modal_faces<T, CUBE>(bface, tface, E, a.mc, F);
       } else {
         T ue[8][3];
 #pragma unroll
@@ -270,24 +322,28 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
             ue[j][c] = bot[j][c];
             ue[4 + j][c] = top[j][c];
           }
-        if (CUBE)
-          element_apply_cube<T>(ue, E, a.mc, f);
-        else
-          element_apply<T>(ue, E, a.mc, f);
+        element_modal<T, CUBE>(ue, E, a.mc, F);
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) f[j][c] = T(0);
+      for (int c = 0; c < NC; ++c)
+        F.cx[c] = F.cy[c] = F.cz[c] = F.cxy[c] = F.cxz[c] = F.cyz[c] = F.cxyz[c] = T(0);
     }
-    // x-neighbour sums (DPP) for the nodes (tx,ty) and (tx,ty+1) of both planes
+    // node plane k from this layer's bottom face and the last layer's top
+    // face; x-neighbour sums (DPP) for the nodes (tx,ty) and (tx,ty+1)
+    T fc[4][NC];
+    face_backward<T, NC>(F, btop, fc);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      xb[c] = f[0][c] + dpp_shr1(f[1][c]);
-      yb[c] = f[3][c] + dpp_shr1(f[2][c]);
-      xt[c] = f[4][c] + dpp_shr1(f[5][c]);
-      yt[c] = f[7][c] + dpp_shr1(f[6][c]);
+      xs[c] = fc[0][c] + dpp_shr1(fc[1][c]);
+      ys[c] = fc[3][c] + dpp_shr1(fc[2][c]);
+      // the x sum is formed here, next to its DPP shift (which then folds
+      // into the add), not where the next layer uses it
+      asm volatile("" : "+v"(xs[c]));
+      // y exchange of plane k for the row above, read after the next
+      // layer's barrier; that buffer was last read before this layer's
+      // barrier
+      xc[(k - kz0 + 1) & 1][ty][tx][c] = ys[c];
     }
     if (kFaces) {
       bface = tface;
@@ -299,13 +355,15 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
     }
     return true;
   };
+  // plane k+1 is staged in st[(k - kz0 + 1) % PF]: with PF = 2 the two calls
+  // of a trip use the two stages, with PF = 1 both use st[0]
   for (int k = kz0 - 1;; k += 2) {
-    if (!layer(k)) break;
-    if (!layer(k + 1)) break;
+    if (!layer(k, st[0])) break;
+    if (!layer(k + 1, st[PF - 1])) break;
   }
   if (a.partials) {
     // block partial of u.y; the consumer kernel sums the partials
-    const double bs = block_reduce_sum256(dot, red);
+    const double bs = block_reduce_sum256((double)dot, red);
     if (threadIdx.x == 0) a.partials[pidx] = bs;
   }
 }

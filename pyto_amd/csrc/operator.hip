@@ -98,8 +98,11 @@ __global__ void k_inv_diag(const double* __restrict__ E64, const uint8_t* __rest
   }
 }
 
-template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true>
-__global__ __launch_bounds__(256) void k_matvec(MatvecArgs<T> a) {
+// At least 4 waves per SIMD for FP32 and 3 for FP64 (128 / 168 VGPRs).
+template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true,
+          int WPE = sizeof(T) == 4 ? 4 : 3>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_matvec(
+    MatvecArgs<T> a) {
   matvec_tile<T, CUBE, NC, MASK_IN>(a, xcd_logical_block(blockIdx.x, gridDim.x), blockIdx.x);
 }
 
@@ -343,7 +346,10 @@ int pyto_create(pyto_handle** out, int nelx, int nely, int nelz, double hx, doub
   PYTO_REQUIRE(nu > -1.0 && nu < 0.5, PYTO_EINVAL, "Poisson ratio must be in (-1, 0.5)");
   PYTO_REQUIRE(fixed_mask, PYTO_EINVAL, "fixed_mask is null");
   const int64_t nn = (int64_t)(nelx + 1) * (nely + 1) * (nelz + 1);
-  PYTO_REQUIRE(3 * nn < (int64_t)INT32_MAX * 4, PYTO_EINVAL, "grid too large");
+  // the operator addresses DOF vectors with 32-bit byte offsets (FP64: 24 B per node)
+  PYTO_REQUIRE(24 * nn <= (int64_t)UINT32_MAX, PYTO_EINVAL,
+               "grid too large: %lld nodes, at most %lld", (long long)nn,
+               (long long)(UINT32_MAX / 24));
   pyto_handle* h = new pyto_handle();
   h->E0 = E0; h->nu = nu; h->penal = penal; h->eps_void = eps_void;
   h->mc64 = make_modal_consts<double>(E0, nu, hx, hy, hz);
@@ -370,7 +376,9 @@ int pyto_create_thermal(pyto_handle** out, int nelx, int nely, int nelz, double 
   PYTO_REQUIRE(k0 > 0, PYTO_EINVAL, "conductivity must be positive");
   PYTO_REQUIRE(fixed_mask, PYTO_EINVAL, "fixed_mask is null");
   const int64_t nn = (int64_t)(nelx + 1) * (nely + 1) * (nelz + 1);
-  PYTO_REQUIRE(nn < (int64_t)INT32_MAX * 4, PYTO_EINVAL, "grid too large");
+  PYTO_REQUIRE(8 * nn <= (int64_t)UINT32_MAX, PYTO_EINVAL,
+               "grid too large: %lld nodes, at most %lld", (long long)nn,
+               (long long)(UINT32_MAX / 8));
   pyto_handle* h = new pyto_handle();
   h->E0 = k0; h->nu = 0.0; h->penal = penal; h->eps_void = k_void;
   h->mc64 = make_thermal_consts<double>(k0, hx, hy, hz);
