@@ -112,6 +112,16 @@ int pyto_info(pyto_handle* h, int64_t* nnodes, int64_t* nelems, int64_t* ndof,
  * (src/deflation.py:514-521). */
 int pyto_set_density(pyto_handle* h, const double* x, void* stream);
 
+/* pyto_set_density for n distinct handles at once (the designs of a batch):
+ * every handle's E_e, Jacobi diagonal and Galerkin matrix, then one batched
+ * FP64 Cholesky factorisation and inverse of all their coarse matrices
+ * (rocSOLVER's batched potrf/potri on copies padded with identity to the
+ * largest order) and one synchronisation of `stream` for the pivot check.
+ * x[i] holds handle i's densities.  No reference counterpart (the reference
+ * updates one design per solve). */
+int pyto_set_density_batch(int n, pyto_handle* const* handles, const double* const* x,
+                           void* stream);
+
 /* Deflation groups: node_group (int32 per node, 0..ngroups-1) and group
  * centres (FP64, ngroups x 3).  W is implicit (rigid-body modes about the
  * centres, column map u:{0,4,5} v:{1,3,5} w:{2,3,4}); replaces

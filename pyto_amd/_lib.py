@@ -78,6 +78,7 @@ _SIGS = {
     "pyto_destroy": ([_vp], _i),
     "pyto_info": ([_vp, _vp, _vp, _vp, _vp], _i),
     "pyto_set_density": ([_vp, _vp, _vp], _i),
+    "pyto_set_density_batch": ([_i, _vp, _vp, _vp], _i),
     "pyto_set_groups": ([_vp, _vp, _i, _vp, _vp], _i),
     "pyto_matvec_f64": ([_vp, _vp, _vp, _vp], _i),
     "pyto_matvec_f32": ([_vp, _vp, _vp, _vp], _i),

@@ -58,6 +58,21 @@ class BatchSolver:
         except Exception:
             pass
 
+    def set_densities(self, xs):
+        """New densities for every design at once (one per operator, in
+        operator order): element moduli, Jacobi diagonals and Galerkin
+        matrices per design, then one batched FP64 Cholesky factorisation and
+        inverse of all the designs' coarse matrices."""
+        uniq, seen = [], {}
+        for op, x in zip(self.ops, xs):
+            if id(op) in seen:
+                if seen[id(op)] is not x:
+                    raise ValueError("a design listed twice needs the same density array")
+                continue
+            seen[id(op)] = x
+            uniq.append((op, x))
+        set_density_batch([op for op, _ in uniq], [x for _, x in uniq])
+
     def info(self):
         z, m, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(_lib.lib().pyto_batch_info(self._b, ctypes.byref(z), ctypes.byref(m), ctypes.byref(v)))
@@ -98,3 +113,24 @@ class BatchSolver:
                                          int(max_iters), int(max_outer), st, stream_ptr())
         check(rc, "batch solve")
         return us, [st[i].as_dict() for i in range(self.nsys)]
+
+
+def set_density_batch(operators, xs):
+    """pyto_set_density for several distinct GridOperators (one density
+    array per operator): one batched coarse factorisation for all of them."""
+    ops = list(operators)
+    if len(ops) != len(xs):
+        raise ValueError("one density array per operator")
+    if len(set(id(op) for op in ops)) != len(ops):
+        raise ValueError("an operator is listed twice")
+    xt = [op.as_elem(x) for op, x in zip(ops, xs)]
+    for op, x in zip(ops, xt):
+        if x.numel() != op.num_elems:
+            raise ValueError("density of %d entries for a grid of %d elements"
+                             % (x.numel(), op.num_elems))
+    hs = (ctypes.c_void_p * len(ops))(*[op.handle for op in ops])
+    xp = (ctypes.c_void_p * len(ops))(*[x.data_ptr() for x in xt])
+    check(_lib.lib().pyto_set_density_batch(len(ops), hs, xp, stream_ptr()),
+          "pyto_set_density_batch")
+    for op, x in zip(ops, xt):
+        op.x = x

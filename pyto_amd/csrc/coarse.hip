@@ -26,6 +26,8 @@
 
 namespace pyto {
 
+int density_elements(Handle* h, const double* x, hipStream_t s);  // operator.hip
+
 // rigid-mode weights of node (ix,iy,iz) relative to centre c, per component:
 // returns the 6 column coefficients for component comp.
 __device__ __forceinline__ void w_row(int comp, double rx, double ry, double rz, double w[6]) {
@@ -922,8 +924,9 @@ void release_blas(Handle* h) {
   h->blas = nullptr;
 }
 
-int coarse_setup(Handle* h, hipStream_t s) {
-  h->coarse_ready = false;   // until the new factor is complete
+// The Galerkin matrix of the current densities into h->Ec (symmetrised and
+// shifted, identity on the padding); no factorisation.
+static int coarse_assemble(Handle* h, hipStream_t s) {
   const int ld = round_up64(h->nc);
   if (h->nsblocks > 0) {
     const int64_t n = (int64_t)h->nsblocks * kSpecialBlockChunks * 64;
@@ -950,10 +953,124 @@ int coarse_setup(Handle* h, hipStream_t s) {
   hipLaunchKernelGGL(k_symmetrize, dim3((ld + 255) / 256, ld), dim3(256), 0, s, h->Ec, h->nc, ld,
                      h->shift_abs, h->shift_rel);
   PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+int coarse_setup(Handle* h, hipStream_t s) {
+  h->coarse_ready = false;   // until the new factor is complete
+  PYTO_TRY(coarse_assemble(h, s));
   PYTO_CHECK_HIP(hipMemsetAsync(h->flags + F_COARSE_PIVOT, 0, sizeof(int), s));
   PYTO_TRY(factorize_coarse(h, s));
   h->coarse_ready = true;
   return PYTO_OK;
+}
+
+// Copy an nc x nc symmetric matrix (leading dimension ld) into an n x n
+// matrix with identity padding: blockdiag(E, I) has inverse blockdiag(E^-1, I).
+__global__ void k_pad_identity(const double* __restrict__ E, int ld, int nc,
+                               double* __restrict__ P, int n) {
+  const int r = blockIdx.y;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    P[(int64_t)r * n + c] = (r < nc && c < nc) ? E[(int64_t)r * ld + c] : (r == c ? 1.0 : 0.0);
+}
+
+// Einv (nc x nc of leading dimension ld, full symmetric) from the lower
+// triangle of a column-major n x n inverse: A(i, j), i >= j, at P[i + j n].
+__global__ void k_unpad_symmetric(const double* __restrict__ P, int n, double* __restrict__ Einv,
+                                  int ld, int nc) {
+  const int r = blockIdx.y;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    const int i = max(r, c), j = min(r, c);
+    Einv[(int64_t)r * ld + c] = P[i + (int64_t)j * n];
+  }
+}
+
+// Density update of many handles: every handle's element kernels and Galerkin
+// matrix, then ONE batched Cholesky factorisation and inverse of all the
+// coarse matrices (padded with identity to the largest order), one host
+// synchronisation for the pivot check.
+int set_density_many(int n, Handle* const* hs, const double* const* x, hipStream_t s) {
+  std::vector<Handle*> defl;
+  int nmax = 0;
+  for (int i = 0; i < n; ++i) {
+    Handle* h = hs[i];
+    PYTO_TRY(density_elements(h, x[i], s));
+    if (h->ngroups > 0) {
+      h->coarse_ready = false;
+      PYTO_TRY(coarse_assemble(h, s));
+      defl.push_back(h);
+      nmax = std::max(nmax, h->nc);
+    }
+  }
+  if (defl.empty()) return PYTO_OK;
+  const int nb = (int)defl.size();
+  Handle* h0 = defl[0];
+  if (!h0->blas) {
+    rocblas_handle bh = nullptr;
+    PYTO_REQUIRE(rocblas_create_handle(&bh) == rocblas_status_success, PYTO_EHIP,
+                 "rocblas_create_handle failed");
+    h0->blas = bh;
+  }
+  rocblas_handle bh = (rocblas_handle)h0->blas;
+  PYTO_REQUIRE(rocblas_set_stream(bh, s) == rocblas_status_success, PYTO_EHIP,
+               "rocblas_set_stream failed");
+  // workspace: nb padded matrices, their pointer table and pivot flags
+  double* P = nullptr;
+  double** ptrs = nullptr;
+  int* info = nullptr;
+  const size_t msz = (size_t)nmax * nmax;
+  auto release = [&]() {
+    if (P) hipFree(P);
+    if (ptrs) hipFree(ptrs);
+    if (info) hipFree(info);
+  };
+  int rc = PYTO_OK;
+  if (hipMalloc(&P, msz * nb * sizeof(double)) != hipSuccess ||
+      hipMalloc(&ptrs, nb * sizeof(double*)) != hipSuccess ||
+      hipMalloc(&info, nb * sizeof(int)) != hipSuccess) {
+    release();
+    set_error("device allocation of the batched coarse workspace (%d x %d^2) failed", nb, nmax);
+    return PYTO_ENOMEM;
+  }
+  std::vector<double*> hp(nb);
+  for (int i = 0; i < nb; ++i) hp[i] = P + msz * i;
+  std::vector<int> hinfo(nb, 0);
+  do {
+    if (hipMemcpyAsync(ptrs, hp.data(), nb * sizeof(double*), hipMemcpyHostToDevice, s) !=
+        hipSuccess) { rc = PYTO_EHIP; set_error("pointer table copy failed"); break; }
+    for (int i = 0; i < nb; ++i)
+      hipLaunchKernelGGL(k_pad_identity, dim3((nmax + 255) / 256, nmax), dim3(256), 0, s,
+                         (const double*)defl[i]->Ec, round_up64(defl[i]->nc), defl[i]->nc, hp[i],
+                         nmax);
+    if (hipGetLastError() != hipSuccess) { rc = PYTO_EHIP; set_error("pad kernel failed"); break; }
+    if (rocsolver_dpotrf_batched(bh, rocblas_fill_lower, nmax, ptrs, nmax, info, nb) !=
+        rocblas_status_success) { rc = PYTO_EHIP; set_error("rocsolver_dpotrf_batched failed"); break; }
+    if (hipMemcpyAsync(hinfo.data(), info, nb * sizeof(int), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) { rc = PYTO_EHIP; set_error("pivot read-back failed"); break; }
+    for (int i = 0; i < nb && rc == PYTO_OK; ++i)
+      if (hinfo[i] != 0) {
+        rc = PYTO_ENUMERIC;
+        set_error("coarse matrix W^T K W of handle %d of the batch is not positive definite "
+                  "(leading minor %d of %d); a coarse diagonal shift "
+                  "(pyto_set_coarse_shift) regularises it", i, hinfo[i], defl[i]->nc);
+      }
+    if (rc != PYTO_OK) break;
+    if (rocsolver_dpotri_batched(bh, rocblas_fill_lower, nmax, ptrs, nmax, info, nb) !=
+        rocblas_status_success) { rc = PYTO_EHIP; set_error("rocsolver_dpotri_batched failed"); break; }
+    for (int i = 0; i < nb; ++i)
+      hipLaunchKernelGGL(k_unpad_symmetric, dim3((defl[i]->nc + 255) / 256, defl[i]->nc),
+                         dim3(256), 0, s, (const double*)hp[i], nmax, defl[i]->Einv,
+                         round_up64(defl[i]->nc), defl[i]->nc);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      rc = PYTO_EHIP;
+      set_error("batched coarse inverse failed");
+      break;
+    }
+    for (Handle* h : defl) h->coarse_ready = true;
+  } while (false);
+  release();
+  return rc;
 }
 
 // Free a handle's coarse space and return it to the no-groups state.
@@ -1323,6 +1440,18 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   }
   if (h->density_set) PYTO_TRY(coarse_setup(h, s));
   return PYTO_OK;
+}
+
+int pyto_set_density_batch(int n, pyto_handle* const* handles, const double* const* x,
+                           void* stream) {
+  PYTO_REQUIRE(n >= 1 && handles && x, PYTO_EINVAL, "null argument or n < 1");
+  for (int i = 0; i < n; ++i) {
+    PYTO_REQUIRE(handles[i] && x[i], PYTO_EINVAL, "handle or density %d is null", i);
+    for (int j = 0; j < i; ++j)
+      PYTO_REQUIRE(handles[j] != handles[i], PYTO_EINVAL, "handle %d is listed twice", i);
+  }
+  std::vector<Handle*> hs(handles, handles + n);
+  return set_density_many(n, hs.data(), x, as_stream(stream));
 }
 
 int pyto_set_coarse_shift(pyto_handle* h, double abs_shift, double rel_shift) {

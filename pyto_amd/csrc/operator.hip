@@ -310,6 +310,23 @@ static int create_common(pyto_handle* h, int nelx, int nely, int nelz, double hx
   return PYTO_OK;
 }
 
+// E_e and the Jacobi inverse diagonal of densities x: the element part of
+// pyto_set_density (the coarse matrix follows in coarse.hip).
+int density_elements(Handle* h, const double* x, hipStream_t s) {
+  const int64_t ne = h->g.nelems;
+  int blocks = (int)std::min<int64_t>((ne + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_simp, dim3(blocks), dim3(256), 0, s, x, h->Ee64, h->Ee32, h->x64, ne,
+                     h->penal, h->eps_void);
+  PYTO_CHECK_HIP(hipGetLastError());
+  blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(256), 0, s, h->Ee64, h->fixmask, h->g,
+                     h->ncomp, h->kediag[0], h->kediag[1], h->kediag[2], h->invd64, h->invd32,
+                     (double*)nullptr);
+  PYTO_CHECK_HIP(hipGetLastError());
+  h->density_set = true;
+  return PYTO_OK;
+}
+
 }  // namespace pyto
 
 using namespace pyto;
@@ -432,17 +449,7 @@ int pyto_info(pyto_handle* h, int64_t* nnodes, int64_t* nelems, int64_t* ndof,
 int pyto_set_density(pyto_handle* h, const double* x, void* stream) {
   PYTO_REQUIRE(h && x, PYTO_EINVAL, "null argument");
   hipStream_t s = as_stream(stream);
-  const int64_t ne = h->g.nelems;
-  int blocks = (int)std::min<int64_t>((ne + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_simp, dim3(blocks), dim3(256), 0, s, x, h->Ee64, h->Ee32, h->x64, ne,
-                     h->penal, h->eps_void);
-  PYTO_CHECK_HIP(hipGetLastError());
-  blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(256), 0, s, h->Ee64, h->fixmask, h->g,
-                     h->ncomp, h->kediag[0], h->kediag[1], h->kediag[2], h->invd64, h->invd32,
-                     (double*)nullptr);
-  PYTO_CHECK_HIP(hipGetLastError());
-  h->density_set = true;
+  PYTO_TRY(density_elements(h, x, s));
   if (h->ngroups > 0) PYTO_TRY(coarse_setup(h, s));
   return PYTO_OK;
 }
