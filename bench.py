@@ -389,8 +389,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 32-design line reported beside `value`")
     ap.add_argument("--operator-only", action="store_true",
-                    help="only 20 launches of the FP32 operator on the workload (for "
-                         "counter collection); prints their HIP-event average")
+                    help="only 20 launches each of the FP32 operator, the FP64 one and the "
+                         "FP32 matrix-core variant on the workload (for counter collection); "
+                         "prints their HIP-event averages")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -431,22 +432,34 @@ def main():
         % (t_setup_host * 1e3, t_setup_dev * 1e3))
     x_np = problems.config_density(args.config, grid)
     if args.operator_only:
+        # 20 back-to-back launches each of the FP32 operator, the FP64 one
+        # and the FP32 variant with the dense KE on the matrix cores
         op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, None, penal=3.0, eps_void=1e-9,
                           fixed_mask=mask)
         op.set_density(torch.from_numpy(x_np).to(dev))
         p32 = torch.randn(op.num_dofs, dtype=torch.float32, device=dev)
-        q32 = torch.empty_like(p32)
-        for _ in range(3):
-            op.matvec(p32, q32)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(20):
-            op.matvec(p32, q32)
-        e1.record()
-        torch.cuda.synchronize()
-        print(json.dumps({"operator_only": True, "kernel": "k_matvec<float>",
-                          "us_per_launch": e0.elapsed_time(e1) / 20 * 1e3}))
+        p64 = p32.double()
+        res = {"operator_only": True}
+        for name, fn, u in (("k_matvec<float>", op.matvec, p32),
+                            ("k_matvec<double>", op.matvec, p64),
+                            ("k_matvec_mfma", op.matvec_mfma, p32)):
+            q = torch.empty_like(u)
+            for _ in range(3):
+                fn(u, q)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                fn(u, q)
+            e1.record()
+            torch.cuda.synchronize()
+            res[name + "_us"] = e0.elapsed_time(e1) / 20 * 1e3
+            res[name + "_out"] = q
+        y32, y64, ym = (res.pop(k + "_out") for k in ("k_matvec<float>", "k_matvec<double>",
+                                                      "k_matvec_mfma"))
+        res["f32_vs_f64_relmax"] = float((y32.double() - y64).abs().max() / y64.abs().max())
+        res["mfma_vs_f64_relmax"] = float((ym.double() - y64).abs().max() / y64.abs().max())
+        print(json.dumps(res))
         return
     log("grid", grid, "dof", 3 * mesh.num_nodes)
     op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, None, penal=3.0, eps_void=1e-9,

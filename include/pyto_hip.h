@@ -135,6 +135,12 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
  * (src/deflation.py:540,558,573; src/hex_structural_fea.py:101). */
 int pyto_matvec_f64(pyto_handle* h, const double* u, double* y, void* stream);
 int pyto_matvec_f32(pyto_handle* h, const float* u, float* y, void* stream);
+/* The same FP32 product with the element contraction on the matrix cores
+ * (the dense 24x24 KE by v_mfma_f32_32x32x2_f32 instead of the modal
+ * element).  No reference counterpart: a measurement variant, slower than
+ * pyto_matvec_f32 (the dense contraction does about 10x the flops of the
+ * modal element). */
+int pyto_matvec_f32_mfma(pyto_handle* h, const float* u, float* y, void* stream);
 
 /* y = K_fc g: the operator with g's fixed entries kept and y's fixed entries
  * zeroed (FP64).  f_f - K_fc g is the right-hand side with the non-zero

@@ -82,6 +82,7 @@ _SIGS = {
     "pyto_set_groups": ([_vp, _vp, _i, _vp, _vp], _i),
     "pyto_matvec_f64": ([_vp, _vp, _vp, _vp], _i),
     "pyto_matvec_f32": ([_vp, _vp, _vp, _vp], _i),
+    "pyto_matvec_f32_mfma": ([_vp, _vp, _vp, _vp], _i),
     "pyto_diagonal_f64": ([_vp, _vp, _vp], _i),
     "pyto_restrict_f64": ([_vp, _vp, _vp, _vp], _i),
     "pyto_prolong_f64": ([_vp, _vp, _vp, _vp], _i),

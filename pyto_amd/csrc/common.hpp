@@ -193,6 +193,7 @@ struct Handle {
   int* host_flags = nullptr;    // pinned mirror of flags
   double* elem_buf = nullptr;   // N_el x 6 FP64 element workspace
   void* stencil = nullptr;      // cone-filter stencil on the device (postproc.hip)
+  float* ke32 = nullptr;        // dense FP32 KE for the MFMA operator variant (lazy)
   double stencil_rel = 0.0;     // its radius in units of hx (0: none built yet)
   hipStream_t gstream = nullptr;  // capture/replay stream (default priority)
   hipEvent_t ev_a = nullptr, ev_b = nullptr;

@@ -30,7 +30,48 @@ struct MatvecArgs {
   const int* done;     // skip when *done != 0 (null: never)
   double* zero_vec;    // zeroed by block 0 (null: none)
   int zero_n;
+  const float* ke32;   // dense 24x24 KE (row-major FP32), MFMA variant only
 };
+
+// Dense-KE element contraction on the matrix cores (FP32 operator variant):
+// f = KE (E u) for the 64 elements of a wave as two v_mfma_f32_32x32x2_f32
+// sequences of 12 K-steps (rows: the 24 DOFs of KE padded to 32; columns: 32
+// elements).  A holds KE[l % 32][2s + l / 32] (ka, 12 values per lane).  The
+// B operand of set 0 (elements of lanes 0-31) wants lane l + 32 to hold DOF
+// 2s+1 of element l, and set 1 the mirror, which v_permlane32_swap forms
+// from (DOF 2s, DOF 2s+1) of every lane in one instruction; the 32x32
+// accumulators hold rows 8q + 4h + m (h = lane / 32) and are swapped back
+// the same way, so every lane ends with the 24 forces of its own element.
+typedef float mfma_v16f __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ void element_mfma(const float u[8][3], float E, const float ka[12],
+                                             float f[8][3]) {
+  float ue[24];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ue[3 * j + c] = E * u[j][c];
+  mfma_v16f acc0 = {}, acc1 = {};
+#pragma unroll
+  for (int s = 0; s < 12; ++s) {
+    const auto bw = __builtin_amdgcn_permlane32_swap(__float_as_uint(ue[2 * s]),
+                                                     __float_as_uint(ue[2 * s + 1]), false, false);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ka[s], __uint_as_float(bw[0]), acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ka[s], __uint_as_float(bw[1]), acc1, 0, 0, 0);
+  }
+  float ra[16], rb[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const auto w = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]),
+                                                    __float_as_uint(acc1[r]), false, false);
+    ra[r] = __uint_as_float(w[0]);   // rows 8q + m   (m < 4) of this lane's element
+    rb[r] = __uint_as_float(w[1]);   // rows 8q + 4 + m
+  }
+#pragma unroll
+  for (int d = 0; d < 24; ++d) {
+    const int q = d / 8, m = d % 8;
+    f[d / 3][d % 3] = m < 4 ? ra[4 * q + m] : rb[4 * q + m - 4];
+  }
+}
 
 // Lane (tx-1)'s value within the 16-lane row (0 for tx = 0): DPP row_shr:1
 // with bound_ctrl, which reads 0 where the row has no source lane, so no
@@ -106,8 +147,9 @@ struct PlaneStage {
 
 // PF: node planes in flight ahead of the one being stored.  One suffices: a
 // second plane in flight (11 more VGPRs in FP32) measured no faster.
-template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true, int PF = 1>
+template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true, int PF = 1, bool MFMA = false>
 __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int pidx) {
+  static_assert(!MFMA || (sizeof(T) == 4 && NC == 3), "the MFMA variant is FP32 structural");
   if (a.done && *a.done) return;
   __shared__ T pl[2][kPl][kPl][NC];
   __shared__ T xc[2][kTile][kTile][NC];   // odd stride (NC = 3 or 1): no bank conflicts between rows
@@ -212,7 +254,20 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
   // FP64 keeps the corner values instead: carrying the face stages costs it
   // 8 VGPRs more and one wave per SIMD.
   // (one temperature per node: the faces are small in both precisions)
-  constexpr bool kFaces = sizeof(T) == 4 || NC == 1;
+  constexpr bool kFaces = (sizeof(T) == 4 || NC == 1) && !MFMA;
+  // MFMA variant: this lane's A operands (KE entries) and the top-face
+  // corner forces of the last element layer
+  float ka[12];
+  float ctop[4][3];
+  if constexpr (MFMA) {
+    const int l = threadIdx.x & 63, row = l & 31;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) ka[q] = row < 24 ? a.ke32[row * 24 + 2 * q + (l >> 5)] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ctop[j][c] = 0.0f;
+  }
   Face<T, NC> bface;
   T bot[4][NC];
   T ucur[NC];
@@ -304,6 +359,39 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
     }
     if (kFaces) face_xy<T, NC>(top[0], top[1], top[2], top[3], tface);
     mq = (mq << 8) | (uint32_t)pm[b][ty][tx];
+    if constexpr (MFMA) {
+      float ue[8][3], f[8][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          ue[j][c] = bot[j][c];
+          ue[4 + j][c] = top[j][c];
+        }
+      // the wave-uniform layer test as below; E = 0 outside the grid
+      const float E = (k >= 0 && k < g.nz && col_valid) ? (float)es[b][ty][tx] : 0.0f;
+      element_mfma(ue, E, ka, f);
+      float fc[4][3];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          fc[j][c] = f[j][c] + ctop[j][c];
+          ctop[j][c] = f[4 + j][c];
+        }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        xs[c] = fc[0][c] + dpp_shr1(fc[1][c]);
+        ys[c] = fc[3][c] + dpp_shr1(fc[2][c]);
+        asm volatile("" : "+v"(xs[c]));
+        xc[(k - kz0 + 1) & 1][ty][tx][c] = ys[c];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) bot[j][c] = top[j][c];
+      return true;
+    }
     ModalN<T, NC> F;
     // wave-uniform test only: a column outside the grid applies E = 0 (its
     // corner values are 0 as well), so no per-lane merge of the results

@@ -106,6 +106,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
   matvec_tile<T, CUBE, NC, MASK_IN>(a, xcd_logical_block(blockIdx.x, gridDim.x), blockIdx.x);
 }
 
+// The operator with the element contraction on the matrix cores: the dense
+// 24x24 KE applied by v_mfma_f32_32x32x2_f32 (matvec.hpp, element_mfma).
+// Kept for the measurement that compares it with the modal form.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_matvec_mfma(
+    MatvecArgs<float> a) {
+  matvec_tile<float, true, 3, true, 1, true>(a, xcd_logical_block(blockIdx.x, gridDim.x),
+                                             blockIdx.x);
+}
+
 // tileflags[tile * sz + kp] = 1 when node plane kp of the tile's 17x17
 // staging window holds a node with a fixed DOF.  Tile bxi stages nodes
 // x in [15 bxi - 1, 15 bxi + 15] (same in y).  Racing stores write the same
@@ -205,6 +214,7 @@ int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalCon
   a.done = done;
   a.zero_vec = zero_vec;
   a.zero_n = zero_n;
+  a.ke32 = nullptr;
   const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
   if (h->ncomp == 1)   // thermal: the modal weights already carry h
     hipLaunchKernelGGL((k_matvec<T, true, 1>), grid, dim3(256), 0, s, a);
@@ -421,7 +431,8 @@ int pyto_destroy(pyto_handle* h) {
                   h->cmu, h->cres, h->r64, h->p64, h->q64, h->r32, h->p32, h->q32, h->x32,
                   h->b32, h->partials, h->counters, h->scal, h->flags, h->elem_buf,
                   h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
-                  h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32, h->stencil};
+                  h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32, h->stencil,
+                  h->ke32};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& gc : h->graphs)
@@ -510,6 +521,41 @@ int pyto_matvec_f32(pyto_handle* h, const float* u, float* y, void* stream) {
   PYTO_REQUIRE(h->density_set, PYTO_ESTATE, "pyto_set_density must be called first");
   return launch_matvec<float>(h, u, y, h->Ee32, h->mc32, nullptr, nullptr, nullptr, 0,
                               as_stream(stream));
+}
+
+int pyto_matvec_f32_mfma(pyto_handle* h, const float* u, float* y, void* stream) {
+  PYTO_REQUIRE(h && u && y, PYTO_EINVAL, "null argument");
+  PYTO_REQUIRE(u != y, PYTO_EINVAL, "u and y must not alias");
+  PYTO_REQUIRE(h->ncomp == 3, PYTO_EINVAL, "the MFMA operator variant is structural");
+  PYTO_REQUIRE(h->density_set, PYTO_ESTATE, "pyto_set_density must be called first");
+  hipStream_t s = as_stream(stream);
+  if (!h->ke32) {
+    double ke[576];
+    host_ke(h, ke);
+    float kf[576];
+    for (int i = 0; i < 576; ++i) kf[i] = (float)ke[i];
+    PYTO_TRY(device_alloc(h, (void**)&h->ke32, sizeof(kf)));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->ke32, kf, sizeof(kf), hipMemcpyHostToDevice, s));
+    PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  }
+  MatvecArgs<float> a;
+  a.u = u;
+  a.y = y;
+  a.Ee = h->Ee32;
+  a.mask = h->fixmask;
+  a.tileflags = h->tileflags;
+  a.g = h->g;
+  a.mc = h->mc32;
+  dim3 grid;
+  matvec_launch_dims<float>(h, grid, a.zchunk, a.tiles_x, a.ntiles);
+  a.partials = nullptr;
+  a.done = nullptr;
+  a.zero_vec = nullptr;
+  a.zero_n = 0;
+  a.ke32 = h->ke32;
+  hipLaunchKernelGGL(k_matvec_mfma, grid, dim3(256), 0, s, a);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
 }
 
 }  // extern "C"

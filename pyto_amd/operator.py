@@ -153,6 +153,16 @@ class GridOperator:
 
     __matmul__ = matvec
 
+    def matvec_mfma(self, u, out=None):
+        """FP32 y = K_ff u with the element contraction on the matrix cores
+        (the dense KE, pyto_matvec_f32_mfma): a measurement variant of
+        matvec for float32 vectors."""
+        out = torch.empty_like(u) if out is None else out
+        check(_lib.lib().pyto_matvec_f32_mfma(self.handle, ptr(u, F32, self.num_dofs, "u"),
+                                              ptr(out, F32, self.num_dofs, "out"), stream_ptr()),
+              "pyto_matvec_f32_mfma")
+        return out
+
     def lift(self, g, out=None):
         """K_fc g: the operator with g's fixed entries kept and the output's
         fixed entries zero (FP64); f - lift(g) eliminates the Dirichlet

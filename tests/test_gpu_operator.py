@@ -33,6 +33,21 @@ def test_matvec_f32_close(gpu, dims):
     assert rel_err(y, c.Kff_full(u.astype(np.float32).astype(np.float64))) <= 2e-6
 
 
+@pytest.mark.parametrize("dims,h", [((3, 2, 2), (1, 1, 1)), ((31, 17, 5), (1, 1, 1)),
+                                    ((60, 20, 10), (1, 1, 1)), ((9, 7, 6), (0.5, 1.25, 0.8))])
+def test_matvec_f32_mfma_close(gpu, dims, h):
+    """The dense-KE matrix-core variant of the FP32 operator against the
+    assembled CSR K (FP32 tolerance) and against the modal FP32 operator."""
+    print("matvec f32 mfma", dims, h, flush=True)
+    c = Case(dims, seed=2, h=h)
+    op = c.operator(gpu, groups=False)
+    u = np.random.default_rng(8).normal(size=3 * c.mesh.num_nodes)
+    u32 = torch.from_numpy(u).to(gpu).float()
+    y = op.matvec_mfma(u32).double().cpu().numpy()
+    assert rel_err(y, c.Kff_full(u.astype(np.float32).astype(np.float64))) <= 2e-6
+    assert rel_err(y, op.matvec(u32).double().cpu().numpy()) <= 2e-6
+
+
 def test_anisotropic_brick_and_material(gpu):
     print("matvec anisotropic", flush=True)
     c = Case((9, 7, 6), seed=3, h=(0.5, 1.25, 0.8), E0=7.0, nu=0.41)
