@@ -130,6 +130,16 @@ int pyto_set_density_batch(int n, pyto_handle* const* handles, const double* con
 int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
                     const double* centers, void* stream);
 
+/* Operator tile shape: 16 x rows element columns per workgroup (rows = 8,
+ * 16 or 32; 16 rows by default), re-labels the tiles' fixed-node planes.
+ * Results are the same for every shape (each node is summed by one tile in
+ * a fixed order); the dot-product partials change their grouping.  No
+ * reference counterpart: a launch-shape setting of this operator, chosen by
+ * measurement (GridOperator.autotune_tile_rows).  A batch sets its
+ * systems' handles to its own rows (pyto_batch_set_tile_rows). */
+int pyto_set_tile_rows(pyto_handle* h, int rows, void* stream);
+int pyto_get_tile_rows(const pyto_handle* h);
+
 /* y = K_ff u (fixed DOFs of u ignored, of y zeroed): the SIMP-scaled
  * matrix-free product replacing K@p with the assembled CSR K
  * (src/deflation.py:540,558,573; src/hex_structural_fea.py:101). */
@@ -282,13 +292,19 @@ int pyto_oc_update_noalloc(const double* x, const double* dc, int64_t n, double 
  * All handles must be structural (pyto_create; a thermal handle is
  * refused with PYTO_EINVAL) and use cube elements, or all bricks.  New densities keep a
  * batch valid; new deflation groups on a handle need a new batch
- * (pyto_batch_solve returns PYTO_ESTATE). */
+ * (pyto_batch_solve returns PYTO_ESTATE).  The batch sets its handles'
+ * operator tile rows to its own (16 at creation). */
 typedef struct pyto_batch pyto_batch;
 int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, void* stream);
 int pyto_batch_destroy(pyto_batch* b);
 /* Node planes per workgroup of the batched operator (the tile depth; the
  * autotuner times candidates with pyto_batch_time_matvec). */
 int pyto_batch_set_zchunk(pyto_batch* b, int zchunk, void* stream);
+/* Tile rows of the batched operator (8, 16 or 32; 16 at creation).  Sets
+ * every system's handle to the same rows (pyto_set_tile_rows).  No reference
+ * counterpart: the batch autotuner's second launch-shape axis. */
+int pyto_batch_set_tile_rows(pyto_batch* b, int rows, void* stream);
+int pyto_batch_get_tile_rows(const pyto_batch* b);
 int pyto_batch_info(pyto_batch* b, int* zchunk, int* matvec_blocks, int* vector_blocks);
 /* Average duration (ms) of `reps` launches of the batched FP32 operator. */
 int pyto_batch_time_matvec(pyto_batch* b, int reps, float* ms_per_launch, void* stream);

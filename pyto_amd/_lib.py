@@ -83,6 +83,8 @@ _SIGS = {
     "pyto_matvec_f64": ([_vp, _vp, _vp, _vp], _i),
     "pyto_matvec_f32": ([_vp, _vp, _vp, _vp], _i),
     "pyto_matvec_f32_mfma": ([_vp, _vp, _vp, _vp], _i),
+    "pyto_set_tile_rows": ([_vp, _i, _vp], _i),
+    "pyto_get_tile_rows": ([_vp], _i),
     "pyto_diagonal_f64": ([_vp, _vp, _vp], _i),
     "pyto_restrict_f64": ([_vp, _vp, _vp, _vp], _i),
     "pyto_prolong_f64": ([_vp, _vp, _vp, _vp], _i),
@@ -100,6 +102,8 @@ _SIGS = {
     "pyto_batch_create": ([ctypes.POINTER(_vp), _i, _vp, _vp], _i),
     "pyto_batch_destroy": ([_vp], _i),
     "pyto_batch_set_zchunk": ([_vp, _i, _vp], _i),
+    "pyto_batch_set_tile_rows": ([_vp, _i, _vp], _i),
+    "pyto_batch_get_tile_rows": ([_vp], _i),
     "pyto_batch_info": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)], _i),
     "pyto_batch_time_matvec": ([_vp, _i, ctypes.POINTER(ctypes.c_float), _vp], _i),
     "pyto_batch_solve": ([_vp, _vp, _vp, _d, _d, _i, _i, _vp, _vp], _i),

@@ -8,8 +8,9 @@ the whole batch, captured as a hipGraph), each with its own scalars and
 stopping test.  Per system: PCG in FP32 inside FP64 iterative refinement,
 deflated with the operator's coarse space when its handle has deflation
 groups (attach them, with ``set_coarse_shift(0, 0)``, before
-``set_density``), Jacobi PCG otherwise.  ``autotune()`` picks the operator's tile depth (node planes per
-workgroup) for this batch by timing the batched FP32 operator.
+``set_density``), Jacobi PCG otherwise.  ``autotune()`` picks the operator's tile shape (element
+rows of the 16-column tiles, node planes per workgroup) for this batch by
+timing the batched FP32 operator.
 """
 import ctypes
 
@@ -86,17 +87,34 @@ class BatchSolver:
         check(_lib.lib().pyto_batch_time_matvec(self._b, int(reps), ctypes.byref(ms), stream_ptr()))
         return ms.value
 
-    def autotune(self, candidates=(4, 6, 8, 12, 16, 24, 32, 48, 62), reps=10):
-        """Time the batched FP32 operator for each tile depth; keep the fastest.
-        Densities must be set (the operator reads E)."""
+    @property
+    def tile_rows(self):
+        return int(_lib.lib().pyto_batch_get_tile_rows(self._b))
+
+    def set_tile_rows(self, rows):
+        """Operator tile rows (8, 16 or 32) of the batch and of its designs'
+        operators."""
+        check(_lib.lib().pyto_batch_set_tile_rows(self._b, int(rows), stream_ptr()),
+              "pyto_batch_set_tile_rows")
+
+    def autotune(self, candidates=(4, 6, 8, 12, 16, 24, 32, 48, 62), rows=(8, 16, 32),
+                 reps=10):
+        """Time the batched FP32 operator for each tile shape (element rows of
+        the 16-column tiles x node planes per workgroup) and keep the fastest.
+        Densities must be set (the operator reads E).  The operator's result
+        does not depend on the shape."""
         times = {}
-        for z in candidates:
-            self.set_zchunk(z)
-            self.time_matvec(2)
-            times[z] = self.time_matvec(reps)
-        best = min(times, key=times.get)
-        self.set_zchunk(best)
-        self.tuning = {"zchunk": best, "ms_per_matvec": times}
+        for r in rows:
+            self.set_tile_rows(r)
+            times[int(r)] = {}
+            for z in candidates:
+                self.set_zchunk(z)
+                self.time_matvec(2)
+                times[int(r)][int(z)] = self.time_matvec(reps)
+        best = min(((r, z) for r in times for z in times[r]), key=lambda rz: times[rz[0]][rz[1]])
+        self.set_tile_rows(best[0])
+        self.set_zchunk(best[1])
+        self.tuning = {"rows": best[0], "zchunk": best[1], "ms_per_matvec": times}
         return self.tuning
 
     def solve(self, rtol=1e-8, inner_rtol=1e-2, max_iters=200000, max_outer=60, out=None):

@@ -93,8 +93,8 @@ __device__ __forceinline__ int find_sys(const int* __restrict__ pre, int n, int 
 }
 
 // ---- operator --------------------------------------------------------------
-template <bool CUBE>
-__global__ __launch_bounds__(256) void k_bmatvec_inner(const BatchSys* __restrict__ sys,
+template <bool CUBE, int TY>
+__global__ __launch_bounds__(kTile * TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_bmatvec_inner(const BatchSys* __restrict__ sys,
                                                        const int* __restrict__ pre, int nsys,
                                                        const int* __restrict__ flags,
                                                        double* __restrict__ partials) {
@@ -116,13 +116,13 @@ __global__ __launch_bounds__(256) void k_bmatvec_inner(const BatchSys* __restric
   a.done = flags + s * kNumFlags + F_DONE;
   a.zero_vec = nullptr;
   a.zero_n = 0;
-  matvec_tile<float, CUBE>(a, gb - pre[s], gb);
+  matvec_tile<float, CUBE, 3, true, 1, false, TY>(a, gb - pre[s], gb);
 }
 
 // q = K x (FP32) for the systems still refining: the start of a deflated
 // inner run (x = x0).
-template <bool CUBE>
-__global__ __launch_bounds__(256) void k_bmatvec_start(const BatchSys* __restrict__ sys,
+template <bool CUBE, int TY>
+__global__ __launch_bounds__(kTile * TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_bmatvec_start(const BatchSys* __restrict__ sys,
                                                        const int* __restrict__ pre, int nsys,
                                                        const int* __restrict__ flags) {
   const int gb = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -143,11 +143,11 @@ __global__ __launch_bounds__(256) void k_bmatvec_start(const BatchSys* __restric
   a.done = flags + s * kNumFlags + F_ODONE;
   a.zero_vec = nullptr;
   a.zero_n = 0;
-  matvec_tile<float, CUBE>(a, gb - pre[s], gb);
+  matvec_tile<float, CUBE, 3, true, 1, false, TY>(a, gb - pre[s], gb);
 }
 
-template <bool CUBE>
-__global__ __launch_bounds__(256) void k_bmatvec_outer(const BatchSys* __restrict__ sys,
+template <bool CUBE, int TY>
+__global__ __launch_bounds__(kTile * TY) __attribute__((amdgpu_waves_per_eu(CUBE ? 3 : 2, 8))) void k_bmatvec_outer(const BatchSys* __restrict__ sys,
                                                        const int* __restrict__ pre, int nsys,
                                                        const int* __restrict__ flags) {
   const int gb = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void k_bmatvec_outer(const BatchSys* __restric
   a.done = flags + s * kNumFlags + F_ODONE;
   a.zero_vec = nullptr;
   a.zero_n = 0;
-  matvec_tile<double, CUBE>(a, gb - pre[s], gb);
+  matvec_tile<double, CUBE, 3, true, 1, false, TY>(a, gb - pre[s], gb);
 }
 
 // ---- vector kernels: a block works on a contiguous share of one system ----
@@ -857,6 +857,7 @@ struct Batch {
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   hipGraphExec_t graph = nullptr;
   int zchunk = 16;
+  int rows = 16;                   // operator tile rows of every system
   // deflation: prefix sums over the deflated systems' special blocks, groups
   // and coarse rows (host and device)
   bool defl = false;
@@ -890,7 +891,15 @@ static int upload_prefix(const std::vector<int>& pre, int** dev, hipStream_t s) 
   return PYTO_OK;
 }
 
-static int batch_geometry(Batch* b, int zc, hipStream_t s) {
+// Tile rows (the systems' handles are set to them) and z-chunk of the
+// batched operator; rebuilds the block tables.
+static int batch_geometry(Batch* b, int zc, int rows, hipStream_t s) {
+  for (int i = 0; i < b->nsys; ++i) {
+    PYTO_TRY(set_tile_rows(b->handles[i], rows, s));
+    BatchSys& S = b->hs[i];
+    S.ntiles = (int)tiles_for(S.g, rows, &S.tiles_x);
+  }
+  b->rows = rows;
   b->h_mpre.assign(b->nsys + 1, 0);
   b->h_vpre.assign(b->nsys + 1, 0);
   for (int i = 0; i < b->nsys; ++i) {
@@ -922,14 +931,30 @@ static int batch_geometry(Batch* b, int zc, hipStream_t s) {
   return PYTO_OK;
 }
 
+// f(cube, rows) with both as compile-time constants for the batch's shape.
+template <typename F>
+static void with_shape(const Batch* b, F&& f) {
+  using std::integral_constant;
+  auto rows = [&](auto cube) {
+    switch (b->rows) {
+      case 8: f(cube, integral_constant<int, 8>{}); break;
+      case 32: f(cube, integral_constant<int, 32>{}); break;
+      default: f(cube, integral_constant<int, 16>{}); break;
+    }
+  };
+  if (b->cube)
+    rows(integral_constant<bool, true>{});
+  else
+    rows(integral_constant<bool, false>{});
+}
+
 static void launch_bmatvec_inner(Batch* b, hipStream_t s) {
   const int nm = b->h_mpre[b->nsys];
-  if (b->cube)
-    hipLaunchKernelGGL(k_bmatvec_inner<true>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
-                       b->nsys, (const int*)b->flags, b->part_mv);
-  else
-    hipLaunchKernelGGL(k_bmatvec_inner<false>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
-                       b->nsys, (const int*)b->flags, b->part_mv);
+  with_shape(b, [&](auto cube, auto ty) {
+    hipLaunchKernelGGL((k_bmatvec_inner<decltype(cube)::value, decltype(ty)::value>), dim3(nm),
+                       dim3(kTile * decltype(ty)::value), 0, s, b->sys, b->mpre, b->nsys,
+                       (const int*)b->flags, b->part_mv);
+  });
 }
 
 // mu = E^-1 W^T K (M r + beta p) for every deflated system still iterating:
@@ -1053,8 +1078,6 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
     S.mask = h->fixmask;
     S.tileflags = h->tileflags;
     S.ndof = 3 * h->g.nnodes;
-    S.tiles_x = (h->g.sx + kOwn - 1) / kOwn;
-    S.ntiles = S.tiles_x * ((h->g.sy + kOwn - 1) / kOwn);
     const size_t n32 = (size_t)S.ndof * 4, n64 = (size_t)S.ndof * 8;
     float* f32[5];
     for (int k = 0; k < 5; ++k) {
@@ -1184,7 +1207,7 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
   }
   hipMemsetAsync(b->scal, 0, (size_t)nsys * kNumScalars * sizeof(double), s);
   hipMemsetAsync(b->flags, 0, (size_t)nsys * kNumFlags * sizeof(int), s);
-  int rc = batch_geometry(b, 16, s);
+  int rc = batch_geometry(b, 16, kTile, s);
   if (rc) return fail(rc);
   int64_t nvtot = b->h_vpre[nsys];
   if (hipMalloc((void**)&b->part_v, (size_t)nvtot * sizeof(double)) != hipSuccess) {
@@ -1220,8 +1243,17 @@ int pyto_batch_destroy(pyto_batch* b) {
 
 int pyto_batch_set_zchunk(pyto_batch* b, int zchunk, void* stream) {
   PYTO_REQUIRE(b && zchunk >= 1, PYTO_EINVAL, "bad arguments");
-  return batch_geometry(b, zchunk, as_stream(stream));
+  return batch_geometry(b, zchunk, b->rows, as_stream(stream));
 }
+
+int pyto_batch_set_tile_rows(pyto_batch* b, int rows, void* stream) {
+  PYTO_REQUIRE(b, PYTO_EINVAL, "null batch");
+  PYTO_REQUIRE(rows == 8 || rows == 16 || rows == 32, PYTO_EINVAL,
+               "operator tile rows must be 8, 16 or 32 (got %d)", rows);
+  return batch_geometry(b, b->zchunk, rows, as_stream(stream));
+}
+
+int pyto_batch_get_tile_rows(const pyto_batch* b) { return b ? b->rows : PYTO_EINVAL; }
 
 int pyto_batch_info(pyto_batch* b, int* zchunk, int* matvec_blocks, int* vector_blocks) {
   PYTO_REQUIRE(b, PYTO_EINVAL, "null batch");
@@ -1283,12 +1315,11 @@ int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, do
   for (int outer = 0; outer < max_outer; ++outer) {
     const bool first = outer == 0;
     if (!first) {
-      if (b->cube)
-        hipLaunchKernelGGL(k_bmatvec_outer<true>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
+      with_shape(b, [&](auto cube, auto ty) {
+        hipLaunchKernelGGL((k_bmatvec_outer<decltype(cube)::value, decltype(ty)::value>),
+                           dim3(nm), dim3(kTile * decltype(ty)::value), 0, s, b->sys, b->mpre,
                            nsys, (const int*)b->flags);
-      else
-        hipLaunchKernelGGL(k_bmatvec_outer<false>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
-                           nsys, (const int*)b->flags);
+      });
     }
     hipLaunchKernelGGL(k_bresid64, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
                        (const int*)b->flags, first, b->part_v);
@@ -1305,12 +1336,11 @@ int pyto_batch_solve(pyto_batch* b, const double* const* f, double* const* u, do
       // deflated start: x0 = W E^-1 W^T b per deflated system (x0 = 0 for the
       // others), r = b - K x0, then the projected first direction
       launch_bstart_x(b, s);
-      if (b->cube)
-        hipLaunchKernelGGL(k_bmatvec_start<true>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
+      with_shape(b, [&](auto cube, auto ty) {
+        hipLaunchKernelGGL((k_bmatvec_start<decltype(cube)::value, decltype(ty)::value>),
+                           dim3(nm), dim3(kTile * decltype(ty)::value), 0, s, b->sys, b->mpre,
                            nsys, (const int*)b->flags);
-      else
-        hipLaunchKernelGGL(k_bmatvec_start<false>, dim3(nm), dim3(256), 0, s, b->sys, b->mpre,
-                           nsys, (const int*)b->flags);
+      });
       hipLaunchKernelGGL(k_binit_q, dim3(nv), dim3(256), 0, s, b->sys, b->vpre, nsys,
                          (const int*)b->flags, b->part_v);
       hipLaunchKernelGGL(k_bsetup, dim3(nsys), dim3(256), 0, s, b->vpre, nsys, b->scal,

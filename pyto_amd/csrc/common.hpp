@@ -122,10 +122,12 @@ struct Handle {
   ModalConsts<float> mc32{};
   double kediag[3] = {0, 0, 0};
   int ncomp = 3;                // values per node: 3 (structural) or 1 (thermal)
+  int tile_rows = 16;           // operator tile: 16 x tile_rows element columns (8, 16 or 32)
 
   // constants / per-density state
   uint8_t* fixmask = nullptr;   // per node
-  uint8_t* tileflags = nullptr; // per (matvec tile, node plane): any fixed node in its window
+  uint8_t* tileflags = nullptr; // per (matvec tile, node plane): any fixed node in its window,
+                                // sized for the smallest tile (8 rows)
   double* Ee64 = nullptr;       // per element E_e/E0 (SIMP)
   float* Ee32 = nullptr;
   double* x64 = nullptr;        // copy of densities
@@ -243,6 +245,8 @@ enum CounterSlot { C_MATVEC = 0, C_UPDATE = 1, C_DOT = 2, C_ELEM = 3, C_NORM = 4
 hipStream_t as_stream(void* s);
 int ensure_solver_workspace(Handle* h);
 int device_alloc(Handle* h, void** p, size_t bytes);
+// Operator tile rows (8, 16 or 32) and the tiles' fixed-node flags for them.
+int set_tile_rows(Handle* h, int rows, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Device helpers

@@ -14,6 +14,16 @@ constexpr int kTile = 16;
 constexpr int kOwn = 15;
 constexpr int kPl = 17;
 
+// Node rows a tile of `rows` element rows owns (x: always kOwn).
+constexpr int tile_own_y(int rows) { return rows - 1; }
+
+// Tiles of the operator for tile rows `rows` (and the x count).
+inline int64_t tiles_for(const GridDims& g, int rows, int* tiles_x = nullptr) {
+  const int tx = (g.sx + kOwn - 1) / kOwn, oy = tile_own_y(rows);
+  if (tiles_x) *tiles_x = tx;
+  return (int64_t)tx * ((g.sy + oy - 1) / oy);
+}
+
 template <typename T>
 struct MatvecArgs {
   const T* u;
@@ -147,20 +157,25 @@ struct PlaneStage {
 
 // PF: node planes in flight ahead of the one being stored.  One suffices: a
 // second plane in flight (11 more VGPRs in FP32) measured no faster.
-template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true, int PF = 1, bool MFMA = false>
+// TY: element rows of the tile (16 x TY element columns, 16 TY threads; it
+// owns 15 x (TY - 1) nodes per plane).
+template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true, int PF = 1, bool MFMA = false,
+          int TY = kTile>
 __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int pidx) {
   static_assert(!MFMA || (sizeof(T) == 4 && NC == 3), "the MFMA variant is FP32 structural");
+  static_assert(TY >= 4 && TY <= 64 && TY % 4 == 0, "tile rows: whole waves");
+  constexpr int kThreads = kTile * TY;
   if (a.done && *a.done) return;
-  __shared__ T pl[2][kPl][kPl][NC];
-  __shared__ T xc[2][kTile][kTile][NC];   // odd stride (NC = 3 or 1): no bank conflicts between rows
-  __shared__ T es[2][kTile][kTile];   // E of the element layer below plane b
-  __shared__ uint8_t pm[2][kPl][kPl];  // fixed bits of the staged nodes
-  __shared__ double red[4];
+  __shared__ T pl[2][TY + 1][kPl][NC];
+  __shared__ T xc[2][TY][kTile][NC];   // odd stride (NC = 3 or 1): no bank conflicts between rows
+  __shared__ T es[2][TY][kTile];   // E of the element layer below plane b
+  __shared__ uint8_t pm[2][TY + 1][kPl];  // fixed bits of the staged nodes
+  __shared__ double red[kThreads / 64];
   const GridDims& g = a.g;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const int tile = lb % a.ntiles;
   const int bxi = tile % a.tiles_x, byi = tile / a.tiles_x;
-  const int x0 = bxi * kOwn - 1, y0 = byi * kOwn - 1;
+  const int x0 = bxi * kOwn - 1, y0 = byi * (TY - 1) - 1;
   const int ex = x0 + tx, ey = y0 + ty;
   const int kz0 = (lb / a.ntiles) * a.zchunk;
   const int kz1 = min(kz0 + a.zchunk, g.sz);
@@ -178,11 +193,11 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
   const bool owner = tx >= 1 && ty >= 1 && ex < g.sx && ey < g.sy;
 
   if (a.zero_vec && pidx == 0)
-    for (int i = threadIdx.x; i < a.zero_n; i += 256) a.zero_vec[i] = 0.0;
+    for (int i = threadIdx.x; i < a.zero_n; i += kThreads) a.zero_vec[i] = 0.0;
 
-  // staging-plane entries of this thread: e0 = tid, e1 = tid + 256 (< 289)
-  const int e0 = threadIdx.x, e1 = threadIdx.x + 256;
-  const bool has1 = e1 < kPl * kPl;
+  // staging-plane entries of this thread: e0 = tid, e1 = tid + 16 TY (< 17 (TY + 1))
+  const int e0 = threadIdx.x, e1 = threadIdx.x + kThreads;
+  const bool has1 = e1 < kPl * (TY + 1);
   const int lx0 = e0 % kPl, ly0 = e0 / kPl;
   const int lx1 = has1 ? e1 % kPl : 0, ly1 = has1 ? e1 / kPl : 0;
   PlaneStage<T, NC> st[PF];
@@ -451,8 +466,16 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
   }
   if (a.partials) {
     // block partial of u.y; the consumer kernel sums the partials
-    const double bs = block_reduce_sum256((double)dot, red);
-    if (threadIdx.x == 0) a.partials[pidx] = bs;
+    // block partial: wave sums, then wave 0 adds them in a fixed order
+    const double ws = wave_reduce_sum((double)dot);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double bs = 0.0;
+#pragma unroll
+      for (int w = 0; w < kThreads / 64; ++w) bs += red[w];
+      a.partials[pidx] = bs;
+    }
   }
 }
 

@@ -98,62 +98,96 @@ __global__ void k_inv_diag(const double* __restrict__ E64, const uint8_t* __rest
   }
 }
 
-// At least 4 waves per SIMD for FP32 and 3 for FP64 (128 / 168 VGPRs).
-template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true,
-          int WPE = sizeof(T) == 4 ? 4 : 3>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_matvec(
+// At least 4 waves per SIMD for FP32 and 3 for FP64 (128 / 168 VGPRs); 2
+// for the FP64 brick element, which needs more than 168 VGPRs.
+// TY: element rows of the tile, 16 TY threads per workgroup.
+template <typename T, bool CUBE, int NC = 3, bool MASK_IN = true, int TY = kTile,
+          int WPE = sizeof(T) == 4 ? 4 : (CUBE || NC == 1 ? 3 : 2)>
+__global__ __launch_bounds__(kTile * TY) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_matvec(
     MatvecArgs<T> a) {
-  matvec_tile<T, CUBE, NC, MASK_IN>(a, xcd_logical_block(blockIdx.x, gridDim.x), blockIdx.x);
+  matvec_tile<T, CUBE, NC, MASK_IN, 1, false, TY>(a, xcd_logical_block(blockIdx.x, gridDim.x),
+                                                  blockIdx.x);
 }
 
 // The operator with the element contraction on the matrix cores: the dense
 // 24x24 KE applied by v_mfma_f32_32x32x2_f32 (matvec.hpp, element_mfma).
 // Kept for the measurement that compares it with the modal form.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_matvec_mfma(
+template <int TY>
+__global__ __launch_bounds__(kTile * TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_matvec_mfma(
     MatvecArgs<float> a) {
-  matvec_tile<float, true, 3, true, 1, true>(a, xcd_logical_block(blockIdx.x, gridDim.x),
-                                             blockIdx.x);
+  matvec_tile<float, true, 3, true, 1, true, TY>(a, xcd_logical_block(blockIdx.x, gridDim.x),
+                                                 blockIdx.x);
 }
 
-// tileflags[tile * sz + kp] = 1 when node plane kp of the tile's 17x17
-// staging window holds a node with a fixed DOF.  Tile bxi stages nodes
-// x in [15 bxi - 1, 15 bxi + 15] (same in y).  Racing stores write the same
-// byte value.
+// tileflags[tile * sz + kp] = 1 when node plane kp of the tile's staging
+// window holds a node with a fixed DOF.  Tile (bxi, byi) stages nodes
+// x in [15 bxi - 1, 15 bxi + 15] and y in [oy byi - 1, oy byi + oy] with
+// oy = rows - 1.  Racing stores write the same byte value.
 __global__ __launch_bounds__(256) void k_tile_flags(const uint8_t* __restrict__ mask, GridDims g,
-                                                    int tiles_x, int tiles_y,
+                                                    int tiles_x, int tiles_y, int oy,
                                                     uint8_t* __restrict__ flags) {
   const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (n >= g.nnodes || mask[n] == 0) return;
   const int gx = (int)(n % g.sx), gy = (int)((n / g.sx) % g.sy);
   const int kp = (int)(n / ((int64_t)g.sx * g.sy));
   const int bx0 = max(0, (gx + kOwn - 1) / kOwn - 1), bx1 = min(tiles_x - 1, (gx + 1) / kOwn);
-  const int by0 = max(0, (gy + kOwn - 1) / kOwn - 1), by1 = min(tiles_y - 1, (gy + 1) / kOwn);
+  const int by0 = max(0, (gy + oy - 1) / oy - 1), by1 = min(tiles_y - 1, (gy + 1) / oy);
   for (int by = by0; by <= by1; ++by)
     for (int bx = bx0; bx <= bx1; ++bx) flags[((int64_t)by * tiles_x + bx) * g.sz + kp] = 1;
 }
 
+// Fills tileflags for the handle's tile rows (the buffer holds the flags of
+// the smallest tile, the largest tile count).
 static hipError_t launch_tile_flags(Handle* h, hipStream_t s) {
-  const int tx = (h->g.sx + kOwn - 1) / kOwn, ty = (h->g.sy + kOwn - 1) / kOwn;
+  int tx = 0;
+  const int64_t nt = tiles_for(h->g, h->tile_rows, &tx);
+  const int oy = tile_own_y(h->tile_rows);
+  hipError_t e = hipMemsetAsync(h->tileflags, 0, nt * h->g.sz, s);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_tile_flags, dim3((unsigned)((h->g.nnodes + 255) / 256)), dim3(256), 0, s,
-                     h->fixmask, h->g, tx, ty, h->tileflags);
+                     h->fixmask, h->g, tx, (int)(nt / tx), oy, h->tileflags);
   return hipGetLastError();
 }
 
-// Resident workgroups of k_matvec<T, CUBE, NC> on the whole device.
-template <typename T, bool CUBE, int NC>
+int set_tile_rows(Handle* h, int rows, hipStream_t s) {
+  PYTO_REQUIRE(rows == 8 || rows == 16 || rows == 32, PYTO_EINVAL,
+               "operator tile rows must be 8, 16 or 32 (got %d)", rows);
+  if (rows == h->tile_rows) return PYTO_OK;
+  // captured iteration graphs hold the old launch shape
+  for (auto& gc : h->graphs)
+    if (gc.exec) {
+      hipGraphExecDestroy(gc.exec);
+      gc.exec = nullptr;
+    }
+  h->tile_rows = rows;
+  PYTO_CHECK_HIP(launch_tile_flags(h, s));
+  PYTO_CHECK_HIP(hipStreamSynchronize(s));
+  return PYTO_OK;
+}
+
+// Resident workgroups of a k_matvec variant on the whole device.
+template <typename T, bool CUBE, int NC, bool MASK_IN, int TY>
 static int matvec_capacity() {
   static int cap = -1;
   if (cap < 0) {
     int nb = 0, dev = 0;
     hipGetDevice(&dev);
     hipDeviceProp_t prop;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_matvec<T, CUBE, NC>, 256, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_matvec<T, CUBE, NC, MASK_IN, TY>,
+                                                     kTile * TY, 0) != hipSuccess ||
         hipGetDeviceProperties(&prop, dev) != hipSuccess || nb <= 0)
-      return 1024;
+      return 1024 * 16 / TY;
     cap = nb * prop.multiProcessorCount;
   }
   return cap;
+}
+
+template <typename T, int TY>
+static int capacity_rows(const Handle* h) {
+  const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
+  return h->ncomp == 1 ? matvec_capacity<T, true, 1, true, TY>()
+         : cube        ? matvec_capacity<T, true, 3, true, TY>()
+                       : matvec_capacity<T, false, 3, true, TY>();
 }
 
 // z-chunk length: minimise (workgroup rounds) x (layers per workgroup,
@@ -162,13 +196,10 @@ template <typename T>
 static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& tiles_x,
                                int& ntiles) {
   const GridDims& g = h->g;
-  tiles_x = (g.sx + kOwn - 1) / kOwn;
-  const int tiles_y = (g.sy + kOwn - 1) / kOwn;
-  const int64_t tiles = (int64_t)tiles_x * tiles_y;
-  const bool cube = g.hx == g.hy && g.hy == g.hz;
-  const int cap = h->ncomp == 1 ? matvec_capacity<T, true, 1>()
-                  : cube        ? matvec_capacity<T, true, 3>()
-                                : matvec_capacity<T, false, 3>();
+  const int64_t tiles = tiles_for(g, h->tile_rows, &tiles_x);
+  const int cap = h->tile_rows == 8    ? capacity_rows<T, 8>(h)
+                  : h->tile_rows == 32 ? capacity_rows<T, 32>(h)
+                                       : capacity_rows<T, 16>(h);
   int best = 16;
   double best_cost = 1e300;
   for (int zc = 4; zc <= 62; ++zc) {   // planes kz0-1 .. kz1 fit one 64-bit mask
@@ -184,6 +215,28 @@ static void matvec_launch_dims(const Handle* h, dim3& grid, int& zchunk, int& ti
   zchunk = best;
   ntiles = (int)tiles;
   grid = dim3((unsigned)(tiles * ((g.sz + zchunk - 1) / zchunk)), 1, 1);
+}
+
+// k_matvec for the handle's element kind and tile rows.
+template <typename T, bool MASK_IN, int TY>
+static void launch_rows(const Handle* h, dim3 grid, hipStream_t s, const MatvecArgs<T>& a) {
+  const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
+  const dim3 block(kTile * TY);
+  if (h->ncomp == 1)   // thermal: the modal weights already carry h
+    hipLaunchKernelGGL((k_matvec<T, true, 1, MASK_IN, TY>), grid, block, 0, s, a);
+  else if (cube)
+    hipLaunchKernelGGL((k_matvec<T, true, 3, MASK_IN, TY>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_matvec<T, false, 3, MASK_IN, TY>), grid, block, 0, s, a);
+}
+
+template <typename T, bool MASK_IN>
+static void launch_kernel(const Handle* h, dim3 grid, hipStream_t s, const MatvecArgs<T>& a) {
+  switch (h->tile_rows) {
+    case 8: launch_rows<T, MASK_IN, 8>(h, grid, s, a); break;
+    case 32: launch_rows<T, MASK_IN, 32>(h, grid, s, a); break;
+    default: launch_rows<T, MASK_IN, 16>(h, grid, s, a); break;
+  }
 }
 
 template <typename T>
@@ -215,13 +268,7 @@ int launch_matvec(const Handle* h, const T* u, T* y, const T* Ee, const ModalCon
   a.zero_vec = zero_vec;
   a.zero_n = zero_n;
   a.ke32 = nullptr;
-  const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
-  if (h->ncomp == 1)   // thermal: the modal weights already carry h
-    hipLaunchKernelGGL((k_matvec<T, true, 1>), grid, dim3(256), 0, s, a);
-  else if (cube)
-    hipLaunchKernelGGL((k_matvec<T, true, 3>), grid, dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_matvec<T, false, 3>), grid, dim3(256), 0, s, a);
+  launch_kernel<T, true>(h, grid, s, a);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -288,7 +335,7 @@ static int create_common(pyto_handle* h, int nelx, int nely, int nelz, double hx
   h->g.hx = hx; h->g.hy = hy; h->g.hz = hz;
   const int64_t ne = h->g.nelems;
   int rc = PYTO_OK;
-  const int64_t ntiles = (int64_t)((h->g.sx + kOwn - 1) / kOwn) * ((h->g.sy + kOwn - 1) / kOwn);
+  const int64_t ntiles = tiles_for(h->g, 8);   // the largest tile count
   if ((rc = device_alloc(h, (void**)&h->fixmask, nn)) ||
       (rc = device_alloc(h, (void**)&h->tileflags, ntiles * h->g.sz)) ||
       (rc = device_alloc(h, (void**)&h->Ee64, ne * 8)) ||
@@ -308,7 +355,6 @@ static int create_common(pyto_handle* h, int nelx, int nely, int nelz, double hx
     return PYTO_ENOMEM;
   }
   if (hipMemcpyAsync(h->fixmask, fixed_mask, nn, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-      hipMemsetAsync(h->tileflags, 0, ntiles * h->g.sz, s) != hipSuccess ||
       launch_tile_flags(h, s) != hipSuccess ||
       hipMemsetAsync(h->counters, 0, kNumCounters * sizeof(unsigned), s) != hipSuccess ||
       hipMemsetAsync(h->scal, 0, kNumScalars * 8, s) != hipSuccess ||
@@ -457,6 +503,13 @@ int pyto_info(pyto_handle* h, int64_t* nnodes, int64_t* nelems, int64_t* ndof,
   return PYTO_OK;
 }
 
+int pyto_set_tile_rows(pyto_handle* h, int rows, void* stream) {
+  PYTO_REQUIRE(h, PYTO_EINVAL, "null handle");
+  return set_tile_rows(h, rows, as_stream(stream));
+}
+
+int pyto_get_tile_rows(const pyto_handle* h) { return h ? h->tile_rows : PYTO_EINVAL; }
+
 int pyto_set_density(pyto_handle* h, const double* x, void* stream) {
   PYTO_REQUIRE(h && x, PYTO_EINVAL, "null argument");
   hipStream_t s = as_stream(stream);
@@ -503,14 +556,8 @@ int pyto_matvec_lift_f64(pyto_handle* h, const double* g, double* y, void* strea
   a.done = nullptr;
   a.zero_vec = nullptr;
   a.zero_n = 0;
-  hipStream_t s = as_stream(stream);
-  const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
-  if (h->ncomp == 1)
-    hipLaunchKernelGGL((k_matvec<double, true, 1, false>), grid, dim3(256), 0, s, a);
-  else if (cube)
-    hipLaunchKernelGGL((k_matvec<double, true, 3, false>), grid, dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_matvec<double, false, 3, false>), grid, dim3(256), 0, s, a);
+  a.ke32 = nullptr;
+  launch_kernel<double, false>(h, grid, as_stream(stream), a);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -553,7 +600,11 @@ int pyto_matvec_f32_mfma(pyto_handle* h, const float* u, float* y, void* stream)
   a.zero_vec = nullptr;
   a.zero_n = 0;
   a.ke32 = h->ke32;
-  hipLaunchKernelGGL(k_matvec_mfma, grid, dim3(256), 0, s, a);
+  switch (h->tile_rows) {
+    case 8: hipLaunchKernelGGL(k_matvec_mfma<8>, grid, dim3(kTile * 8), 0, s, a); break;
+    case 32: hipLaunchKernelGGL(k_matvec_mfma<32>, grid, dim3(kTile * 32), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_matvec_mfma<16>, grid, dim3(kTile * 16), 0, s, a); break;
+  }
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }

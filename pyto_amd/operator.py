@@ -182,6 +182,39 @@ class GridOperator:
     def shape(self):
         return (self.num_dofs, self.num_dofs)
 
+    # -- launch shape ----------------------------------------------------------
+    @property
+    def tile_rows(self):
+        """Element rows of the operator's 16-column tiles (8, 16 or 32)."""
+        return int(_lib.lib().pyto_get_tile_rows(self.handle))
+
+    def set_tile_rows(self, rows):
+        check(_lib.lib().pyto_set_tile_rows(self.handle, int(rows), stream_ptr()),
+              "pyto_set_tile_rows")
+
+    def autotune_tile_rows(self, candidates=(8, 16, 32), reps=20, precision="fp32"):
+        """Times the operator for each tile shape on this grid (HIP events on
+        the current stream, after one warm-up launch) and keeps the fastest.
+        The operator's result does not depend on the shape.  Needs
+        set_density first.  Returns {rows: microseconds per launch}."""
+        dtype = F32 if precision == "fp32" else F64
+        u = torch.ones(self.num_dofs, dtype=dtype, device=self.device)
+        y = torch.empty_like(u)
+        times = {}
+        for rows in candidates:
+            self.set_tile_rows(rows)
+            self.matvec(u, y)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                self.matvec(u, y)
+            e1.record()
+            e1.synchronize()
+            times[int(rows)] = e0.elapsed_time(e1) * 1e3 / reps
+        self.set_tile_rows(min(times, key=times.get))
+        return times
+
     # -- coarse space --------------------------------------------------------
     def restrict(self, v):
         t = torch.empty(6 * self.ngroups, dtype=F64, device=self.device)

@@ -165,16 +165,23 @@ def test_deflated_batch_brick_elements(gpu):
 
 
 def test_batch_autotune_and_resolve(gpu):
-    """The autotuner picks a tile depth among the candidates; solutions after
-    re-tuning (new graph) and a repeated solve (graph reuse) agree."""
+    """The autotuner picks a tile shape (rows x depth) among the candidates;
+    solutions after re-tuning (new graph) and a repeated solve (graph reuse)
+    agree, and every shape gives the same operator bits."""
     from pyto_amd.batch import BatchSolver
     print("batch autotune", flush=True)
     specs = [((32, 16, 16), 7, 2), ((48, 24, 16), 8, 1)]
     ops, loads = zip(*[_design(g, s, n, gpu) for g, s, n in specs])
     bs = BatchSolver(ops, loads)
     us0, _ = bs.solve(rtol=1e-9)
+    y16 = [op.matvec(u) for op, u in zip(ops, us0[::2])]
     tune = bs.autotune(candidates=(4, 8, 16, 32))
     assert tune["zchunk"] in (4, 8, 16, 32) and bs.info()["zchunk"] == tune["zchunk"]
+    assert tune["rows"] in (8, 16, 32) and bs.tile_rows == tune["rows"]
+    assert all(op.tile_rows == tune["rows"] for op in ops)
+    assert sorted(tune["ms_per_matvec"]) == [8, 16, 32]
+    for op, u, y in zip(ops, us0[::2], y16):
+        assert torch.equal(op.matvec(u), y)
     us1, st1 = bs.solve(rtol=1e-9)
     us2, st2 = bs.solve(rtol=1e-9)
     for a, b, c in zip(us0, us1, us2):

@@ -139,3 +139,61 @@ def test_bad_calls_raise(gpu):
         op.set_groups(np.full(op.num_nodes, 3, np.int32), np.zeros((2, 3)))
     with pytest.raises(ValueError):
         GridOperator((0, 2, 2), (1, 1, 1), 1.0, 0.3, [])
+
+
+@pytest.mark.parametrize("dims,h", [((31, 17, 5), (1, 1, 1)), ((16, 45, 7), (1, 1, 1)),
+                                    ((9, 23, 6), (0.5, 1.25, 0.8))])
+def test_tile_rows_same_result(gpu, dims, h):
+    """Every tile shape (16 x 8, 16 x 16, 16 x 32 element columns) gives the
+    same bits: each node is summed by one tile in a fixed order.  Grids whose
+    y extent is not a multiple of any tile's owned rows, with fixed nodes on
+    tile seams; FP64 also against the assembled CSR K."""
+    print("tile rows", dims, h, flush=True)
+    c = Case(dims, seed=9, h=h)
+    op = c.operator(gpu, groups=False)
+    assert op.tile_rows == 16
+    rng = np.random.default_rng(10)
+    u = torch.from_numpy(rng.normal(size=3 * c.mesh.num_nodes)).to(gpu)
+    ref64, ref32, reflift = op.matvec(u), op.matvec(u.float()), op.lift(u)
+    refm = op.matvec_mfma(u.float()) if h == (1, 1, 1) else None
+    for rows in (8, 32, 16):
+        op.set_tile_rows(rows)
+        assert op.tile_rows == rows
+        y64 = op.matvec(u)
+        assert torch.equal(y64, ref64)
+        assert rel_err(y64.cpu().numpy(), c.Kff_full(u.cpu().numpy())) <= 1e-13
+        assert torch.equal(op.matvec(u.float()), ref32)
+        assert torch.equal(op.lift(u), reflift)
+        if refm is not None:
+            assert torch.equal(op.matvec_mfma(u.float()), refm)
+
+
+def test_tile_rows_thermal_and_solve(gpu):
+    """Tile shapes on the thermal operator, and a deflated mixed solve after a
+    shape change (the captured iteration graphs are rebuilt)."""
+    from pyto_amd.operator import ThermalOperator
+    print("tile rows thermal + solve", flush=True)
+    dims = (20, 33, 6)
+    idx = np.arange((dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1))
+    op = ThermalOperator(dims, (1.0, 1.0, 1.0), 1.0, idx[idx % (dims[0] + 1) == 0], penal=3.0,
+                         k_void=1e-6)
+    op.set_density(np.random.default_rng(11).uniform(0.05, 1.0, op.num_elems))
+    T = torch.from_numpy(np.random.default_rng(12).normal(size=op.num_nodes)).to(gpu)
+    ref = op.matvec(T)
+    for rows in (8, 32):
+        op.set_tile_rows(rows)
+        assert torch.equal(op.matvec(T), ref)
+        assert torch.equal(op.matvec(T.float()), op.matvec(T.float()))
+
+    c = Case((24, 40, 8), seed=13)
+    s = c.operator(gpu, groups=True)
+    f = torch.from_numpy(c.f).to(gpu)
+    u16, info16 = s.solve(f, rtol=1e-8, precision="mixed")
+    for rows in (32, 8):
+        s.set_tile_rows(rows)
+        u, info = s.solve(f, rtol=1e-8, precision="mixed")
+        assert info["converged"]
+        assert abs(info["iterations"] - info16["iterations"]) <= 3
+        assert rel_err(u.cpu().numpy(), u16.cpu().numpy()) <= 1e-7
+    with pytest.raises(ValueError):
+        s.set_tile_rows(12)
