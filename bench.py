@@ -388,6 +388,9 @@ def main():
                     help="skip the solves with the reference's 2000-group coarse space")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 32-design line reported beside `value`")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="only the warm-up, --steps mixed solves and --fp64-steps FP64 solves "
+                         "of the workload (for kernel traces); prints their times")
     ap.add_argument("--operator-only", action="store_true",
                     help="only 20 launches each of the FP32 operator, the FP64 one and the "
                          "FP32 matrix-core variant on the workload (for counter collection); "
@@ -497,6 +500,16 @@ def main():
     t_mixed = (time.perf_counter() - t0) / args.steps
     u_mixed = u.clone()
     log("mixed: %.3f ms/solve" % (t_mixed * 1e3), stats[-1])
+    if args.solve_only:
+        t0 = time.perf_counter()
+        for _ in range(args.fp64_steps):
+            st64 = step("fp64")
+        torch.cuda.synchronize()
+        t_fp64 = (time.perf_counter() - t0) / max(1, args.fp64_steps)
+        print(json.dumps({"solve_only": True, "mixed_ms": t_mixed * 1e3,
+                          "fp64_ms": t_fp64 * 1e3, "mixed": stats[-1], "fp64": st64,
+                          "groups": ngroups}))
+        return
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
