@@ -255,7 +255,7 @@ def batch_line(torch, dev, inner_rtol=None, boxes=12):
     design, or a number (a box edge in elements: each design gets
     max(1, round(n / edge)) boxes per axis), or None (Jacobi PCG only)."""
     from pyto_amd import deflation, problems
-    from pyto_amd.batch import BatchSolver
+    from pyto_amd.batch import BatchSolver, measure_grid_barrier
     from pyto_amd.operator import GridOperator
     designs = problems.batch_designs()
     ops = []
@@ -306,7 +306,11 @@ def batch_line(torch, dev, inner_rtol=None, boxes=12):
            "outer_iterations_max": max(s["outer_iterations"] for s in stats),
            "deflation_boxes": (None if boxes is None else list(boxes)
                                if isinstance(boxes, (tuple, list)) else "edge %g" % boxes),
-           "zchunk": tune["zchunk"]}
+           "zchunk": tune["zchunk"], "tile_rows": tune["rows"],
+           "ms_per_lockstep_iteration": t * 1e3 / max(it),
+           # the persistent-kernel alternative: one grid-wide barrier per
+           # dependent phase of an iteration instead of a graph node
+           "grid_barrier_us": {str(b): measure_grid_barrier(b, rounds=2000) for b in (1, 2)}}
     bs.close()
     for op in ops:
         op.close()
@@ -388,6 +392,8 @@ def main():
                     help="skip the solves with the reference's 2000-group coarse space")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 32-design line reported beside `value`")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched 32-design line (for kernel traces)")
     ap.add_argument("--solve-only", action="store_true",
                     help="only the warm-up, --steps mixed solves and --fp64-steps FP64 solves "
                          "of the workload (for kernel traces); prints their times")
@@ -404,6 +410,10 @@ def main():
     import torch
     from pyto_amd import deflation, problems
     from pyto_amd.operator import GridOperator
+    if args.batch_only:
+        print(json.dumps({"batch_only": True, "batch_32_designs": batch_line(torch,
+                                                                              torch.device("cuda"))}))
+        return
 
     from pyto_amd import builders
 

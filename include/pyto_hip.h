@@ -308,6 +308,16 @@ int pyto_batch_get_tile_rows(const pyto_batch* b);
 int pyto_batch_info(pyto_batch* b, int* zchunk, int* matvec_blocks, int* vector_blocks);
 /* Average duration (ms) of `reps` launches of the batched FP32 operator. */
 int pyto_batch_time_matvec(pyto_batch* b, int reps, float* ms_per_launch, void* stream);
+/* Microseconds per grid-wide barrier of a persistent kernel (one workgroup
+ * of 256 threads on each of blocks_per_cu x CU count slots, cooperative
+ * launch, counter barrier with a release fence and an acquiring poll; the
+ * launch itself subtracted).  No reference counterpart: the measurement
+ * behind replaying captured iteration graphs instead of one persistent
+ * launch (each batched iteration has one dependent phase per kernel).
+ * PYTO_ENOCONV if a wait exceeded its poll limit (all workgroups then
+ * leave). */
+int pyto_measure_grid_barrier(int blocks_per_cu, int rounds, float* us_per_barrier,
+                              void* stream);
 /* Solve all systems: f[i], u[i] are device FP64 vectors of system i's DOFs
  * (u overwritten).  Densities must be set on the handles.  stats (nsys
  * entries, may be null) per system.  Synchronises `stream`; PYTO_ENOCONV if

@@ -102,6 +102,7 @@ _SIGS = {
     "pyto_batch_create": ([ctypes.POINTER(_vp), _i, _vp, _vp], _i),
     "pyto_batch_destroy": ([_vp], _i),
     "pyto_batch_set_zchunk": ([_vp, _i, _vp], _i),
+    "pyto_measure_grid_barrier": ([_i, _i, ctypes.POINTER(ctypes.c_float), _vp], _i),
     "pyto_batch_set_tile_rows": ([_vp, _i, _vp], _i),
     "pyto_batch_get_tile_rows": ([_vp], _i),
     "pyto_batch_info": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)], _i),

@@ -152,3 +152,16 @@ def set_density_batch(operators, xs):
           "pyto_set_density_batch")
     for op, x in zip(ops, xt):
         op.x = x
+
+
+def measure_grid_barrier(blocks_per_cu=1, rounds=2000):
+    """Microseconds per grid-wide barrier of a persistent kernel on this GPU
+    (pyto_measure_grid_barrier): one 256-thread workgroup on each of
+    blocks_per_cu x CU count slots, cooperative launch, the launch itself
+    subtracted.  A persistent form of the batched iteration would pay one
+    such barrier per dependent phase (per kernel of the captured graph)."""
+    _lib.require_gpu()
+    us = ctypes.c_float()
+    check(_lib.lib().pyto_measure_grid_barrier(int(blocks_per_cu), int(rounds), ctypes.byref(us),
+                                               stream_ptr()), "pyto_measure_grid_barrier")
+    return float(us.value)

@@ -833,6 +833,40 @@ __global__ void k_bset_flag(int* flags, int nsys, int slot, int value) {
     flags[s * kNumFlags + slot] = value;
 }
 
+// Grid-barrier probe (a measurement for the persistent-kernel alternative to
+// replaying captured iteration graphs): `rounds` barriers of all gridDim.x
+// workgroups, which a cooperative launch keeps resident together.  Barrier
+// r: each block's thread 0 publishes its writes (release fence), adds 1 to
+// the counter and waits until it reaches (r + 1) * blocks.  The wait is
+// bounded: after spin_limit polls it sets *timed_out and every block leaves.
+__global__ __launch_bounds__(256) void k_grid_barrier_probe(unsigned* __restrict__ counter,
+                                                            int rounds, unsigned spin_limit,
+                                                            int* __restrict__ timed_out) {
+  __shared__ int quit;
+  if (threadIdx.x == 0) quit = 0;
+  const unsigned nb = gridDim.x;
+  for (int r = 0; r < rounds; ++r) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(counter, 1u);
+      const unsigned target = (unsigned)(r + 1) * nb;
+      unsigned polls = 0;
+      while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++polls > spin_limit || __hip_atomic_load(timed_out, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)) {
+          atomicExch(timed_out, 1);
+          quit = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    if (quit) return;
+  }
+}
+
 // ---- host side ----------------------------------------------------------------
 struct Batch {
   int device = 0;
@@ -1281,6 +1315,57 @@ int pyto_batch_time_matvec(pyto_batch* b, int reps, float* ms_per_launch, void* 
   hipEventDestroy(e1);
   *ms_per_launch = ms / reps;
   PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
+int pyto_measure_grid_barrier(int blocks_per_cu, int rounds, float* us_per_barrier,
+                              void* stream) {
+  PYTO_REQUIRE(blocks_per_cu >= 1 && blocks_per_cu <= 4 && rounds >= 1 && rounds <= 100000 &&
+                   us_per_barrier,
+               PYTO_EINVAL, "bad arguments");
+  hipStream_t s = as_stream(stream);
+  int dev = 0, coop = 0, per_cu = 0;
+  hipDeviceProp_t prop;
+  PYTO_CHECK_HIP(hipGetDevice(&dev));
+  PYTO_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
+  PYTO_CHECK_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  PYTO_REQUIRE(coop, PYTO_EINVAL, "the device has no cooperative launch");
+  PYTO_CHECK_HIP(
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_grid_barrier_probe, 256, 0));
+  PYTO_REQUIRE(per_cu >= blocks_per_cu, PYTO_EINVAL,
+               "%d workgroups per CU do not fit (at most %d)", blocks_per_cu, per_cu);
+  const int nb = prop.multiProcessorCount * blocks_per_cu;
+  unsigned* counter = nullptr;
+  PYTO_CHECK_HIP(hipMalloc((void**)&counter, 2 * sizeof(unsigned)));
+  int* timed_out = reinterpret_cast<int*>(counter + 1);
+  const unsigned spin_limit = 1u << 22;
+  float ms[2] = {0.f, 0.f};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t err = hipEventCreate(&e0);
+  if (err == hipSuccess) err = hipEventCreate(&e1);
+  // warm-up, then the same launch with 0 rounds (the launch itself) and
+  // with `rounds` barriers
+  const int plan[3] = {8, 0, rounds};
+  for (int k = 0; k < 3 && err == hipSuccess; ++k) {
+    err = hipMemsetAsync(counter, 0, 2 * sizeof(unsigned), s);
+    int rr = plan[k];
+    void* args[] = {(void*)&counter, (void*)&rr, (void*)&spin_limit, (void*)&timed_out};
+    if (err == hipSuccess) err = hipEventRecord(e0, s);
+    if (err == hipSuccess)
+      err = hipLaunchCooperativeKernel((const void*)k_grid_barrier_probe, dim3(nb), dim3(256),
+                                       args, 0, s);
+    if (err == hipSuccess) err = hipEventRecord(e1, s);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    if (err == hipSuccess && k > 0) err = hipEventElapsedTime(&ms[k - 1], e0, e1);
+  }
+  int flag = 0;
+  if (err == hipSuccess) err = hipMemcpy(&flag, timed_out, sizeof(int), hipMemcpyDeviceToHost);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(counter);
+  PYTO_CHECK_HIP(err);
+  PYTO_REQUIRE(!flag, PYTO_ENOCONV, "grid barrier probe: a wait exceeded its poll limit");
+  *us_per_barrier = (ms[1] - ms[0]) * 1e3f / (float)rounds;
   return PYTO_OK;
 }
 

@@ -296,3 +296,17 @@ def test_batch_matches_oracle_direct_solves(gpu):
         print(d, k, st["iterations"], "err u %.1e C %.1e" % (eu, eC), flush=True)
         assert eu <= 1e-7 and eC <= 1e-9
     bs.close()
+
+
+def test_grid_barrier_probe(gpu):
+    """The persistent-kernel measurement: a grid-wide barrier costs a finite,
+    positive number of microseconds at 1 and 2 workgroups per CU (every wait
+    bounded; a timed-out wait would raise), and bad arguments are refused."""
+    from pyto_amd.batch import measure_grid_barrier
+    print("grid barrier probe", flush=True)
+    for bpc in (1, 2):
+        us = measure_grid_barrier(bpc, rounds=500)
+        print("blocks per CU", bpc, "us per barrier", us, flush=True)
+        assert 0.0 < us < 1000.0
+    with pytest.raises(ValueError):
+        measure_grid_barrier(0, rounds=10)
