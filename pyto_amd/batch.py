@@ -165,3 +165,68 @@ def measure_grid_barrier(blocks_per_cu=1, rounds=2000):
     check(_lib.lib().pyto_measure_grid_barrier(int(blocks_per_cu), int(rounds), ctypes.byref(us),
                                                stream_ptr()), "pyto_measure_grid_barrier")
     return float(us.value)
+
+
+def block_dpcg(op, forces, rtol=1e-8, max_iters=5000):
+    """Block deflated PCG (O'Leary's block CG with the deflation projection
+    of src/deflation.py:504-584 applied per column) over the load cases of
+    one design, in FP64 on the GPU: the operator, Jacobi diagonal and
+    coarse space are the handle's (GridOperator.matvec, diagonal, restrict,
+    coarse_solve, prolong), the s x s algebra is torch's.  One shared Krylov
+    space for all s right-hand sides: Q = K P, alpha = (P^T Q)^-1 (Z^T R),
+    X += P alpha, R -= Q alpha, Z = M^-1 R, beta = (Z^T R)_old^-1 (Z^T R),
+    P = (I - W E^-1 W^T K)(Z + P beta).  Every column stops on the
+    reference's test sqrt(r^T M r / r0^T M r0) <= rtol; the iteration ends
+    when all have.  With s = 1 it is the per-system deflated PCG in the same
+    form, the baseline of the block measurement (the projection costs a
+    second operator per column here, in both forms).  No reference
+    counterpart: the reference solves one load case at a time.
+    Returns (U as an s x n tensor, {"iterations", "converged", "rel"})."""
+    F = torch.stack([torch.as_tensor(f, dtype=F64).to(op.device) for f in forces])
+    s = F.shape[0]
+    d = op.diagonal()
+    free = d > 0
+    invd = torch.where(free, 1.0 / torch.where(free, d, torch.ones_like(d)), torch.zeros_like(d))
+    F = F * free
+    defl = op.ngroups > 0
+
+    def kmul(V):
+        return torch.stack([op.matvec(v) for v in V])
+
+    def project(V):   # V - W E^-1 W^T K V, row by row
+        if not defl:
+            return V
+        return torch.stack([v - op.prolong(op.coarse_solve(op.restrict(op.matvec(v)))) for v in V])
+
+    if defl:
+        X = torch.stack([op.prolong(op.coarse_solve(op.restrict(f))) for f in F])
+    else:
+        X = torch.zeros_like(F)
+    R = F - kmul(X)
+    Z = invd * R
+    P = project(Z)
+    D = Z @ R.T                       # (Z^T R), s x s
+    rz0 = torch.diagonal(D).clone()
+    rz0 = torch.where(rz0 > 0, rz0, torch.ones_like(rz0))
+    it, rel = 0, float("inf")
+    converged = bool((torch.diagonal(D) <= 0).all())
+    def small_solve(A, B):   # s x s systems on the host (FP64)
+        a = A.cpu().numpy()
+        return torch.from_numpy(np.linalg.solve(0.5 * (a + a.T), B.cpu().numpy())).to(op.device)
+
+    while not converged and it < max_iters:
+        Q = kmul(P)
+        alpha = small_solve(P @ Q.T, D)
+        X = X + alpha.T @ P
+        R = R - alpha.T @ Q
+        Z = invd * R
+        Dn = Z @ R.T
+        it += 1
+        rel = float(torch.sqrt(torch.clamp(torch.diagonal(Dn), min=0.0) / rz0).max())
+        if rel <= rtol:
+            converged = True
+            break
+        beta = small_solve(D, Dn)
+        P = project(Z + beta.T @ P)
+        D = Dn
+    return X, {"iterations": it, "converged": converged, "rel": rel, "columns": s}

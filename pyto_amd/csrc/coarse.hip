@@ -435,18 +435,27 @@ __global__ __launch_bounds__(256) void k_dir_update(
   V* pv = reinterpret_cast<V*>(p);
   V* xv = reinterpret_cast<V*>(x);
   const V* zv = reinterpret_cast<const V*>(z);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t d0 = i * W;
-    const int64_t n0 = d0 / 3;
-    const int c0 = (int)(d0 - 3 * n0);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // the vector's first node n0, its first component c0 and (with groups) the
+  // node's coordinates: formed once, then advanced by the constant step of
+  // the grid-stride loop (stride * W DOFs = kn nodes + kc components, kn at
+  // (sxn, syn, szn)) instead of divided out on every trip
+  int64_t n0 = (i * W) / 3;
+  int c0 = (int)(i * W - 3 * n0);
+  const int64_t kn = (stride * W) / 3;
+  const int kc = (int)(stride * W - 3 * kn);
+  int ix = 0, iy = 0, iz = 0, sxn = 0, syn = 0, szn = 0;
+  if (grp && i < nv) {
+    node_coords(n0, g, ix, iy, iz);
+    node_coords(kn, g, sxn, syn, szn);
+  }
+  for (; i < nv; i += stride) {
     // W mu of the vector's first node (0..2) and of the next node (3..5);
     // fixed bits likewise (bit c of the first node, 3 + c of the next)
     double w6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     uint32_t fx = 0;
     if (grp) {
-      int ix, iy, iz;
-      node_coords(n0, g, ix, iy, iz);
       const bool wx = ix + 1 == g.sx;
       const int jx = wx ? 0 : ix + 1;
       const bool wy = wx && iy + 1 == g.sy;
@@ -476,6 +485,24 @@ __global__ __launch_bounds__(256) void k_dir_update(
       DV::set(pp, j, (T)((double)DV::get(zz, j) + beta * (double)DV::get(pp, j) - wm));
     }
     pv[i] = pp;
+    // next trip: c0 + kc may carry one node (so may x and then y)
+    c0 += kc;
+    const int extra = c0 >= 3 ? 1 : 0;
+    c0 -= 3 * extra;
+    n0 += kn + extra;
+    if (grp) {
+      ix += sxn + extra;
+      iy += syn;
+      iz += szn;
+      if (ix >= g.sx) {
+        ix -= g.sx;
+        ++iy;
+      }
+      if (iy >= g.sy) {
+        iy -= g.sy;
+        ++iz;
+      }
+    }
   }
   // tail: DOFs past the last whole vector, one at a time by block 0
   if (blockIdx.x == 0)
@@ -527,13 +554,19 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
   }
   const double beta = first ? 0.0 : scal[S_BETA];
   const double alpha = first ? 0.0 : scal[S_ALPHA_X];
-  for (int64_t nd = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; nd < g.nnodes;
-       nd += (int64_t)gridDim.x * blockDim.x) {
+  // the node's coordinates, formed once and advanced by the constant step of
+  // the grid-stride loop (stride nodes, at (sxn, syn, szn))
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t nd = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int ix = 0, iy = 0, iz = 0, sxn = 0, syn = 0, szn = 0;
+  if (grp && nd < g.nnodes) {
+    node_coords(nd, g, ix, iy, iz);
+    node_coords(stride, g, sxn, syn, szn);
+  }
+  for (; nd < g.nnodes; nd += stride) {
     double w[3] = {0.0, 0.0, 0.0};
     uint32_t fx = 0;
     if (grp) {
-      int ix, iy, iz;
-      node_coords(nd, g, ix, iy, iz);
       fx = mask[nd];
       const int q = grp[nd];
       const double X = ix * g.hx, Y = iy * g.hy, Z = iz * g.hz;
@@ -576,6 +609,19 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
     for (int c = 0; c < 3; ++c) {
       const double wm = ((fx >> c) & 1u) ? 0.0 : w[c];
       pn[c] = (T)((double)zz[c] + beta * (double)pp[c] - wm);
+    }
+    if (grp) {
+      ix += sxn;
+      iy += syn;
+      iz += szn;
+      if (ix >= g.sx) {
+        ix -= g.sx;
+        ++iy;
+      }
+      if (iy >= g.sy) {
+        iy -= g.sy;
+        ++iz;
+      }
     }
   }
 }
