@@ -312,9 +312,32 @@ def batch_line(torch, dev, inner_rtol=None, boxes=12):
            # dependent phase of an iteration instead of a graph node
            "grid_barrier_us": {str(b): measure_grid_barrier(b, rounds=2000) for b in (1, 2)}}
     bs.close()
+    out["block_pcg"] = block_pcg_probe(torch, ops, designs)
     for op in ops:
         op.close()
     return out
+
+
+def block_pcg_probe(torch, ops, designs, count=2):
+    """Block deflated PCG (one Krylov space per design's load cases) against
+    the per-load-case form of the same GPU code (s = 1 per column), FP64 to
+    rel res 1e-8, on the `count` designs with the most load cases (largest
+    grid first on ties).  The lockstep batch runs as many iterations as its
+    hardest system, so the figure of merit is the block iteration count over
+    the per-case maximum."""
+    from pyto_amd.batch import block_dpcg
+    order = sorted(range(len(designs)), key=lambda i: (-len(designs[i]["loads"]),
+                                                       -int(np.prod(designs[i]["grid"]))))
+    res = []
+    for i in order[:count]:
+        op, loads = ops[i], designs[i]["loads"]
+        _, sb = block_dpcg(op, loads, rtol=1e-8)
+        its = [block_dpcg(op, [f], rtol=1e-8)[1]["iterations"] for f in loads]
+        res.append({"grid": list(designs[i]["grid"]), "load_cases": len(loads),
+                    "groups": op.ngroups, "block_iterations": sb["iterations"],
+                    "per_case_iterations": its, "block_converged": sb["converged"],
+                    "block_over_per_case_max": sb["iterations"] / max(its)})
+    return res
 
 
 def thermal_line(torch, dev, grid, x):
@@ -397,6 +420,8 @@ def main():
     ap.add_argument("--solve-only", action="store_true",
                     help="only the warm-up, --steps mixed solves and --fp64-steps FP64 solves "
                          "of the workload (for kernel traces); prints their times")
+    ap.add_argument("--tile-rows", type=int, default=16,
+                    help="operator tile rows for --operator-only (8, 16 or 32)")
     ap.add_argument("--operator-only", action="store_true",
                     help="only 20 launches each of the FP32 operator, the FP64 one and the "
                          "FP32 matrix-core variant on the workload (for counter collection); "
@@ -450,6 +475,7 @@ def main():
         op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, None, penal=3.0, eps_void=1e-9,
                           fixed_mask=mask)
         op.set_density(torch.from_numpy(x_np).to(dev))
+        op.set_tile_rows(args.tile_rows)
         p32 = torch.randn(op.num_dofs, dtype=torch.float32, device=dev)
         p64 = p32.double()
         res = {"operator_only": True}
@@ -472,6 +498,14 @@ def main():
                                                       "k_matvec_mfma"))
         res["f32_vs_f64_relmax"] = float((y32.double() - y64).abs().max() / y64.abs().max())
         res["mfma_vs_f64_relmax"] = float((ym.double() - y64).abs().max() / y64.abs().max())
+        res["tile_rows"] = op.tile_rows
+        # the PMC calibration: three copies of one FP32 DOF vector (known bytes)
+        cal_src = torch.randn(op.num_dofs, dtype=torch.float32, device=dev)
+        cal_dst = torch.empty_like(cal_src)
+        for _ in range(3):
+            cal_dst.copy_(cal_src)
+        torch.cuda.synchronize()
+        res["calibration_copy_bytes"] = 4 * op.num_dofs
         print(json.dumps(res))
         return
     log("grid", grid, "dof", 3 * mesh.num_nodes)

@@ -310,3 +310,24 @@ def test_grid_barrier_probe(gpu):
         assert 0.0 < us < 1000.0
     with pytest.raises(ValueError):
         measure_grid_barrier(0, rounds=10)
+
+
+def test_block_dpcg_matches_single_solves(gpu):
+    """Block deflated PCG over a design's load cases (shared Krylov space):
+    every column agrees with the design's own single-system FP64 solve and
+    meets rtol, in no more iterations than the per-column form of the same
+    code (s = 1) takes for the hardest column; Jacobi-only as well."""
+    from pyto_amd.batch import block_dpcg
+    print("block dpcg", flush=True)
+    for boxes in ((3, 1, 1), None):
+        op, loads = _design((32, 16, 16), 51, 5, gpu, boxes)
+        U, st = block_dpcg(op, loads, rtol=1e-9)
+        assert st["converged"] and st["rel"] <= 1e-9
+        single = [block_dpcg(op, [f], rtol=1e-9)[1]["iterations"] for f in loads]
+        print(boxes, "block", st["iterations"], "per column", single, flush=True)
+        assert st["iterations"] <= max(single)
+        for f, u in zip(loads, U):
+            ref, _ = op.solve(torch.from_numpy(f).to(gpu), rtol=1e-11, max_iters=20000)
+            fd = torch.from_numpy(f).to(gpu)
+            C, Cr = float(torch.dot(fd, u)), float(torch.dot(fd, ref))
+            assert abs(C - Cr) <= 1e-7 * abs(Cr)
