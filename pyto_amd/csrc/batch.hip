@@ -463,9 +463,7 @@ __global__ __launch_bounds__(256) void k_bdir_defl(const BatchSys* __restrict__ 
     for (int q = threadIdx.x; q < S.ngroups; q += blockDim.x) coeffs(q, aw + 6 * q);
     __syncthreads();
   }
-  auto wmu3 = [&](int64_t n, double* w) {
-    int ix, iy, iz;
-    node_coords(n, S.g, ix, iy, iz);
+  auto wmu3 = [&](int64_t n, int ix, int iy, int iz, double* w) {
     const int q = S.node_group[n];
     double e[6];
     if (lds) {
@@ -489,14 +487,33 @@ __global__ __launch_bounds__(256) void k_bdir_defl(const BatchSys* __restrict__ 
     const double wm = ((m >> c) & 1u) ? 0.0 : w3[c];
     S.p[d] = (float)((double)S.invd32[d] * (double)S.r[d] + be * (double)pd - wm);
   };
-  // one node (three DOFs) per thread: W mu once per node
+  // one node (three DOFs) per thread: W mu once per node; the node's
+  // coordinates formed once and advanced by the loop's constant step
   const int64_t nn = S.ndof / 3;
-  for (int64_t nd = (int64_t)v.lb * 256 + threadIdx.x; nd < nn; nd += (int64_t)v.nb * 256) {
+  const int64_t step = (int64_t)v.nb * 256;
+  int64_t nd = (int64_t)v.lb * 256 + threadIdx.x;
+  int ix = 0, iy = 0, iz = 0, sxn = 0, syn = 0, szn = 0;
+  if (nd < nn) {
+    node_coords(nd, S.g, ix, iy, iz);
+    node_coords(step, S.g, sxn, syn, szn);
+  }
+  for (; nd < nn; nd += step) {
     double w3[3];
-    wmu3(nd, w3);
+    wmu3(nd, ix, iy, iz, w3);
     const uint32_t m = S.mask[nd];
 #pragma unroll
     for (int c = 0; c < 3; ++c) dof(3 * nd + c, nd, w3, m);
+    ix += sxn;
+    iy += syn;
+    iz += szn;
+    if (ix >= S.g.sx) {
+      ix -= S.g.sx;
+      ++iy;
+    }
+    if (iy >= S.g.sy) {
+      iy -= S.g.sy;
+      ++iz;
+    }
   }
 }
 
