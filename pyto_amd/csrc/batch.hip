@@ -56,6 +56,7 @@ struct BatchSys {
   // system's own coarse vectors
   int ngroups, nc, ld;
   const int32_t* node_group;
+  const uint16_t* gmask;   // group | fixed bits << 12 (the handle's)
   const double* centers;
   // the handle's block-padded restriction tables
   const int2* pwave;
@@ -463,8 +464,7 @@ __global__ __launch_bounds__(256) void k_bdir_defl(const BatchSys* __restrict__ 
     for (int q = threadIdx.x; q < S.ngroups; q += blockDim.x) coeffs(q, aw + 6 * q);
     __syncthreads();
   }
-  auto wmu3 = [&](int64_t n, int ix, int iy, int iz, double* w) {
-    const int q = S.node_group[n];
+  auto wmu3 = [&](int q, int ix, int iy, int iz, double* w) {
     double e[6];
     if (lds) {
 #pragma unroll
@@ -498,9 +498,11 @@ __global__ __launch_bounds__(256) void k_bdir_defl(const BatchSys* __restrict__ 
     node_coords(step, S.g, sxn, syn, szn);
   }
   for (; nd < nn; nd += step) {
+    // the node's group and fixed bits in one 16-bit word
+    const uint32_t a = S.gmask[nd];
     double w3[3];
-    wmu3(nd, ix, iy, iz, w3);
-    const uint32_t m = S.mask[nd];
+    wmu3((int)(a & 0xfffu), ix, iy, iz, w3);
+    const uint32_t m = a >> 12;
 #pragma unroll
     for (int c = 0; c < 3; ++c) dof(3 * nd + c, nd, w3, m);
     ix += sxn;
@@ -1168,6 +1170,7 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
       S.nc = h->nc;
       S.ld = (h->nc + 63) / 64 * 64;
       S.node_group = h->node_group;
+      S.gmask = h->gmask;
       S.centers = h->centers;
       S.pwave = h->pwave;
       S.pslot = h->pslot;

@@ -382,7 +382,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_dir_update(
     T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
     const double* __restrict__ mu, const int32_t* __restrict__ grp,
-    const double* __restrict__ centers, int ngroups, const uint8_t* __restrict__ mask,
+    const double* __restrict__ centers, int ngroups, const uint16_t* __restrict__ gm,
     GridDims g, int nc, const double* __restrict__ scal, bool first,
     const int* __restrict__ done) {
   if (done && *done) return;
@@ -462,9 +462,11 @@ __global__ __launch_bounds__(256) void k_dir_update(
       const int jy = wx ? (wy ? 0 : iy + 1) : iy;
       const int jz = wy ? iz + 1 : iz;
       const bool two = c0 + W > 3 && n0 + 1 < g.nnodes;
-      const int g0 = grp[n0];
-      const int g1 = two ? grp[n0 + 1] : g0;
-      fx = (uint32_t)mask[n0] | (two ? (uint32_t)mask[n0 + 1] << 3 : 0u);
+      const uint32_t a0 = gm[n0];
+      const uint32_t a1 = two ? (uint32_t)gm[n0 + 1] : a0;
+      const int g0 = (int)(a0 & 0xfffu);
+      const int g1 = (int)(a1 & 0xfffu);
+      fx = (a0 >> 12) | (two ? (a1 >> 12) << 3 : 0u);
       wmu3(ix, iy, iz, g0, w6);
       wmu3(jx, jy, jz, g1, w6 + 3);
     }
@@ -510,11 +512,11 @@ __global__ __launch_bounds__(256) void k_dir_update(
       const int64_t nd = d / 3;
       const int c = (int)(d - 3 * nd);
       double wm = 0.0;
-      if (grp && !((mask[nd] >> c) & 1u)) {
+      if (grp && !((gm[nd] >> (12 + c)) & 1u)) {
         int ix, iy, iz;
         node_coords(nd, g, ix, iy, iz);
         double w3[3];
-        wmu3(ix, iy, iz, grp[nd], w3);
+        wmu3(ix, iy, iz, (int)(gm[nd] & 0xfffu), w3);
         wm = c == 0 ? w3[0] : (c == 1 ? w3[1] : w3[2]);
       }
       const T pd = first ? T(0) : p[d];
@@ -532,7 +534,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_dir_update_nodes(
     T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
     const double* __restrict__ mu, const int32_t* __restrict__ grp,
-    const double* __restrict__ centers, int ngroups, const uint8_t* __restrict__ mask,
+    const double* __restrict__ centers, int ngroups, const uint16_t* __restrict__ gm,
     GridDims g, const double* __restrict__ scal, bool first, const int* __restrict__ done) {
   if (done && *done) return;
   // 6 values per group, sized by the launch (ngroups * 48 bytes when staged)
@@ -567,8 +569,9 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
     double w[3] = {0.0, 0.0, 0.0};
     uint32_t fx = 0;
     if (grp) {
-      fx = mask[nd];
-      const int q = grp[nd];
+      const uint32_t a = gm[nd];
+      fx = a >> 12;
+      const int q = (int)(a & 0xfffu);
       const double X = ix * g.hx, Y = iy * g.hy, Z = iz * g.hz;
       double e[6];
       if (lds) {
@@ -639,7 +642,8 @@ int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* 
     hipLaunchKernelGGL(k_dir_update_nodes<T>, dim3(blocks), dim3(256), lds, s, x, p, z,
                        (const double*)h->cmu,
                        h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
-                       h->ngroups, h->fixmask, h->g, (const double*)h->scal, first, done);
+                       h->ngroups, (const uint16_t*)h->gmask, h->g, (const double*)h->scal, first,
+                       done);
     PYTO_CHECK_HIP(hipGetLastError());
     return PYTO_OK;
   }
@@ -648,8 +652,8 @@ int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* 
   hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), lds, s, x, p, z,
                      (const double*)h->cmu,
                      h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
-                     h->ngroups, h->fixmask, h->g, h->ncomp, (const double*)h->scal, first,
-                     done);
+                     h->ngroups, (const uint16_t*)h->gmask, h->g, h->ncomp,
+                     (const double*)h->scal, first, done);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
@@ -1121,7 +1125,7 @@ int set_density_many(int n, Handle* const* hs, const double* const* x, hipStream
 
 // Free a handle's coarse space and return it to the no-groups state.
 static void release_coarse(Handle* h) {
-  void* olds[] = {h->node_group, h->centers, h->sinfo, h->chunks, h->cpart,
+  void* olds[] = {h->node_group, h->gmask, h->centers, h->sinfo, h->chunks, h->cpart,
                   h->gptr, h->gidx,
                   h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
                   h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres,
@@ -1129,7 +1133,7 @@ static void release_coarse(Handle* h) {
                   h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32};
   for (void* p : olds)
     if (p) hipFree(p);
-  h->node_group = nullptr; h->centers = nullptr; h->chunks = nullptr;
+  h->node_group = nullptr; h->gmask = nullptr; h->centers = nullptr; h->chunks = nullptr;
   h->sinfo = nullptr;
   h->cpart = nullptr; h->gptr = nullptr; h->gidx = nullptr;
   h->nsblocks = 0;
@@ -1179,6 +1183,11 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   for (int64_t n = 0; n < nn; ++n)
     PYTO_REQUIRE(grp[n] >= 0 && grp[n] < ngroups, PYTO_EINVAL,
                  "node_group[%lld] = %d out of range [0,%d)", (long long)n, grp[n], ngroups);
+  // the direction updates read a node's group and fixed bits as one 16-bit
+  // word (groups < kMaxCoarseGroups = 4096 take 12 bits)
+  static_assert(kMaxCoarseGroups <= 4096, "group ids must fit 12 bits");
+  std::vector<uint16_t> gmask(nn);
+  for (int64_t n = 0; n < nn; ++n) gmask[n] = (uint16_t)(grp[n] | ((mask[n] & 7) << 12));
   // special elements and their group sets
   const GridDims& g = h->g;
   std::vector<std::pair<std::array<int, 8>, int32_t>> list;
@@ -1406,6 +1415,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     h->nsblocks = (int)sblocks.size();
     const int ld = round_up64(h->nc);
     PYTO_TRY(device_alloc(h, (void**)&h->node_group, nn * 4));
+    PYTO_TRY(device_alloc(h, (void**)&h->gmask, nn * 2));
     PYTO_TRY(device_alloc(h, (void**)&h->centers, (size_t)ngroups * 3 * 8));
     PYTO_TRY(device_alloc(h, (void**)&h->sinfo, sinfo.size() * sizeof(SpecialElem)));
     PYTO_TRY(device_alloc(h, (void**)&h->chunks, chunks.size() * sizeof(SpecialChunk)));
@@ -1437,6 +1447,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     PYTO_TRY(device_alloc(h, (void**)&h->cmu, (size_t)ld * 8));
     PYTO_TRY(device_alloc(h, (void**)&h->cres, (size_t)ld * 8));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->node_group, node_group, nn * 4, hipMemcpyDeviceToDevice, s));
+    PYTO_CHECK_HIP(hipMemcpyAsync(h->gmask, gmask.data(), nn * 2, hipMemcpyHostToDevice, s));
     PYTO_CHECK_HIP(hipMemcpyAsync(h->centers, centers, (size_t)ngroups * 24, hipMemcpyDeviceToDevice, s));
     if (!special.empty()) {
       PYTO_CHECK_HIP(hipMemcpyAsync(h->sinfo, sinfo.data(), sinfo.size() * sizeof(SpecialElem),

@@ -138,6 +138,7 @@ struct Handle {
   // deflation
   int ngroups = 0, nc = 0;      // nc = 6*ngroups
   int32_t* node_group = nullptr;
+  uint16_t* gmask = nullptr;    // per node: group (bits 0-11) | fixed bits << 12
   double* centers = nullptr;
   int nspecial = 0, nchunks = 0;
   SpecialElem* sinfo = nullptr; // same order, with packed slots and fixed bits

@@ -471,7 +471,7 @@ int pyto_create_thermal(pyto_handle** out, int nelx, int nely, int nelz, double 
 int pyto_destroy(pyto_handle* h) {
   if (!h) return PYTO_OK;
   void* ptrs[] = {h->fixmask, h->tileflags, h->Ee64, h->Ee32, h->x64, h->invd64, h->invd32,
-                  h->node_group, h->centers, h->sinfo, h->chunks, h->cpart,
+                  h->node_group, h->gmask, h->centers, h->sinfo, h->chunks, h->cpart,
                   h->gptr, h->gidx, h->gnodes, h->nchunk,
                   h->nchunk_grp, h->npart, h->nptr, h->nidx, h->Ec, h->Einv, h->cwork, h->ct,
                   h->cmu, h->cres, h->r64, h->p64, h->q64, h->r32, h->p32, h->q32, h->x32,
