@@ -378,10 +378,11 @@ constexpr int kLdsGroups = 512;
 // on a 192 x 64 x 64 grid).
 constexpr int kMaxCoarseGroups = 4096;
 
-template <typename T>
+// INPLACE: pin == pout (one restrict-qualified pointer; otherwise the two
+// are distinct buffers and both are restrict-qualified).
+template <typename T, bool FLUSH, bool INPLACE>
 __global__ __launch_bounds__(256) void k_dir_update(
-    T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
-    const double* __restrict__ mu, const int32_t* __restrict__ grp,
+    DirIO<T> io, const double* __restrict__ mu, const int32_t* __restrict__ grp,
     const double* __restrict__ centers, int ngroups, const uint16_t* __restrict__ gm,
     GridDims g, int nc, const double* __restrict__ scal, bool first,
     const int* __restrict__ done) {
@@ -429,12 +430,18 @@ __global__ __launch_bounds__(256) void k_dir_update(
   using V = typename DV::V;
   constexpr int W = DV::W;
   const double beta = first ? 0.0 : scal[S_BETA];
-  const double alpha = first ? 0.0 : scal[S_ALPHA_X];
+  const double alpha = (first || !io.xupd) ? 0.0 : scal[S_ALPHA_X];
   const int64_t n = nc * g.nnodes;   // groups (grp) only with nc = 3
   const int64_t nv = n / W;
-  V* pv = reinterpret_cast<V*>(p);
+  T* __restrict__ x = io.x;
+  T* __restrict__ pio = io.pout;   // INPLACE: the one direction buffer
+  const T* __restrict__ p = INPLACE ? pio : io.pin;
+  const T* __restrict__ z = io.z;
+  const V* pv = reinterpret_cast<const V*>(p);
+  V* pov = reinterpret_cast<V*>(pio);
   V* xv = reinterpret_cast<V*>(x);
   const V* zv = reinterpret_cast<const V*>(z);
+
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   // the vector's first node n0, its first component c0 and (with groups) the
@@ -472,7 +479,27 @@ __global__ __launch_bounds__(256) void k_dir_update(
     }
     V pp = first ? V{} : pv[i];
     const V zz = zv[i];
-    if (alpha != 0.0) {
+    if (FLUSH) {
+      // x += the deferred alpha_s p_s of the window (FP64 sum)
+      V xx = xv[i];
+      double acc[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) acc[j] = (double)DV::get(xx, j);
+      const V* rv = reinterpret_cast<const V*>(io.ring);
+      const int64_t ldv = io.ld / W;
+#pragma unroll
+      for (int sl = 0; sl < kDirSlots; ++sl) {
+        if (sl < io.nflush) {
+          const V r = rv[sl * ldv + i];
+          const double a = io.al[sl];
+#pragma unroll
+          for (int j = 0; j < W; ++j) acc[j] += a * (double)DV::get(r, j);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < W; ++j) DV::set(xx, j, (T)acc[j]);
+      xv[i] = xx;
+    } else if (alpha != 0.0) {
       V xx = xv[i];
 #pragma unroll
       for (int j = 0; j < W; ++j)
@@ -486,7 +513,7 @@ __global__ __launch_bounds__(256) void k_dir_update(
       const double wm = ((fx >> (c0 + j)) & 1u) ? 0.0 : w;
       DV::set(pp, j, (T)((double)DV::get(zz, j) + beta * (double)DV::get(pp, j) - wm));
     }
-    pv[i] = pp;
+    pov[i] = pp;
     // next trip: c0 + kc may carry one node (so may x and then y)
     c0 += kc;
     const int extra = c0 >= 3 ? 1 : 0;
@@ -520,8 +547,14 @@ __global__ __launch_bounds__(256) void k_dir_update(
         wm = c == 0 ? w3[0] : (c == 1 ? w3[1] : w3[2]);
       }
       const T pd = first ? T(0) : p[d];
-      if (alpha != 0.0) x[d] = (T)((double)x[d] + alpha * (double)pd);
-      p[d] = (T)((double)z[d] + beta * (double)pd - wm);
+      if (FLUSH) {
+        double acc = (double)x[d];
+        for (int sl = 0; sl < io.nflush; ++sl) acc += io.al[sl] * (double)io.ring[sl * io.ld + d];
+        x[d] = (T)acc;
+      } else if (alpha != 0.0) {
+        x[d] = (T)((double)x[d] + alpha * (double)pd);
+      }
+      pio[d] = (T)((double)z[d] + beta * (double)pd - wm);
     }
 }
 
@@ -530,12 +563,12 @@ __global__ __launch_bounds__(256) void k_dir_update(
 // instead of for both nodes a 16-byte vector can span, and the per-DOF
 // arithmetic is the vector kernel's.  (In FP32 a node is 12 bytes, loaded as
 // separate dwords; the vector kernel is faster there.)
-template <typename T>
+template <typename T, bool FLUSH, bool INPLACE>
 __global__ __launch_bounds__(256) void k_dir_update_nodes(
-    T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
     const double* __restrict__ mu, const int32_t* __restrict__ grp,
     const double* __restrict__ centers, int ngroups, const uint16_t* __restrict__ gm,
-    GridDims g, const double* __restrict__ scal, bool first, const int* __restrict__ done) {
+    GridDims g, const double* __restrict__ scal, bool first, const int* __restrict__ done,
+    DirIO<T> io) {
   if (done && *done) return;
   // 6 values per group, sized by the launch (ngroups * 48 bytes when staged)
   extern __shared__ double aw[];
@@ -555,7 +588,8 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
     __syncthreads();
   }
   const double beta = first ? 0.0 : scal[S_BETA];
-  const double alpha = first ? 0.0 : scal[S_ALPHA_X];
+  const double alpha = (first || !io.xupd) ? 0.0 : scal[S_ALPHA_X];
+
   // the node's coordinates, formed once and advanced by the constant step of
   // the grid-stride loop (stride nodes, at (sxn, syn, szn))
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -592,16 +626,32 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
       w[1] = e[1] + e[5] * X - e[3] * Z;
       w[2] = e[2] + e[3] * Y - e[4] * X;
     }
-    T* pn = p + 3 * nd;
-    T* xn = x + 3 * nd;
-    const T* zn = z + 3 * nd;
+    T* __restrict__ po = io.pout + 3 * nd;
+    const T* __restrict__ pn = INPLACE ? po : io.pin + 3 * nd;
+    T* __restrict__ xn = io.x + 3 * nd;
+    const T* __restrict__ zn = io.z + 3 * nd;
     T pp[3], zz[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       pp[c] = first ? T(0) : pn[c];
       zz[c] = zn[c];
     }
-    if (alpha != 0.0) {
+    if (FLUSH) {
+      // x += the deferred alpha_s p_s of the window (FP64 sum)
+      double acc[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[c] = (double)xn[c];
+#pragma unroll
+      for (int sl = 0; sl < kDirSlots; ++sl) {
+        if (sl < io.nflush) {
+          const T* r = io.ring + sl * io.ld + 3 * nd;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[c] += io.al[sl] * (double)r[c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xn[c] = (T)acc[c];
+    } else if (alpha != 0.0) {
       T xx[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) xx[c] = xn[c];
@@ -611,7 +661,7 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const double wm = ((fx >> c) & 1u) ? 0.0 : w[c];
-      pn[c] = (T)((double)zz[c] + beta * (double)pp[c] - wm);
+      po[c] = (T)((double)zz[c] + beta * (double)pp[c] - wm);
     }
     if (grp) {
       ix += sxn;
@@ -630,26 +680,34 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
 }
 
 template <typename T>
-int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* done,
-                      hipStream_t s) {
+int launch_dir_update(Handle* h, const DirIO<T>& io, bool first, const int* done, hipStream_t s) {
   // 2048 workgroups (8 per CU) striding over the vectors: each stages the
   // group maps once, so fewer, longer-lived blocks keep those L2 reads small
   // the group table in LDS takes only what the groups need, so that all
   // workgroups fit on the chip at once
+  PYTO_REQUIRE(io.nflush >= 0 && io.nflush <= kDirSlots && (io.nflush == 0 || io.ld % 4 == 0),
+               PYTO_EINVAL, "bad flush window");
   const size_t lds = (h->ngroups > 0 && h->ngroups <= kLdsGroups) ? (size_t)h->ngroups * 48 : 0;
   if (h->ncomp == 3 && sizeof(T) == 8) {
     const int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_dir_update_nodes<T>, dim3(blocks), dim3(256), lds, s, x, p, z,
-                       (const double*)h->cmu,
+    const bool inp = io.pin == io.pout;
+    auto kernel = io.nflush > 0 ? (inp ? k_dir_update_nodes<T, true, true>
+                                       : k_dir_update_nodes<T, true, false>)
+                                : (inp ? k_dir_update_nodes<T, false, true>
+                                       : k_dir_update_nodes<T, false, false>);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), lds, s, (const double*)h->cmu,
                        h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
                        h->ngroups, (const uint16_t*)h->gmask, h->g, (const double*)h->scal, first,
-                       done);
+                       done, io);
     PYTO_CHECK_HIP(hipGetLastError());
     return PYTO_OK;
   }
   const int64_t nvec = h->ncomp * h->g.nnodes / (16 / (int64_t)sizeof(T));
   const int blocks = (int)std::min<int64_t>((nvec + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_dir_update<T>, dim3(blocks), dim3(256), lds, s, x, p, z,
+  const bool inp = io.pin == io.pout;
+  auto kernel = io.nflush > 0 ? (inp ? k_dir_update<T, true, true> : k_dir_update<T, true, false>)
+                              : (inp ? k_dir_update<T, false, true> : k_dir_update<T, false, false>);
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), lds, s, io,
                      (const double*)h->cmu,
                      h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
                      h->ngroups, (const uint16_t*)h->gmask, h->g, h->ncomp,
@@ -657,9 +715,9 @@ int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* 
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
-template int launch_dir_update<double>(Handle*, double*, double*, const double*, bool,
-                                       const int*, hipStream_t);
-template int launch_dir_update<float>(Handle*, float*, float*, const float*, bool, const int*,
+template int launch_dir_update<double>(Handle*, const DirIO<double>&, bool, const int*,
+                                       hipStream_t);
+template int launch_dir_update<float>(Handle*, const DirIO<float>&, bool, const int*,
                                       hipStream_t);
 
 // Galerkin matrix E = W^T K W over the special elements, without atomics.

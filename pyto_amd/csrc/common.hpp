@@ -185,10 +185,15 @@ struct Handle {
 
   // solver workspace (allocated on first use); in the DPCG, r holds
   // z = M^-1 r
-  double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr;
-  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr,
-        *b32 = nullptr;
+  double *r64 = nullptr, *q64 = nullptr;
+  float *r32 = nullptr, *q32 = nullptr, *x32 = nullptr, *b32 = nullptr;
   double* partials = nullptr;   // block partial sums: 2 x kMaxPartials (p.q, then r.Mr)
+  // direction rings (kDirSlots DOF vectors in FP64, one in FP32), the alphas
+  // of a window and the slot of each precision's last direction (resumed runs)
+  double* pring64 = nullptr;
+  float* pring32 = nullptr;
+  double* alring = nullptr;
+  int ring_last[2] = {0, 0};
   unsigned* counters = nullptr; // last-block counters (kNumCounters)
   double* scal = nullptr;       // device scalars (kNumScalars)
   int* flags = nullptr;         // device flags (kNumFlags)
@@ -205,6 +210,31 @@ struct Handle {
 };
 
 constexpr int kMaxPartials = 8192;
+
+// The DPCG keeps its last kDirSlots directions in a ring (slot k mod
+// kDirSlots holds p_k) and adds alpha_k p_k to x once per kDirSlots
+// iterations instead of on every iteration.
+constexpr int kDirSlots = 8;
+// Elements between ring slots for n-element DOF vectors: n rounded up to a
+// multiple of 4, so that every slot starts on a 16-byte boundary (the vector
+// kernels read 16 bytes per thread).
+inline int64_t ring_stride(int64_t n) { return (n + 3) / 4 * 4; }
+
+// Direction-update I/O: p_out = z + beta p_in - W mu; with xupd, x += alpha
+// p_in (alpha = scal[S_ALPHA_X]); with nflush > 0, x += sum over s <
+// nflush of al[s] ring[s * ld + i] (the deferred updates of a window).
+template <typename T>
+struct DirIO {
+  T* x;
+  const T* pin;
+  T* pout;
+  const T* z;
+  bool xupd;
+  const T* ring;
+  int64_t ld;
+  const double* al;
+  int nflush;
+};
 constexpr int kNumCounters = 16;
 constexpr int kNumScalars = 32;
 constexpr int kNumFlags = 10;

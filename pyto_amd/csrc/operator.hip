@@ -284,13 +284,15 @@ int ensure_solver_workspace(Handle* h) {
   if (h->r64) return PYTO_OK;
   const size_t nd = (size_t)(h->ncomp * h->g.nnodes);
   PYTO_TRY(device_alloc(h, (void**)&h->r64, nd * 8));
-  PYTO_TRY(device_alloc(h, (void**)&h->p64, nd * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->q64, nd * 8));
   PYTO_TRY(device_alloc(h, (void**)&h->r32, nd * 4));
-  PYTO_TRY(device_alloc(h, (void**)&h->p32, nd * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->q32, nd * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->x32, nd * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->b32, nd * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->pring64, kDirSlots * ring_stride(nd) * 8));
+  // the FP32 DPCG keeps its direction in place: one slot
+  PYTO_TRY(device_alloc(h, (void**)&h->pring32, ring_stride(nd) * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->alring, kDirSlots * 8));
   return PYTO_OK;
 }
 
@@ -474,11 +476,11 @@ int pyto_destroy(pyto_handle* h) {
                   h->node_group, h->gmask, h->centers, h->sinfo, h->chunks, h->cpart,
                   h->gptr, h->gidx, h->gnodes, h->nchunk,
                   h->nchunk_grp, h->npart, h->nptr, h->nidx, h->Ec, h->Einv, h->cwork, h->ct,
-                  h->cmu, h->cres, h->r64, h->p64, h->q64, h->r32, h->p32, h->q32, h->x32,
+                  h->cmu, h->cres, h->r64, h->q64, h->r32, h->q32, h->x32,
                   h->b32, h->partials, h->counters, h->scal, h->flags, h->elem_buf,
                   h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
                   h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32, h->stencil,
-                  h->ke32};
+                  h->ke32, h->pring64, h->pring32, h->alring};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& gc : h->graphs)

@@ -45,8 +45,7 @@ int coarse_apply(Handle* h, const double* t1, const double* t2, const double* be
 template <typename T>
 int matvec_blocks(const Handle* h);
 template <typename T>
-int launch_dir_update(Handle* h, T* x, T* p, const T* z, bool first, const int* done,
-                      hipStream_t s);
+int launch_dir_update(Handle* h, const DirIO<T>& io, bool first, const int* done, hipStream_t s);
 
 constexpr int kVecBlocks = 2048;
 
@@ -76,14 +75,18 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_cg_update_z(
     T* __restrict__ z, const T* __restrict__ q, const T* __restrict__ invd, int64_t n,
     double* __restrict__ scal, const int* __restrict__ flags,
-    const double* __restrict__ part_pq, int npq, double* __restrict__ part_rz) {
+    const double* __restrict__ part_pq, int npq, double* __restrict__ part_rz,
+    double* __restrict__ al_slot) {
   if (flags[F_DONE]) return;
   __shared__ double red[5];
   const double pkp = block_sum_partials(part_pq, npq, red);
   double acc = 0.0;
   const bool ok = pkp > 0.0;   // false for NaN
   const double alpha = ok ? scal[S_RZ] / pkp : 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_ALPHA_X] = alpha;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[S_ALPHA_X] = alpha;
+    if (al_slot) *al_slot = alpha;   // the direction ring's alpha of this iteration
+  }
   if (ok) {
     auto lane = [&](T& zi, const T qi, const T di) {
       zi = (T)((double)zi - alpha * ((double)di * (double)qi));
@@ -115,33 +118,23 @@ __global__ __launch_bounds__(256) void k_cg_update_z(
   if (threadIdx.x == 0) part_rz[blockIdx.x] = bs;
 }
 
-// x += scal[S_ALPHA_X] p (in FP64, stored in T): the update of the iteration
-// that stopped the solve, whose direction update was skipped.
+// x += sum over j < count of al[a] ring[r * ld + i], a = (a0 + j) mod
+// kDirSlots, r = (r0 + j) mod kDirSlots (r0 < 0: slot 0 throughout), summed
+// in FP64: the updates still pending when a DPCG run ends (the last, partial
+// window when x is deferred; the stopping iteration's alpha p otherwise).
 template <typename T>
-__global__ __launch_bounds__(256) void k_axpy_x(T* __restrict__ x, const T* __restrict__ p,
-                                               int64_t n, const double* __restrict__ scal) {
-  const double alpha = scal[S_ALPHA_X];
-  if (alpha == 0.0) return;
-  using V = typename Vec4<T>::type;
-  constexpr int W = Vec4<T>::width;
-  const int64_t nv = n / W;
-  V* xv = reinterpret_cast<V*>(x);
-  const V* pv = reinterpret_cast<const V*>(p);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
+__global__ __launch_bounds__(256) void k_flush_x(T* __restrict__ x, const T* __restrict__ ring,
+                                                int64_t ld, const double* __restrict__ al,
+                                                int a0, int r0, int count, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    V xx = xv[i];
-    const V pp = pv[i];
-    xx.x = (T)((double)xx.x + alpha * (double)pp.x);
-    xx.y = (T)((double)xx.y + alpha * (double)pp.y);
-    if constexpr (W == 4) {
-      xx.z = (T)((double)xx.z + alpha * (double)pp.z);
-      xx.w = (T)((double)xx.w + alpha * (double)pp.w);
+    double acc = (double)x[i];
+    for (int j = 0; j < count; ++j) {
+      const int r = r0 < 0 ? 0 : (r0 + j) % kDirSlots;
+      acc += al[(a0 + j) % kDirSlots] * (double)ring[r * ld + i];
     }
-    xv[i] = xx;
+    x[i] = (T)acc;
   }
-  if (blockIdx.x == 0)
-    for (int64_t i = nv * W + threadIdx.x; i < n; i += blockDim.x)
-      x[i] = (T)((double)x[i] + alpha * (double)p[i]);
 }
 
 // z = invd o (f - q) (in T); per-block partials of r^T (invd o r).
@@ -257,17 +250,18 @@ __global__ void k_resume_scalars(double* scal) {
 
 template <typename T>
 struct Vecs {
-  T *r, *p, *q;
+  T *r, *q;
   const T* invd;
   const T* Ee;
   ModalConsts<T> mc;
+  T* ring;   // kDirSlots directions: slot k mod kDirSlots holds p_k
 };
 
 static Vecs<double> vecs64(Handle* h) {
-  return {h->r64, h->p64, h->q64, h->invd64, h->Ee64, h->mc64};
+  return {h->r64, h->q64, h->invd64, h->Ee64, h->mc64, h->pring64};
 }
 static Vecs<float> vecs32(Handle* h) {
-  return {h->r32, h->p32, h->q32, h->invd32, h->Ee32, h->mc32};
+  return {h->r32, h->q32, h->invd32, h->Ee32, h->mc32, h->pring32};
 }
 
 template <typename T>
@@ -330,7 +324,7 @@ static int iteration_graph(Handle* h, const void* x, int tsize, F&& one_iteratio
   hipGraph_t graph;
   PYTO_CHECK_HIP(hipStreamBeginCapture(h->gstream, hipStreamCaptureModeThreadLocal));
   int rc = PYTO_OK;
-  for (int k = 0; k < kGraphBatch && rc == PYTO_OK; ++k) rc = one_iteration(h->gstream);
+  for (int k = 0; k < kGraphBatch && rc == PYTO_OK; ++k) rc = one_iteration(h->gstream, k);
   hipError_t e = hipStreamEndCapture(h->gstream, &graph);
   if (rc != PYTO_OK) return rc;
   PYTO_CHECK_HIP(e);
@@ -377,6 +371,21 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   const int64_t n = h->ncomp * h->g.nnodes;
   const bool defl = h->ngroups > 0;
   double* ct = h->ct;
+  // Directions and alphas.  alpha_k goes to alring[k mod kDirSlots] (graph
+  // replays start at multiples of kDirSlots, so every position's slots are
+  // fixed).  FP64 defers x: p_k goes to ring slot k mod kDirSlots and x takes
+  // a window's alpha_k p_k at once, from the direction update of the window's
+  // last iteration (the FP64 update moves 40 B per DOF and the x pass is 16
+  // of them).  FP32 updates p in place in slot 0 and x on every iteration:
+  // its vectors (51 MB each at 12.8M DOF) stay largely in the 256 MB
+  // last-level cache, which a rotating ring defeats; deferred, it measured
+  // slower.
+  static_assert(kGraphBatch % kDirSlots == 0, "graph batches hold whole windows");
+  const bool defer = sizeof(T) == 8;
+  const int tslot = sizeof(T) == 8 ? 0 : 1;
+  const int64_t ld = ring_stride(n);   // slots start on 16-byte boundaries
+  auto slot = [&](int k) { return defer ? v.ring + (int64_t)(k % kDirSlots) * ld : v.ring; };
+  T* p_prev = slot(h->ring_last[tslot]);
   PYTO_TRY(ensure_streams(h));
   if (defl) {
     PYTO_TRY(launch_restrict_full<T>(h, b, ct, s));
@@ -399,7 +408,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     // beta = rz / rz_prev; no x update
     hipLaunchKernelGGL(k_resume_scalars, dim3(1), dim3(1), 0, s, h->scal);
     if (defl) {
-      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, v.p, part_rz, kVecBlocks,
+      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, p_prev, part_rz, kVecBlocks,
                                                 h->scal + S_RZPREV, nullptr, s));
       PYTO_TRY(launch_coarse_gather(h, ct, nullptr, s));
       PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
@@ -408,8 +417,12 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     PYTO_TRY(launch_restrict_special<T>(h, v.r, ct, nullptr, s));
     PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
   }
-  // p0 = z0 - W mu, or (resume) z0 + beta p_prev - W mu
-  PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, !resume, nullptr, s));
+  // p0 = z0 - W mu, or (resume) z0 + beta p_prev - W mu, into slot 0 (alpha
+  // is 0 here: no x update)
+  {
+    const DirIO<T> io0{x, p_prev, slot(0), v.r, false, nullptr, 0, nullptr, 0};
+    PYTO_TRY(launch_dir_update<T>(h, io0, !resume, nullptr, s));
+  }
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RTOL, rtol);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RELMIN, 1e300);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_FLOOR,
@@ -424,14 +437,19 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   }
   const int* done = h->flags + F_DONE;
   // one iteration's launches on stream `st`
-  auto one_iteration = [&](hipStream_t st) -> int {
-    PYTO_TRY(launch_matvec<T>(h, v.p, v.q, v.Ee, v.mc, part_pq, done, nullptr, 0, st));
+  // iteration k (from 0 in each run): p_k from slot k, p_{k+1} to slot k+1
+  auto one_iteration = [&](hipStream_t st, int k) -> int {
+    T* pk = slot(k);
+    T* pk1 = slot(k + 1);
+    const int pos = k % kDirSlots;
+    PYTO_TRY(launch_matvec<T>(h, pk, v.q, v.Ee, v.mc, part_pq, done, nullptr, 0, st));
     hipLaunchKernelGGL(k_cg_update_z<T>, dim3(kVecBlocks), dim3(256), 0, st, v.r, v.q, v.invd,
-                       n, h->scal, (const int*)h->flags, (const double*)part_pq, npq, part_rz);
+                       n, h->scal, (const int*)h->flags, (const double*)part_pq, npq, part_rz,
+                       h->alring + pos);
     if (defl) {
       // beta is formed inside the restriction from the update's partials, so
       // the scalar logic runs beside the per-group gather in one launch
-      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, v.p, part_rz, kVecBlocks,
+      PYTO_TRY(launch_restrict_special_parts<T>(h, v.r, pk, part_rz, kVecBlocks,
                                                 h->scal + S_RZ, done, st));
       PYTO_TRY(launch_gather_scalars(h, ct, part_pq, npq, part_rz, kVecBlocks, st));
       PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, done, st, false));
@@ -439,7 +457,10 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
       hipLaunchKernelGGL(k_scalars, dim3(1), dim3(256), 0, st, h->scal, h->flags,
                          (const double*)part_pq, npq, (const double*)part_rz, kVecBlocks);
     }
-    PYTO_TRY(launch_dir_update<T>(h, x, v.p, v.r, false, done, st));
+    // deferred: x takes the window's alpha p at its last position
+    const DirIO<T> io{x, pk, pk1, v.r, !defer, v.ring, ld, h->alring,
+                      (defer && pos == kDirSlots - 1) ? kDirSlots : 0};
+    PYTO_TRY(launch_dir_update<T>(h, io, false, done, st));
     PYTO_CHECK_HIP(hipGetLastError());
     return PYTO_OK;
   };
@@ -460,7 +481,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
         launched += kGraphBatch;
       } else {
         const int rest = max_iters - launched;
-        for (int k = 0; k < rest; ++k) PYTO_TRY(one_iteration(h->gstream));
+        for (int k = 0; k < rest; ++k) PYTO_TRY(one_iteration(h->gstream, launched + k));
         launched += rest;
       }
       PYTO_TRY(read_back(h, h->gstream));
@@ -472,7 +493,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     int batch = 8;
     while (launched < max_iters) {
       const int nb = std::min(batch, max_iters - launched);
-      for (int k = 0; k < nb; ++k) PYTO_TRY(one_iteration(s));
+      for (int k = 0; k < nb; ++k) PYTO_TRY(one_iteration(s, launched + k));
       launched += nb;
       PYTO_TRY(read_back(h, s));
       if (h->host_flags[F_DONE]) break;
@@ -485,11 +506,20 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   res.stagnated = h->host_flags[F_BREAKDOWN] == 2;
   res.converged = h->host_flags[F_DONE] != 0 && !res.breakdown;
   res.rel = h->host_scal[S_REL];
-  // the iteration that set the stop flag skipped its direction update, and
-  // with it x += alpha p (alpha 0 after a breakdown)
-  if (h->host_flags[F_DONE])
-    hipLaunchKernelGGL(k_axpy_x<T>, dim3(kVecBlocks), dim3(256), 0, s, x, (const T*)v.p, n,
-                       (const double*)h->scal);
+  // the pending alpha_k p_k: alphas were formed by `a` iterations, of which
+  // all but a stopping one ran their direction update.  Deferred, the window
+  // flushes ran at every kDirSlots-th of those and the last, partial window
+  // is pending; otherwise only the stopping iteration's.  A breakdown
+  // publishes alpha 0.
+  const int a = h->host_flags[F_ITERS];
+  const int dirs = h->host_flags[F_DONE] ? std::max(0, a - 1) : a;
+  const int a0 = defer ? 0 : (a - 1 + kDirSlots) % kDirSlots;
+  const int count = defer ? a - (dirs / kDirSlots) * kDirSlots : a - dirs;
+  if (count > 0)
+    hipLaunchKernelGGL(k_flush_x<T>, dim3(kVecBlocks), dim3(256), 0, s, x, (const T*)v.ring, ld,
+                       (const double*)h->alring, a0, defer ? 0 : -1, count, n);
+  // p_dirs, the last direction formed, for a resumed run
+  h->ring_last[tslot] = defer ? dirs % kDirSlots : 0;
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
