@@ -133,6 +133,9 @@ struct Handle {
   double* x64 = nullptr;        // copy of densities
   double* invd64 = nullptr;     // per DOF
   float* invd32 = nullptr;
+  float* esum32 = nullptr;      // per node (3 values per node only): FP32 sum of E_e over
+                                // the node's elements, from which the FP32 CG update
+                                // forms invd32 on the fly (jacobi_inv32)
   bool density_set = false;
 
   // deflation
@@ -463,6 +466,25 @@ __device__ __forceinline__ void wave_flush6(const double acc[6], int gid, double
 }
 
 // Node index -> (ix, iy, iz) without integer division by runtime values.
+// The FP32 Jacobi inverse of one DOF from its node's FP32 sum of E_e and
+// the element diagonal entry kd of its component: 0 for a fixed DOF,
+// d = kd s with |d| < 1e-12 -> 1 (src/linear_solvers.py:84-85), 1/d.  d
+// (0 when fixed) is returned too.  invd32 is written with it and the FP32 CG
+// update forms it again, so both hold the same bits.
+__device__ __forceinline__ float jacobi_inv32(float kd, float s, bool fixed, float& d) {
+  float dd = kd * s;
+  if (fabsf(dd) < 1e-12f) dd = 1.0f;
+  d = fixed ? 0.0f : dd;
+  return fixed ? 0.0f : 1.0f / dd;
+}
+
+// Per-node source of the FP32 Jacobi (jacobi_inv32).
+struct JacobiNodes {
+  const float* esum;      // per node
+  const uint8_t* mask;    // per node, fixed bits
+  float kd0, kd1, kd2;
+};
+
 __device__ __forceinline__ void node_coords(int64_t n, const GridDims& g, int& ix, int& iy,
                                             int& iz) {
   const double plane = (double)g.sx * (double)g.sy;

@@ -70,13 +70,17 @@ struct Vec4<double> {
 // direction update applies (or, for the iteration that stops the solve, the
 // host after the loop).  With pKp <= 0 or NaN the block leaves z untouched
 // and alpha is published as 0 (the scalar step flags the breakdown).  Vector
-// loads of 16 bytes; block 0 takes the tail.
-template <typename T>
+// loads of 16 bytes; block 0 takes the tail.  NODE (FP32, 3 values per
+// node): invd is not read; each DOF's inverse and d = 1/invd come from its
+// node's sum of E_e and fixed bits (jn, jacobi_inv32: the bits of invd32),
+// 5 bytes per node instead of 12, and r^T M^-1 r sums z^2 d.
+template <typename T, bool NODE>
 __global__ __launch_bounds__(256) void k_cg_update_z(
-    T* __restrict__ z, const T* __restrict__ q, const T* __restrict__ invd, int64_t n,
-    double* __restrict__ scal, const int* __restrict__ flags,
+    T* __restrict__ z, const T* __restrict__ q, const T* __restrict__ invd, JacobiNodes jn,
+    int64_t n, double* __restrict__ scal, const int* __restrict__ flags,
     const double* __restrict__ part_pq, int npq, double* __restrict__ part_rz,
     double* __restrict__ al_slot) {
+  static_assert(!NODE || sizeof(T) == 4, "the node form is the FP32 update");
   if (flags[F_DONE]) return;
   __shared__ double red[5];
   const double pkp = block_sum_partials(part_pq, npq, red);
@@ -92,6 +96,14 @@ __global__ __launch_bounds__(256) void k_cg_update_z(
       zi = (T)((double)zi - alpha * ((double)di * (double)qi));
       if (di > T(0)) acc += (double)zi * (double)(zi / di);
     };
+    // node form: the DOF's inverse and d from its node's sum and fixed bits
+    auto nlane = [&](T& zi, const T qi, float s, uint8_t m, int c) {
+      const float kd = c == 0 ? jn.kd0 : (c == 1 ? jn.kd1 : jn.kd2);
+      float d;
+      const float di = jacobi_inv32(kd, s, (m >> c) & 1, d);
+      zi = (T)((double)zi - alpha * ((double)di * (double)qi));
+      acc += (double)zi * (double)zi * (double)d;
+    };
     using V = typename Vec4<T>::type;
     constexpr int W = Vec4<T>::width;
     const int64_t nv = n / W;
@@ -101,17 +113,39 @@ __global__ __launch_bounds__(256) void k_cg_update_z(
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
          i += (int64_t)gridDim.x * blockDim.x) {
       V zz = zv[i];
-      const V qq = qv[i], dd = dv[i];
-      lane(zz.x, qq.x, dd.x);
-      lane(zz.y, qq.y, dd.y);
-      if constexpr (W == 4) {
-        lane(zz.z, qq.z, dd.z);
-        lane(zz.w, qq.w, dd.w);
+      const V qq = qv[i];
+      if constexpr (NODE && W == 4) {
+        // DOFs 4i..4i+3 lie on nodes n0 and n0 + 1 (components c0.. and 0..)
+        const int64_t n0 = (4 * i) / 3;
+        const int c0 = (int)(4 * i - 3 * n0);
+        const float s0 = jn.esum[n0], s1 = jn.esum[n0 + 1];
+        const uint8_t m0 = jn.mask[n0], m1 = jn.mask[n0 + 1];
+        T* zl[4] = {&zz.x, &zz.y, &zz.z, &zz.w};
+        const T ql[4] = {qq.x, qq.y, qq.z, qq.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int c = c0 + j;
+          const bool second = c >= 3;
+          nlane(*zl[j], ql[j], second ? s1 : s0, second ? m1 : m0, second ? c - 3 : c);
+        }
+      } else {
+        const V dd = dv[i];
+        lane(zz.x, qq.x, dd.x);
+        lane(zz.y, qq.y, dd.y);
+        if constexpr (W == 4) {
+          lane(zz.z, qq.z, dd.z);
+          lane(zz.w, qq.w, dd.w);
+        }
       }
       zv[i] = zz;
     }
     if (blockIdx.x == 0)
-      for (int64_t i = nv * W + threadIdx.x; i < n; i += blockDim.x) lane(z[i], q[i], invd[i]);
+      for (int64_t i = nv * W + threadIdx.x; i < n; i += blockDim.x) {
+        if constexpr (NODE)
+          nlane(z[i], q[i], jn.esum[i / 3], jn.mask[i / 3], (int)(i % 3));
+        else
+          lane(z[i], q[i], invd[i]);
+      }
   }
   __shared__ double red4[4];
   const double bs = block_reduce_sum256(acc, red4);
@@ -436,6 +470,11 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     return PYTO_OK;
   }
   const int* done = h->flags + F_DONE;
+  // FP32 with 3 values per node: the CG update forms the Jacobi from the
+  // node sums (k_cg_update_z NODE)
+  const bool node_jacobi = sizeof(T) == 4 && h->esum32 != nullptr;
+  const JacobiNodes jn{h->esum32, h->fixmask, (float)h->kediag[0], (float)h->kediag[1],
+                       (float)h->kediag[2]};
   // one iteration's launches on stream `st`
   // iteration k (from 0 in each run): p_k from slot k, p_{k+1} to slot k+1
   auto one_iteration = [&](hipStream_t st, int k) -> int {
@@ -443,9 +482,18 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     T* pk1 = slot(k + 1);
     const int pos = k % kDirSlots;
     PYTO_TRY(launch_matvec<T>(h, pk, v.q, v.Ee, v.mc, part_pq, done, nullptr, 0, st));
-    hipLaunchKernelGGL(k_cg_update_z<T>, dim3(kVecBlocks), dim3(256), 0, st, v.r, v.q, v.invd,
-                       n, h->scal, (const int*)h->flags, (const double*)part_pq, npq, part_rz,
-                       h->alring + pos);
+    bool updated = false;
+    if constexpr (sizeof(T) == 4)
+      if (node_jacobi) {
+        hipLaunchKernelGGL((k_cg_update_z<T, true>), dim3(kVecBlocks), dim3(256), 0, st, v.r,
+                           v.q, v.invd, jn, n, h->scal, (const int*)h->flags,
+                           (const double*)part_pq, npq, part_rz, h->alring + pos);
+        updated = true;
+      }
+    if (!updated)
+      hipLaunchKernelGGL((k_cg_update_z<T, false>), dim3(kVecBlocks), dim3(256), 0, st, v.r, v.q,
+                         v.invd, jn, n, h->scal, (const int*)h->flags, (const double*)part_pq,
+                         npq, part_rz, h->alring + pos);
     if (defl) {
       // beta is formed inside the restriction from the update's partials, so
       // the scalar logic runs beside the per-group gather in one launch
