@@ -898,9 +898,9 @@ struct Batch {
   std::vector<int> h_mpre, h_vpre;
   int* mpre = nullptr;
   int* vpre = nullptr;
-  double* scal = nullptr;
-  int* flags = nullptr;
-  double* host_scal = nullptr;
+  double* scal = nullptr;          // nsys x kNumScalars, then the flags
+  int* flags = nullptr;            // nsys x kNumFlags, inside scal's allocation
+  double* host_scal = nullptr;     // pinned mirror of both (nsys x kCtlBytes)
   int* host_flags = nullptr;
   double* part_mv = nullptr;
   double* part_v = nullptr;
@@ -1070,10 +1070,8 @@ static int batch_iteration(Batch* b, hipStream_t s) {
 }
 
 static int batch_read(Batch* b, hipStream_t s) {
-  PYTO_CHECK_HIP(hipMemcpyAsync(b->host_flags, b->flags, (size_t)b->nsys * kNumFlags * sizeof(int),
-                                hipMemcpyDeviceToHost, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(b->host_scal, b->scal,
-                                (size_t)b->nsys * kNumScalars * sizeof(double),
+  // every system's scalars, then every system's flags: one block
+  PYTO_CHECK_HIP(hipMemcpyAsync(b->host_scal, b->scal, (size_t)b->nsys * kCtlBytes,
                                 hipMemcpyDeviceToHost, s));
   PYTO_CHECK_HIP(hipStreamSynchronize(s));
   return PYTO_OK;
@@ -1250,15 +1248,15 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
     if (rc) return fail(rc);
   }
   if (hipMalloc((void**)&b->sys, nsys * sizeof(BatchSys)) != hipSuccess ||
-      hipMalloc((void**)&b->scal, (size_t)nsys * kNumScalars * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&b->flags, (size_t)nsys * kNumFlags * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void**)&b->host_scal, (size_t)nsys * kNumScalars * sizeof(double),
-                    hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&b->host_flags, (size_t)nsys * kNumFlags * sizeof(int),
-                    hipHostMallocDefault) != hipSuccess) {
+      hipMalloc((void**)&b->scal, (size_t)nsys * kCtlBytes) != hipSuccess ||
+      hipHostMalloc((void**)&b->host_scal, (size_t)nsys * kCtlBytes, hipHostMallocDefault) !=
+          hipSuccess) {
     set_error("batch allocation failed");
     return fail(PYTO_ENOMEM);
   }
+  // the flags of all systems follow their scalars in the same allocations
+  b->flags = reinterpret_cast<int*>(b->scal + (size_t)nsys * kNumScalars);
+  b->host_flags = reinterpret_cast<int*>(b->host_scal + (size_t)nsys * kNumScalars);
   hipMemsetAsync(b->scal, 0, (size_t)nsys * kNumScalars * sizeof(double), s);
   hipMemsetAsync(b->flags, 0, (size_t)nsys * kNumFlags * sizeof(int), s);
   int rc = batch_geometry(b, 16, kTile, s);
@@ -1281,7 +1279,7 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
 int pyto_batch_destroy(pyto_batch* b) {
   if (!b) return PYTO_OK;
   for (void* p : b->bufs) hipFree(p);
-  void* ptrs[] = {b->sys, b->mpre, b->vpre, b->scal, b->flags, b->part_mv, b->part_v,
+  void* ptrs[] = {b->sys, b->mpre, b->vpre, b->scal /* and flags */, b->part_mv, b->part_v,
                   b->rpre, b->gpre, b->npre, b->dpre, b->drun};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -1289,8 +1287,7 @@ int pyto_batch_destroy(pyto_batch* b) {
   if (b->ev_a) hipEventDestroy(b->ev_a);
   if (b->ev_b) hipEventDestroy(b->ev_b);
   if (b->gstream) hipStreamDestroy(b->gstream);
-  if (b->host_scal) hipHostFree(b->host_scal);
-  if (b->host_flags) hipHostFree(b->host_flags);
+  if (b->host_scal) hipHostFree(b->host_scal);   // holds host_flags too
   delete b;
   return PYTO_OK;
 }

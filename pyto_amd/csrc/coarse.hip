@@ -268,6 +268,32 @@ __global__ void k_special_E(const SpecialSlot* __restrict__ pslot, int64_t n,
   pE32[i] = e >= 0 ? E32[e] : 0.f;
 }
 
+// acc = this lane's share of a group's chunk partials: cpart[6 gidx[k] ..]
+// for k = k0 + lane, k0 + lane + 64, ... below k1, added in that order.  The
+// index and partial loads go out 4 trips at a time (a plain loop waits for
+// both loads of every trip in turn).
+__device__ __forceinline__ void group_partials_sum(const double* __restrict__ cpart,
+                                                   const int32_t* __restrict__ gidx, int k0,
+                                                   int k1, int lane, double acc[6]) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) acc[j] = 0.0;
+  for (int k = k0 + lane; k < k1; k += 4 * 64) {
+    int id[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) id[u] = k + 64 * u < k1 ? gidx[k + 64 * u] : -1;
+    double c[4][6];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) c[u][j] = id[u] >= 0 ? cpart[(int64_t)id[u] * 6 + j] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (id[u] >= 0)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc[j] += c[u][j];
+  }
+}
+
 // t[6g..6g+5] = sum of the chunk partials of group g: one wave per group,
 // lanes stride over the group's partials, then a wave reduction (fixed
 // order).  Replaces atomics: tens of thousands of FP64 atomics on a few
@@ -281,12 +307,8 @@ __global__ __launch_bounds__(256) void k_coarse_gather(const double* __restrict_
   const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (g >= ngroups) return;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = gptr[g] + lane; k < gptr[g + 1]; k += 64) {
-    const double* c = cpart + (int64_t)gidx[k] * 6;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) acc[j] += c[j];
-  }
+  double acc[6];
+  group_partials_sum(cpart, gidx, gptr[g], gptr[g + 1], lane, acc);
   const double s = wave_reduce6(acc);
   if ((lane & 7) == 0 && lane < 48) t[6 * g + (lane >> 3)] = s;
 }
@@ -319,12 +341,8 @@ __global__ __launch_bounds__(256) void k_gather_scalars(
   const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (g >= ngroups) return;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = gptr[g] + lane; k < gptr[g + 1]; k += 64) {
-    const double* c = cpart + (int64_t)gidx[k] * 6;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) acc[j] += c[j];
-  }
+  double acc[6];
+  group_partials_sum(cpart, gidx, gptr[g], gptr[g + 1], lane, acc);
   const double sj = wave_reduce6(acc);
   if ((lane & 7) == 0 && lane < 48) t[6 * g + (lane >> 3)] = sj;
 }

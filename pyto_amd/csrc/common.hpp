@@ -198,10 +198,10 @@ struct Handle {
   double* alring = nullptr;
   int ring_last[2] = {0, 0};
   unsigned* counters = nullptr; // last-block counters (kNumCounters)
-  double* scal = nullptr;       // device scalars (kNumScalars)
-  int* flags = nullptr;         // device flags (kNumFlags)
-  double* host_scal = nullptr;  // pinned mirror of scal
-  int* host_flags = nullptr;    // pinned mirror of flags
+  double* scal = nullptr;       // device scalars (kNumScalars), then the flags
+  int* flags = nullptr;         // device flags (kNumFlags), inside scal's allocation
+  double* host_scal = nullptr;  // pinned mirror of scal and flags (kCtlBytes)
+  int* host_flags = nullptr;    // inside host_scal's allocation
   double* elem_buf = nullptr;   // N_el x 6 FP64 element workspace
   void* stencil = nullptr;      // cone-filter stencil on the device (postproc.hip)
   float* ke32 = nullptr;        // dense FP32 KE for the MFMA operator variant (lazy)
@@ -241,6 +241,8 @@ struct DirIO {
 constexpr int kNumCounters = 16;
 constexpr int kNumScalars = 32;
 constexpr int kNumFlags = 10;
+// a handle's device scalars followed by its flags, one allocation
+constexpr size_t kCtlBytes = kNumScalars * sizeof(double) + kNumFlags * sizeof(int);
 
 // Device scalar slots.
 enum ScalarSlot {
@@ -345,10 +347,28 @@ __device__ __forceinline__ double block_reduce_sum256(double v, double* lds4) {
 // returned to every thread (blockDim.x == 256).  Kernel boundaries make the
 // partials visible, so no fence or atomic is involved; every block that
 // calls this gets bit-identical results.
+// One thread's strided share of n partials, part[t], part[t + stride], ...,
+// summed in that order.  The loads go out 8 at a time before they are added
+// (a loop of load-then-add with a run-time trip count waits for each load,
+// and for every load issued before it, in turn).
+__device__ __forceinline__ double strided_sum(const double* __restrict__ part, int n, int t,
+                                              int stride) {
+  double s = 0.0;
+  int i = t;
+  for (; i + 7 * stride < n; i += 8 * stride) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = part[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  for (; i < n; i += stride) s += part[i];
+  return s;
+}
+
 __device__ __forceinline__ double block_sum_partials(const double* __restrict__ part, int n,
                                                      double* lds5) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  double s = strided_sum(part, n, threadIdx.x, 256);
   s = wave_reduce_sum(s);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
@@ -396,9 +416,8 @@ __device__ __forceinline__ void pcg_scalar_step(double* sc, int* fl, double pkp,
 __device__ __forceinline__ void block_sum_partials2(const double* __restrict__ pa, int na,
                                                     const double* __restrict__ pb, int nb,
                                                     double* lds10, double& sa, double& sb) {
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < na; i += 256) a += pa[i];
-  for (int i = threadIdx.x; i < nb; i += 256) b += pb[i];
+  double a = strided_sum(pa, na, threadIdx.x, 256);
+  double b = strided_sum(pb, nb, threadIdx.x, 256);
   a = wave_reduce_sum(a);
   b = wave_reduce_sum(b);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -434,9 +453,7 @@ __device__ __forceinline__ void grid_reduce_finish(double block_sum, double* par
   __syncthreads();
   if (!am_last) return;
   __threadfence();
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
-    s += partials[i];
+  double s = strided_sum(partials, nblocks, threadIdx.x, blockDim.x);
   __shared__ double red[4];
   double tot = block_reduce_sum256(s, red);
   if (threadIdx.x == 0) {

@@ -352,15 +352,16 @@ static int create_common(pyto_handle* h, int nelx, int nely, int nelz, double hx
       (nc == 3 && (rc = device_alloc(h, (void**)&h->esum32, nn * 4))) ||
       (rc = device_alloc(h, (void**)&h->partials, 2 * kMaxPartials * 8)) ||
       (rc = device_alloc(h, (void**)&h->counters, kNumCounters * sizeof(unsigned))) ||
-      (rc = device_alloc(h, (void**)&h->scal, kNumScalars * 8)) ||
-      (rc = device_alloc(h, (void**)&h->flags, kNumFlags * sizeof(int))))
+      (rc = device_alloc(h, (void**)&h->scal, kCtlBytes)))
     return rc;
-  if (hipHostMalloc((void**)&h->host_scal, kNumScalars * 8, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&h->host_flags, kNumFlags * sizeof(int), hipHostMallocDefault) !=
-          hipSuccess) {
+  // scalars and flags in one block (device and host), so that the solver
+  // reads both back with one copy
+  h->flags = reinterpret_cast<int*>(h->scal + kNumScalars);
+  if (hipHostMalloc((void**)&h->host_scal, kCtlBytes, hipHostMallocDefault) != hipSuccess) {
     set_error("hipHostMalloc failed");
     return PYTO_ENOMEM;
   }
+  h->host_flags = reinterpret_cast<int*>(h->host_scal + kNumScalars);
   if (hipMemcpyAsync(h->fixmask, fixed_mask, nn, hipMemcpyDeviceToDevice, s) != hipSuccess ||
       launch_tile_flags(h, s) != hipSuccess ||
       hipMemsetAsync(h->counters, 0, kNumCounters * sizeof(unsigned), s) != hipSuccess ||
@@ -483,7 +484,7 @@ int pyto_destroy(pyto_handle* h) {
                   h->gptr, h->gidx, h->gnodes, h->nchunk,
                   h->nchunk_grp, h->npart, h->nptr, h->nidx, h->Ec, h->Einv, h->cwork, h->ct,
                   h->cmu, h->cres, h->r64, h->q64, h->r32, h->q32, h->x32,
-                  h->b32, h->partials, h->counters, h->scal, h->flags, h->elem_buf,
+                  h->b32, h->partials, h->counters, h->scal /* and flags */, h->elem_buf,
                   h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
                   h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32, h->stencil,
                   h->ke32, h->pring64, h->pring32, h->alring};
@@ -495,8 +496,7 @@ int pyto_destroy(pyto_handle* h) {
   if (h->ev_a) hipEventDestroy(h->ev_a);
   if (h->ev_b) hipEventDestroy(h->ev_b);
   if (h->gstream) hipStreamDestroy(h->gstream);
-  if (h->host_scal) hipHostFree(h->host_scal);
-  if (h->host_flags) hipHostFree(h->host_flags);
+  if (h->host_scal) hipHostFree(h->host_scal);   // holds host_flags too
   delete h;
   return PYTO_OK;
 }

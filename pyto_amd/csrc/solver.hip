@@ -315,10 +315,8 @@ struct DpcgResult {
 };
 
 static int read_back(Handle* h, hipStream_t s) {
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->host_flags, h->flags, kNumFlags * sizeof(int),
-                                hipMemcpyDeviceToHost, s));
-  PYTO_CHECK_HIP(hipMemcpyAsync(h->host_scal, h->scal, kNumScalars * sizeof(double),
-                                hipMemcpyDeviceToHost, s));
+  // scalars and flags are one block (kCtlBytes)
+  PYTO_CHECK_HIP(hipMemcpyAsync(h->host_scal, h->scal, kCtlBytes, hipMemcpyDeviceToHost, s));
   PYTO_CHECK_HIP(hipStreamSynchronize(s));
   return PYTO_OK;
 }
