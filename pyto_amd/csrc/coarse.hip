@@ -389,6 +389,29 @@ struct DirVec<double> {
   }
 };
 
+// Non-temporal 16-byte accesses, for x: the direction update reads and
+// writes it once per iteration (FP64: once per window) and nothing reads it
+// in between, so its lines are not kept; the vectors the next kernels read
+// (p, q, z, the element moduli) then stay in the last-level cache.
+typedef float pyto_f4 __attribute__((ext_vector_type(4)));
+typedef double pyto_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 load_nt(const float4* a) {
+  const pyto_f4 t = __builtin_nontemporal_load(reinterpret_cast<const pyto_f4*>(a));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+__device__ __forceinline__ double2 load_nt(const double2* a) {
+  const pyto_d2 t = __builtin_nontemporal_load(reinterpret_cast<const pyto_d2*>(a));
+  return make_double2(t.x, t.y);
+}
+__device__ __forceinline__ void store_nt(float4* a, const float4& v) {
+  const pyto_f4 t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<pyto_f4*>(a));
+}
+__device__ __forceinline__ void store_nt(double2* a, const double2& v) {
+  const pyto_d2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<pyto_d2*>(a));
+}
+
 constexpr int kLdsGroups = 512;
 // Largest dense coarse space: 6 x 4096 = 24576 unknowns, whose FP64 matrix
 // and inverse take 4.8 GB each.  The reference's own heuristic asks for at
@@ -499,7 +522,7 @@ __global__ __launch_bounds__(256) void k_dir_update(
     const V zz = zv[i];
     if (FLUSH) {
       // x += the deferred alpha_s p_s of the window (FP64 sum)
-      V xx = xv[i];
+      V xx = load_nt(xv + i);
       double acc[W];
 #pragma unroll
       for (int j = 0; j < W; ++j) acc[j] = (double)DV::get(xx, j);
@@ -516,13 +539,13 @@ __global__ __launch_bounds__(256) void k_dir_update(
       }
 #pragma unroll
       for (int j = 0; j < W; ++j) DV::set(xx, j, (T)acc[j]);
-      xv[i] = xx;
+      store_nt(xv + i, xx);
     } else if (alpha != 0.0) {
-      V xx = xv[i];
+      V xx = load_nt(xv + i);
 #pragma unroll
       for (int j = 0; j < W; ++j)
         DV::set(xx, j, (T)((double)DV::get(xx, j) + alpha * (double)DV::get(pp, j)));
-      xv[i] = xx;
+      store_nt(xv + i, xx);
     }
 #pragma unroll
     for (int j = 0; j < W; ++j) {
@@ -658,7 +681,7 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
       // x += the deferred alpha_s p_s of the window (FP64 sum)
       double acc[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) acc[c] = (double)xn[c];
+      for (int c = 0; c < 3; ++c) acc[c] = (double)__builtin_nontemporal_load(xn + c);
 #pragma unroll
       for (int sl = 0; sl < kDirSlots; ++sl) {
         if (sl < io.nflush) {
@@ -668,13 +691,14 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
         }
       }
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xn[c] = (T)acc[c];
+      for (int c = 0; c < 3; ++c) __builtin_nontemporal_store((T)acc[c], xn + c);
     } else if (alpha != 0.0) {
       T xx[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xx[c] = xn[c];
+      for (int c = 0; c < 3; ++c) xx[c] = __builtin_nontemporal_load(xn + c);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xn[c] = (T)((double)xx[c] + alpha * (double)pp[c]);
+      for (int c = 0; c < 3; ++c)
+        __builtin_nontemporal_store((T)((double)xx[c] + alpha * (double)pp[c]), xn + c);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {

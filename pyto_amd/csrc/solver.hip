@@ -110,14 +110,19 @@ __global__ __launch_bounds__(256) void k_cg_update_z(
     V* zv = reinterpret_cast<V*>(z);
     const V* qv = reinterpret_cast<const V*>(q);
     const V* dv = reinterpret_cast<const V*>(invd);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // node form: DOFs 4i..4i+3 lie on nodes n0 and n0 + 1 (components c0..
+    // and 0..); n0 and c0 are formed once and advanced by the loop's
+    // constant step (4 stride DOFs = kn nodes + kc components)
+    int64_t n0 = (W * i) / 3;
+    int c0 = (int)(W * i - 3 * n0);
+    const int64_t kn = (W * stride) / 3;
+    const int kc = (int)(W * stride - 3 * kn);
+    for (; i < nv; i += stride) {
       V zz = zv[i];
       const V qq = qv[i];
       if constexpr (NODE && W == 4) {
-        // DOFs 4i..4i+3 lie on nodes n0 and n0 + 1 (components c0.. and 0..)
-        const int64_t n0 = (4 * i) / 3;
-        const int c0 = (int)(4 * i - 3 * n0);
         const float s0 = jn.esum[n0], s1 = jn.esum[n0 + 1];
         const uint8_t m0 = jn.mask[n0], m1 = jn.mask[n0 + 1];
         T* zl[4] = {&zz.x, &zz.y, &zz.z, &zz.w};
@@ -128,6 +133,10 @@ __global__ __launch_bounds__(256) void k_cg_update_z(
           const bool second = c >= 3;
           nlane(*zl[j], ql[j], second ? s1 : s0, second ? m1 : m0, second ? c - 3 : c);
         }
+        c0 += kc;
+        const int carry = c0 >= 3 ? 1 : 0;
+        c0 -= 3 * carry;
+        n0 += kn + carry;
       } else {
         const V dd = dv[i];
         lane(zz.x, qq.x, dd.x);
