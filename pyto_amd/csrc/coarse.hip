@@ -1265,12 +1265,6 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
                "groups; use fewer, larger deflation groups", ngroups, 6 * ngroups,
                kMaxCoarseGroups);
   hipStream_t s = as_stream(stream);
-  // captured iteration graphs hold the old coarse-space pointers
-  for (auto& gc : h->graphs)
-    if (gc.exec) {
-      hipGraphExecDestroy(gc.exec);
-      gc.exec = nullptr;
-    }
   release_coarse(h);
   if (ngroups == 0) return PYTO_OK;
   PYTO_REQUIRE(node_group && centers, PYTO_EINVAL, "null group arrays");

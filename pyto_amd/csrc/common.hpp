@@ -108,12 +108,6 @@ struct SpecialSlot {
   uint4 li;
 };
 
-struct GraphCache {
-  hipGraphExec_t exec = nullptr;
-  const void* x = nullptr;
-  int ngroups = -1, nchunks = -1;
-};
-
 struct Handle {
   int device = 0;
   GridDims g{};
@@ -206,9 +200,10 @@ struct Handle {
   void* stencil = nullptr;      // cone-filter stencil on the device (postproc.hip)
   float* ke32 = nullptr;        // dense FP32 KE for the MFMA operator variant (lazy)
   double stencil_rel = 0.0;     // its radius in units of hx (0: none built yet)
-  hipStream_t gstream = nullptr;  // capture/replay stream (default priority)
-  hipEvent_t ev_a = nullptr, ev_b = nullptr;
-  GraphCache graphs[2];           // FP64, FP32 iteration batches
+  // the DPCG's read-backs of the control block, one per batch of iterations
+  // in flight: two pinned copies (2 x kCtlBytes) and their events
+  double* host_ctl = nullptr;
+  hipEvent_t ev_ctl[2] = {nullptr, nullptr};
   int64_t workspace_bytes = 0;
 };
 

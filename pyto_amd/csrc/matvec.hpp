@@ -408,10 +408,13 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
       return true;
     }
     ModalN<T, NC> F;
-    // wave-uniform test only: a column outside the grid applies E = 0 (its
-    // corner values are 0 as well), so no per-lane merge of the results
-    if (k >= 0 && k < g.nz) {
-      const T E = col_valid ? es[b][ty][tx] : T(0);   // E (or k) of layer k, staged with plane k+1
+    // No branch: an element layer outside the grid (k = -1 or nz at the
+    // chunk ends) and a column outside it apply E = 0, and their corner
+    // values are 0 as well (masked when staged), so F is 0 there.  A branch
+    // on the wave-uniform layer test made the compiler zero the 21 modal
+    // forces before it on every layer.
+    {
+      const T E = (col_valid && k >= 0 && k < g.nz) ? es[b][ty][tx] : T(0);   // E (or k) of layer k, staged with plane k+1
       if constexpr (NC == 1) {
         thermal_modal_faces<T>(bface, tface, E, a.mc, F);
       } else if constexpr (kFaces) {
@@ -427,10 +430,6 @@ __device__ __forceinline__ void matvec_tile(const MatvecArgs<T>& a, int lb, int 
           }
         element_modal<T, CUBE>(ue, E, a.mc, F);
       }
-    } else {
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-        F.cx[c] = F.cy[c] = F.cz[c] = F.cxy[c] = F.cxz[c] = F.cyz[c] = F.cxyz[c] = T(0);
     }
     // node plane k from this layer's bottom face and the last layer's top
     // face; x-neighbour sums (DPP) for the nodes (tx,ty) and (tx,ty+1)

@@ -157,12 +157,6 @@ int set_tile_rows(Handle* h, int rows, hipStream_t s) {
   PYTO_REQUIRE(rows == 8 || rows == 16 || rows == 32, PYTO_EINVAL,
                "operator tile rows must be 8, 16 or 32 (got %d)", rows);
   if (rows == h->tile_rows) return PYTO_OK;
-  // captured iteration graphs hold the old launch shape
-  for (auto& gc : h->graphs)
-    if (gc.exec) {
-      hipGraphExecDestroy(gc.exec);
-      gc.exec = nullptr;
-    }
   h->tile_rows = rows;
   PYTO_CHECK_HIP(launch_tile_flags(h, s));
   PYTO_CHECK_HIP(hipStreamSynchronize(s));
@@ -490,12 +484,10 @@ int pyto_destroy(pyto_handle* h) {
                   h->ke32, h->pring64, h->pring32, h->alring};
   for (void* p : ptrs)
     if (p) hipFree(p);
-  for (auto& gc : h->graphs)
-    if (gc.exec) hipGraphExecDestroy(gc.exec);
   release_blas(h);
-  if (h->ev_a) hipEventDestroy(h->ev_a);
-  if (h->ev_b) hipEventDestroy(h->ev_b);
-  if (h->gstream) hipStreamDestroy(h->gstream);
+  for (hipEvent_t e : h->ev_ctl)
+    if (e) hipEventDestroy(e);
+  if (h->host_ctl) hipHostFree(h->host_ctl);
   if (h->host_scal) hipHostFree(h->host_scal);   // holds host_flags too
   delete h;
   return PYTO_OK;

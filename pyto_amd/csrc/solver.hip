@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 #include "element.hpp"
 
@@ -277,15 +278,6 @@ __global__ __launch_bounds__(256) void k_norm2_diff(const double* __restrict__ a
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
-__global__ void k_set_flags(int* flags, int done, int iters) {
-  flags[F_DONE] = done;
-  flags[F_ITERS] = iters;
-  flags[F_BREAKDOWN] = 0;
-  flags[F_STALL] = 0;
-}
-
-__global__ void k_set_scalar(double* scal, int slot, double v) { scal[slot] = v; }
-
 __global__ void k_resume_scalars(double* scal) {
   scal[S_BETA] = scal[S_RZ] / scal[S_RZPREV];
   scal[S_ALPHA_X] = 0.0;
@@ -330,53 +322,50 @@ static int read_back(Handle* h, hipStream_t s) {
   return PYTO_OK;
 }
 
-constexpr int kGraphBatch = 16;
+// Iteration batches.  The host enqueues the launches of a batch of
+// iterations directly on the caller's stream, then a copy of the control
+// block into pinned memory and an event, and keeps the next batch queued
+// while it waits for the previous one's copy, so the GPU does not idle at
+// batch boundaries.  Far from the stop a batch is kFarBatch iterations; once
+// the convergence rate seen in the read-backs puts the stop within
+// kNearWindow iterations, batches shrink to kNearBatch, so at most about two
+// such batches run after the stopping iteration (their launches return at
+// once on the device flag, a few microseconds each).
+constexpr int kFarBatch = 16;
+constexpr int kNearBatch = 2;
+constexpr int kNearWindow = 24;
+constexpr size_t kCtlStride = 512;   // bytes between the two read-back slots
+static_assert(kCtlBytes <= kCtlStride, "control block fits a read-back slot");
 
-// The handle's capture/replay stream (default priority) and its events,
-// created on first use.
-static int ensure_streams(Handle* h) {
-  if (!h->gstream) {
-    PYTO_CHECK_HIP(hipStreamCreateWithFlags(&h->gstream, hipStreamNonBlocking));
-    PYTO_CHECK_HIP(hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming));
-    PYTO_CHECK_HIP(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
+// The handle's pinned read-back slots and events, created on first use.
+static int ensure_control(Handle* h) {
+  if (!h->host_ctl) {
+    PYTO_CHECK_HIP(hipHostMalloc((void**)&h->host_ctl, 2 * kCtlStride, hipHostMallocDefault));
+    for (hipEvent_t& e : h->ev_ctl)
+      PYTO_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   return PYTO_OK;
 }
+static double* ctl_slot(Handle* h, int slot) {
+  return reinterpret_cast<double*>(reinterpret_cast<char*>(h->host_ctl) + slot * kCtlStride);
+}
+
 // largest ratio of true to recurrence residual norm at which a mixed
 // refinement step continues the previous inner run's search direction
 constexpr double kResumeDrift = 1.25;
 
-// The captured batch of iterations for (precision, x) on this handle: built
-// on first use, reused while the solution buffer stays the same.
-template <typename F>
-static int iteration_graph(Handle* h, const void* x, int tsize, F&& one_iteration,
-                           hipGraphExec_t* out) {
-  PYTO_TRY(ensure_streams(h));
-  const int slot = tsize == 8 ? 0 : 1;
-  GraphCache& gc = h->graphs[slot];
-  if (gc.exec && gc.x == x && gc.ngroups == h->ngroups && gc.nchunks == h->nchunks) {
-    *out = gc.exec;
-    return PYTO_OK;
-  }
-  if (gc.exec) {
-    hipGraphExecDestroy(gc.exec);
-    gc.exec = nullptr;
-  }
-  hipGraph_t graph;
-  PYTO_CHECK_HIP(hipStreamBeginCapture(h->gstream, hipStreamCaptureModeThreadLocal));
-  int rc = PYTO_OK;
-  for (int k = 0; k < kGraphBatch && rc == PYTO_OK; ++k) rc = one_iteration(h->gstream, k);
-  hipError_t e = hipStreamEndCapture(h->gstream, &graph);
-  if (rc != PYTO_OK) return rc;
-  PYTO_CHECK_HIP(e);
-  e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
-  hipGraphDestroy(graph);
-  PYTO_CHECK_HIP(e);
-  gc.x = x;
-  gc.ngroups = h->ngroups;
-  gc.nchunks = h->nchunks;
-  *out = gc.exec;
-  return PYTO_OK;
+// First launch of a run: its tolerance, the stagnation state and the flags;
+// a run whose r0^T M r0 is 0 is done before its first iteration.
+__global__ void k_start_run(double* scal, int* flags, double rtol, double floor) {
+  const bool zero = scal[S_RZ0] == 0.0;
+  scal[S_RTOL] = rtol;
+  scal[S_RELMIN] = 1e300;
+  scal[S_FLOOR] = floor;
+  if (zero) scal[S_REL] = 0.0;
+  flags[F_DONE] = zero ? 1 : 0;
+  flags[F_ITERS] = 0;
+  flags[F_BREAKDOWN] = 0;
+  flags[F_STALL] = 0;
 }
 
 // Deflated PCG on K x = b (b and x full-length in precision T; fixed DOFs of
@@ -412,22 +401,20 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   const int64_t n = h->ncomp * h->g.nnodes;
   const bool defl = h->ngroups > 0;
   double* ct = h->ct;
-  // Directions and alphas.  alpha_k goes to alring[k mod kDirSlots] (graph
-  // replays start at multiples of kDirSlots, so every position's slots are
-  // fixed).  FP64 defers x: p_k goes to ring slot k mod kDirSlots and x takes
+  // Directions and alphas.  alpha_k goes to alring[k mod kDirSlots], k
+  // counted from the run's first iteration.  FP64 defers x: p_k goes to ring slot k mod kDirSlots and x takes
   // a window's alpha_k p_k at once, from the direction update of the window's
   // last iteration (the FP64 update moves 40 B per DOF and the x pass is 16
   // of them).  FP32 updates p in place in slot 0 and x on every iteration:
   // its vectors (51 MB each at 12.8M DOF) stay largely in the 256 MB
   // last-level cache, which a rotating ring defeats; deferred, it measured
   // slower.
-  static_assert(kGraphBatch % kDirSlots == 0, "graph batches hold whole windows");
   const bool defer = sizeof(T) == 8;
   const int tslot = sizeof(T) == 8 ? 0 : 1;
   const int64_t ld = ring_stride(n);   // slots start on 16-byte boundaries
   auto slot = [&](int k) { return defer ? v.ring + (int64_t)(k % kDirSlots) * ld : v.ring; };
   T* p_prev = slot(h->ring_last[tslot]);
-  PYTO_TRY(ensure_streams(h));
+  PYTO_TRY(ensure_control(h));
   if (defl) {
     PYTO_TRY(launch_restrict_full<T>(h, b, ct, s));
     PYTO_TRY(coarse_apply(h, ct, nullptr, nullptr, h->cmu, nullptr, s, false));
@@ -464,18 +451,9 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     const DirIO<T> io0{x, p_prev, slot(0), v.r, false, nullptr, 0, nullptr, 0};
     PYTO_TRY(launch_dir_update<T>(h, io0, !resume, nullptr, s));
   }
-  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RTOL, rtol);
-  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_RELMIN, 1e300);
-  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, h->scal, (int)S_FLOOR,
+  hipLaunchKernelGGL(k_start_run, dim3(1), dim3(1), 0, s, h->scal, h->flags, rtol,
                      sizeof(T) == 8 ? kFloor64 : kFloor32);
-  hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(1), 0, s, h->flags, 0, 0);
   PYTO_CHECK_HIP(hipGetLastError());
-  PYTO_TRY(read_back(h, s));
-  if (h->host_scal[S_RZ0] == 0.0) {
-    res.converged = true;
-    res.rel = 0.0;
-    return PYTO_OK;
-  }
   const int* done = h->flags + F_DONE;
   // FP32 with 3 values per node: the CG update forms the Jacobi from the
   // node sums (k_cg_update_z NODE)
@@ -519,42 +497,54 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     PYTO_CHECK_HIP(hipGetLastError());
     return PYTO_OK;
   };
-  int launched = 0;
-  const char* env = getenv("PYTO_NO_GRAPH");
-  const bool use_graph = !(env && env[0] == '1');
-  if (use_graph) {
-    // Batches of kGraphBatch iterations replayed from a captured hipGraph on
-    // the handle's own stream (capture needs a non-null stream), ordered after
-    // and before the caller's stream by events.
-    hipGraphExec_t exec = nullptr;
-    PYTO_TRY(iteration_graph(h, (const void*)x, sizeof(T), one_iteration, &exec));
-    PYTO_CHECK_HIP(hipEventRecord(h->ev_a, s));
-    PYTO_CHECK_HIP(hipStreamWaitEvent(h->gstream, h->ev_a, 0));
-    while (launched < max_iters) {
-      if (max_iters - launched >= kGraphBatch) {
-        PYTO_CHECK_HIP(hipGraphLaunch(exec, h->gstream));
-        launched += kGraphBatch;
-      } else {
-        const int rest = max_iters - launched;
-        for (int k = 0; k < rest; ++k) PYTO_TRY(one_iteration(h->gstream, launched + k));
-        launched += rest;
-      }
-      PYTO_TRY(read_back(h, h->gstream));
-      if (h->host_flags[F_DONE]) break;
+  // Batches of iterations with one read-back each, the next batch queued
+  // behind the one being waited for (see kFirstBatch).  PYTO_FIXED_BATCH=n
+  // (n >= 1) fixes every batch at n iterations; the iterates do not depend
+  // on the batching.
+  const char* env = getenv("PYTO_FIXED_BATCH");
+  const int fixed = env ? std::max(0, atoi(env)) : 0;
+  int enq = 0;             // iterations enqueued
+  int slots[2], nflight = 0, next_slot = 0;
+  int est_end = -1;        // predicted stopping iteration (-1: no estimate)
+  int it_prev = -1;
+  double rel_prev = 0.0;
+  const double* last = nullptr;
+  for (;;) {
+    while (nflight < 2 && enq < max_iters) {
+      const bool near = est_end >= 0 && est_end - enq <= kNearWindow;
+      int nb = fixed ? fixed : near ? kNearBatch : kFarBatch;
+      nb = std::min(nb, max_iters - enq);
+      for (int k = enq; k < enq + nb; ++k) PYTO_TRY(one_iteration(s, k));
+      const int sl = next_slot;
+      next_slot ^= 1;
+      PYTO_CHECK_HIP(hipMemcpyAsync(ctl_slot(h, sl), h->scal, kCtlBytes, hipMemcpyDeviceToHost, s));
+      PYTO_CHECK_HIP(hipEventRecord(h->ev_ctl[sl], s));
+      slots[nflight++] = sl;
+      enq += nb;
     }
-    PYTO_CHECK_HIP(hipEventRecord(h->ev_b, h->gstream));
-    PYTO_CHECK_HIP(hipStreamWaitEvent(s, h->ev_b, 0));
-  } else {
-    int batch = 8;
-    while (launched < max_iters) {
-      const int nb = std::min(batch, max_iters - launched);
-      for (int k = 0; k < nb; ++k) PYTO_TRY(one_iteration(s, launched + k));
-      launched += nb;
-      PYTO_TRY(read_back(h, s));
-      if (h->host_flags[F_DONE]) break;
-      batch = std::min(batch * 2, 32);
+    if (nflight == 0) break;   // max_iters enqueued and read back
+    const int sl = slots[0];
+    slots[0] = slots[1];
+    nflight--;
+    PYTO_CHECK_HIP(hipEventSynchronize(h->ev_ctl[sl]));
+    last = ctl_slot(h, sl);
+    const int* lf = reinterpret_cast<const int*>(last + kNumScalars);
+    if (lf[F_DONE]) break;
+    // log-linear rate of the relative residual between the last two
+    // read-backs; without a decrease, no estimate
+    const int it = lf[F_ITERS];
+    const double rel = last[S_REL];
+    est_end = -1;
+    if (it_prev >= 0 && it > it_prev && rel > 0.0 && rel < rel_prev) {
+      const double need = std::log(rtol / rel) / (std::log(rel / rel_prev) / (it - it_prev));
+      if (need < 1e6) est_end = it + std::max(1, (int)std::ceil(need));
     }
+    it_prev = it;
+    rel_prev = rel;
   }
+  // the read-back that showed the stop (or the last one) holds the final
+  // control block: later batches of the run returned at once
+  if (last) std::memcpy(h->host_scal, last, kCtlBytes);
   res.iterations = h->host_flags[F_ITERS];
   res.matvecs += res.iterations;
   res.breakdown = h->host_flags[F_BREAKDOWN] != 0;

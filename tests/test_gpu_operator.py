@@ -170,7 +170,7 @@ def test_tile_rows_same_result(gpu, dims, h):
 
 def test_tile_rows_thermal_and_solve(gpu):
     """Tile shapes on the thermal operator, and a deflated mixed solve after a
-    shape change (the captured iteration graphs are rebuilt)."""
+    shape change (the solver launches with the new shape)."""
     from pyto_amd.operator import ThermalOperator
     print("tile rows thermal + solve", flush=True)
     dims = (20, 33, 6)

@@ -107,27 +107,32 @@ def test_unattainable_tolerance_stops_as_stagnated(gpu):
     assert rel_err(un[free], ud) <= 1e-8
 
 
-def test_graph_replay_matches_eager(gpu, monkeypatch):
-    """The hipGraph-replayed iteration batches give the eager result: same
-    iteration count, same solution to rounding, over repeated solves with
-    other work in between."""
-    print("graph vs eager", flush=True)
+def test_batching_does_not_change_iterates(gpu, monkeypatch):
+    """The host enqueues iterations in batches sized by the convergence rate,
+    one read-back each, with the next batch queued behind it.  One iteration
+    per batch (a read-back after every iteration, so no iteration runs past
+    the stop) gives the same iteration counts and the same bits as the
+    adaptive batches, over repeated solves with other work in between, for
+    the plain FP64 solve (deferred x updates from the direction ring) and the
+    mixed one (several inner runs)."""
+    print("fixed vs adaptive batches", flush=True)
     c = Case((40, 12, 10), seed=9, lo=0.05)
     op = c.operator(gpu)
     f = torch.from_numpy(c.f).to(gpu)
-    monkeypatch.setenv("PYTO_NO_GRAPH", "1")
+    monkeypatch.setenv("PYTO_FIXED_BATCH", "1")
     u_e, st_e = op.solve(f, rtol=1e-8)
     ue32, st_e32 = op.solve(f, rtol=1e-8, precision="mixed")
-    monkeypatch.setenv("PYTO_NO_GRAPH", "0")
+    monkeypatch.setenv("PYTO_FIXED_BATCH", "0")
     for rep in range(3):
         u_g, st_g = op.solve(f, rtol=1e-8)
         assert st_g["iterations"] == st_e["iterations"]
-        assert rel_err(u_g.cpu().numpy(), u_e.cpu().numpy()) <= 1e-12
-        _ = op.matvec(u_g)       # eager work between replays
+        assert torch.equal(u_g, u_e)
+        _ = op.matvec(u_g)       # other work between solves
         torch.cuda.synchronize()
         ug32, st_g32 = op.solve(f, rtol=1e-8, precision="mixed")
         assert st_g32["iterations"] == st_e32["iterations"]
-        assert rel_err(ug32.cpu().numpy(), ue32.cpu().numpy()) <= 1e-12
+        assert st_g32["outer_iterations"] == st_e32["outer_iterations"]
+        assert torch.equal(ug32, ue32)
 
 
 @pytest.mark.parametrize("grid", [(64, 32, 32), (96, 48, 48)])
