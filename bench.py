@@ -343,8 +343,10 @@ def block_pcg_probe(torch, ops, designs, count=2):
 def thermal_line(torch, dev, grid, x):
     """The thermal operator on the same grid and densities (conductivity law
     k = x, one temperature per node): FP32/FP64 launch times against the
-    algorithmic bytes, and one Jacobi PCG solve to rel res 1e-8 (sink T = 0
-    on x = 0, uniform heat flux into the x = max face)."""
+    algorithmic bytes, one Jacobi PCG solve to rel res 1e-8 (sink T = 0 on
+    x = 0, uniform heat flux into the x = max face) in FP64 and mixed, and the
+    same solves deflated by 8x4x4 and 16x8x8 box groups (one constant per
+    group)."""
     import numpy as np
     from pyto_amd.operator import ThermalOperator
     nelx, nely, nelz = grid
@@ -380,7 +382,7 @@ def thermal_line(torch, dev, grid, x):
     torch.cuda.synchronize()
     res.update({"grid": list(grid), "nodes": nn, "solve_ms": (time.perf_counter() - t0) * 1e3,
                 "iterations": st["iterations"], "converged": bool(st["converged"]),
-                "solver": "Jacobi PCG FP64 (no deflation space on the thermal operator)"})
+                "solver": "Jacobi PCG FP64 (no deflation groups)"})
     # the same solve in FP32 inside FP64 refinement
     op.solve(q, rtol=1e-8, max_iters=100000, precision="mixed")
     torch.cuda.synchronize()
@@ -392,6 +394,32 @@ def thermal_line(torch, dev, grid, x):
                 "mixed_iterations": sm["iterations"], "mixed_refinements": sm["outer_iterations"],
                 "mixed_converged": bool(sm["converged"]),
                 "mixed_compliance_rel_vs_fp64": abs(Cm - C64) / abs(C64)})
+    # the deflated solves: one constant per box group (the thermal coarse
+    # space), density set-up (Galerkin matrix and its inverse) timed apart
+    from pyto_amd import builders
+    op.set_coarse_shift(0.0, 0.0)
+    deflated = []
+    for boxes in ((8, 4, 4), (16, 8, 8)):
+        ngn, cen, _ = builders.box_groups(grid, (1.0, 1.0, 1.0), boxes, 15, dev)
+        op.set_groups(ngn, cen)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.set_density(x)
+        torch.cuda.synchronize()
+        t_set = time.perf_counter() - t0
+        row = {"boxes": list(boxes), "groups": op.ngroups, "density_setup_ms": t_set * 1e3}
+        for prec in ("fp64", "mixed"):
+            op.solve(q, rtol=1e-8, max_iters=100000, precision=prec)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            Td, sd = op.solve(q, rtol=1e-8, max_iters=100000, precision=prec)
+            torch.cuda.synchronize()
+            row[prec] = {"solve_ms": (time.perf_counter() - t0) * 1e3,
+                         "iterations": sd["iterations"], "converged": bool(sd["converged"]),
+                         "compliance_rel_vs_jacobi_fp64":
+                             abs(float(torch.dot(q, Td)) - C64) / abs(C64)}
+        deflated.append(row)
+    res["deflated"] = deflated
     op.close()
     return res
 

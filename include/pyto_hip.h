@@ -126,6 +126,8 @@ int pyto_set_density_batch(int n, pyto_handle* const* handles, const double* con
  * centres (FP64, ngroups x 3).  W is implicit (rigid-body modes about the
  * centres, column map u:{0,4,5} v:{1,3,5} w:{2,3,4}); replaces
  * DeflationSolver.create_deflation_matrix (src/deflation.py:386-458).
+ * On a thermal handle (pyto_create_thermal) W holds one constant per group
+ * instead (ngroups coarse unknowns; the centres are not used).
  * node_group and centers are device pointers; the handle keeps copies. */
 int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
                     const double* centers, void* stream);
@@ -162,7 +164,8 @@ int pyto_matvec_lift_f64(pyto_handle* h, const double* g, double* y, void* strea
  * _jacobi_preconditioner (src/linear_solvers.py:84). */
 int pyto_diagonal_f64(pyto_handle* h, double* d, void* stream);
 
-/* Coarse operators (need groups + density).  restrict: t = W^T v (6G);
+/* Coarse operators (need groups + density).  restrict: t = W^T v (6G, or G
+ * on a thermal handle);
  * prolong: v = W mu (fixed DOFs 0); coarse_solve: mu = E^{-1} t with the
  * FP64 factor; replaces WT@v, W@mu, spsolve(WKW, .) (src/deflation.py:533-551). */
 int pyto_restrict_f64(pyto_handle* h, const double* v, double* t, void* stream);
@@ -175,7 +178,8 @@ int pyto_coarse_solve_f64(pyto_handle* h, const double* t, double* mu, void* str
  * void elements make coarse entries small); abs = rel = 0 gives the exact
  * Galerkin matrix.  Takes effect at the next pyto_set_density. */
 int pyto_set_coarse_shift(pyto_handle* h, double abs_shift, double rel_shift);
-/* Copy the coarse Galerkin matrix (6G x 6G FP64, device) for inspection. */
+/* Copy the coarse Galerkin matrix (6G x 6G FP64, G x G on a thermal handle,
+ * device) for inspection. */
 int pyto_coarse_matrix_f64(pyto_handle* h, double* E, void* stream);
 
 /* Plain FP64 deflated PCG: the reference algorithm of deflatedPCG_CPU

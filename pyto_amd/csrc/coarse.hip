@@ -41,6 +41,12 @@ __device__ __forceinline__ void w_row(int comp, double rx, double ry, double rz,
 __global__ void k_coarse_gather(const double* __restrict__ cpart, const int32_t* __restrict__ gptr,
                                 const int32_t* __restrict__ gidx, int ngroups,
                                 double* __restrict__ t, const int* __restrict__ done);
+__global__ void k_gather_scalars_t(const double* __restrict__ cpart,
+                                   const int32_t* __restrict__ gptr,
+                                   const int32_t* __restrict__ gidx, int ngroups,
+                                   double* __restrict__ t, const double* __restrict__ part_pq,
+                                   int npq, const double* __restrict__ part_rz, int nrz,
+                                   double* __restrict__ scal, int* __restrict__ flags);
 
 // ---------------------------------------------------------------------------
 // t = W^T v over all nodes (fixed DOFs excluded): one block per run of <= 256
@@ -350,6 +356,12 @@ __global__ __launch_bounds__(256) void k_gather_scalars(
 int launch_gather_scalars(Handle* h, double* t, const double* part_pq, int npq,
                           const double* part_rz, int nrz, hipStream_t s) {
   const int gb = (h->ngroups * 64 + 255) / 256;
+  if (h->ncomp == 1) {
+    hipLaunchKernelGGL(k_gather_scalars_t, dim3(gb + 1), dim3(256), 0, s, h->cpart, h->gptr,
+                       h->gidx, h->ngroups, t, part_pq, npq, part_rz, nrz, h->scal, h->flags);
+    PYTO_CHECK_HIP(hipGetLastError());
+    return PYTO_OK;
+  }
   hipLaunchKernelGGL(k_gather_scalars, dim3(gb + 1), dim3(256), 0, s, h->cpart, h->gptr, h->gidx,
                      h->ngroups, t, part_pq, npq, part_rz, nrz, h->scal, h->flags);
   PYTO_CHECK_HIP(hipGetLastError());
@@ -721,6 +733,314 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Thermal coarse space: one constant mode per group (W[n][g] = 1 for node n
+// of group g, rows of fixed temperatures dropped), so W^T v is a per-group
+// sum, W mu takes mu at the node's group, and E = W^T K_T W is G x G.  The
+// special elements are those of the structural space (corners in more than
+// one group, or a fixed corner): K_T annihilates constants, so an element
+// whose corners are free and in one group adds nothing to W^T K_T v.  The
+// tables (chunks, padded restriction blocks, group and pair lists) are the
+// structural ones; partials keep one value per (chunk, slot) at index
+// chunk * 8 + slot (per node run: run * 8).
+// ---------------------------------------------------------------------------
+
+// t = W^T v over all nodes: one block per run of <= 256 nodes of one group,
+// a block sum per run (k_gather_t then sums the runs of each group).
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict_full_t(const T* __restrict__ v,
+                                                         const int32_t* __restrict__ gnodes,
+                                                         const int2* __restrict__ nchunk,
+                                                         const uint8_t* __restrict__ mask,
+                                                         double* __restrict__ npart) {
+  __shared__ double red[4];
+  const int2 ch = nchunk[blockIdx.x];
+  double c = 0.0;
+  if ((int)threadIdx.x < ch.y) {
+    const int64_t n = gnodes[ch.x + threadIdx.x];
+    c = (mask[n] & 1) ? 0.0 : (double)v[n];
+  }
+  const double tot = block_reduce_sum256(c, red);
+  if (threadIdx.x == 0) npart[(int64_t)blockIdx.x * 8] = tot;
+}
+
+// t[g] = sum of part[idx[k]] over k in [ptr[g], ptr[g+1]): one wave per group,
+// lanes striding over the list, then a wave sum (fixed order).
+__device__ __forceinline__ double group_sum_t(const double* __restrict__ part,
+                                              const int32_t* __restrict__ idx, int k0, int k1,
+                                              int lane) {
+  double acc = 0.0;
+  for (int k = k0 + lane; k < k1; k += 4 * 64) {
+    int id[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) id[u] = k + 64 * u < k1 ? idx[k + 64 * u] : -1;
+    double c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = id[u] >= 0 ? part[id[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += c[u];
+  }
+  return wave_reduce_sum(acc);
+}
+
+__global__ __launch_bounds__(256) void k_gather_t(const double* __restrict__ part,
+                                                  const int32_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ idx, int ngroups,
+                                                  double* __restrict__ t,
+                                                  const int* __restrict__ done) {
+  if (done && *done) return;
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (g >= ngroups) return;
+  const double s = group_sum_t(part, idx, ptr[g], ptr[g + 1], lane);
+  if (lane == 0) t[g] = s;
+}
+
+// k_gather_t and, in one extra workgroup, the iteration's scalars (as
+// k_gather_scalars).
+__global__ __launch_bounds__(256) void k_gather_scalars_t(
+    const double* __restrict__ cpart, const int32_t* __restrict__ gptr,
+    const int32_t* __restrict__ gidx, int ngroups, double* __restrict__ t,
+    const double* __restrict__ part_pq, int npq, const double* __restrict__ part_rz, int nrz,
+    double* __restrict__ scal, int* __restrict__ flags) {
+  if (blockIdx.x == gridDim.x - 1) {
+    __shared__ int done_s;
+    __shared__ double red[10];
+    if (threadIdx.x == 0) done_s = flags[F_DONE];
+    __syncthreads();
+    if (done_s) return;
+    double pkp, rz_new;
+    block_sum_partials2(part_pq, npq, part_rz, nrz, red, pkp, rz_new);
+    if (threadIdx.x == 0) pcg_scalar_step(scal, flags, pkp, rz_new, scal[S_FLOOR]);
+    return;
+  }
+  if (flags[F_DONE]) return;
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (g >= ngroups) return;
+  const double sj = group_sum_t(cpart, gidx, gptr[g], gptr[g + 1], lane);
+  if (lane == 0) t[g] = sj;
+}
+
+// out = W mu: mu of the node's group, 0 at fixed temperatures.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_t(T* __restrict__ out,
+                                                   const double* __restrict__ mu,
+                                                   const uint16_t* __restrict__ gm, int64_t nn,
+                                                   const int* __restrict__ done) {
+  if (done && *done) return;
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nn;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t a = gm[n];
+    out[n] = ((a >> 12) & 1u) ? T(0) : (T)mu[a & 0xfffu];
+  }
+}
+
+// Per-(chunk, slot) partials of W^T K_T v over the special elements, v = z or
+// z + beta p (beta from the update's partials), in the padded blocks of the
+// structural restriction: the block stages v at its sorted corner nodes in
+// LDS, each wave applies the thermal element (modal form, precision T) to its
+// chunk and sums the nodal fluxes per group slot in FP64.
+template <typename T, bool HAS_P>
+__global__ __launch_bounds__(256) void k_restrict_special_t(
+    const T* __restrict__ z, const T* __restrict__ p, const double* __restrict__ part_rz,
+    int nrz, const double* __restrict__ rz_old, const int2* __restrict__ pwave,
+    const SpecialSlot* __restrict__ pslot, const int32_t* __restrict__ pnodes,
+    const T* __restrict__ pE, ModalConsts<T> mc, double* __restrict__ cpart,
+    const int* __restrict__ done) {
+  constexpr int kStage = kSpecialBlockNodes / 256;
+  __shared__ T vs[kSpecialBlockNodes];
+  const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int stopped = done ? *done : 0;
+  const int2 pw = pwave[b * kSpecialBlockChunks + w];   // (chunk or -1, ng)
+  const int64_t slot = ((int64_t)b * kSpecialBlockChunks + w) * 64 + lane;
+  const SpecialSlot se = pslot[slot];
+  const T Ev = pE[slot];
+  uint32_t node[kStage];
+#pragma unroll
+  for (int t = 0; t < kStage; ++t)
+    node[t] = (uint32_t)pnodes[(int64_t)b * kSpecialBlockNodes + threadIdx.x + 256 * t];
+  if (stopped) return;
+  T za[kStage], pa[kStage];
+#pragma unroll
+  for (int t = 0; t < kStage; ++t) {
+    za[t] = z[node[t]];
+    pa[t] = HAS_P ? p[node[t]] : T(0);
+  }
+  __shared__ double redb[5];
+  const double beta = HAS_P ? block_sum_partials(part_rz, nrz, redb) / *rz_old : 0.0;
+#pragma unroll
+  for (int t = 0; t < kStage; ++t)
+    vs[threadIdx.x + 256 * t] = (T)((double)za[t] + (HAS_P ? beta * (double)pa[t] : 0.0));
+  __syncthreads();
+  if (pw.x < 0) return;
+  const bool active = se.e >= 0;
+  T f[8][1];
+  const uint64_t info = active ? se.info : 0;
+  if (active) {
+    const uint32_t lw[4] = {se.li.x, se.li.y, se.li.z, se.li.w};
+    T u[8][1];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int l = (int)((lw[j >> 1] >> (16 * (j & 1))) & 0xffffu);
+      u[j][0] = ((info >> (24 + 3 * j)) & 1) ? T(0) : vs[l];
+    }
+    element_apply_thermal<T>(u, Ev, mc, f);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) f[j][0] = T(0);
+  }
+  for (int sl = 0; sl < pw.y; ++sl) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (active && (int)((info >> (3 * j)) & 7) == sl && !((info >> (24 + 3 * j)) & 1))
+        acc += (double)f[j][0];
+    const double tot = wave_reduce_sum(acc);
+    if (lane == 0) cpart[(int64_t)pw.x * 8 + sl] = tot;
+  }
+}
+
+// Galerkin partials of E = W^T K_T W: one wave per chunk; for each slot sa,
+// the element operator applied to the indicator of sa's free corners, then
+// for every slot sb the sum of the fluxes at sb's free corners, summed over
+// the chunk's elements, at part[off[chunk] + sa ng + sb].
+__global__ __launch_bounds__(256) void k_galerkin_parts_t(
+    const SpecialElem* __restrict__ sinfo, const SpecialChunk* __restrict__ chunks, int nchunks,
+    const int32_t* __restrict__ off, const double* __restrict__ Ee, ModalConsts<double> mc,
+    double* __restrict__ part) {
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (wave >= nchunks) return;
+  const SpecialChunk* ch = chunks + wave;
+  const int ng = ch->ng;
+  const bool active = lane < ch->count;
+  uint64_t info = 0;
+  double Ev = 0.0;
+  if (active) {
+    const SpecialElem se = sinfo[ch->start + lane];
+    info = se.info;
+    Ev = Ee[se.e];
+  }
+  double* out = part + off[wave];
+  for (int sa = 0; sa < ng; ++sa) {
+    double u[8][1], f[8][1];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      u[j][0] = (active && (int)((info >> (3 * j)) & 7) == sa && !((info >> (24 + 3 * j)) & 1))
+                    ? 1.0 : 0.0;
+    element_apply_thermal<double>(u, Ev, mc, f);
+    for (int sb = 0; sb < ng; ++sb) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (active && (int)((info >> (3 * j)) & 7) == sb && !((info >> (24 + 3 * j)) & 1))
+          acc += f[j][0];
+      const double tot = wave_reduce_sum(acc);
+      if (lane == 0) out[sa * ng + sb] = tot;
+    }
+  }
+}
+
+// E[gb][ga] = sum of the pair's chunk partials in list order (one thread per
+// group pair sharing a chunk; fixed order, so E is the same bits each time).
+__global__ __launch_bounds__(256) void k_galerkin_reduce_t(const int2* __restrict__ pairs,
+                                                           const int32_t* __restrict__ pptr,
+                                                           const int32_t* __restrict__ pidx,
+                                                           int npairs,
+                                                           const double* __restrict__ part,
+                                                           double* __restrict__ E, int ld) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  double acc = 0.0;
+  for (int k = pptr[p]; k < pptr[p + 1]; ++k) acc += part[pidx[k]];
+  const int2 gp = pairs[p];
+  E[(int64_t)gp.y * ld + gp.x] = acc;
+}
+
+// Direction update of the thermal deflated PCG: x += alpha p_old (or the
+// window's deferred alpha_s p_s, FLUSH), p = z + beta p_old - W mu with W mu
+// = mu of the node's group (0 at fixed temperatures), one 16-byte vector of
+// nodes per thread, mu staged in LDS (8 bytes per group, sized by the
+// launch).
+template <typename T, bool FLUSH, bool INPLACE>
+__global__ __launch_bounds__(256) void k_dir_update_t(
+    DirIO<T> io, const double* __restrict__ mu, int ngroups, const uint16_t* __restrict__ gm,
+    int64_t nn, const double* __restrict__ scal, bool first, const int* __restrict__ done) {
+  if (done && *done) return;
+  extern __shared__ double amu[];
+  for (int q = threadIdx.x; q < ngroups; q += blockDim.x) amu[q] = mu[q];
+  __syncthreads();
+  using DV = DirVec<T>;
+  using V = typename DV::V;
+  constexpr int W = DV::W;
+  const double beta = first ? 0.0 : scal[S_BETA];
+  const double alpha = (first || !io.xupd) ? 0.0 : scal[S_ALPHA_X];
+  const int64_t nv = nn / W;
+  T* __restrict__ x = io.x;
+  T* __restrict__ pio = io.pout;
+  const T* __restrict__ p = INPLACE ? pio : io.pin;
+  const T* __restrict__ z = io.z;
+  const V* pv = reinterpret_cast<const V*>(p);
+  V* pov = reinterpret_cast<V*>(pio);
+  V* xv = reinterpret_cast<V*>(x);
+  const V* zv = reinterpret_cast<const V*>(z);
+  auto wmu = [&](int64_t n) -> double {
+    const uint32_t a = gm[n];
+    return ((a >> 12) & 1u) ? 0.0 : amu[a & 0xfffu];
+  };
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    V pp = first ? V{} : pv[i];
+    const V zz = zv[i];
+    if (FLUSH) {
+      V xx = load_nt(xv + i);
+      double acc[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) acc[j] = (double)DV::get(xx, j);
+      const V* rv = reinterpret_cast<const V*>(io.ring);
+      const int64_t ldv = io.ld / W;
+#pragma unroll
+      for (int sl = 0; sl < kDirSlots; ++sl) {
+        if (sl < io.nflush) {
+          const V r = rv[sl * ldv + i];
+          const double a = io.al[sl];
+#pragma unroll
+          for (int j = 0; j < W; ++j) acc[j] += a * (double)DV::get(r, j);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < W; ++j) DV::set(xx, j, (T)acc[j]);
+      store_nt(xv + i, xx);
+    } else if (alpha != 0.0) {
+      V xx = load_nt(xv + i);
+#pragma unroll
+      for (int j = 0; j < W; ++j)
+        DV::set(xx, j, (T)((double)DV::get(xx, j) + alpha * (double)DV::get(pp, j)));
+      store_nt(xv + i, xx);
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      DV::set(pp, j, (T)((double)DV::get(zz, j) + beta * (double)DV::get(pp, j) - wmu(i * W + j)));
+    pov[i] = pp;
+  }
+  // nodes past the last whole vector, one at a time by block 0
+  if (blockIdx.x == 0)
+    for (int64_t d = nv * W + threadIdx.x; d < nn; d += blockDim.x) {
+      const T pd = first ? T(0) : p[d];
+      if (FLUSH) {
+        double acc = (double)x[d];
+        for (int sl = 0; sl < io.nflush; ++sl) acc += io.al[sl] * (double)io.ring[sl * io.ld + d];
+        x[d] = (T)acc;
+      } else if (alpha != 0.0) {
+        x[d] = (T)((double)x[d] + alpha * (double)pd);
+      }
+      pio[d] = (T)((double)z[d] + beta * (double)pd - wmu(d));
+    }
+}
+
 template <typename T>
 int launch_dir_update(Handle* h, const DirIO<T>& io, bool first, const int* done, hipStream_t s) {
   // 2048 workgroups (8 per CU) striding over the vectors: each stages the
@@ -729,6 +1049,20 @@ int launch_dir_update(Handle* h, const DirIO<T>& io, bool first, const int* done
   // workgroups fit on the chip at once
   PYTO_REQUIRE(io.nflush >= 0 && io.nflush <= kDirSlots && (io.nflush == 0 || io.ld % 4 == 0),
                PYTO_EINVAL, "bad flush window");
+  if (h->ncomp == 1 && h->ngroups > 0) {
+    // thermal groups: one value per group in LDS (at most 32 KB)
+    const int64_t nn = h->g.nnodes;
+    const int blocks = (int)std::min<int64_t>((nn / (16 / (int64_t)sizeof(T)) + 255) / 256, 2048);
+    const bool inp = io.pin == io.pout;
+    auto kernel = io.nflush > 0 ? (inp ? k_dir_update_t<T, true, true> : k_dir_update_t<T, true, false>)
+                                : (inp ? k_dir_update_t<T, false, true>
+                                       : k_dir_update_t<T, false, false>);
+    hipLaunchKernelGGL(kernel, dim3(std::max(blocks, 1)), dim3(256), (size_t)h->ngroups * 8, s, io,
+                       (const double*)h->cmu, h->ngroups, (const uint16_t*)h->gmask, nn,
+                       (const double*)h->scal, first, done);
+    PYTO_CHECK_HIP(hipGetLastError());
+    return PYTO_OK;
+  }
   const size_t lds = (h->ngroups > 0 && h->ngroups <= kLdsGroups) ? (size_t)h->ngroups * 48 : 0;
   if (h->ncomp == 3 && sizeof(T) == 8) {
     const int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 2048);
@@ -934,6 +1268,15 @@ int coarse_apply(Handle* h, const double* t1, const double* t2, const double* be
 
 template <typename T>
 int launch_restrict_full(Handle* h, const T* v, double* t, hipStream_t s) {
+  if (h->ncomp == 1) {
+    hipLaunchKernelGGL(k_restrict_full_t<T>, dim3(h->nnchunks), dim3(256), 0, s, v, h->gnodes,
+                       h->nchunk, h->fixmask, h->npart);
+    hipLaunchKernelGGL(k_gather_t, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
+                       (const double*)h->npart, h->nptr, h->nidx, h->ngroups, t,
+                       (const int*)nullptr);
+    PYTO_CHECK_HIP(hipGetLastError());
+    return PYTO_OK;
+  }
   hipLaunchKernelGGL(k_restrict_full<T>, dim3(h->nnchunks), dim3(256), 0, s, v, h->gnodes,
                      h->nchunk, h->nchunk_grp, h->centers, h->fixmask, h->g, h->npart);
   hipLaunchKernelGGL(k_coarse_gather, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
@@ -963,7 +1306,16 @@ int launch_restrict_special_parts(Handle* h, const T* z, const T* p, const doubl
                                   hipStream_t s) {
   PYTO_REQUIRE(z && (p == nullptr || (part_rz != nullptr && rz_old != nullptr)),
                PYTO_EINVAL, "special restriction needs z and (with p) the rz partials");
-  if (h->nsblocks > 0) {
+  if (h->nsblocks > 0 && h->ncomp == 1) {
+    auto go = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(h->nsblocks), dim3(256), 0, s, z, p, part_rz, nrz, rz_old,
+                         h->pwave, h->pslot, h->pnodes, pe_of<T>(h), mc_of<T>(h), h->cpart, done);
+    };
+    if (p)
+      go(k_restrict_special_t<T, true>);
+    else
+      go(k_restrict_special_t<T, false>);
+  } else if (h->nsblocks > 0) {
     const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
     auto go = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, dim3(h->nsblocks), dim3(256), 0, s, z, p, part_rz, nrz, rz_old,
@@ -980,6 +1332,12 @@ int launch_restrict_special_parts(Handle* h, const T* z, const T* p, const doubl
 }
 
 int launch_coarse_gather(Handle* h, double* t, const int* done, hipStream_t s) {
+  if (h->ncomp == 1) {
+    hipLaunchKernelGGL(k_gather_t, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
+                       (const double*)h->cpart, h->gptr, h->gidx, h->ngroups, t, done);
+    PYTO_CHECK_HIP(hipGetLastError());
+    return PYTO_OK;
+  }
   hipLaunchKernelGGL(k_coarse_gather, dim3((h->ngroups * 64 + 255) / 256), dim3(256), 0, s,
                      h->cpart, h->gptr, h->gidx, h->ngroups, t, done);
   PYTO_CHECK_HIP(hipGetLastError());
@@ -998,6 +1356,14 @@ template <typename T, bool UPDATE>
 int launch_prolong(Handle* h, T* out, const T* z, const double* mu, const double* beta,
                    bool first, const int* done, hipStream_t s, double* zero_vec, int zero_n) {
   const int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 8192);
+  if (h->ncomp == 1) {
+    PYTO_REQUIRE(!UPDATE && zero_vec == nullptr && h->ngroups > 0, PYTO_EINVAL,
+                 "thermal prolongation is W mu only");
+    hipLaunchKernelGGL(k_prolong_t<T>, dim3(blocks), dim3(256), 0, s, out, mu,
+                       (const uint16_t*)h->gmask, h->g.nnodes, done);
+    PYTO_CHECK_HIP(hipGetLastError());
+    return PYTO_OK;
+  }
   hipLaunchKernelGGL((k_prolong<T, UPDATE>), dim3(blocks), dim3(256), 0, s, out, z, mu,
                      h->ngroups > 0 ? h->node_group : (const int32_t*)nullptr, h->centers,
                      h->fixmask, h->g, beta, first, done, zero_vec, zero_n);
@@ -1084,7 +1450,14 @@ static int coarse_assemble(Handle* h, hipStream_t s) {
                        n, (const double*)h->Ee64, (const float*)h->Ee32, h->pE64, h->pE32);
   }
   PYTO_CHECK_HIP(hipMemsetAsync(h->Ec, 0, sizeof(double) * ld * ld, s));
-  if (h->nchunks > 0) {
+  if (h->nchunks > 0 && h->ncomp == 1) {
+    hipLaunchKernelGGL(k_galerkin_parts_t, dim3((h->nchunks + 3) / 4), dim3(256), 0, s, h->sinfo,
+                       h->chunks, h->nchunks, h->gal_off, h->Ee64, h->mc64, h->gal_part);
+    hipLaunchKernelGGL(k_galerkin_reduce_t, dim3((h->npairs + 255) / 256), dim3(256), 0, s,
+                       h->gal_pairs, h->gal_pptr, h->gal_pidx, h->npairs,
+                       (const double*)h->gal_part, h->Ec, ld);
+    PYTO_CHECK_HIP(hipGetLastError());
+  } else if (h->nchunks > 0) {
     const bool cube = h->g.hx == h->g.hy && h->g.hy == h->g.hz;
     const dim3 grid((h->nchunks + 3) / 4, 6);
     if (cube)
@@ -1258,11 +1631,13 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
                     const double* centers, void* stream) {
   PYTO_REQUIRE(h, PYTO_EINVAL, "null handle");
   PYTO_REQUIRE(ngroups >= 0, PYTO_EINVAL, "ngroups must be >= 0");
-  PYTO_REQUIRE(ngroups == 0 || h->ncomp == 3, PYTO_EINVAL,
-               "rigid-body deflation groups need a structural handle");
+  // structural handles: six rigid-body modes per group; thermal handles
+  // (one temperature per node): one constant per group
+  const bool th = h->ncomp == 1;
+  const int modes = th ? 1 : 6;
   PYTO_REQUIRE(ngroups <= kMaxCoarseGroups, PYTO_EINVAL,
                "coarse space of %d groups (%d unknowns) exceeds the dense GPU limit of %d "
-               "groups; use fewer, larger deflation groups", ngroups, 6 * ngroups,
+               "groups; use fewer, larger deflation groups", ngroups, modes * ngroups,
                kMaxCoarseGroups);
   hipStream_t s = as_stream(stream);
   release_coarse(h);
@@ -1439,7 +1814,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
       gidx.insert(gidx.end(), per[gi].begin(), per[gi].end());
     }
   }
-  // Galerkin partial blocks: per chunk ng^2 blocks of 36 at gal_off[chunk];
+  // Galerkin partial blocks: per chunk ng^2 blocks of modes^2 at gal_off[chunk];
   // per group pair (ga, gb) sharing a chunk, the list of its blocks in chunk
   // order
   std::vector<int32_t> gal_off(chunks.size());
@@ -1455,8 +1830,8 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
       for (int sa = 0; sa < ng; ++sa)
         for (int sb = 0; sb < ng; ++sb)
           ent.push_back({(int64_t)chunks[c].groups[sa] * ngroups + chunks[c].groups[sb],
-                         (int32_t)(galsz + (sa * ng + sb) * 36)});
-      galsz += (int64_t)36 * ng * ng;
+                         (int32_t)(galsz + (sa * ng + sb) * modes * modes)});
+      galsz += (int64_t)modes * modes * ng * ng;
     }
     std::stable_sort(ent.begin(), ent.end(),
                      [](const std::pair<int64_t, int32_t>& x,
@@ -1502,7 +1877,7 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
   // install the new space; a failure part-way leaves the handle without groups
   auto install = [&]() -> int {
     h->ngroups = ngroups;
-    h->nc = 6 * ngroups;
+    h->nc = modes * ngroups;
     h->nnchunks = (int)nchunk.size();
     h->nspecial = (int)special.size();
     h->nchunks = (int)chunks.size();
@@ -1513,11 +1888,13 @@ int pyto_set_groups(pyto_handle* h, const int32_t* node_group, int ngroups,
     PYTO_TRY(device_alloc(h, (void**)&h->centers, (size_t)ngroups * 3 * 8));
     PYTO_TRY(device_alloc(h, (void**)&h->sinfo, sinfo.size() * sizeof(SpecialElem)));
     PYTO_TRY(device_alloc(h, (void**)&h->chunks, chunks.size() * sizeof(SpecialChunk)));
-    PYTO_TRY(device_alloc(h, (void**)&h->cpart, chunks.size() * 48 * sizeof(double)));
+    // partials per (chunk, slot): 6 values (structural) or 1 (thermal) per slot
+    const size_t pstride = th ? 8 : 48;
+    PYTO_TRY(device_alloc(h, (void**)&h->cpart, std::max<size_t>(chunks.size(), 1) * pstride * sizeof(double)));
     PYTO_TRY(device_alloc(h, (void**)&h->gnodes, nn * sizeof(int32_t)));
     PYTO_TRY(device_alloc(h, (void**)&h->nchunk, nchunk.size() * sizeof(int2)));
     PYTO_TRY(device_alloc(h, (void**)&h->nchunk_grp, nchunk.size() * sizeof(int32_t)));
-    PYTO_TRY(device_alloc(h, (void**)&h->npart, nchunk.size() * 48 * sizeof(double)));
+    PYTO_TRY(device_alloc(h, (void**)&h->npart, nchunk.size() * pstride * sizeof(double)));
     PYTO_TRY(device_alloc(h, (void**)&h->nptr, nptr.size() * sizeof(int32_t)));
     PYTO_TRY(device_alloc(h, (void**)&h->nidx, nidx.size() * sizeof(int32_t)));
     PYTO_TRY(device_alloc(h, (void**)&h->gptr, gptr.size() * sizeof(int32_t)));

@@ -7,12 +7,15 @@ by the matrix-free ``ThermalOperator`` (one temperature per node, the 8x8
 KE_T scaled by the conductivity law) and the solve runs on the GPU, with
 non-zero Dirichlet temperatures eliminated by lifting.
 
-Solvers: the thermal operator carries no deflation space, so ``Solvers.DPCG``
-and ``Solvers.PCG`` both run Jacobi PCG on the GPU (the converged temperature
-does not depend on the deflation space; the reference's DPCG takes its
-thermal W as the rows ``W[free_dofs]`` of the structural rigid-body matrix,
-src/linear_solvers.py:184).  Arrays passed as numpy come back as numpy;
-CUDA tensors stay on the device.
+Solvers: ``Solvers.DPCG`` with a DeflationSolver runs the deflated PCG on
+the GPU with one constant per deflation group (the thermal coarse space: the
+temperature analogue of the rigid-body modes); ``Solvers.PCG``, or DPCG
+without a DeflationSolver, runs Jacobi PCG.  The reference's DPCG takes its
+thermal W as the rows ``W[free_dofs]`` of the structural rigid-body matrix
+(src/linear_solvers.py:184), which pairs temperatures with unrelated
+rigid-mode entries; the converged temperature does not depend on W, only the
+iteration count does.  Arrays passed as numpy come back as numpy; CUDA
+tensors stay on the device.
 """
 import ctypes
 
@@ -129,12 +132,17 @@ class HexThermalFEA:
             self._force = op.as_dof(self.bc.force)
             self._force_src = self.bc.force
         solver = self.solver
-        if solver == linear_solvers.Solvers.DPCG:
-            solver = linear_solvers.Solvers.PCG
         kwargs = dict(self.kwargs)
         kwargs.setdefault("maxIters", 50000)
+        if solver == linear_solvers.Solvers.DPCG:
+            if self.dsolver is None or self.dsolver.ws_nGroups == 0:
+                solver = linear_solvers.Solvers.PCG
+            else:
+                kwargs["dsolver"] = self.dsolver
         T = linear_solvers.solve(op, self._force, solver, self.bc, **kwargs)
-        self.last_stats = getattr(op, "last_stats", None)
+        self.last_stats = (getattr(self.dsolver, "last_stats", None)
+                           if solver == linear_solvers.Solvers.DPCG
+                           else getattr(op, "last_stats", None))
         self._T = T
         self.sol = T.cpu().numpy() if as_numpy else T
         return self.sol

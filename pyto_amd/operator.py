@@ -216,27 +216,31 @@ class GridOperator:
         return times
 
     # -- coarse space --------------------------------------------------------
+    # coarse unknowns per group: the six rigid-body modes (one constant per
+    # group on the thermal operator)
+    modes_per_group = 6
+
     def restrict(self, v):
-        t = torch.empty(6 * self.ngroups, dtype=F64, device=self.device)
+        t = torch.empty(self.modes_per_group * self.ngroups, dtype=F64, device=self.device)
         check(_lib.lib().pyto_restrict_f64(self.handle, ptr(self.as_dof(v)), ptr(t), stream_ptr()))
         return t
 
     def prolong(self, mu):
         v = torch.empty(self.num_dofs, dtype=F64, device=self.device)
         mu = torch.as_tensor(mu, dtype=F64, device=self.device).contiguous()
-        check(_lib.lib().pyto_prolong_f64(self.handle, ptr(mu, F64, 6 * self.ngroups), ptr(v),
+        check(_lib.lib().pyto_prolong_f64(self.handle, ptr(mu, F64, self.modes_per_group * self.ngroups), ptr(v),
                                           stream_ptr()))
         return v
 
     def coarse_solve(self, t):
         t = torch.as_tensor(t, dtype=F64, device=self.device).contiguous()
         mu = torch.empty_like(t)
-        check(_lib.lib().pyto_coarse_solve_f64(self.handle, ptr(t, F64, 6 * self.ngroups),
+        check(_lib.lib().pyto_coarse_solve_f64(self.handle, ptr(t, F64, self.modes_per_group * self.ngroups),
                                                ptr(mu), stream_ptr()))
         return mu
 
     def coarse_matrix(self):
-        n = 6 * self.ngroups
+        n = self.modes_per_group * self.ngroups
         E = torch.empty((n, n), dtype=F64, device=self.device)
         check(_lib.lib().pyto_coarse_matrix_f64(self.handle, ptr(E), stream_ptr()))
         return E
@@ -368,9 +372,12 @@ class ThermalOperator(GridOperator):
 
     Replaces the assembled ``stiff_mtrx`` of ``HexThermalFEA.solve``
     (src/hex_thermal_fea.py:74-117).  matvec, diagonal, set_density, solve
-    (Jacobi PCG; no deflation groups) and compliance act on vectors of
-    length num_nodes.
+    and compliance act on vectors of length num_nodes.  Deflation groups
+    (``set_groups``) span one constant per group: W[n, g] = 1 for the free
+    nodes n of group g, E = W^T K_T W is G x G, and solve runs the deflated
+    PCG with that coarse space (Jacobi PCG without groups).
     """
+    modes_per_group = 1
 
     def __init__(self, num_elems, elem_size, k0, fixed_nodes, penal=1.0, k_void=1e-8,
                  device=None):
@@ -394,11 +401,6 @@ class ThermalOperator(GridOperator):
         self._h = h
         self.ngroups = 0
         self.x = None
-
-    def set_groups(self, node_group, centers):
-        if np.asarray(centers).reshape(-1, 3).shape[0] > 0:
-            raise NotImplementedError("deflation groups are not supported on the thermal "
-                                      "operator; its solves are Jacobi PCG")
 
     def gradient(self, T):
         """Centre temperature gradient in reference coordinates, 3 x num_elems

@@ -158,10 +158,14 @@ def solve_thermal_adjoint(x, d, fe_thermal_solver, fe_structural_solver, materia
     bcAdjoint = bound_cond.BC(force=force, fixed_dofs=ts.bc.fixed_dofs,
                               dirichlet_values=np.zeros(np.asarray(ts.bc.fixed_dofs).size))
     solver = ts.solver
-    if solver == linear_solvers.Solvers.DPCG:
-        solver = linear_solvers.Solvers.PCG   # the thermal operator has no deflation space
     kwargs = dict(ts.kwargs)
     kwargs.setdefault("maxIters", 50000)
+    if solver == linear_solvers.Solvers.DPCG:
+        # the thermal coarse space of the forward solve, when it has one
+        if ts.dsolver is None or ts.dsolver.ws_nGroups == 0:
+            solver = linear_solvers.Solvers.PCG
+        else:
+            kwargs["dsolver"] = ts.dsolver
     lam = linear_solvers.solve(opT, rhs, solver, bcAdjoint, **kwargs)
     return _like(d, lam)
 

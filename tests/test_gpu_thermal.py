@@ -250,3 +250,108 @@ def test_thermal_mixed_matches_fp64(gpu, dims):
     assert abs(C64 - Cm) <= 1e-6 * abs(C64)
     assert rel_err(Tm.cpu().numpy(), T64.cpu().numpy()) <= 1e-5
     op.close()
+
+
+def thermal_groups(c, boxes):
+    """Box groups of the case's nodes (the reference's box rule on a unit
+    grid, no merge) and the oracle W: one constant per group on the free
+    nodes."""
+    import scipy.sparse as sp
+    m = c.mesh
+    n = m.node_indices[:, :3].astype(np.int64)
+    size = np.array(c.dims, dtype=np.int64) + 1
+    b = np.array(boxes, dtype=np.int64)
+    box = np.minimum(n * b // size, b - 1)
+    ngn = (box[:, 0] + b[0] * (box[:, 1] + b[1] * box[:, 2])).astype(np.int32)
+    G = int(b.prod())
+    cen = np.zeros((G, 3))
+    for i in range(3):
+        cen[:, i] = np.bincount(ngn, weights=m.node_xyz[:, i], minlength=G) / np.bincount(ngn, minlength=G)
+    free = np.setdiff1d(np.arange(m.num_nodes), c.fixed)
+    W = sp.csr_matrix((np.ones(free.size), (np.arange(free.size), ngn[free])), shape=(free.size, G))
+    return ngn, cen, W, free
+
+
+@pytest.mark.parametrize("dims,boxes", [((12, 6, 4), (3, 2, 2)), ((17, 9, 6), (4, 2, 3))])
+def test_thermal_coarse_space_matches_oracle(gpu, dims, boxes):
+    """Thermal deflation groups (one constant per group): E = W^T K_T,ff W,
+    W^T v and W mu against the oracle's assembled K_T and explicit W."""
+    print("thermal coarse space", dims, boxes, flush=True)
+    c = TCase(dims, seed=7, penal=3.0, kvoid=1e-6)
+    ngn, cen, W, free = thermal_groups(c, boxes)
+    op = c.operator(gpu)
+    op.set_coarse_shift(0.0, 0.0)
+    op.set_groups(ngn, cen)
+    op.set_density(c.x)
+    Kff = c.K.tocsr()[free][:, free]
+    E_ref = (W.T @ (Kff @ W)).toarray()
+    E = op.coarse_matrix().cpu().numpy()
+    assert E.shape == E_ref.shape
+    assert rel_err(E, E_ref) <= 1e-12
+    v = np.random.default_rng(3).normal(size=c.mesh.num_nodes)
+    t = op.restrict(torch.from_numpy(v).to(gpu)).cpu().numpy()
+    assert rel_err(t, W.T @ v[free]) <= 1e-13
+    mu = np.random.default_rng(4).normal(size=E.shape[0])
+    pv = op.prolong(torch.from_numpy(mu).to(gpu)).cpu().numpy()
+    ref = np.zeros(c.mesh.num_nodes)
+    ref[free] = W @ mu
+    assert np.array_equal(pv, ref)
+    op.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_thermal_deflated_solve(gpu, precision):
+    """Deflated PCG on the thermal operator: the temperatures of the sparse
+    direct solve, far fewer iterations than Jacobi PCG, and (FP64) the
+    iteration count of the oracle's deflated PCG with the same W within 2."""
+    import scipy.sparse.linalg as spla
+    print("thermal deflated solve", precision, flush=True)
+    dims = (32, 16, 12)
+    c = TCase(dims, seed=8, penal=3.0, kvoid=1e-6, lo=0.01)
+    c.gv[:] = 0.0    # homogeneous temperatures: the operator solves K_ff T_f = f_f
+    ngn, cen, W, free = thermal_groups(c, (4, 2, 2))
+    op = c.operator(gpu)
+    op.set_coarse_shift(0.0, 0.0)
+    f = torch.from_numpy(c.f).to(gpu)
+    Tj, sj = op.solve(f, rtol=1e-10, max_iters=100000, precision=precision)
+    op.set_groups(ngn, cen)
+    op.set_density(c.x)
+    T, st = op.solve(f, rtol=1e-10, max_iters=100000, precision=precision)
+    print("jacobi", sj["iterations"], "deflated", st["iterations"], flush=True)
+    assert st["converged"] and sj["converged"]
+    A, b, _ = OF.impose_dirichlet(c.K, c.f, c.fixed, c.gv)
+    T_ref = np.zeros(c.mesh.num_nodes)
+    T_ref[free] = spla.spsolve(A.tocsc(), b)
+    assert rel_err(T.cpu().numpy(), T_ref) <= 1e-7
+    assert st["iterations"] < 0.6 * sj["iterations"]
+    if precision == "fp64":
+        from oracle import deflation as OD
+        _, info = OD.deflated_pcg(A.tocsr(), b, W, OF.jacobi_inverse_diag(A), rtol=1e-10,
+                                  maxIters=5000, shift=0.0)
+        assert abs(st["iterations"] - info["iterations"]) <= 2
+    op.close()
+
+
+def test_thermal_fea_dpcg_uses_groups(gpu):
+    """HexThermalFEA with Solvers.DPCG and a DeflationSolver runs the
+    deflated solve (fewer iterations than PCG, same temperatures)."""
+    from pyto_amd import bound_cond, deflation, hex_mesher, linear_solvers, mat_lib
+    from pyto_amd.hex_thermal_fea import HexThermalFEA
+    print("thermal fea dpcg", flush=True)
+    dims = (24, 12, 8)
+    c = TCase(dims, seed=9, k0=50.0, lo=0.2)
+    mesh = hex_mesher.HexMesher()
+    mesh.grid_mesh(dims, (1.0, 1.0, 1.0))
+    bc = bound_cond.BC(force=c.f, fixed_dofs=c.fixed, dirichlet_values=c.gv)
+    mat = mat_lib.Material("T", 1.0, 0.3, 1.0, 50.0, 1.0, 1e-5)
+    ds = deflation.DeflationSolver()
+    ds.create_box_groups(mesh, (4, 2, 2))
+    fe_d = HexThermalFEA(mesh, mat, bc, linear_solvers.Solvers.DPCG, dsolver=ds, rtol=1e-12)
+    Td = fe_d.solve(c.x, None)
+    fe_p = HexThermalFEA(mesh, mat, bc, linear_solvers.Solvers.PCG, rtol=1e-12)
+    Tp = fe_p.solve(c.x, None)
+    print("dpcg", fe_d.last_stats["iterations"], "pcg", fe_p.last_stats["iterations"], flush=True)
+    assert fe_d.last_stats["converged"]
+    assert fe_d.last_stats["iterations"] < fe_p.last_stats["iterations"]
+    assert rel_err(Td, Tp) <= 1e-9
+    assert rel_err(Td, c.solve_oracle()) <= 1e-9
