@@ -127,6 +127,8 @@ struct Handle {
   double* x64 = nullptr;        // copy of densities
   double* invd64 = nullptr;     // per DOF
   float* invd32 = nullptr;
+  double* esum64 = nullptr;     // per node (3 values per node only): the FP64 sum of E_e
+                                // from which the FP64 CG update forms invd64 on the fly
   float* esum32 = nullptr;      // per node (3 values per node only): FP32 sum of E_e over
                                 // the node's elements, from which the FP32 CG update
                                 // forms invd32 on the fly (jacobi_inv32)
@@ -492,9 +494,11 @@ __device__ __forceinline__ float jacobi_inv32(float kd, float s, bool fixed, flo
 
 // Per-node source of the FP32 Jacobi (jacobi_inv32).
 struct JacobiNodes {
-  const float* esum;      // per node
+  const float* esum;      // per node (FP32 solver)
   const uint8_t* mask;    // per node, fixed bits
   float kd0, kd1, kd2;
+  const double* esum64;   // per node (FP64 solver)
+  double kd64[3];
 };
 
 __device__ __forceinline__ void node_coords(int64_t n, const GridDims& g, int& ix, int& iy,

@@ -68,7 +68,8 @@ __global__ void k_simp(const double* __restrict__ x, double* __restrict__ E64,
 __global__ void k_inv_diag(const double* __restrict__ E64, const uint8_t* __restrict__ mask,
                            GridDims g, int nc, double kd0, double kd1, double kd2,
                            double* __restrict__ invd64, float* __restrict__ invd32,
-                           double* __restrict__ diag_out, float* __restrict__ esum32) {
+                           double* __restrict__ diag_out, float* __restrict__ esum32,
+                           double* __restrict__ esum64) {
   for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < g.nnodes;
        n += (int64_t)gridDim.x * blockDim.x) {
     int ix, iy, iz;
@@ -88,6 +89,7 @@ __global__ void k_inv_diag(const double* __restrict__ E64, const uint8_t* __rest
     const double kd[3] = {kd0, kd1, kd2};
     const float s32 = (float)s;
     if (esum32) esum32[n] = s32;
+    if (esum64) esum64[n] = s;
     for (int c = 0; c < nc; ++c) {
       const bool fixed = (m >> c) & 1;
       double d = kd[c] * s;
@@ -344,6 +346,7 @@ static int create_common(pyto_handle* h, int nelx, int nely, int nelz, double hx
       (rc = device_alloc(h, (void**)&h->invd64, nc * nn * 8)) ||
       (rc = device_alloc(h, (void**)&h->invd32, nc * nn * 4)) ||
       (nc == 3 && (rc = device_alloc(h, (void**)&h->esum32, nn * 4))) ||
+      (nc == 3 && (rc = device_alloc(h, (void**)&h->esum64, nn * 8))) ||
       (rc = device_alloc(h, (void**)&h->partials, 2 * kMaxPartials * 8)) ||
       (rc = device_alloc(h, (void**)&h->counters, kNumCounters * sizeof(unsigned))) ||
       (rc = device_alloc(h, (void**)&h->scal, kCtlBytes)))
@@ -379,7 +382,7 @@ int density_elements(Handle* h, const double* x, hipStream_t s) {
   blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 4096);
   hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(256), 0, s, h->Ee64, h->fixmask, h->g,
                      h->ncomp, h->kediag[0], h->kediag[1], h->kediag[2], h->invd64, h->invd32,
-                     (double*)nullptr, h->esum32);
+                     (double*)nullptr, h->esum32, h->esum64);
   PYTO_CHECK_HIP(hipGetLastError());
   h->density_set = true;
   return PYTO_OK;
@@ -473,7 +476,7 @@ int pyto_create_thermal(pyto_handle** out, int nelx, int nely, int nelz, double 
 int pyto_destroy(pyto_handle* h) {
   if (!h) return PYTO_OK;
   void* ptrs[] = {h->fixmask, h->tileflags, h->Ee64, h->Ee32, h->x64, h->invd64, h->invd32,
-                  h->esum32,
+                  h->esum32, h->esum64,
                   h->node_group, h->gmask, h->centers, h->sinfo, h->chunks, h->cpart,
                   h->gptr, h->gidx, h->gnodes, h->nchunk,
                   h->nchunk_grp, h->npart, h->nptr, h->nidx, h->Ec, h->Einv, h->cwork, h->ct,
@@ -524,7 +527,7 @@ int pyto_diagonal_f64(pyto_handle* h, double* d, void* stream) {
   int blocks = (int)std::min<int64_t>((h->g.nnodes + 255) / 256, 4096);
   hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(256), 0, as_stream(stream), h->Ee64,
                      h->fixmask, h->g, h->ncomp, h->kediag[0], h->kediag[1], h->kediag[2],
-                     (double*)nullptr, (float*)nullptr, d, (float*)nullptr);
+                     (double*)nullptr, (float*)nullptr, d, (float*)nullptr, (double*)nullptr);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }

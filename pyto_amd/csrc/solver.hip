@@ -71,17 +71,19 @@ struct Vec4<double> {
 // direction update applies (or, for the iteration that stops the solve, the
 // host after the loop).  With pKp <= 0 or NaN the block leaves z untouched
 // and alpha is published as 0 (the scalar step flags the breakdown).  Vector
-// loads of 16 bytes; block 0 takes the tail.  NODE (FP32, 3 values per
-// node): invd is not read; each DOF's inverse and d = 1/invd come from its
-// node's sum of E_e and fixed bits (jn, jacobi_inv32: the bits of invd32),
-// 5 bytes per node instead of 12, and r^T M^-1 r sums z^2 d.
+// loads of 16 bytes; block 0 takes the tail.  NODE (3 values per node):
+// invd is not read; each DOF's inverse comes from its node's sum of E_e and
+// fixed bits.  FP32: jacobi_inv32 (the bits of invd32), 5 bytes per node
+// instead of 12, and r^T M^-1 r sums z^2 d.  FP64: the operations that form
+// invd64 (the same bits, and the same rz terms), 9 bytes per node instead of
+// 24.
 template <typename T, bool NODE>
 __global__ __launch_bounds__(256) void k_cg_update_z(
     T* __restrict__ z, const T* __restrict__ q, const T* __restrict__ invd, JacobiNodes jn,
     int64_t n, double* __restrict__ scal, const int* __restrict__ flags,
     const double* __restrict__ part_pq, int npq, double* __restrict__ part_rz,
     double* __restrict__ al_slot) {
-  static_assert(!NODE || sizeof(T) == 4, "the node form is the FP32 update");
+
   if (flags[F_DONE]) return;
   __shared__ double red[5];
   const double pkp = block_sum_partials(part_pq, npq, red);
@@ -104,6 +106,15 @@ __global__ __launch_bounds__(256) void k_cg_update_z(
       const float di = jacobi_inv32(kd, s, (m >> c) & 1, d);
       zi = (T)((double)zi - alpha * ((double)di * (double)qi));
       acc += (double)zi * (double)zi * (double)d;
+    };
+    // FP64 node form: the inverse with invd64's operations (k_inv_diag), so
+    // the same bits as reading it, and the same rz terms as `lane`
+    auto nlane64 = [&](T& zi, const T qi, double s, uint8_t m, int c) {
+      double d = jn.kd64[c] * s;
+      if (fabs(d) < 1e-12) d = 1.0;
+      const double di = ((m >> c) & 1) ? 0.0 : 1.0 / d;
+      zi = (T)((double)zi - alpha * (di * (double)qi));
+      if (di > 0.0) acc += (double)zi * ((double)zi / di);
     };
     using V = typename Vec4<T>::type;
     constexpr int W = Vec4<T>::width;
@@ -138,6 +149,20 @@ __global__ __launch_bounds__(256) void k_cg_update_z(
         const int carry = c0 >= 3 ? 1 : 0;
         c0 -= 3 * carry;
         n0 += kn + carry;
+      } else if constexpr (NODE && W == 2) {
+        // DOFs 2i, 2i+1: component c0 of node n0, then c0 + 1 (node n0 + 1
+        // when c0 is 2)
+        const double s0 = jn.esum64[n0];
+        const uint8_t m0 = jn.mask[n0];
+        const bool two = c0 == 2;
+        const double s1 = two ? jn.esum64[n0 + 1] : s0;
+        const uint8_t m1 = two ? jn.mask[n0 + 1] : m0;
+        nlane64(zz.x, qq.x, s0, m0, c0);
+        nlane64(zz.y, qq.y, s1, m1, two ? 0 : c0 + 1);
+        c0 += kc;
+        const int carry = c0 >= 3 ? 1 : 0;
+        c0 -= 3 * carry;
+        n0 += kn + carry;
       } else {
         const V dd = dv[i];
         lane(zz.x, qq.x, dd.x);
@@ -151,8 +176,10 @@ __global__ __launch_bounds__(256) void k_cg_update_z(
     }
     if (blockIdx.x == 0)
       for (int64_t i = nv * W + threadIdx.x; i < n; i += blockDim.x) {
-        if constexpr (NODE)
+        if constexpr (NODE && sizeof(T) == 4)
           nlane(z[i], q[i], jn.esum[i / 3], jn.mask[i / 3], (int)(i % 3));
+        else if constexpr (NODE)
+          nlane64(z[i], q[i], jn.esum64[i / 3], jn.mask[i / 3], (int)(i % 3));
         else
           lane(z[i], q[i], invd[i]);
       }
@@ -457,9 +484,10 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   const int* done = h->flags + F_DONE;
   // FP32 with 3 values per node: the CG update forms the Jacobi from the
   // node sums (k_cg_update_z NODE)
-  const bool node_jacobi = sizeof(T) == 4 && h->esum32 != nullptr;
+  const bool node_jacobi = sizeof(T) == 4 ? h->esum32 != nullptr : h->esum64 != nullptr;
   const JacobiNodes jn{h->esum32, h->fixmask, (float)h->kediag[0], (float)h->kediag[1],
-                       (float)h->kediag[2]};
+                       (float)h->kediag[2], h->esum64,
+                       {h->kediag[0], h->kediag[1], h->kediag[2]}};
   // one iteration's launches on stream `st`
   // iteration k (from 0 in each run): p_k from slot k, p_{k+1} to slot k+1
   auto one_iteration = [&](hipStream_t st, int k) -> int {
@@ -468,13 +496,12 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     const int pos = k % kDirSlots;
     PYTO_TRY(launch_matvec<T>(h, pk, v.q, v.Ee, v.mc, part_pq, done, nullptr, 0, st));
     bool updated = false;
-    if constexpr (sizeof(T) == 4)
-      if (node_jacobi) {
-        hipLaunchKernelGGL((k_cg_update_z<T, true>), dim3(kVecBlocks), dim3(256), 0, st, v.r,
-                           v.q, v.invd, jn, n, h->scal, (const int*)h->flags,
-                           (const double*)part_pq, npq, part_rz, h->alring + pos);
-        updated = true;
-      }
+    if (node_jacobi) {
+      hipLaunchKernelGGL((k_cg_update_z<T, true>), dim3(kVecBlocks), dim3(256), 0, st, v.r,
+                         v.q, v.invd, jn, n, h->scal, (const int*)h->flags,
+                         (const double*)part_pq, npq, part_rz, h->alring + pos);
+      updated = true;
+    }
     if (!updated)
       hipLaunchKernelGGL((k_cg_update_z<T, false>), dim3(kVecBlocks), dim3(256), 0, st, v.r, v.q,
                          v.invd, jn, n, h->scal, (const int*)h->flags, (const double*)part_pq,
