@@ -443,6 +443,9 @@ def main():
                     help="skip the solves with the reference's 2000-group coarse space")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 32-design line reported beside `value`")
+    ap.add_argument("--thermal-only", action="store_true",
+                    help="only the thermal line (operator launches, Jacobi and deflated "
+                         "solves) on the workload's grid and densities (for kernel traces)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched 32-design line (for kernel traces)")
     ap.add_argument("--solve-only", action="store_true",
@@ -471,6 +474,11 @@ def main():
     from pyto_amd import builders
 
     cfg = problems.CONFIGS[args.config]
+    if args.thermal_only:
+        x_t = torch.from_numpy(problems.config_density(args.config, cfg["grid"])).to("cuda")
+        print(json.dumps({"thermal_only": True,
+                          "thermal": thermal_line(torch, torch.device("cuda"), cfg["grid"], x_t)}))
+        return
     grid = cfg["grid"]
     boxes = tuple(int(v) for v in args.boxes.split(","))
     dev = torch.device("cuda")
