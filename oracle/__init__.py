@@ -29,10 +29,16 @@ Pinning (what checks each function independently of its own formula):
   weights {1, 1/3, (1.5-sqrt 2)/1.5}, symmetry.
 * ``ocm.oc_update``: the bisection meets the volume constraint; move limits.
 
-Parity unpinned: the exact DPCG iteration trajectory of the reference
-(``src/deflation.py:504-584`` with SuperLU refactorised each iteration) and the
-deflation-group assignment on non-trivial grids (``src/deflation.py:86-229``)
-are restated from source only; the reference cannot be imported here (numba,
-pyvista, pyamg, numpy-stl are absent), so nothing outside this restatement
-confirms them.  Converged displacements and compliance do not depend on them.
+* ``deflation.create_groups`` (``src/deflation.py:86-229``): box counts per
+  axis decided in exact rational arithmetic, the node-to-box assignment against
+  an integer brute force, centres against closed-form lattice means, and the
+  small-box merge on hand-built grids written out by hand
+  (tests/test_oracle_groups.py).
+
+Beyond these pins, identity with the reference's own DPCG iterate sequence
+(``src/deflation.py:504-584``, SuperLU refactorised each iteration) is not
+checked element by element: the reference cannot be imported here (numba,
+pyvista, pyamg, numpy-stl are absent).  Converged displacements and compliance
+do not depend on the iterate sequence, and the oracle's DPCG is pinned by the
+direct solves above.
 """

@@ -19,8 +19,9 @@ its gradient at xval; the subproblem solution matches an independent dual
 solve (closed-form x(lambda), maximised dual) and satisfies the subproblem's
 KKT conditions; the outer loop reaches Svanberg's published optimum of the
 five-segment cantilever (f = 1.340) and the SLSQP optimum of a two-constraint
-toy problem.  No golden vector of mmapy exists in the reference, so the exact
-iterate sequence of mmapy itself is parity unpinned.
+toy problem.  No golden vector of mmapy exists in the reference, so identity
+with mmapy's own iterates, beyond these pins of the published algorithm, is
+not checked.
 """
 import numpy as np
 
