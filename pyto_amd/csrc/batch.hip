@@ -19,7 +19,10 @@
 // solve, so an inner run can continue the previous run's search direction
 // (reliable update: beta = r^T M r / rz_prev) while that run converged and its
 // recurrence residual still tracks the new true one (within 1.25 in norm);
-// otherwise it restarts from p = M r.
+// otherwise it restarts from p = M r.  As in the single-system solver, the
+// inner run stores z = M^-1 r instead of r: the CG update forms z -= alpha
+// M^-1 q with the inverse taken from each node's sum of E_e, and the
+// direction updates and restrictions read z alone.
 //
 // Deflation: a system whose handle has deflation groups runs the deflated PCG
 // of the single-system solver inside the same launches.  Each inner run starts
@@ -46,7 +49,12 @@ struct BatchSys {
   const double* invd64;
   const uint8_t* mask;
   const uint8_t* tileflags;
-  float *r, *p, *q, *x, *b;
+  // FP32 Jacobi from the node sums of E_e (jacobi_inv32): the handle's sums
+  // and the element's diagonal entry per component
+  const float* esum32;
+  float kd[3];
+  // the inner DPCG stores z = M^-1 r, as the single-system solver does
+  float *z, *p, *q, *x, *b;
   double *r64, *q64;
   const double* f;
   double* u;
@@ -185,7 +193,14 @@ __device__ __forceinline__ VecShare vec_share(const int* __restrict__ vpre, int 
   return v;
 }
 
-// x = 0, r = b, partials of r^T M r (systems still refining).
+// z = M^-1 r and the r^T M^-1 r term of one DOF from r (FP64) and the
+// stored inverse di: z = di r, term r di r.
+__device__ __forceinline__ float z_of_r(double r, float di, double& term) {
+  term = r * (double)di * r;
+  return (float)((double)di * r);
+}
+
+// x = 0, z = M^-1 b, partials of b^T M^-1 b (systems still refining).
 __global__ __launch_bounds__(256) void k_binit(const BatchSys* __restrict__ sys,
                                                const int* __restrict__ vpre, int nsys,
                                                const int* __restrict__ flags,
@@ -197,22 +212,28 @@ __global__ __launch_bounds__(256) void k_binit(const BatchSys* __restrict__ sys,
   const int64_t nv = S.ndof / 4;
   const float4* bv = reinterpret_cast<const float4*>(S.b);
   const float4* dv = reinterpret_cast<const float4*>(S.invd32);
-  float4* rv = reinterpret_cast<float4*>(S.r);
+  float4* zv = reinterpret_cast<float4*>(S.z);
   float4* xv = reinterpret_cast<float4*>(S.x);
-  double acc = 0.0;
+  double acc = 0.0, t;
   for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < nv; i += (int64_t)v.nb * 256) {
     const float4 bb = bv[i], dd = dv[i];
-    rv[i] = bb;
+    float4 zz;
+    zz.x = z_of_r((double)bb.x, dd.x, t);
+    acc += t;
+    zz.y = z_of_r((double)bb.y, dd.y, t);
+    acc += t;
+    zz.z = z_of_r((double)bb.z, dd.z, t);
+    acc += t;
+    zz.w = z_of_r((double)bb.w, dd.w, t);
+    acc += t;
+    zv[i] = zz;
     xv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    acc += (double)bb.x * dd.x * bb.x + (double)bb.y * dd.y * bb.y +
-           (double)bb.z * dd.z * bb.z + (double)bb.w * dd.w * bb.w;
   }
   if (v.lb == 0)
     for (int64_t i = nv * 4 + threadIdx.x; i < S.ndof; i += 256) {
-      const float bb = S.b[i];
-      S.r[i] = bb;
+      S.z[i] = z_of_r((double)S.b[i], S.invd32[i], t);
+      acc += t;
       S.x[i] = 0.f;
-      acc += (double)bb * S.invd32[i] * bb;
     }
   const double bs = block_reduce_sum256(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
@@ -249,9 +270,9 @@ __global__ __launch_bounds__(256) void k_bstart_x(const BatchSys* __restrict__ s
   }
 }
 
-// Start of a deflated inner run: r = b - q (q = K x0), partials of r^T M r,
-// in k_binit's order (a system without groups has q = 0 and gets k_binit's
-// bits).
+// Start of a deflated inner run: z = M^-1 (b - q) (q = K x0, r formed in
+// FP64), partials of r^T M^-1 r, in k_binit's order (a system without groups
+// has q = 0 and gets k_binit's bits).
 __global__ __launch_bounds__(256) void k_binit_q(const BatchSys* __restrict__ sys,
                                                  const int* __restrict__ vpre, int nsys,
                                                  const int* __restrict__ flags,
@@ -264,20 +285,25 @@ __global__ __launch_bounds__(256) void k_binit_q(const BatchSys* __restrict__ sy
   const float4* bv = reinterpret_cast<const float4*>(S.b);
   const float4* qv = reinterpret_cast<const float4*>(S.q);
   const float4* dv = reinterpret_cast<const float4*>(S.invd32);
-  float4* rv = reinterpret_cast<float4*>(S.r);
-  double acc = 0.0;
+  float4* zv = reinterpret_cast<float4*>(S.z);
+  double acc = 0.0, t;
   for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < nv; i += (int64_t)v.nb * 256) {
     const float4 bb = bv[i], qq = qv[i], dd = dv[i];
-    const float4 rr = make_float4(bb.x - qq.x, bb.y - qq.y, bb.z - qq.z, bb.w - qq.w);
-    rv[i] = rr;
-    acc += (double)rr.x * dd.x * rr.x + (double)rr.y * dd.y * rr.y +
-           (double)rr.z * dd.z * rr.z + (double)rr.w * dd.w * rr.w;
+    float4 zz;
+    zz.x = z_of_r((double)bb.x - (double)qq.x, dd.x, t);
+    acc += t;
+    zz.y = z_of_r((double)bb.y - (double)qq.y, dd.y, t);
+    acc += t;
+    zz.z = z_of_r((double)bb.z - (double)qq.z, dd.z, t);
+    acc += t;
+    zz.w = z_of_r((double)bb.w - (double)qq.w, dd.w, t);
+    acc += t;
+    zv[i] = zz;
   }
   if (v.lb == 0)
     for (int64_t i = nv * 4 + threadIdx.x; i < S.ndof; i += 256) {
-      const float ri = S.b[i] - S.q[i];
-      S.r[i] = ri;
-      acc += (double)ri * S.invd32[i] * ri;
+      S.z[i] = z_of_r((double)S.b[i] - (double)S.q[i], S.invd32[i], t);
+      acc += t;
     }
   const double bs = block_reduce_sum256(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
@@ -316,7 +342,11 @@ __global__ __launch_bounds__(256) void k_bsetup(const int* __restrict__ vpre, in
   }
 }
 
-// alpha = rz / p^T q (system's matvec partials), r -= alpha q, r^T M r partials.
+// alpha = rz / p^T q (the system's matvec partials), z -= alpha M^-1 q with
+// the inverse formed from the node sums (cg_z_node32, the single-system FP32
+// update's arithmetic), r^T M^-1 r partials.  A thread owns 16-byte vectors
+// (4 DOFs on nodes n0 and n0 + 1); n0 and the component c0 are formed once
+// and advanced by the loop's constant step.
 __global__ __launch_bounds__(256) void k_bcg_update(const BatchSys* __restrict__ sys,
                                                     const int* __restrict__ vpre,
                                                     const int* __restrict__ mpre, int nsys,
@@ -330,28 +360,47 @@ __global__ __launch_bounds__(256) void k_bcg_update(const BatchSys* __restrict__
   if (flags[v.s * kNumFlags + F_DONE]) return;
   const BatchSys& S = sys[v.s];
   const double pkp = block_sum_partials(part_mv + mpre[v.s], mpre[v.s + 1] - mpre[v.s], red);
-  const float a = (float)(scal[v.s * kNumScalars + S_RZ] / pkp);
+  // pKp <= 0 or NaN: z unchanged (the scalar step flags the breakdown)
+  const double alpha = pkp > 0.0 ? scal[v.s * kNumScalars + S_RZ] / pkp : 0.0;
   const int64_t nv = S.ndof / 4;
-  float4* rv = reinterpret_cast<float4*>(S.r);
+  float4* zv = reinterpret_cast<float4*>(S.z);
   const float4* qv = reinterpret_cast<const float4*>(S.q);
-  const float4* dv = reinterpret_cast<const float4*>(S.invd32);
-  double acc = 0.0;
-  for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < nv; i += (int64_t)v.nb * 256) {
-    float4 rr = rv[i];
-    const float4 qq = qv[i], dd = dv[i];
-    rr.x -= a * qq.x;
-    rr.y -= a * qq.y;
-    rr.z -= a * qq.z;
-    rr.w -= a * qq.w;
-    rv[i] = rr;
-    acc += (double)rr.x * dd.x * rr.x + (double)rr.y * dd.y * rr.y +
-           (double)rr.z * dd.z * rr.z + (double)rr.w * dd.w * rr.w;
+  const float kd[3] = {S.kd[0], S.kd[1], S.kd[2]};
+  double acc = 0.0, t;
+  const int64_t stride = (int64_t)v.nb * 256;
+  int64_t i = (int64_t)v.lb * 256 + threadIdx.x;
+  int64_t n0 = (4 * i) / 3;
+  int c0 = (int)(4 * i - 3 * n0);
+  const int64_t kn = (4 * stride) / 3;
+  const int kc = (int)(4 * stride - 3 * kn);
+  for (; i < nv; i += stride) {
+    float4 zz = zv[i];
+    const float4 qq = qv[i];
+    const float s0 = S.esum32[n0], s1 = S.esum32[n0 + 1];
+    const uint8_t m0 = S.mask[n0], m1 = S.mask[n0 + 1];
+    float* zl[4] = {&zz.x, &zz.y, &zz.z, &zz.w};
+    const float ql[4] = {qq.x, qq.y, qq.z, qq.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = c0 + j;
+      const bool second = c >= 3;
+      const int cl = second ? c - 3 : c;
+      *zl[j] = cg_z_node32(*zl[j], ql[j], kd[cl], second ? s1 : s0,
+                           ((second ? m1 : m0) >> cl) & 1, alpha, t);
+      acc += t;
+    }
+    zv[i] = zz;
+    c0 += kc;
+    const int carry = c0 >= 3 ? 1 : 0;
+    c0 -= 3 * carry;
+    n0 += kn + carry;
   }
   if (v.lb == 0)
-    for (int64_t i = nv * 4 + threadIdx.x; i < S.ndof; i += 256) {
-      const float ri = S.r[i] - a * S.q[i];
-      S.r[i] = ri;
-      acc += (double)ri * S.invd32[i] * ri;
+    for (int64_t d = nv * 4 + threadIdx.x; d < S.ndof; d += 256) {
+      const int64_t nd = d / 3;
+      const int c = (int)(d - 3 * nd);
+      S.z[d] = cg_z_node32(S.z[d], S.q[d], kd[c], S.esum32[nd], (S.mask[nd] >> c) & 1, alpha, t);
+      acc += t;
     }
   const double bs = block_reduce_sum256(acc, red4);
   if (threadIdx.x == 0) part_v[blockIdx.x] = bs;
@@ -375,10 +424,10 @@ __global__ __launch_bounds__(256) void k_bscalars(const int* __restrict__ vpre,
   if (threadIdx.x == 0) pcg_scalar_step(sc, fl, pkp, rz_new, kFloor32);
 }
 
-// x += alpha p (unless FIRST), p = invd o r + beta p.  FIRST: the start of an
-// inner run, p = invd o r, or with F_RESUME p = invd o r + (rz / rz_prev) p
-// (x is 0 then).  Also the direction update of the systems without groups in
-// a batch that has deflated ones, so that they take the same bits.
+// x += alpha p (unless FIRST), p = z + beta p.  FIRST: the start of an inner
+// run, p = z, or with F_RESUME p = z + (rz / rz_prev) p (x is 0 then).  Also
+// the direction update of the systems without groups in a batch that has
+// deflated ones, so that they take the same bits.
 template <bool FIRST>
 __device__ __forceinline__ void bdir_plain(const BatchSys& S, const VecShare& v,
                                            const double* sc, bool resume) {
@@ -387,30 +436,29 @@ __device__ __forceinline__ void bdir_plain(const BatchSys& S, const VecShare& v,
   const int64_t nv = S.ndof / 4;
   float4* pv = reinterpret_cast<float4*>(S.p);
   float4* xv = reinterpret_cast<float4*>(S.x);
-  const float4* rv = reinterpret_cast<const float4*>(S.r);
-  const float4* dv = reinterpret_cast<const float4*>(S.invd32);
+  const float4* zv = reinterpret_cast<const float4*>(S.z);
   for (int64_t i = (int64_t)v.lb * 256 + threadIdx.x; i < nv; i += (int64_t)v.nb * 256) {
-    const float4 rr = rv[i], dd = dv[i];
+    const float4 zz = zv[i];
     float4 pp = (FIRST && !resume) ? make_float4(0.f, 0.f, 0.f, 0.f) : pv[i];
     if (!FIRST) {
-      float4 xx = xv[i];
+      float4 xx = load_nt(xv + i);
       xx.x += al * pp.x;
       xx.y += al * pp.y;
       xx.z += al * pp.z;
       xx.w += al * pp.w;
-      xv[i] = xx;
+      store_nt(xv + i, xx);
     }
-    pp.x = dd.x * rr.x + be * pp.x;
-    pp.y = dd.y * rr.y + be * pp.y;
-    pp.z = dd.z * rr.z + be * pp.z;
-    pp.w = dd.w * rr.w + be * pp.w;
+    pp.x = zz.x + be * pp.x;
+    pp.y = zz.y + be * pp.y;
+    pp.z = zz.z + be * pp.z;
+    pp.w = zz.w + be * pp.w;
     pv[i] = pp;
   }
   if (v.lb == 0)
     for (int64_t i = nv * 4 + threadIdx.x; i < S.ndof; i += 256) {
       const float pp = (FIRST && !resume) ? 0.f : S.p[i];
       if (!FIRST) S.x[i] += al * pp;
-      S.p[i] = S.invd32[i] * S.r[i] + be * pp;
+      S.p[i] = S.z[i] + be * pp;
     }
 }
 
@@ -426,7 +474,7 @@ __global__ __launch_bounds__(256) void k_bdir(const BatchSys* __restrict__ sys,
 }
 
 // The direction update of deflated systems: x += alpha p (unless FIRST),
-// p = M r + beta p - W mu (FIRST: beta as in k_bdir), formed in FP64 as the
+// p = z + beta p - W mu (FIRST: beta as in k_bdir), formed in FP64 as the
 // single-system solver does (z and W mu cancel in near-void groups).  A block
 // works on one system; W mu of group q at position X is a_q + w_q x X with
 // w = (mu3, mu4, mu5), a = (mu0, mu1, mu2) - w x c_q, staged in LDS.
@@ -480,32 +528,64 @@ __global__ __launch_bounds__(256) void k_bdir_defl(const BatchSys* __restrict__ 
   const double al = FIRST ? 0.0 : sc[S_ALPHA];
   const double be = FIRST ? (resume ? sc[S_RZ] / sc[S_RZPREV] : 0.0) : sc[S_BETA];
   const bool read_p = !FIRST || resume;
-  auto dof = [&](int64_t d, int64_t node, const double* w3, uint32_t m) {
-    const int c = (int)(d - 3 * node);
-    const float pd = read_p ? S.p[d] : 0.f;
-    if (!FIRST) S.x[d] = (float)((double)S.x[d] + al * (double)pd);
-    const double wm = ((m >> c) & 1u) ? 0.0 : w3[c];
-    S.p[d] = (float)((double)S.invd32[d] * (double)S.r[d] + be * (double)pd - wm);
-  };
-  // one node (three DOFs) per thread: W mu once per node; the node's
-  // coordinates formed once and advanced by the loop's constant step
-  const int64_t nn = S.ndof / 3;
-  const int64_t step = (int64_t)v.nb * 256;
-  int64_t nd = (int64_t)v.lb * 256 + threadIdx.x;
+  // A thread owns 16-byte vectors (4 DOFs on nodes n0 and n0 + 1, the
+  // single-system FP32 direction update's layout): W mu and the fixed bits
+  // for both nodes, each DOF selecting its own; the node index, component and
+  // coordinates formed once and advanced by the loop's constant step.
+  const int64_t nv = S.ndof / 4;
+  const int64_t stride = (int64_t)v.nb * 256;
+  int64_t i = (int64_t)v.lb * 256 + threadIdx.x;
+  int64_t n0 = (4 * i) / 3;
+  int c0 = (int)(4 * i - 3 * n0);
+  const int64_t kn = (4 * stride) / 3;
+  const int kc = (int)(4 * stride - 3 * kn);
   int ix = 0, iy = 0, iz = 0, sxn = 0, syn = 0, szn = 0;
-  if (nd < nn) {
-    node_coords(nd, S.g, ix, iy, iz);
-    node_coords(step, S.g, sxn, syn, szn);
+  if (i < nv) {
+    node_coords(n0, S.g, ix, iy, iz);
+    node_coords(kn, S.g, sxn, syn, szn);
   }
-  for (; nd < nn; nd += step) {
-    // the node's group and fixed bits in one 16-bit word
-    const uint32_t a = S.gmask[nd];
-    double w3[3];
-    wmu3((int)(a & 0xfffu), ix, iy, iz, w3);
-    const uint32_t m = a >> 12;
+  float4* pv = reinterpret_cast<float4*>(S.p);
+  float4* xv = reinterpret_cast<float4*>(S.x);
+  const float4* zv = reinterpret_cast<const float4*>(S.z);
+  for (; i < nv; i += stride) {
+    // the next node's coordinates (n0 + 1 exists: DOF 4i + 3 lies on it)
+    const bool wx = ix + 1 == S.g.sx;
+    const int jx = wx ? 0 : ix + 1;
+    const bool wy = wx && iy + 1 == S.g.sy;
+    const int jy = wx ? (wy ? 0 : iy + 1) : iy;
+    const int jz = wy ? iz + 1 : iz;
+    // each node's group and fixed bits in one 16-bit word
+    const uint32_t a0 = S.gmask[n0], a1 = S.gmask[n0 + 1];
+    double w6[6];
+    wmu3((int)(a0 & 0xfffu), ix, iy, iz, w6);
+    wmu3((int)(a1 & 0xfffu), jx, jy, jz, w6 + 3);
+    const uint32_t fx = (a0 >> 12) | ((a1 >> 12) << 3);
+    float4 pp = read_p ? pv[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 zz = zv[i];
+    if (!FIRST) {
+      float4 xx = load_nt(xv + i);
+      xx.x = (float)((double)xx.x + al * (double)pp.x);
+      xx.y = (float)((double)xx.y + al * (double)pp.y);
+      xx.z = (float)((double)xx.z + al * (double)pp.z);
+      xx.w = (float)((double)xx.w + al * (double)pp.w);
+      store_nt(xv + i, xx);
+    }
+    float* pl[4] = {&pp.x, &pp.y, &pp.z, &pp.w};
+    const float zl[4] = {zz.x, zz.y, zz.z, zz.w};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) dof(3 * nd + c, nd, w3, m);
-    ix += sxn;
+    for (int j = 0; j < 4; ++j) {
+      // component c0 + j of the pair of nodes; c0 is 0, 1 or 2
+      const double w = c0 == 0 ? w6[j] : (c0 == 1 ? w6[j + 1] : w6[j + 2]);
+      const double wm = ((fx >> (c0 + j)) & 1u) ? 0.0 : w;
+      *pl[j] = (float)((double)zl[j] + be * (double)*pl[j] - wm);
+    }
+    pv[i] = pp;
+    // next trip: c0 + kc may carry one node (so may x and then y)
+    c0 += kc;
+    const int extra = c0 >= 3 ? 1 : 0;
+    c0 -= 3 * extra;
+    n0 += kn + extra;
+    ix += sxn + extra;
     iy += syn;
     iz += szn;
     if (ix >= S.g.sx) {
@@ -517,6 +597,21 @@ __global__ __launch_bounds__(256) void k_bdir_defl(const BatchSys* __restrict__ 
       ++iz;
     }
   }
+  // DOFs past the last whole vector, one at a time by the system's block 0
+  if (v.lb == 0)
+    for (int64_t d = nv * 4 + threadIdx.x; d < S.ndof; d += 256) {
+      const int64_t nd = d / 3;
+      const int c = (int)(d - 3 * nd);
+      int jx, jy, jz;
+      node_coords(nd, S.g, jx, jy, jz);
+      const uint32_t a = S.gmask[nd];
+      double w3[3];
+      wmu3((int)(a & 0xfffu), jx, jy, jz, w3);
+      const double wm = (((a >> 12) >> c) & 1u) ? 0.0 : w3[c];
+      const float pd = read_p ? S.p[d] : 0.f;
+      if (!FIRST) S.x[d] = (float)((double)S.x[d] + al * (double)pd);
+      S.p[d] = (float)((double)S.z[d] + be * (double)pd - wm);
+    }
 }
 
 // The x += alpha p that the stopping iteration leaves pending (converged
@@ -536,7 +631,7 @@ __global__ __launch_bounds__(256) void k_bfinal(const BatchSys* __restrict__ sys
 
 // ---- deflation ----------------------------------------------------------------
 // Chunk partials of W^T K v over a deflated system's special elements, v =
-// M r + beta p (beta: the iteration's, or at the start of an inner run the
+// z + beta p (z = M^-1 r as stored) (beta: the iteration's, or at the start of an inner run the
 // resumed direction's rz / rz_prev, 0 when it restarts): one block per special
 // block of each deflated system (prefix sums rpre), the single-system
 // restriction's staging and element arithmetic.
@@ -573,16 +668,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     (&csh[0][0])[threadIdx.x] = S.pcen[(int64_t)b * kSpecialBlockChunks * 24 + threadIdx.x];
   const double beta = start ? (fl[F_RESUME] ? sc[S_RZ] / sc[S_RZPREV] : 0.0) : sc[S_BETA];
   const bool has_p = beta != 0.0;
-  float ra[kStage], da[kStage], pa[kStage];
+  float za[kStage], pa[kStage];
 #pragma unroll
   for (int t = 0; t < kStage; ++t) {
-    ra[t] = S.r[dof[t]];
-    da[t] = S.invd32[dof[t]];
+    za[t] = S.z[dof[t]];
     pa[t] = has_p ? S.p[dof[t]] : 0.f;
   }
 #pragma unroll
   for (int t = 0; t < kStage; ++t)
-    vs[threadIdx.x + 256 * t] = (float)((double)(da[t] * ra[t]) + beta * (double)pa[t]);
+    vs[threadIdx.x + 256 * t] = (float)((double)za[t] + beta * (double)pa[t]);
   __syncthreads();
   if (pw.x < 0) return;
   const bool active = se.e >= 0;
@@ -1010,7 +1104,7 @@ static void launch_bmatvec_inner(Batch* b, hipStream_t s) {
   });
 }
 
-// mu = E^-1 W^T K (M r + beta p) for every deflated system still iterating:
+// mu = E^-1 W^T K (z + beta p) for every deflated system still iterating:
 // special restriction, per-group gather, coarse GEMV.
 static void launch_bcoarse(Batch* b, hipStream_t s, bool start) {
   const int nr = b->h_rpre[b->nsys], ng = b->h_gpre[b->nsys];
@@ -1141,7 +1235,9 @@ int pyto_batch_create(pyto_batch** out, int nsys, pyto_handle* const* handles, v
       hipMemsetAsync(p, 0, n32, s);
       f32[k] = (float*)p;
     }
-    S.r = f32[0];
+    S.z = f32[0];
+    S.esum32 = h->esum32;
+    for (int c = 0; c < 3; ++c) S.kd[c] = (float)h->kediag[c];
     S.p = f32[1];
     S.q = f32[2];
     S.x = f32[3];

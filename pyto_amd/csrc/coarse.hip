@@ -401,29 +401,6 @@ struct DirVec<double> {
   }
 };
 
-// Non-temporal 16-byte accesses, for x: the direction update reads and
-// writes it once per iteration (FP64: once per window) and nothing reads it
-// in between, so its lines are not kept; the vectors the next kernels read
-// (p, q, z, the element moduli) then stay in the last-level cache.
-typedef float pyto_f4 __attribute__((ext_vector_type(4)));
-typedef double pyto_d2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float4 load_nt(const float4* a) {
-  const pyto_f4 t = __builtin_nontemporal_load(reinterpret_cast<const pyto_f4*>(a));
-  return make_float4(t.x, t.y, t.z, t.w);
-}
-__device__ __forceinline__ double2 load_nt(const double2* a) {
-  const pyto_d2 t = __builtin_nontemporal_load(reinterpret_cast<const pyto_d2*>(a));
-  return make_double2(t.x, t.y);
-}
-__device__ __forceinline__ void store_nt(float4* a, const float4& v) {
-  const pyto_f4 t = {v.x, v.y, v.z, v.w};
-  __builtin_nontemporal_store(t, reinterpret_cast<pyto_f4*>(a));
-}
-__device__ __forceinline__ void store_nt(double2* a, const double2& v) {
-  const pyto_d2 t = {v.x, v.y};
-  __builtin_nontemporal_store(t, reinterpret_cast<pyto_d2*>(a));
-}
-
 constexpr int kLdsGroups = 512;
 // Largest dense coarse space: 6 x 4096 = 24576 unknowns, whose FP64 matrix
 // and inverse take 4.8 GB each.  The reference's own heuristic asks for at

@@ -501,6 +501,42 @@ struct JacobiNodes {
   double kd64[3];
 };
 
+// Non-temporal 16-byte accesses, for x: the direction updates read and
+// write it once per iteration (FP64: once per window) and nothing reads it
+// in between, so its lines are not kept; the vectors the next kernels read
+// (p, q, z, the element moduli) then stay in the last-level cache.
+typedef float pyto_f4 __attribute__((ext_vector_type(4)));
+typedef double pyto_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 load_nt(const float4* a) {
+  const pyto_f4 t = __builtin_nontemporal_load(reinterpret_cast<const pyto_f4*>(a));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+__device__ __forceinline__ double2 load_nt(const double2* a) {
+  const pyto_d2 t = __builtin_nontemporal_load(reinterpret_cast<const pyto_d2*>(a));
+  return make_double2(t.x, t.y);
+}
+__device__ __forceinline__ void store_nt(float4* a, const float4& v) {
+  const pyto_f4 t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<pyto_f4*>(a));
+}
+__device__ __forceinline__ void store_nt(double2* a, const double2& v) {
+  const pyto_d2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<pyto_d2*>(a));
+}
+
+// One DOF of the z-form CG update z - alpha M^-1 q with the FP32 Jacobi
+// inverse formed from the node's sum of E_e (jacobi_inv32), and its
+// r^T M^-1 r term z^2 d (d the diagonal entry, 0 on fixed DOFs).
+__device__ __forceinline__ float cg_z_node32(float z, float q, float kd, float s, bool fixed,
+                                             double alpha, double& term) {
+  float d;
+  const float di = jacobi_inv32(kd, s, fixed, d);
+  const float zn = (float)((double)z - alpha * ((double)di * (double)q));
+  term = (double)zn * (double)zn * (double)d;
+  return zn;
+}
+
+
 __device__ __forceinline__ void node_coords(int64_t n, const GridDims& g, int& ix, int& iy,
                                             int& iz) {
   const double plane = (double)g.sx * (double)g.sy;
