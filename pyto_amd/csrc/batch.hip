@@ -838,11 +838,30 @@ __global__ __launch_bounds__(256) void k_bgemv(const BatchSys* __restrict__ sys,
   double acc[kBatchRhs];
 #pragma unroll
   for (int k = 0; k < kBatchRhs; ++k) acc[k] = 0.0;
-  for (int j = lane; j < S.nc; j += 64) {
-    const double aj = a[j];
+  // the row's loads of 4 trips go out together; each lane adds its terms in
+  // the order of j
+  // (indices past the row are clamped to its last entry, their terms left out)
+  constexpr int U = 4;
+  for (int j0 = lane; j0 < S.nc; j0 += 64 * U) {
+    double av[U];
+    int jj[U];
 #pragma unroll
-    for (int k = 0; k < kBatchRhs; ++k)
-      if (act[k]) acc[k] += aj * t[k][j];
+    for (int u = 0; u < U; ++u) {
+      jj[u] = min(j0 + 64 * u, S.nc - 1);
+      av[u] = a[jj[u]];
+    }
+#pragma unroll
+    for (int k = 0; k < kBatchRhs; ++k) {
+      if (!act[k]) continue;
+      double tv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) tv[u] = t[k][jj[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double p = av[u] * tv[u];
+        if (j0 + 64 * u < S.nc) acc[k] += p;
+      }
+    }
   }
 #pragma unroll
   for (int k = 0; k < kBatchRhs; ++k) {

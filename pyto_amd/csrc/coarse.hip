@@ -1203,10 +1203,26 @@ __global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ A, int 
   const double beta = (x2 && beta_ptr) ? *beta_ptr : 0.0;
   double s = 0.0;
   const double* a = A + (int64_t)row * ld;
-  if (x2)
-    for (int j = lane; j < n; j += 64) s += a[j] * (x[j] + beta * x2[j]);
-  else
-    for (int j = lane; j < n; j += 64) s += a[j] * x[j];
+  // the loads of 8 trips go out together (a plain loop waits for each trip's
+  // loads in turn); each lane still adds its terms in the order of j
+  // (indices past the row are clamped to its last entry, and their terms
+  // left out)
+  constexpr int U = 8;
+  for (int j0 = lane; j0 < n; j0 += 64 * U) {
+    double av[U], xv[U], yv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = min(j0 + 64 * u, n - 1);
+      av[u] = a[j];
+      xv[u] = x[j];
+      yv[u] = x2 ? x2[j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double t = av[u] * (x2 ? xv[u] + beta * yv[u] : xv[u]);
+      if (j0 + 64 * u < n) s += t;
+    }
+  }
   s = wave_reduce_sum(s);
   if (lane == 0) y[row] = (b ? b[row] : 0.0) + sign * s;
 }
