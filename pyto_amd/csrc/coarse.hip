@@ -529,6 +529,9 @@ __global__ __launch_bounds__(256) void k_dir_update(
 #pragma unroll
       for (int j = 0; j < W; ++j) DV::set(xx, j, (T)acc[j]);
       store_nt(xv + i, xx);
+      // the window's first slot is the new direction's: its loads above stay
+      // before the direction's store below (the pointers are restrict-qualified)
+      asm volatile("" ::: "memory");
     } else if (alpha != 0.0) {
       V xx = load_nt(xv + i);
 #pragma unroll
@@ -681,6 +684,9 @@ __global__ __launch_bounds__(256) void k_dir_update_nodes(
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) __builtin_nontemporal_store((T)acc[c], xn + c);
+      // the window's first slot is the new direction's (po): its loads above
+      // stay before the stores below (the pointers are restrict-qualified)
+      asm volatile("" ::: "memory");
     } else if (alpha != 0.0) {
       T xx[3];
 #pragma unroll
@@ -991,6 +997,9 @@ __global__ __launch_bounds__(256) void k_dir_update_t(
 #pragma unroll
       for (int j = 0; j < W; ++j) DV::set(xx, j, (T)acc[j]);
       store_nt(xv + i, xx);
+      // the window's first slot is the new direction's: its loads above stay
+      // before the direction's store below (the pointers are restrict-qualified)
+      asm volatile("" ::: "memory");
     } else if (alpha != 0.0) {
       V xx = load_nt(xv + i);
 #pragma unroll

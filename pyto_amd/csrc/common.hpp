@@ -213,8 +213,12 @@ constexpr int kMaxPartials = 8192;
 
 // The DPCG keeps its last kDirSlots directions in a ring (slot k mod
 // kDirSlots holds p_k) and adds alpha_k p_k to x once per kDirSlots
-// iterations instead of on every iteration.
+// iterations instead of on every iteration (FP64).  The FP32 solver keeps
+// one slot (kDirSlots32) and updates x on every iteration: its vectors stay
+// largely in the last-level cache, which even a second slot defeats.
 constexpr int kDirSlots = 8;
+constexpr int kDirSlots32 = 1;
+static_assert(kDirSlots32 >= 1 && kDirSlots32 <= kDirSlots, "the FP32 ring fits the alphas");
 // Elements between ring slots for n-element DOF vectors: n rounded up to a
 // multiple of 4, so that every slot starts on a 16-byte boundary (the vector
 // kernels read 16 bytes per thread).

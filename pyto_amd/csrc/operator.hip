@@ -290,8 +290,7 @@ int ensure_solver_workspace(Handle* h) {
   PYTO_TRY(device_alloc(h, (void**)&h->x32, nd * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->b32, nd * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->pring64, kDirSlots * ring_stride(nd) * 8));
-  // the FP32 DPCG keeps its direction in place: one slot
-  PYTO_TRY(device_alloc(h, (void**)&h->pring32, ring_stride(nd) * 4));
+  PYTO_TRY(device_alloc(h, (void**)&h->pring32, kDirSlots32 * ring_stride(nd) * 4));
   PYTO_TRY(device_alloc(h, (void**)&h->alring, kDirSlots * 8));
   return PYTO_OK;
 }

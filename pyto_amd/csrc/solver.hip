@@ -428,18 +428,19 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   const int64_t n = h->ncomp * h->g.nnodes;
   const bool defl = h->ngroups > 0;
   double* ct = h->ct;
-  // Directions and alphas.  alpha_k goes to alring[k mod kDirSlots], k
-  // counted from the run's first iteration.  FP64 defers x: p_k goes to ring slot k mod kDirSlots and x takes
-  // a window's alpha_k p_k at once, from the direction update of the window's
-  // last iteration (the FP64 update moves 40 B per DOF and the x pass is 16
-  // of them).  FP32 updates p in place in slot 0 and x on every iteration:
-  // its vectors (51 MB each at 12.8M DOF) stay largely in the 256 MB
-  // last-level cache, which a rotating ring defeats; deferred, it measured
-  // slower.
-  const bool defer = sizeof(T) == 8;
+  // Directions and alphas.  x is deferred over a ring of R directions: p_k
+  // goes to ring slot k mod R, alpha_k to alring[k mod R] (k counted from the
+  // run's first iteration), and x takes a window's alpha_k p_k at once, from
+  // the direction update of the window's last iteration.  FP64: R = 8 (its
+  // update moves 40 B per DOF and the x pass is 16 of them).  FP32: R =
+  // kDirSlots32 = 1, p updated in place and x on every iteration; its vectors
+  // (51 MB each at 12.8M DOF) stay largely in the 256 MB last-level cache,
+  // which a ring of 2 or 8 defeats (both measured slower).
+  const int R = sizeof(T) == 8 ? kDirSlots : kDirSlots32;
+  const bool defer = R > 1;
   const int tslot = sizeof(T) == 8 ? 0 : 1;
   const int64_t ld = ring_stride(n);   // slots start on 16-byte boundaries
-  auto slot = [&](int k) { return defer ? v.ring + (int64_t)(k % kDirSlots) * ld : v.ring; };
+  auto slot = [&](int k) { return defer ? v.ring + (int64_t)(k % R) * ld : v.ring; };
   T* p_prev = slot(h->ring_last[tslot]);
   PYTO_TRY(ensure_control(h));
   if (defl) {
@@ -493,7 +494,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   auto one_iteration = [&](hipStream_t st, int k) -> int {
     T* pk = slot(k);
     T* pk1 = slot(k + 1);
-    const int pos = k % kDirSlots;
+    const int pos = k % R;
     PYTO_TRY(launch_matvec<T>(h, pk, v.q, v.Ee, v.mc, part_pq, done, nullptr, 0, st));
     bool updated = false;
     if (node_jacobi) {
@@ -519,7 +520,7 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
     }
     // deferred: x takes the window's alpha p at its last position
     const DirIO<T> io{x, pk, pk1, v.r, !defer, v.ring, ld, h->alring,
-                      (defer && pos == kDirSlots - 1) ? kDirSlots : 0};
+                      (defer && pos == R - 1) ? R : 0};
     PYTO_TRY(launch_dir_update<T>(h, io, false, done, st));
     PYTO_CHECK_HIP(hipGetLastError());
     return PYTO_OK;
@@ -585,13 +586,13 @@ static int dpcg(Handle* h, const T* b, T* x, double rtol, int max_iters, DpcgRes
   // publishes alpha 0.
   const int a = h->host_flags[F_ITERS];
   const int dirs = h->host_flags[F_DONE] ? std::max(0, a - 1) : a;
-  const int a0 = defer ? 0 : (a - 1 + kDirSlots) % kDirSlots;
-  const int count = defer ? a - (dirs / kDirSlots) * kDirSlots : a - dirs;
+  const int a0 = defer ? 0 : (a - 1 + R) % R;
+  const int count = defer ? a - (dirs / R) * R : a - dirs;
   if (count > 0)
     hipLaunchKernelGGL(k_flush_x<T>, dim3(kVecBlocks), dim3(256), 0, s, x, (const T*)v.ring, ld,
                        (const double*)h->alring, a0, defer ? 0 : -1, count, n);
   // p_dirs, the last direction formed, for a resumed run
-  h->ring_last[tslot] = defer ? dirs % kDirSlots : 0;
+  h->ring_last[tslot] = defer ? dirs % R : 0;
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
