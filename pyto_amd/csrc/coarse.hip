@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -1244,6 +1245,124 @@ __global__ void k_axpy_small(double* __restrict__ y, const double* __restrict__ 
 }
 
 
+// Symmetric GEMV for large coarse spaces.  E^-1 is symmetric, so its upper
+// triangle of 64 x 64 blocks (nb (nb + 1) / 2 blocks, block (I, J), I <= J,
+// at index I nb - I (I - 1) / 2 + J - I, row-major inside, entries past the
+// order n zero) holds all of it: the GEMV reads half the bytes of the dense
+// one.  One workgroup per stored block writes A_IJ x_J to spart[I][J] and,
+// off the diagonal, A_IJ^T x_I to spart[J][I]; a second launch sums
+// spart[I][J] over J in order for every row: no atomics, fixed order.
+constexpr int kSymBlock = 64;
+// coarse orders from which the packed form is used (the dense GEMV is
+// bandwidth-bound there: 12,000 unknowns are a 1.15 GB inverse)
+constexpr int kSymMinOrder = 6144;
+
+__device__ __forceinline__ int64_t sym_block_index(int I, int J, int nb) {
+  return (int64_t)I * nb - (int64_t)I * (I - 1) / 2 + (J - I);
+}
+
+__global__ __launch_bounds__(256) void k_pack_upper(const double* __restrict__ A, int ld, int n,
+                                                    int nb, double* __restrict__ P) {
+  // one workgroup per stored block; blockIdx.x = I, blockIdx.y = J - I
+  const int I = blockIdx.x, J = I + blockIdx.y;
+  if (J >= nb) return;
+  double* dst = P + sym_block_index(I, J, nb) * (kSymBlock * kSymBlock);
+  for (int e = threadIdx.x; e < kSymBlock * kSymBlock; e += blockDim.x) {
+    const int r = I * kSymBlock + e / kSymBlock, c = J * kSymBlock + e % kSymBlock;
+    dst[e] = (r < n && c < n) ? A[(int64_t)r * ld + c] : 0.0;
+  }
+}
+
+// spart[I][J][0..63] = A_IJ x_J; spart[J][I][0..63] = A_IJ^T x_I (I < J).
+// The block is staged in LDS (rows padded to 65 doubles), x_I and x_J beside
+// it; threads 0..63 form the row sums, threads 64..127 the column sums.
+__global__ __launch_bounds__(256) void k_symv_blocks(const double* __restrict__ P, int nb, int n,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ spart,
+                                                     const int* __restrict__ done) {
+  if (done && *done) return;
+  const int I = blockIdx.x, J = I + blockIdx.y;
+  if (J >= nb) return;
+  __shared__ double a[kSymBlock][kSymBlock + 1];
+  __shared__ double xi[kSymBlock], xj[kSymBlock];
+  const double* src = P + sym_block_index(I, J, nb) * (kSymBlock * kSymBlock);
+  double v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = src[threadIdx.x + 256 * k];
+  if (threadIdx.x < kSymBlock) {
+    const int ri = I * kSymBlock + threadIdx.x, rj = J * kSymBlock + threadIdx.x;
+    xi[threadIdx.x] = ri < n ? x[ri] : 0.0;
+    xj[threadIdx.x] = rj < n ? x[rj] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int e = threadIdx.x + 256 * k;
+    a[e / kSymBlock][e % kSymBlock] = v[k];
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < kSymBlock) {
+    double sr = 0.0;
+    for (int c = 0; c < kSymBlock; ++c) sr += a[t][c] * xj[c];
+    spart[((int64_t)I * nb + J) * kSymBlock + t] = sr;
+  } else if (t < 2 * kSymBlock && I != J) {
+    const int c = t - kSymBlock;
+    double sc = 0.0;
+    for (int r = 0; r < kSymBlock; ++r) sc += a[r][c] * xi[r];
+    spart[((int64_t)J * nb + I) * kSymBlock + c] = sc;
+  }
+}
+
+// y[r] = sum over J of spart[r / 64][J][r % 64], J in order (rows < n).
+__global__ __launch_bounds__(256) void k_symv_sum(const double* __restrict__ spart, int nb, int n,
+                                                  double* __restrict__ y,
+                                                  const int* __restrict__ done) {
+  if (done && *done) return;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int I = r / kSymBlock, l = r % kSymBlock;
+  const double* p = spart + (int64_t)I * nb * kSymBlock + l;
+  double s = 0.0;
+  // the loads of 8 block columns go out together; the sum runs over J in
+  // order (clamped indices past the last column are left out)
+  constexpr int U = 8;
+  for (int J0 = 0; J0 < nb; J0 += U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = p[(int64_t)min(J0 + u, nb - 1) * kSymBlock];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (J0 + u < nb) s += v[u];
+  }
+  y[r] = s;
+}
+
+// The packed copy of a large coarse inverse (after every factorisation).
+// PYTO_SYMV_MIN_ORDER=n moves the threshold (tests use it on small spaces).
+static int pack_coarse_inverse(Handle* h, hipStream_t s) {
+  const char* env = getenv("PYTO_SYMV_MIN_ORDER");
+  const int min_order = env ? std::max(1, atoi(env)) : kSymMinOrder;
+  if (h->nc < min_order) {
+    h->pk_nb = 0;   // the dense GEMV (a kept packed copy is not refreshed)
+    return PYTO_OK;
+  }
+  const int ld = round_up64(h->nc), nb = ld / kSymBlock;
+  if (!h->Epk || h->pk_nb != nb) {
+    if (h->Epk) hipFree(h->Epk);
+    if (h->spart) hipFree(h->spart);
+    h->Epk = h->spart = nullptr;
+    h->pk_nb = 0;
+    const size_t nblk = (size_t)nb * (nb + 1) / 2;
+    PYTO_TRY(device_alloc(h, (void**)&h->Epk, nblk * kSymBlock * kSymBlock * sizeof(double)));
+    PYTO_TRY(device_alloc(h, (void**)&h->spart, (size_t)nb * nb * kSymBlock * sizeof(double)));
+    h->pk_nb = nb;
+  }
+  hipLaunchKernelGGL(k_pack_upper, dim3(nb, nb), dim3(256), 0, s, (const double*)h->Einv, ld,
+                     h->nc, nb, h->Epk);
+  PYTO_CHECK_HIP(hipGetLastError());
+  return PYTO_OK;
+}
+
 // mu = E^{-1} (t1 + beta t2) with the FP64 inverse (t2/beta optional).  With
 // `refine`, one step of refinement on the coarse residual follows:
 // r = t - E mu; mu += E^{-1} r (used by the public coarse solve; the PCG
@@ -1252,6 +1371,15 @@ int coarse_apply(Handle* h, const double* t1, const double* t2, const double* be
                  double* mu, const int* done, hipStream_t s, bool refine) {
   const int n = h->nc, ld = round_up64(h->nc);
   const int blocks = (n * 64 + 255) / 256;
+  if (h->pk_nb > 0 && t2 == nullptr && !refine) {
+    const int nb = h->pk_nb;
+    hipLaunchKernelGGL(k_symv_blocks, dim3(nb, nb), dim3(256), 0, s, (const double*)h->Epk, nb,
+                       n, t1, h->spart, done);
+    hipLaunchKernelGGL(k_symv_sum, dim3((n + 255) / 256), dim3(256), 0, s,
+                       (const double*)h->spart, nb, n, mu, done);
+    PYTO_CHECK_HIP(hipGetLastError());
+    return PYTO_OK;
+  }
   hipLaunchKernelGGL(k_gemv, dim3(blocks), dim3(256), 0, s, h->Einv, ld, n, t1, t2, beta,
                      (const double*)nullptr, 1.0, mu, done);
   if (refine) {
@@ -1486,6 +1614,7 @@ int coarse_setup(Handle* h, hipStream_t s) {
   PYTO_TRY(coarse_assemble(h, s));
   PYTO_CHECK_HIP(hipMemsetAsync(h->flags + F_COARSE_PIVOT, 0, sizeof(int), s));
   PYTO_TRY(factorize_coarse(h, s));
+  PYTO_TRY(pack_coarse_inverse(h, s));
   h->coarse_ready = true;
   return PYTO_OK;
 }
@@ -1592,7 +1721,11 @@ int set_density_many(int n, Handle* const* hs, const double* const* x, hipStream
       set_error("batched coarse inverse failed");
       break;
     }
-    for (Handle* h : defl) h->coarse_ready = true;
+    for (Handle* h : defl) {
+      if ((rc = pack_coarse_inverse(h, s)) != PYTO_OK) break;
+      h->coarse_ready = true;
+    }
+    if (rc != PYTO_OK) break;
   } while (false);
   release();
   return rc;
@@ -1603,7 +1736,7 @@ static void release_coarse(Handle* h) {
   void* olds[] = {h->node_group, h->gmask, h->centers, h->sinfo, h->chunks, h->cpart,
                   h->gptr, h->gidx,
                   h->gnodes, h->nchunk, h->nchunk_grp, h->npart, h->nptr, h->nidx,
-                  h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres,
+                  h->Ec, h->Einv, h->cwork, h->ct, h->cmu, h->cres, h->Epk, h->spart,
                   h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
                   h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32};
   for (void* p : olds)
@@ -1615,6 +1748,8 @@ static void release_coarse(Handle* h) {
   h->gnodes = nullptr; h->nchunk = nullptr; h->nchunk_grp = nullptr; h->npart = nullptr;
   h->nptr = nullptr; h->nidx = nullptr; h->nnchunks = 0;
   h->Ec = h->Einv = h->cwork = h->ct = h->cmu = h->cres = nullptr;
+  h->Epk = h->spart = nullptr;
+  h->pk_nb = 0;
   h->gal_off = nullptr; h->gal_part = nullptr; h->gal_pairs = nullptr; h->gal_pptr = nullptr;
   h->gal_pidx = nullptr; h->npairs = 0;
   h->pwave = nullptr; h->pslot = nullptr; h->pnodes = nullptr; h->pcen = nullptr;

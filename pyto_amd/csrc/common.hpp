@@ -172,6 +172,11 @@ struct Handle {
   int32_t* gal_pidx = nullptr;
   double* Ec = nullptr;         // coarse matrix (nc x nc), then Cholesky factor
   double* Einv = nullptr;       // coarse inverse (nc x nc)
+  // large coarse spaces: the inverse's upper-triangle 64 x 64 blocks packed
+  // (kSymBlock), and the block partials of the symmetric GEMV
+  double* Epk = nullptr;
+  double* spart = nullptr;
+  int pk_nb = 0;                // block rows of Epk (0: not packed)
   double* cwork = nullptr;      // ld workspace (coarse refinement correction)
   void* blas = nullptr;         // rocblas_handle of the dense coarse factorisation
   double* ct = nullptr;         // coarse rhs: W^T K z (ld), then W^T K p (ld) at ct + ld

@@ -479,7 +479,7 @@ int pyto_destroy(pyto_handle* h) {
                   h->node_group, h->gmask, h->centers, h->sinfo, h->chunks, h->cpart,
                   h->gptr, h->gidx, h->gnodes, h->nchunk,
                   h->nchunk_grp, h->npart, h->nptr, h->nidx, h->Ec, h->Einv, h->cwork, h->ct,
-                  h->cmu, h->cres, h->r64, h->q64, h->r32, h->q32, h->x32,
+                  h->cmu, h->cres, h->Epk, h->spart, h->r64, h->q64, h->r32, h->q32, h->x32,
                   h->b32, h->partials, h->counters, h->scal /* and flags */, h->elem_buf,
                   h->gal_off, h->gal_part, h->gal_pairs, h->gal_pptr, h->gal_pidx,
                   h->pwave, h->pslot, h->pnodes, h->pcen, h->pE64, h->pE32, h->stencil,

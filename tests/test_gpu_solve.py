@@ -195,3 +195,37 @@ def test_more_groups_than_the_lds_table(gpu):
         assert st["converged"] and st["iterations"] < s_ref["iterations"]
         assert abs(float(torch.dot(f, u)) - C_ref) <= 1e-8 * abs(C_ref)
         assert rel_err(u.cpu().numpy(), u_ref.cpu().numpy()) <= 1e-6
+
+
+def test_packed_symmetric_coarse_inverse(gpu, monkeypatch):
+    """Large coarse spaces apply E^-1 from the packed 64 x 64 blocks of its
+    upper triangle (half the bytes of the dense GEMV; block partials summed
+    in a fixed order).  Forced here on 640 groups (3840 coarse unknowns, the
+    last block row partly padding): the FP64 and mixed deflated solves agree
+    with the dense-GEMV solves."""
+    from pyto_amd import deflation, problems
+    from pyto_amd.operator import GridOperator
+    print("packed coarse inverse", flush=True)
+    grid = (48, 24, 24)
+    mesh, _, bc = problems.cantilever(grid)
+    x = problems.density_uniform(mesh.num_elems, 0.05, 1.0, 9)
+    f = torch.from_numpy(bc.force).to(gpu)
+    out = {}
+    for order in ("100000000", "1"):
+        monkeypatch.setenv("PYTO_SYMV_MIN_ORDER", order)
+        op = GridOperator(grid, (1, 1, 1), 1.0, 0.3, bc.fixed_dofs, eps_void=1e-9)
+        op.set_coarse_shift(0.0, 0.0)
+        ds = deflation.DeflationSolver()
+        ds.create_box_groups(mesh, (10, 8, 8))
+        ds.attach(op)
+        op.set_density(x)      # packs the inverse when its order reaches the threshold
+        for prec in ("fp64", "mixed"):
+            out[order, prec] = op.solve(f, rtol=1e-10, max_iters=5000, precision=prec)
+    for prec in ("fp64", "mixed"):
+        (u_d, st_d), (u_p, st_p) = out["100000000", prec], out["1", prec]
+        print(prec, st_d["iterations"], st_p["iterations"], flush=True)
+        assert st_d["converged"] and st_p["converged"]
+        assert abs(st_p["iterations"] - st_d["iterations"]) <= 2
+        C_d, C_p = float(torch.dot(f, u_d)), float(torch.dot(f, u_p))
+        assert abs(C_p - C_d) <= 1e-9 * abs(C_d)
+        assert rel_err(u_p.cpu().numpy(), u_d.cpu().numpy()) <= 1e-7
