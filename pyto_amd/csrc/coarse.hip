@@ -1253,9 +1253,11 @@ __global__ void k_axpy_small(double* __restrict__ y, const double* __restrict__ 
 // off the diagonal, A_IJ^T x_I to spart[J][I]; a second launch sums
 // spart[I][J] over J in order for every row: no atomics, fixed order.
 constexpr int kSymBlock = 64;
-// coarse orders from which the packed form is used (the dense GEMV is
-// bandwidth-bound there: 12,000 unknowns are a 1.15 GB inverse)
-constexpr int kSymMinOrder = 6144;
+// coarse orders from which the packed form is used.  Measured on the
+// 256 x 128 x 128 workload (mixed solve): 432 groups (2592 unknowns) 46.9 ->
+// 46.2 ms, 1024 groups 51.7 -> 46.4 ms, 2000 groups 66.4 -> 53.2 ms; below
+// that the dense GEMV's single launch wins.
+constexpr int kSymMinOrder = 2048;
 
 __device__ __forceinline__ int64_t sym_block_index(int I, int J, int nb) {
   return (int64_t)I * nb - (int64_t)I * (I - 1) / 2 + (J - I);

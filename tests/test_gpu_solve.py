@@ -200,9 +200,9 @@ def test_more_groups_than_the_lds_table(gpu):
 def test_packed_symmetric_coarse_inverse(gpu, monkeypatch):
     """Large coarse spaces apply E^-1 from the packed 64 x 64 blocks of its
     upper triangle (half the bytes of the dense GEMV; block partials summed
-    in a fixed order).  Forced here on 640 groups (3840 coarse unknowns, the
-    last block row partly padding): the FP64 and mixed deflated solves agree
-    with the dense-GEMV solves."""
+    in a fixed order).  On 640 groups (3840 coarse unknowns, the last block
+    row partly padding), packed by default, against the dense GEMV forced by
+    a threshold above the order: the FP64 and mixed deflated solves agree."""
     from pyto_amd import deflation, problems
     from pyto_amd.operator import GridOperator
     print("packed coarse inverse", flush=True)
