@@ -451,6 +451,9 @@ def main():
     ap.add_argument("--solve-only", action="store_true",
                     help="only the warm-up, --steps mixed solves and --fp64-steps FP64 solves "
                          "of the workload (for kernel traces); prints their times")
+    ap.add_argument("--reference-groups-only", action="store_true",
+                    help="only the workload with the reference's own coarse space (2000 "
+                         "groups) after one warm-up solve (for kernel traces)")
     ap.add_argument("--tile-rows", type=int, default=16,
                     help="operator tile rows for --operator-only (8, 16 or 32)")
     ap.add_argument("--operator-only", action="store_true",
@@ -559,6 +562,11 @@ def main():
         _, st = op.solve(f, rtol=1e-8, max_iters=5000, precision=precision,
                          inner_rtol=args.inner_rtol, out=u)
         return st
+
+    if args.reference_groups_only:
+        print(json.dumps({"reference_groups_only": True,
+                          "reference_groups": reference_groups_line(op, mesh, f, u, step, dev)}))
+        return
 
     for _ in range(args.warmup):
         st = step("mixed")

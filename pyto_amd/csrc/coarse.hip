@@ -1275,68 +1275,92 @@ __global__ __launch_bounds__(256) void k_pack_upper(const double* __restrict__ A
   }
 }
 
+// Row and column of stored block b (b = I nb - I (I - 1) / 2 + J - I).
+__device__ __forceinline__ void sym_block_rc(int64_t b, int nb, int& I, int& J) {
+  const double m = 2.0 * nb + 1.0;
+  int i = (int)((m - sqrt(m * m - 8.0 * (double)b)) * 0.5);
+  i = max(0, min(i, nb - 1));
+  while (i > 0 && sym_block_index(i, i, nb) > b) --i;
+  while (i + 1 < nb && sym_block_index(i + 1, i + 1, nb) <= b) ++i;
+  I = i;
+  J = i + (int)(b - sym_block_index(i, i, nb));
+}
+
 // spart[I][J][0..63] = A_IJ x_J; spart[J][I][0..63] = A_IJ^T x_I (I < J).
-// The block is staged in LDS (rows padded to 65 doubles), x_I and x_J beside
+// A fixed grid of workgroups strides over the stored blocks (a launch that
+// returns at once after the stop flag then dispatches only that grid).  Each
+// block is staged in LDS (rows padded to 65 doubles) with x_I and x_J beside
 // it; threads 0..63 form the row sums, threads 64..127 the column sums.
+constexpr int kSymvGrid = 1024;
 __global__ __launch_bounds__(256) void k_symv_blocks(const double* __restrict__ P, int nb, int n,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ spart,
                                                      const int* __restrict__ done) {
   if (done && *done) return;
-  const int I = blockIdx.x, J = I + blockIdx.y;
-  if (J >= nb) return;
   __shared__ double a[kSymBlock][kSymBlock + 1];
   __shared__ double xi[kSymBlock], xj[kSymBlock];
-  const double* src = P + sym_block_index(I, J, nb) * (kSymBlock * kSymBlock);
-  double v[16];
+  const int64_t nblk = (int64_t)nb * (nb + 1) / 2;
+  for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
+    int I, J;
+    sym_block_rc(b, nb, I, J);
+    const double* src = P + b * (kSymBlock * kSymBlock);
+    double v[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = src[threadIdx.x + 256 * k];
-  if (threadIdx.x < kSymBlock) {
-    const int ri = I * kSymBlock + threadIdx.x, rj = J * kSymBlock + threadIdx.x;
-    xi[threadIdx.x] = ri < n ? x[ri] : 0.0;
-    xj[threadIdx.x] = rj < n ? x[rj] : 0.0;
-  }
+    for (int k = 0; k < 16; ++k) v[k] = src[threadIdx.x + 256 * k];
+    __syncthreads();   // the previous block's LDS reads are done
+    if (threadIdx.x < kSymBlock) {
+      const int ri = I * kSymBlock + threadIdx.x, rj = J * kSymBlock + threadIdx.x;
+      xi[threadIdx.x] = ri < n ? x[ri] : 0.0;
+      xj[threadIdx.x] = rj < n ? x[rj] : 0.0;
+    }
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int e = threadIdx.x + 256 * k;
-    a[e / kSymBlock][e % kSymBlock] = v[k];
-  }
-  __syncthreads();
-  const int t = threadIdx.x;
-  if (t < kSymBlock) {
-    double sr = 0.0;
-    for (int c = 0; c < kSymBlock; ++c) sr += a[t][c] * xj[c];
-    spart[((int64_t)I * nb + J) * kSymBlock + t] = sr;
-  } else if (t < 2 * kSymBlock && I != J) {
-    const int c = t - kSymBlock;
-    double sc = 0.0;
-    for (int r = 0; r < kSymBlock; ++r) sc += a[r][c] * xi[r];
-    spart[((int64_t)J * nb + I) * kSymBlock + c] = sc;
+    for (int k = 0; k < 16; ++k) {
+      const int e = threadIdx.x + 256 * k;
+      a[e / kSymBlock][e % kSymBlock] = v[k];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < kSymBlock) {
+      double sr = 0.0;
+      for (int c = 0; c < kSymBlock; ++c) sr += a[t][c] * xj[c];
+      spart[((int64_t)I * nb + J) * kSymBlock + t] = sr;
+    } else if (t < 2 * kSymBlock && I != J) {
+      const int c = t - kSymBlock;
+      double sc = 0.0;
+      for (int r = 0; r < kSymBlock; ++r) sc += a[r][c] * xi[r];
+      spart[((int64_t)J * nb + I) * kSymBlock + c] = sc;
+    }
   }
 }
 
-// y[r] = sum over J of spart[r / 64][J][r % 64], J in order (rows < n).
+// y[r] = sum over J of spart[r / 64][J][r % 64] (rows < n): four lanes per
+// row, lane k summing the k-th quarter of J in order (loads of 8 columns in
+// flight), then ((q0 + q1) + (q2 + q3)).
 __global__ __launch_bounds__(256) void k_symv_sum(const double* __restrict__ spart, int nb, int n,
                                                   double* __restrict__ y,
                                                   const int* __restrict__ done) {
   if (done && *done) return;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const int I = r / kSymBlock, l = r % kSymBlock;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = gt >> 2, k = gt & 3;
+  const bool row = r < n;
+  const int I = (row ? r : 0) / kSymBlock, l = (row ? r : 0) % kSymBlock;
   const double* p = spart + (int64_t)I * nb * kSymBlock + l;
+  const int q = (nb + 3) / 4, j0 = k * q, j1 = min(nb, j0 + q);
   double s = 0.0;
-  // the loads of 8 block columns go out together; the sum runs over J in
-  // order (clamped indices past the last column are left out)
   constexpr int U = 8;
-  for (int J0 = 0; J0 < nb; J0 += U) {
+  for (int J0 = j0; J0 < j1; J0 += U) {
     double v[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = p[(int64_t)min(J0 + u, nb - 1) * kSymBlock];
+    for (int u = 0; u < U; ++u) v[u] = p[(int64_t)min(J0 + u, j1 - 1) * kSymBlock];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (J0 + u < nb) s += v[u];
+      if (J0 + u < j1) s += v[u];
   }
-  y[r] = s;
+  // lanes 4m .. 4m + 3 of a wave hold one row's quarters
+  const double s1 = __shfl_xor(s, 1, 64);
+  const double h = (k & 1) ? s1 + s : s + s1;   // (q0 + q1) on lanes 0, 1; (q2 + q3) on 2, 3
+  const double h2 = __shfl_xor(h, 2, 64);
+  if (row && k == 0) y[r] = h + h2;
 }
 
 // The packed copy of a large coarse inverse (after every factorisation).
@@ -1375,9 +1399,10 @@ int coarse_apply(Handle* h, const double* t1, const double* t2, const double* be
   const int blocks = (n * 64 + 255) / 256;
   if (h->pk_nb > 0 && t2 == nullptr && !refine) {
     const int nb = h->pk_nb;
-    hipLaunchKernelGGL(k_symv_blocks, dim3(nb, nb), dim3(256), 0, s, (const double*)h->Epk, nb,
-                       n, t1, h->spart, done);
-    hipLaunchKernelGGL(k_symv_sum, dim3((n + 255) / 256), dim3(256), 0, s,
+    const int64_t nblk = (int64_t)nb * (nb + 1) / 2;
+    hipLaunchKernelGGL(k_symv_blocks, dim3((unsigned)std::min<int64_t>(nblk, kSymvGrid)),
+                       dim3(256), 0, s, (const double*)h->Epk, nb, n, t1, h->spart, done);
+    hipLaunchKernelGGL(k_symv_sum, dim3((4 * n + 255) / 256), dim3(256), 0, s,
                        (const double*)h->spart, nb, n, mu, done);
     PYTO_CHECK_HIP(hipGetLastError());
     return PYTO_OK;
