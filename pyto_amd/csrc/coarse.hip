@@ -1178,19 +1178,39 @@ __global__ __launch_bounds__(256) void k_galerkin_reduce(const int2* __restrict_
 // E <- (E + E^T)/2 + (sa + sr |E_ii|) I on the nc x nc block, identity on the
 // padding.  A diagonal that is exactly zero (a group whose nodes are all
 // fixed, so W has no rows for it) becomes 1: that unknown decouples.
-__global__ void k_symmetrize(double* __restrict__ E, int nc, int ld, double sa, double sr) {
-  const int i = blockIdx.y;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j <= i && j < ld; j += gridDim.x * blockDim.x) {
+// One workgroup per pair of 64 x 64 tiles (I, J), J <= I.  Both tiles are
+// read with whole rows into LDS, each entry pair (i, j), i >= j, is formed
+// once as 0.5 (E_ij + E_ji), and both tiles are written back with whole rows.
+// ld is a multiple of 64.
+__global__ __launch_bounds__(256) void k_symmetrize_tiles(double* __restrict__ E, int nc, int ld,
+                                                          double sa, double sr) {
+  const int I = blockIdx.x, J = blockIdx.y;
+  if (J > I) return;
+  __shared__ double A[64][65], B[64][65];   // tile (I, J) and tile (J, I)
+  for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    A[r][c] = E[(int64_t)(I * 64 + r) * ld + J * 64 + c];
+    B[r][c] = E[(int64_t)(J * 64 + r) * ld + I * 64 + c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    if (I == J && c > r) continue;   // a diagonal tile's pairs from its lower half
+    const int i = I * 64 + r, j = J * 64 + c;   // i >= j
+    double v;
     if (i < nc && j < nc) {
-      double v = 0.5 * (E[(int64_t)i * ld + j] + E[(int64_t)j * ld + i]);
+      v = 0.5 * (A[r][c] + B[c][r]);
       if (i == j) v = (v == 0.0) ? 1.0 : v + sa + sr * fabs(v);
-      E[(int64_t)i * ld + j] = v;
-      E[(int64_t)j * ld + i] = v;
     } else {
-      const double v = (i == j) ? 1.0 : 0.0;
-      E[(int64_t)i * ld + j] = v;
-      E[(int64_t)j * ld + i] = v;
+      v = (i == j) ? 1.0 : 0.0;
     }
+    A[r][c] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    E[(int64_t)(I * 64 + r) * ld + J * 64 + c] = (I == J && c > r) ? A[c][r] : A[r][c];
+    if (I != J) E[(int64_t)(J * 64 + r) * ld + I * 64 + c] = A[c][r];
   }
 }
 
@@ -1630,8 +1650,8 @@ static int coarse_assemble(Handle* h, hipStream_t s) {
                        (const double*)h->gal_part, h->Ec, ld);
     PYTO_CHECK_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_symmetrize, dim3((ld + 255) / 256, ld), dim3(256), 0, s, h->Ec, h->nc, ld,
-                     h->shift_abs, h->shift_rel);
+  hipLaunchKernelGGL(k_symmetrize_tiles, dim3(ld / 64, ld / 64), dim3(256), 0, s, h->Ec, h->nc,
+                     ld, h->shift_abs, h->shift_rel);
   PYTO_CHECK_HIP(hipGetLastError());
   return PYTO_OK;
 }
